@@ -1,0 +1,131 @@
+/*
+ * picugpu: MI355X (gfx950) big-integer multiply engine behind a plain C ABI.
+ *
+ * This library replaces the limb product inside BigFloat::mul
+ * (reference BigFloat.cpp:377-441) -- the FFT/NTT convolution engines of
+ * FFT.h:4-9 / NTT.h:4-9 and their calls -- with HIP kernels.  The metadata
+ * logic of BigFloat::mul (zero test, precision chop, sign, exponent and the
+ * one-limb trim, BigFloat.cpp:333-375,411-412,436-437) stays with the caller;
+ * piculator_amd/host/BigFloat.cpp is such a caller.
+ *
+ * Limbs are little-endian uint32_t in radix 2^32, as BigFloat::T
+ * (BigFloat.h:36).  Every entry point computes the EXACT integer product of
+ * the two limb vectors; the result never depends on the engine, the primes,
+ * the transform length or batching.
+ *
+ * Conventions: no C++ types cross this boundary; every function returns a
+ * pcg_status; pcg_last_error_string() describes the last failure of the
+ * calling thread.  Calls are thread-safe: they serialise on one internal
+ * lock per library instance (one GPU, one stream).
+ */
+#ifndef PICUGPU_H
+#define PICUGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pcg_status {
+  PCG_OK = 0,
+  PCG_ERR_INVALID = 1,    /* bad argument (null pointer, bad flag)                          */
+  PCG_ERR_NOT_INIT = 2,   /* pcg_init() was not called or failed                            */
+  PCG_ERR_NO_DEVICE = 3,  /* no HIP device / the device cannot be used                      */
+  PCG_ERR_HIP = 4,        /* a HIP runtime call failed                                      */
+  PCG_ERR_TOO_LARGE = 5,  /* operands exceed the engine limits; analog of the reference's   */
+                          /* throw "FFT length is too small." (NTT.cpp:352-353)              */
+  PCG_ERR_OOM = 6,        /* device or pinned-host allocation failed                        */
+  PCG_ERR_INTERNAL = 7    /* an internal consistency check failed (e.g. carry overflow)     */
+} pcg_status;
+
+/* Engine flags (the reference picks FFT for k < 27, NTT otherwise,
+ * BigFloat.cpp:386; here AUTO picks per size class). */
+#define PCG_ENGINE_AUTO 0u
+#define PCG_ENGINE_NTT 1u  /* two-prime NTT, always exact                            */
+#define PCG_ENGINE_FFT 2u  /* FP64 complex FFT with round-off check and NTT recompute */
+#define PCG_ENGINE_MASK 3u
+
+/* Limits: na + nb <= 2^30 limbs and min(na, nb) <= 2^29 limbs. */
+#define PCG_MAX_TOTAL_LIMBS (1ull << 30)
+#define PCG_MAX_MIN_LIMBS (1ull << 29)
+
+/* Initialise on HIP device `device` (tables, stream).  Replaces the table
+ * pre-sizing of Pi() (PI.h:145-146) and NNT::init / fft_ensure_table
+ * (NTT.cpp:169-188,297-304; FFT.cpp:31-60).  Idempotent. */
+int pcg_init(int device);
+
+/* Release device memory and the stream. */
+void pcg_shutdown(void);
+
+/* Text of the last error on the calling thread ("" if none). */
+const char* pcg_last_error_string(void);
+
+/* Make sure scratch for products up to `total_limbs` = na + nb is allocated
+ * (the analog of ntt_ensure_table(k) / fft_ensure_table(k), NTT.h:4, FFT.h:4). */
+int pcg_reserve(uint64_t total_limbs);
+
+/* out[0 .. na+nb) = a[0 .. na) * b[0 .. nb), host pointers, untrimmed
+ * (out may have one zero top limb; the caller trims as BigFloat.cpp:411-412).
+ * Replaces the engine block of BigFloat::mul, BigFloat.cpp:377-441.
+ * na == 0 or nb == 0 writes na+nb zero limbs. */
+int pcg_mul(const uint32_t* a, uint64_t na, const uint32_t* b, uint64_t nb, uint32_t* out,
+            uint32_t flags);
+
+/* Same on device pointers, enqueued on the library stream; returns after
+ * enqueueing unless PCG_SYNC is set in flags. */
+#define PCG_SYNC 0x100u
+int pcg_mul_dev(const uint32_t* d_a, uint64_t na, const uint32_t* d_b, uint64_t nb,
+                uint32_t* d_out, uint32_t flags);
+
+/* One product of a batch. */
+typedef struct pcg_mul_desc {
+  const uint32_t* a;
+  uint64_t na;
+  const uint32_t* b;
+  uint64_t nb;
+  uint32_t* out; /* na + nb limbs */
+} pcg_mul_desc;
+
+/* A batch of independent products (host pointers).  Replaces `count`
+ * calls of the BigFloat::mul engine block (BigFloat.cpp:377-441) made by one
+ * level of the binary-splitting tree (PI.h:110-112): products are grouped
+ * into size classes and each class runs as one batched launch sequence.
+ * PCG_BATCH_PLAIN instead runs one launch sequence per product (the plain
+ * method, for comparison). */
+#define PCG_BATCH_PLAIN 0x200u
+int pcg_mul_batch(const pcg_mul_desc* descs, uint64_t count, uint32_t flags);
+
+/* Same with device pointers in the descriptors (descs itself is host memory). */
+int pcg_mul_batch_dev(const pcg_mul_desc* descs, uint64_t count, uint32_t flags);
+
+/* Wait for all work enqueued on the library stream. */
+int pcg_sync(void);
+
+/* Device memory helpers for callers without their own allocator. */
+int pcg_dev_alloc(void** ptr, uint64_t bytes);
+int pcg_dev_free(void* ptr);
+int pcg_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
+int pcg_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
+
+/* FFT-engine round-off report of the last PCG_ENGINE_FFT product:
+ * max distance to the nearest integer over all output coefficients and
+ * whether the product was recomputed on the NTT (distance > 0.25). */
+int pcg_fft_last_roundoff(double* max_dist, int* recomputed);
+
+/* Timing of the last product run with PCG_PROFILE in flags: per-stage
+ * kernel durations in milliseconds measured with HIP events on the library
+ * stream.  stage_ms[0] forward(a), [1] forward(b)+pointwise+inverse,
+ * [2] carry; total_ms = whole enqueued sequence. */
+#define PCG_PROFILE 0x400u
+int pcg_last_profile(double* stage_ms, int n_stage, double* total_ms);
+
+/* Static facts for reports: transform length (log2) chosen for na + nb
+ * limbs by the NTT engine, and the number of passes. */
+int pcg_ntt_plan(uint64_t total_limbs, int* log2_points, int* passes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

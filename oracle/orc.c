@@ -1,0 +1,955 @@
+/*
+ * Oracle: CPU restatement of the reference Piculator arithmetic (C11).
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+ * CPU-baseline leg of bench.py as the checker / baseline.  The product never
+ * links or calls it.
+ *
+ * Written from the reference's behaviour, function by function:
+ *   NTT engine     NTT.cpp:85-123, 169-223, 258-369 (primes 125-158)
+ *   FFT engine     FFT.cpp:31-571
+ *   BigFloat       BigFloat.cpp:14-567, BigFloat.h:6 (extra precision 2)
+ *   Chudnovsky     PI.h:64-210
+ * MSVC intrinsics are restated portably: _umul128/_udiv128 as the x86-64
+ * `mul`/`div` instructions (hardware 128/64 divide, as the reference does),
+ * _addcarry/_subborrow as plain carries.  AVX code in FFT.cpp is restated as
+ * scalar complex arithmetic with the same twiddle tables and recursion.
+ * std::async / std::thread splitting is restated with pthreads.
+ *
+ * Not inherited (undefined behaviour in the reference): NTT tables exist for
+ * every k (the reference returns early for k <= 26, NTT.cpp:299) and k < 11
+ * uses the k = 11 prime (the table starts at 11, NTT.cpp:126);
+ * (uint64_t)(x + 0.5) of a negative x (FFT.cpp:546) is clamped to 0.
+ */
+#define _GNU_SOURCE
+#include "orc.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+#define EXTRA_PRECISION 2 /* BigFloat.h:6 */
+
+static _Atomic uint64_t g_mul_count;
+uint64_t orc_mul_count(void) { return atomic_load(&g_mul_count); }
+
+/* ------------------------------------------------------------------ */
+/* 64-bit modular arithmetic (NTT.cpp:85-100, 258-292)                 */
+/* ------------------------------------------------------------------ */
+
+/* a*b mod p through a 64x64->128 multiply and a 128/64 hardware divide
+ * (NTT.cpp:277-282).  Requires a, b < p so the quotient fits 64 bits. */
+uint64_t orc_modmul(uint64_t a, uint64_t b, uint64_t p) {
+#if defined(__x86_64__)
+  uint64_t hi, lo, q, r;
+  __asm__("mulq %3" : "=a"(lo), "=d"(hi) : "a"(a), "rm"(b) : "cc");
+  __asm__("divq %4" : "=a"(q), "=d"(r) : "a"(lo), "d"(hi), "rm"(p) : "cc");
+  (void)q;
+  return r;
+#else
+  return (uint64_t)(((unsigned __int128)a * b) % p);
+#endif
+}
+
+uint64_t orc_modpow(uint64_t a, uint64_t e, uint64_t p) {
+  uint64_t result = 1;
+  while (e > 0) {
+    if (e & 1) result = orc_modmul(result, a, p);
+    e >>= 1;
+    a = orc_modmul(a, a, p);
+  }
+  return result;
+}
+
+/* NNT::modadd, NTT.cpp:258-265: add with carry, -p on carry, -p if >= p. */
+static inline uint64_t modadd(uint64_t a, uint64_t b, uint64_t p) {
+  uint64_t d = a + b;
+  int cy = d < a;
+  if (cy) d -= p;
+  if (d >= p) d -= p;
+  return d;
+}
+/* NNT::modsub, NTT.cpp:268-274: subtract with borrow, +p on borrow. */
+static inline uint64_t modsub(uint64_t a, uint64_t b, uint64_t p) {
+  uint64_t d = a - b;
+  if (a < b) d += p;
+  return d;
+}
+
+/* Prime table k -> p, NTT.cpp:125-158 (k < 11 uses the k = 11 entry). */
+uint64_t orc_ntt_prime(int k) {
+  static const struct { int k; uint64_t p; } tab[] = {
+      {11, 9223372036854675457ull}, {12, 9223372036854497281ull}, {13, 9223372036854497281ull},
+      {14, 9223372036854497281ull}, {15, 9223372036853661697ull}, {16, 9223372036853661697ull},
+      {17, 9223372036844421121ull}, {18, 9223372036836950017ull}, {19, 9223372036836950017ull},
+      {20, 9223372036836950017ull}, {21, 9223372036752015361ull}, {22, 9223372036737335297ull},
+      {23, 9223372036737335297ull}, {24, 9223372036737335297ull}, {25, 9223372036083023873ull},
+      {26, 9223372035915251713ull}, {27, 9223372035915251713ull}, {28, 9223372034170421249ull},
+      {29, 9223372034170421249ull}, {30, 9223372006790004737ull}, {31, 9223372006790004737ull},
+      {32, 9223372006790004737ull}, {33, 9223371564408373249ull}, {34, 9223371280940531713ull},
+      {35, 9223371280940531713ull}, {36, 9223371280940531713ull}, {37, 9223369837831520257ull},
+      {38, 9223369837831520257ull}, {39, 9223369837831520257ull}, {40, 9223369837831520257ull},
+      {41, 9223369837831520257ull}, {42, 9223341250529198081ull}};
+  if (k < 11) k = 11;
+  for (size_t i = 0; i < sizeof(tab) / sizeof(tab[0]); ++i)
+    if (tab[i].k == k) return tab[i].p;
+  return 0;
+}
+
+/* Smallest primitive root, by trial factorisation of p-1 and the same
+ * candidate order 2, 3, 4, ... (NTT.cpp:104-123). */
+uint64_t orc_ntt_generator(uint64_t p) {
+  uint64_t fact[64];
+  int nf = 0;
+  uint64_t phi = p - 1, n = phi;
+  for (uint64_t i = 2; i * i <= n; ++i)
+    if (n % i == 0) {
+      fact[nf++] = i;
+      while (n % i == 0) n /= i;
+    }
+  if (n > 1) fact[nf++] = n;
+  for (uint64_t res = 2; res <= p; ++res) {
+    int ok = 1;
+    for (int i = 0; i < nf && ok; ++i) ok &= orc_modpow(res, phi / fact[i], p) != 1;
+    if (ok) return res;
+  }
+  return (uint64_t)-1;
+}
+
+/* Per-k transform constants, NNT::init (NTT.cpp:169-188), cached per k.
+ * Built under a mutex, so concurrent first use is safe (unlike NTT.cpp:295-304). */
+typedef struct {
+  int ready;
+  uint64_t p, r, W, iW, rN;
+} nnt_t;
+static nnt_t g_nnt[64];
+static pthread_mutex_t g_nnt_mu = PTHREAD_MUTEX_INITIALIZER;
+static uint64_t g_gen_p[32], g_gen_r[32];
+static int g_gen_n;
+
+static const nnt_t* nnt_get(int k) {
+  pthread_mutex_lock(&g_nnt_mu);
+  nnt_t* t = &g_nnt[k];
+  if (!t->ready) {
+    uint64_t n = 1ull << k;
+    uint64_t p = orc_ntt_prime(k);
+    uint64_t r = 0;
+    for (int i = 0; i < g_gen_n; ++i)
+      if (g_gen_p[i] == p) r = g_gen_r[i];
+    if (!r) {
+      r = orc_ntt_generator(p);
+      if (g_gen_n < 32) { g_gen_p[g_gen_n] = p; g_gen_r[g_gen_n] = r; ++g_gen_n; }
+    }
+    uint64_t L = (p - 1) / n;
+    t->p = p;
+    t->r = r;
+    t->W = orc_modpow(r, L, p);
+    t->iW = orc_modpow(r, p - 1 - L, p);
+    t->rN = orc_modpow(n, p - 2, p);
+    t->ready = 1;
+  }
+  pthread_mutex_unlock(&g_nnt_mu);
+  return t;
+}
+
+void orc_ntt_roots(int k, uint64_t* p, uint64_t* W, uint64_t* iW, uint64_t* rN) {
+  const nnt_t* t = nnt_get(k);
+  if (p) *p = t->p;
+  if (W) *W = t->W;
+  if (iW) *iW = t->iW;
+  if (rN) *rN = t->rN;
+}
+
+/* NNT::NTT_fast, NTT.cpp:191-223: recursive radix-2, natural order in and
+ * out; src is clobbered and used as scratch; dst receives the transform. */
+typedef struct {
+  uint64_t *dst, *src, n, w, p;
+  int threads;
+} ntt_job;
+static void ntt_fast(uint64_t* dst, uint64_t* src, uint64_t n, uint64_t w, uint64_t p, int threads);
+static void* ntt_thread(void* arg) {
+  ntt_job* j = (ntt_job*)arg;
+  ntt_fast(j->dst, j->src, j->n, j->w, j->p, j->threads);
+  return NULL;
+}
+static void ntt_fast(uint64_t* dst, uint64_t* src, uint64_t n, uint64_t w, uint64_t p, int threads) {
+  if (n <= 1) {
+    if (n == 1) dst[0] = src[0];
+    return;
+  }
+  uint64_t i, j, n2 = n >> 1, w2 = orc_modmul(w, w, p);
+  for (i = 0, j = 0; i < n2; i++, j += 2) dst[i] = src[j];
+  for (j = 1; i < n; i++, j += 2) dst[i] = src[j];
+  if (threads <= 1) {
+    ntt_fast(src, dst, n2, w2, p, 1);
+    ntt_fast(src + n2, dst + n2, n2, w2, p, 1);
+  } else {
+    int tds0 = threads / 2, tds1 = threads - tds0;
+    ntt_job job = {src, dst, n2, w2, p, tds0};
+    pthread_t th;
+    int spawned = pthread_create(&th, NULL, ntt_thread, &job) == 0;
+    if (!spawned) ntt_thread(&job);
+    ntt_fast(src + n2, dst + n2, n2, w2, p, tds1);
+    if (spawned) pthread_join(th, NULL);
+  }
+  uint64_t a0, a1;
+  for (w2 = 1, i = 0, j = n2; i < n2; i++, j++, w2 = orc_modmul(w2, w, p)) {
+    a0 = src[i];
+    a1 = orc_modmul(src[j], w2, p);
+    dst[i] = modadd(a0, a1, p);
+    dst[j] = modsub(a0, a1, p);
+  }
+}
+
+void orc_ntt_forward(uint64_t* T, int k, int threads) {
+  const nnt_t* t = nnt_get(k);
+  size_t n = (size_t)1 << k;
+  uint64_t* D = (uint64_t*)malloc(n * sizeof(uint64_t));
+  ntt_fast(D, T, n, t->W, t->p, threads);
+  memcpy(T, D, n * sizeof(uint64_t));
+  free(D);
+}
+
+void orc_ntt_inverse(uint64_t* T, int k, int threads) {
+  const nnt_t* t = nnt_get(k);
+  size_t n = (size_t)1 << k;
+  uint64_t* D = (uint64_t*)malloc(n * sizeof(uint64_t));
+  ntt_fast(D, T, n, t->iW, t->p, threads);
+  for (size_t i = 0; i < n; i++) D[i] = orc_modmul(D[i], t->rN, t->p);
+  memcpy(T, D, n * sizeof(uint64_t));
+  free(D);
+}
+
+void orc_ntt_pointwise(uint64_t* T, const uint64_t* A, int k) {
+  const nnt_t* t = nnt_get(k);
+  size_t n = (size_t)1 << k;
+  for (size_t i = 0; i < n; i++) T[i] = orc_modmul(T[i], A[i], t->p);
+}
+
+/* NTT.cpp:330-341: two 16-bit digits per limb, zero padded. */
+void orc_int_to_ntt(uint64_t* T, int k, const uint32_t* A, size_t AL) {
+  for (size_t i = 0; i < AL; i++) {
+    uint32_t X = A[i];
+    T[2 * i] = X % 65536;
+    X /= 65536;
+    T[2 * i + 1] = X % 65536;
+  }
+  for (size_t i = 2 * AL; i < ((size_t)1 << k); i++) T[i] = 0;
+}
+
+/* NTT.cpp:343-369: serial base-2^16 carry, two digits per output limb. */
+int orc_ntt_to_int(const uint64_t* T, int k, uint32_t* A, size_t AL) {
+  size_t len = (size_t)1 << k;
+  if (len < 2 * AL) return -1; /* "FFT length is too small." */
+  uint64_t carry = 0;
+  for (size_t c = 0; c < AL; c++) {
+    uint32_t word;
+    carry += T[2 * c];
+    word = (uint32_t)(carry % 65536);
+    carry /= 65536;
+    carry += T[2 * c + 1];
+    word += (uint32_t)((carry % 65536) * 65536);
+    carry /= 65536;
+    A[c] = word;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* FP64 complex FFT (FFT.cpp)                                          */
+/* ------------------------------------------------------------------ */
+
+/* twiddle_table[k][c] = (cos(2 pi c / 2^k), sin(2 pi c / 2^k)), c < 2^(k-1),
+ * built one level at a time (FFT.cpp:31-60). */
+static double* g_tw[48];
+static int g_tw_k = -1;
+static pthread_mutex_t g_tw_mu = PTHREAD_MUTEX_INITIALIZER;
+
+void orc_fft_ensure_table(int k) {
+  pthread_mutex_lock(&g_tw_mu);
+  while (g_tw_k < k) {
+    int lv = g_tw_k + 1;
+    size_t length = (size_t)1 << lv;
+    double omega = 2 * M_PI / length;
+    length /= 2;
+    double* sub = (double*)malloc((length ? length : 1) * 2 * sizeof(double));
+    for (size_t c = 0; c < length; c++) {
+      double angle = omega * c;
+      sub[2 * c] = cos(angle);
+      sub[2 * c + 1] = sin(angle);
+    }
+    g_tw[lv] = sub;
+    g_tw_k = lv;
+  }
+  pthread_mutex_unlock(&g_tw_mu);
+}
+
+typedef struct {
+  double* T;
+  int k, threads;
+} fft_job;
+static void* fft_fwd_thread(void* arg) {
+  fft_job* j = (fft_job*)arg;
+  orc_fft_forward(j->T, j->k, j->threads);
+  return NULL;
+}
+static void* fft_inv_thread(void* arg) {
+  fft_job* j = (fft_job*)arg;
+  orc_fft_inverse(j->T, j->k, j->threads);
+  return NULL;
+}
+
+/* Decimation in frequency: T[c] = a + b, T[c + N/2] = (a - b) e^{+2 pi i c/N},
+ * then both halves recursively; output in bit-reversed order (FFT.cpp:64-265).
+ * The hand-unrolled k = 2, 3 cases of the reference compute the same
+ * butterflies with the exact twiddles 1, i, (1+i)/sqrt2 ...; the general
+ * rule with table twiddles is used for them here. */
+void orc_fft_forward(double* T, int k, int threads) {
+  if (k <= 0) return;
+  if (k == 1) {
+    double ar = T[0], ai = T[1], br = T[2], bi = T[3];
+    T[0] = ar + br; T[1] = ai + bi;
+    T[2] = ar - br; T[3] = ai - bi;
+    return;
+  }
+  size_t half = (size_t)1 << (k - 1);
+  const double* tw = g_tw[k];
+  for (size_t c = 0; c < half; c++) {
+    double ar = T[2 * c], ai = T[2 * c + 1];
+    double br = T[2 * (c + half)], bi = T[2 * (c + half) + 1];
+    double dr = ar - br, di = ai - bi;
+    double wr = tw[2 * c], wi = tw[2 * c + 1];
+    T[2 * c] = ar + br;
+    T[2 * c + 1] = ai + bi;
+    T[2 * (c + half)] = dr * wr - di * wi;
+    T[2 * (c + half) + 1] = di * wr + dr * wi;
+  }
+  if (threads <= 1) {
+    orc_fft_forward(T, k - 1, 1);
+    orc_fft_forward(T + 2 * half, k - 1, 1);
+  } else {
+    int tds0 = threads / 2, tds1 = threads - tds0;
+    fft_job job = {T, k - 1, tds0};
+    pthread_t th;
+    int spawned = pthread_create(&th, NULL, fft_fwd_thread, &job) == 0;
+    if (!spawned) fft_fwd_thread(&job);
+    orc_fft_forward(T + 2 * half, k - 1, tds1);
+    if (spawned) pthread_join(th, NULL);
+  }
+}
+
+/* Decimation in time with conjugate twiddles, bit-reversed input, natural
+ * output, unscaled (FFT.cpp:266-463). */
+void orc_fft_inverse(double* T, int k, int threads) {
+  if (k <= 0) return;
+  if (k == 1) {
+    double ar = T[0], ai = T[1], br = T[2], bi = T[3];
+    T[0] = ar + br; T[1] = ai + bi;
+    T[2] = ar - br; T[3] = ai - bi;
+    return;
+  }
+  size_t half = (size_t)1 << (k - 1);
+  if (threads <= 1) {
+    orc_fft_inverse(T, k - 1, 1);
+    orc_fft_inverse(T + 2 * half, k - 1, 1);
+  } else {
+    int tds0 = threads / 2, tds1 = threads - tds0;
+    fft_job job = {T, k - 1, tds0};
+    pthread_t th;
+    int spawned = pthread_create(&th, NULL, fft_inv_thread, &job) == 0;
+    if (!spawned) fft_inv_thread(&job);
+    orc_fft_inverse(T + 2 * half, k - 1, tds1);
+    if (spawned) pthread_join(th, NULL);
+  }
+  const double* tw = g_tw[k];
+  for (size_t c = 0; c < half; c++) {
+    double ar = T[2 * c], ai = T[2 * c + 1];
+    double br = T[2 * (c + half)], bi = T[2 * (c + half) + 1];
+    double wr = tw[2 * c], wi = -tw[2 * c + 1];
+    double cr = br * wr - bi * wi, ci = bi * wr + br * wi;
+    T[2 * c] = ar + cr;
+    T[2 * c + 1] = ai + ci;
+    T[2 * (c + half)] = ar - cr;
+    T[2 * (c + half) + 1] = ai - ci;
+  }
+}
+
+/* FFT.cpp:464-484: complex pointwise product T *= A. */
+void orc_fft_pointwise(double* T, const double* A, int k) {
+  size_t n = (size_t)1 << k;
+  for (size_t c = 0; c < n; c++) {
+    double ar = T[2 * c], ai = T[2 * c + 1], br = A[2 * c], bi = A[2 * c + 1];
+    T[2 * c] = ar * br - ai * bi;
+    T[2 * c + 1] = ai * br + ar * bi;
+  }
+}
+
+/* FFT.cpp:485-519: four 8-bit digits per limb as real parts. */
+void orc_int_to_fft(double* T, int k, const uint32_t* A, size_t AL) {
+  size_t n = (size_t)1 << k;
+  size_t i = 0;
+  for (size_t c = 0; c < AL; c++) {
+    uint32_t word = A[c];
+    for (int d = 0; d < 4; d++) {
+      T[2 * i] = (double)(d < 3 ? word % 256 : word);
+      T[2 * i + 1] = 0.0;
+      word /= 256;
+      ++i;
+    }
+  }
+  for (; i < n; i++) T[2 * i] = T[2 * i + 1] = 0.0;
+}
+
+/* FFT.cpp:520-571: scale by 1/N, round with (uint64_t)(x + 0.5), carry in
+ * base 256, four digits per limb. */
+int orc_fft_to_int(const double* T, int k, uint32_t* A, size_t AL) {
+  size_t n = (size_t)1 << k;
+  double scale = 1. / n;
+  if (n < 4 * AL) return -1; /* "FFT length is too small." */
+  uint64_t carry = 0;
+  size_t i = 0;
+  for (size_t c = 0; c < AL; c++) {
+    uint32_t word = 0;
+    for (int d = 0; d < 4; d++) {
+      double f = T[2 * i++] * scale;
+      double g = f + 0.5;
+      uint64_t ip = g > 0 ? (uint64_t)g : 0;
+      carry += ip;
+      word += (uint32_t)((carry % 256) << (8 * d));
+      carry /= 256;
+    }
+    A[c] = word;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* The limb product of BigFloat::mul (BigFloat.cpp:377-441)            */
+/* ------------------------------------------------------------------ */
+int orc_mul_limbs(const uint32_t* AT, size_t AL, const uint32_t* BT, size_t BL, uint32_t* out,
+                  int engine, int threads) {
+  size_t zL = AL + BL;
+  int k = 0;
+  size_t length = 1;
+  while (length < 4 * zL) {
+    length <<= 1;
+    k++;
+  }
+  int use_fft = engine == ORC_ENGINE_FFT || (engine == ORC_ENGINE_REFERENCE && k < 27);
+  int rc;
+  atomic_fetch_add(&g_mul_count, 1);
+  if (use_fft) {
+    double* Ta = (double*)aligned_alloc(64, length * 16);
+    double* Tb = (double*)aligned_alloc(64, length * 16);
+    if (!Ta || !Tb) { free(Ta); free(Tb); return -2; }
+    orc_fft_ensure_table(k);
+    orc_int_to_fft(Ta, k, AT, AL);
+    orc_int_to_fft(Tb, k, BT, BL);
+    orc_fft_forward(Ta, k, k > 15 ? threads : 1);
+    orc_fft_forward(Tb, k, k > 15 ? threads : 1);
+    orc_fft_pointwise(Ta, Tb, k);
+    orc_fft_inverse(Ta, k, k > 15 ? threads : 1);
+    rc = orc_fft_to_int(Ta, k, out, zL);
+    free(Ta);
+    free(Tb);
+  } else {
+    uint64_t* Ta = (uint64_t*)malloc(length * 8);
+    uint64_t* Tb = (uint64_t*)malloc(length * 8);
+    if (!Ta || !Tb) { free(Ta); free(Tb); return -2; }
+    orc_int_to_ntt(Ta, k, AT, AL);
+    orc_int_to_ntt(Tb, k, BT, BL);
+    orc_ntt_forward(Ta, k, threads);
+    orc_ntt_forward(Tb, k, threads);
+    orc_ntt_pointwise(Ta, Tb, k);
+    orc_ntt_inverse(Ta, k, threads);
+    rc = orc_ntt_to_int(Ta, k, out, zL);
+    free(Ta);
+    free(Tb);
+  }
+  return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* BigFloat (BigFloat.cpp)                                             */
+/* ------------------------------------------------------------------ */
+static void bf_init(orc_bf* z) {
+  z->sign = 1;
+  z->exp = 0;
+  z->L = 0;
+  z->T = NULL;
+}
+void orc_bf_free(orc_bf* x) {
+  free(x->T);
+  bf_init(x);
+}
+static void bf_move(orc_bf* dst, orc_bf* src) {
+  if (dst == src) return;
+  free(dst->T);
+  *dst = *src;
+  bf_init(src);
+}
+
+void orc_bf_from_u32(orc_bf* z, uint32_t x, int sign) {
+  free(z->T);
+  bf_init(z);
+  if (x == 0) return;
+  z->sign = sign;
+  z->L = 1;
+  z->T = (uint32_t*)malloc(sizeof(uint32_t));
+  z->T[0] = x;
+}
+
+void orc_bf_copy(orc_bf* z, const orc_bf* x) {
+  if (z == x) return;
+  uint32_t* T = (uint32_t*)malloc((x->L ? x->L : 1) * sizeof(uint32_t));
+  if (x->L) memcpy(T, x->T, x->L * sizeof(uint32_t));
+  free(z->T);
+  z->sign = x->sign;
+  z->exp = x->exp;
+  z->L = x->L;
+  z->T = T;
+}
+
+/* BigFloat.cpp:109-122 */
+static inline uint32_t word_at(const orc_bf* a, int64_t mag) {
+  if (mag < a->exp) return 0;
+  if (mag >= a->exp + (int64_t)a->L) return 0;
+  return a->T[(size_t)(mag - a->exp)];
+}
+
+int orc_bf_ucmp(const orc_bf* a, const orc_bf* x) {
+  int64_t magA = a->exp + (int64_t)a->L;
+  int64_t magB = x->exp + (int64_t)x->L;
+  if (magA > magB) return 1;
+  if (magA < magB) return -1;
+  int64_t mag = magA;
+  while (mag >= a->exp || mag >= x->exp) {
+    uint32_t wA = word_at(a, mag), wB = word_at(x, mag);
+    if (wA < wB) return -1;
+    if (wA > wB) return 1;
+    mag--;
+  }
+  return 0;
+}
+
+void orc_bf_negate(orc_bf* a) {
+  if (a->L == 0) return;
+  a->sign = !a->sign;
+}
+
+void orc_bf_mul_u32(orc_bf* out, const orc_bf* a, uint32_t x) {
+  orc_bf z;
+  bf_init(&z);
+  if (a->L == 0 || x == 0) {
+    bf_move(out, &z);
+    return;
+  }
+  z.sign = a->sign;
+  z.exp = a->exp;
+  z.L = a->L;
+  z.T = (uint32_t*)malloc((z.L + 1) * sizeof(uint32_t));
+  uint64_t carry = 0;
+  for (size_t c = 0; c < a->L; c++) {
+    carry += (uint64_t)a->T[c] * x;
+    z.T[c] = (uint32_t)(carry % 4294967296ull);
+    carry >>= 32;
+  }
+  if (carry != 0) z.T[z.L++] = (uint32_t)carry;
+  bf_move(out, &z);
+}
+
+/* BigFloat.cpp:184-232 */
+static void bf_uadd(orc_bf* out, const orc_bf* a, const orc_bf* x, size_t p) {
+  int64_t magA = a->exp + (int64_t)a->L;
+  int64_t magB = x->exp + (int64_t)x->L;
+  int64_t top = magA > magB ? magA : magB;
+  int64_t bot = a->exp < x->exp ? a->exp : x->exp;
+  int64_t TL = top - bot;
+  if (p == 0)
+    p = (size_t)TL;
+  else
+    p += EXTRA_PRECISION;
+  if (TL > (int64_t)p) {
+    bot = top - (int64_t)p;
+    TL = (int64_t)p;
+  }
+  orc_bf z;
+  bf_init(&z);
+  z.sign = a->sign;
+  z.exp = bot;
+  z.L = (uint32_t)TL;
+  z.T = (uint32_t*)malloc((z.L + 1) * sizeof(uint32_t));
+  uint32_t carry = 0;
+  for (size_t c = 0; bot < top; bot++, c++) {
+    uint64_t s = (uint64_t)word_at(a, bot) + word_at(x, bot) + carry;
+    z.T[c] = (uint32_t)s;
+    carry = (uint32_t)(s >> 32);
+  }
+  if (carry != 0) z.T[z.L++] = 1;
+  bf_move(out, &z);
+}
+
+/* BigFloat.cpp:233-287; requires |a| >= |x|. */
+static void bf_usub(orc_bf* out, const orc_bf* a, const orc_bf* x, size_t p) {
+  int64_t magA = a->exp + (int64_t)a->L;
+  int64_t magB = x->exp + (int64_t)x->L;
+  int64_t top = magA > magB ? magA : magB;
+  int64_t bot = a->exp < x->exp ? a->exp : x->exp;
+  int64_t TL = top - bot;
+  if (p == 0)
+    p = (size_t)TL;
+  else
+    p += EXTRA_PRECISION;
+  if (TL > (int64_t)p) {
+    bot = top - (int64_t)p;
+    TL = (int64_t)p;
+  }
+  orc_bf z;
+  bf_init(&z);
+  z.sign = a->sign;
+  z.exp = bot;
+  z.L = (uint32_t)TL;
+  z.T = (uint32_t*)malloc((z.L ? z.L : 1) * sizeof(uint32_t));
+  uint32_t borrow = 0;
+  for (size_t c = 0; bot < top; bot++, c++) {
+    uint64_t d = (uint64_t)word_at(a, bot) - word_at(x, bot) - borrow;
+    z.T[c] = (uint32_t)d;
+    borrow = (uint32_t)((d >> 32) & 1);
+  }
+  while (z.L > 0 && z.T[z.L - 1] == 0) z.L--;
+  if (z.L == 0) {
+    z.exp = 0;
+    z.sign = 1;
+    free(z.T);
+    z.T = NULL;
+  }
+  bf_move(out, &z);
+}
+
+void orc_bf_add(orc_bf* out, const orc_bf* a, const orc_bf* x, size_t p) {
+  if (a->sign == x->sign) {
+    bf_uadd(out, a, x, p);
+  } else if (orc_bf_ucmp(a, x) > 0) {
+    bf_usub(out, a, x, p);
+  } else {
+    bf_usub(out, x, a, p);
+  }
+}
+
+void orc_bf_sub(orc_bf* out, const orc_bf* a, const orc_bf* x, size_t p) {
+  if (a->sign != x->sign) {
+    bf_uadd(out, a, x, p);
+  } else if (orc_bf_ucmp(a, x) > 0) {
+    bf_usub(out, a, x, p);
+  } else {
+    orc_bf z;
+    bf_init(&z);
+    bf_usub(&z, x, a, p);
+    orc_bf_negate(&z);
+    bf_move(out, &z);
+  }
+}
+
+/* BigFloat.cpp:326-444 */
+void orc_bf_mul(orc_bf* out, const orc_bf* a, const orc_bf* x, size_t p, int engine, int threads) {
+  orc_bf z;
+  bf_init(&z);
+  if (a->L == 0 || x->L == 0) {
+    bf_move(out, &z);
+    return;
+  }
+  if (p == 0)
+    p = a->L + x->L;
+  else
+    p += EXTRA_PRECISION;
+  int64_t Aexp = a->exp, Bexp = x->exp;
+  size_t AL = a->L, BL = x->L;
+  const uint32_t* AT = a->T;
+  const uint32_t* BT = x->T;
+  if (AL > p) {
+    size_t chop = AL - p;
+    AL = p;
+    Aexp += (int64_t)chop;
+    AT += chop;
+  }
+  if (BL > p) {
+    size_t chop = BL - p;
+    BL = p;
+    Bexp += (int64_t)chop;
+    BT += chop;
+  }
+  z.sign = a->sign == x->sign;
+  z.exp = Aexp + Bexp;
+  z.L = AL + BL;
+  z.T = (uint32_t*)malloc(z.L * sizeof(uint32_t));
+  orc_mul_limbs(AT, AL, BT, BL, z.T, engine, threads);
+  if (z.T[z.L - 1] == 0) z.L--;
+  bf_move(out, &z);
+}
+
+/* BigFloat.cpp:445-511: Newton reciprocal, r1 = r0 - (r0*x - 1)*r0. */
+int orc_bf_rcp(orc_bf* out, const orc_bf* a, size_t p, int engine, int threads) {
+  if (a->L == 0) return -1; /* "Divide by Zero" */
+  int64_t Aexp = a->exp;
+  size_t AL = a->L;
+  const uint32_t* AT = a->T;
+  if (p == 0) {
+    p = 3;
+    if (AL > p) {
+      size_t chop = AL - p;
+      AL = p;
+      Aexp += (int64_t)chop;
+      AT += chop;
+    }
+    double val = AT[0];
+    if (AL >= 2) val += AT[1] * 4294967296.;
+    if (AL >= 3) val += AT[2] * 4294967296. * 4294967296.;
+    val = 1. / val;
+    Aexp = -Aexp;
+    while (val < 4294967296.) {
+      val *= 4294967296.;
+      Aexp--;
+    }
+    uint64_t val64 = (uint64_t)val;
+    orc_bf o;
+    bf_init(&o);
+    o.sign = a->sign;
+    o.T = (uint32_t*)malloc(2 * sizeof(uint32_t));
+    o.T[0] = (uint32_t)(val64 % 4294967296ull);
+    o.T[1] = (uint32_t)(val64 / 4294967296ull);
+    o.L = 2;
+    o.exp = Aexp;
+    bf_move(out, &o);
+    return 0;
+  }
+  size_t s = p / 2 + 1;
+  if (p == 1) s = 0;
+  if (p == 2) s = 1;
+  orc_bf T, t1, one, t2, t3;
+  bf_init(&T); bf_init(&t1); bf_init(&one); bf_init(&t2); bf_init(&t3);
+  int rc = orc_bf_rcp(&T, a, s, engine, threads);
+  if (rc) return rc;
+  orc_bf_mul(&t1, a, &T, p, engine, threads);
+  orc_bf_from_u32(&one, 1, 1);
+  orc_bf_sub(&t2, &t1, &one, p);
+  orc_bf_mul(&t3, &t2, &T, p, engine, threads);
+  orc_bf_sub(out, &T, &t3, p);
+  orc_bf_free(&T); orc_bf_free(&t1); orc_bf_free(&one); orc_bf_free(&t2); orc_bf_free(&t3);
+  return 0;
+}
+
+int orc_bf_div(orc_bf* out, const orc_bf* a, const orc_bf* x, size_t p, int engine, int threads) {
+  orc_bf r;
+  bf_init(&r);
+  int rc = orc_bf_rcp(&r, x, p, engine, threads);
+  if (rc) return rc;
+  orc_bf_mul(out, a, &r, p, engine, threads);
+  orc_bf_free(&r);
+  return 0;
+}
+
+/* BigFloat.cpp:516-567: r1 = r0 - (r0^2*x - 1)/2 * r0. */
+int orc_bf_invsqrt(orc_bf* out, uint32_t x, size_t p, int engine, int threads) {
+  if (x == 0) return -1; /* "Divide by Zero" */
+  if (p == 0) {
+    double val = 1. / sqrt((double)x);
+    int64_t exponent = 0;
+    while (val < 4294967296.) {
+      val *= 4294967296.;
+      exponent--;
+    }
+    uint64_t val64 = (uint64_t)val;
+    orc_bf o;
+    bf_init(&o);
+    o.sign = 1;
+    o.T = (uint32_t*)malloc(2 * sizeof(uint32_t));
+    o.T[0] = (uint32_t)(val64 % 4294967296ull);
+    o.T[1] = (uint32_t)(val64 / 4294967296ull);
+    o.L = 2;
+    o.exp = exponent;
+    bf_move(out, &o);
+    return 0;
+  }
+  size_t s = p / 2 + 1;
+  if (p == 1) s = 0;
+  if (p == 2) s = 1;
+  orc_bf T, temp, xb, one, t;
+  bf_init(&T); bf_init(&temp); bf_init(&xb); bf_init(&one); bf_init(&t);
+  int rc = orc_bf_invsqrt(&T, x, s, engine, threads);
+  if (rc) return rc;
+  orc_bf_mul(&temp, &T, &T, p, engine, threads);       /* r0^2           */
+  orc_bf_from_u32(&xb, x, 1);
+  orc_bf_mul(&t, &temp, &xb, 0, engine, threads);       /* r0^2 * x, p=0  */
+  bf_move(&temp, &t);
+  orc_bf_from_u32(&one, 1, 1);
+  orc_bf_sub(&t, &temp, &one, p);                       /* r0^2 * x - 1   */
+  bf_move(&temp, &t);
+  orc_bf_mul_u32(&t, &temp, 2147483648u);               /* ... / 2        */
+  bf_move(&temp, &t);
+  temp.exp--;
+  orc_bf_mul(&t, &temp, &T, p, engine, threads);        /* ... * r0       */
+  bf_move(&temp, &t);
+  orc_bf_sub(out, &T, &temp, p);
+  orc_bf_free(&T); orc_bf_free(&temp); orc_bf_free(&xb); orc_bf_free(&one); orc_bf_free(&t);
+  return 0;
+}
+
+/* BigFloat.cpp:95-106: top limb in decimal, '.', then setw(8) lowercase hex
+ * limbs walking down, truncated to 2 + digits characters. */
+long orc_bf_to_hex(const orc_bf* a, size_t digits, char* out, size_t cap) {
+  if (a->L == 0) return -1;
+  size_t nl = (digits + 7) / 8;
+  if (nl > a->L - 1) return -1; /* the reference would read below T[0] */
+  char head[16];
+  int hn = snprintf(head, sizeof head, "%u.", a->T[a->L - 1]);
+  size_t total = (size_t)hn + 8 * nl;
+  size_t want = 2 + digits < total ? 2 + digits : total;
+  if (cap < want + 1) return -1;
+  char* buf = (char*)malloc(total + 1);
+  memcpy(buf, head, (size_t)hn);
+  size_t loc = a->L - 1, pos = (size_t)hn;
+  for (size_t i = 0; i < nl; i++) {
+    loc--;
+    snprintf(buf + pos, 9, "%08x", a->T[loc]);
+    pos += 8;
+  }
+  memcpy(out, buf, want);
+  out[want] = 0;
+  free(buf);
+  return (long)want;
+}
+
+/* ------------------------------------------------------------------ */
+/* Chudnovsky binary splitting (PI.h)                                  */
+/* ------------------------------------------------------------------ */
+void orc_pi_sizes(size_t decimal_digits, size_t* hex_digits, size_t* p, size_t* terms) {
+  size_t hex = (size_t)(decimal_digits * log(10) / log(16));
+  size_t pp = (hex + 9) / 9;
+  size_t p_dec = (decimal_digits + 9) / 9;
+  size_t t = (size_t)(p_dec * 0.6346230241342037371474889163921741077188431452678) + 1;
+  if (hex_digits) *hex_digits = hex;
+  if (p) *p = pp;
+  if (terms) *terms = t;
+}
+
+typedef struct {
+  orc_bf *P, *Q, *R;
+  uint32_t a, b;
+  size_t p;
+  int engine, threads;
+} bsr_job;
+static void* bsr_thread(void* arg) {
+  bsr_job* j = (bsr_job*)arg;
+  orc_pi_bsr(j->P, j->Q, j->R, j->a, j->b, j->p, j->engine, j->threads);
+  return NULL;
+}
+
+/* PI.h:64-112 (progress printing omitted). */
+void orc_pi_bsr(orc_bf* P, orc_bf* Q, orc_bf* R, uint32_t a, uint32_t b, size_t p, int engine,
+                int tds) {
+  if (b - a == 1) {
+    orc_bf t, c;
+    bf_init(&t); bf_init(&c);
+    /* P = (13591409 + 545140134 b)(2b-1)(6b-5)(6b-1) (-1)^b */
+    orc_bf_from_u32(&t, b, 1);
+    orc_bf_mul_u32(P, &t, 545140134u);
+    orc_bf_from_u32(&c, 13591409u, 1);
+    orc_bf_add(&t, P, &c, 0);
+    orc_bf_mul_u32(P, &t, 2 * b - 1);
+    orc_bf_mul_u32(&t, P, 6 * b - 5);
+    orc_bf_mul_u32(P, &t, 6 * b - 1);
+    if (b % 2 == 1) orc_bf_negate(P);
+    /* Q = 10939058860032000 * b^3 */
+    orc_bf_from_u32(&t, b, 1);
+    orc_bf_mul_u32(Q, &t, b);
+    orc_bf_mul_u32(&t, Q, b);
+    orc_bf_mul_u32(Q, &t, 26726400u);
+    orc_bf_mul_u32(&t, Q, 409297880u);
+    bf_move(Q, &t);
+    /* R = (2b-1)(6b-5)(6b-1) */
+    orc_bf_from_u32(&t, 2 * b - 1, 1);
+    orc_bf_mul_u32(R, &t, 6 * b - 5);
+    orc_bf_mul_u32(&t, R, 6 * b - 1);
+    bf_move(R, &t);
+    orc_bf_free(&c);
+    return;
+  }
+  uint32_t m = (a + b) / 2;
+  orc_bf P0, Q0, R0, P1, Q1, R1;
+  bf_init(&P0); bf_init(&Q0); bf_init(&R0); bf_init(&P1); bf_init(&Q1); bf_init(&R1);
+  if (b - a < 1000 || tds <= 1) {
+    orc_pi_bsr(&P0, &Q0, &R0, a, m, p, engine, tds);
+    orc_pi_bsr(&P1, &Q1, &R1, m, b, p, engine, tds);
+  } else {
+    int tds0 = tds / 2, tds1 = tds - tds0;
+    bsr_job j0 = {&P0, &Q0, &R0, a, m, p, engine, tds0};
+    bsr_job j1 = {&P1, &Q1, &R1, m, b, p, engine, tds1};
+    pthread_t t0, t1;
+    int s0 = pthread_create(&t0, NULL, bsr_thread, &j0) == 0;
+    if (!s0) bsr_thread(&j0);
+    int s1 = pthread_create(&t1, NULL, bsr_thread, &j1) == 0;
+    if (!s1) bsr_thread(&j1);
+    if (s0) pthread_join(t0, NULL);
+    if (s1) pthread_join(t1, NULL);
+  }
+  orc_bf x, y;
+  bf_init(&x); bf_init(&y);
+  orc_bf_mul(&x, &P0, &Q1, p, engine, tds);
+  orc_bf_mul(&y, &P1, &R0, p, engine, tds);
+  orc_bf_add(P, &x, &y, p);
+  orc_bf_mul(Q, &Q0, &Q1, p, engine, tds);
+  orc_bf_mul(R, &R0, &R1, p, engine, tds);
+  orc_bf_free(&x); orc_bf_free(&y);
+  orc_bf_free(&P0); orc_bf_free(&Q0); orc_bf_free(&R0);
+  orc_bf_free(&P1); orc_bf_free(&Q1); orc_bf_free(&R1);
+}
+
+static double wall(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* PI.h:138-210 (console output and the file write omitted). */
+long orc_pi_hex(size_t decimal_digits, int engine, int threads, char* out, size_t cap,
+                double* times) {
+  size_t hex, p, terms;
+  orc_pi_sizes(decimal_digits, &hex, &p, &terms);
+  if ((uint32_t)terms != terms) return -1; /* "Limit Exceeded" */
+  orc_bf P, Q, R, t, u;
+  bf_init(&P); bf_init(&Q); bf_init(&R); bf_init(&t); bf_init(&u);
+  double t0 = wall();
+  orc_pi_bsr(&P, &Q, &R, 0, (uint32_t)terms, p, engine, threads);
+  orc_bf_mul_u32(&t, &Q, 13591409u);
+  orc_bf_add(&u, &t, &P, p);
+  bf_move(&P, &u);
+  orc_bf_mul_u32(&t, &Q, 4270934400u);
+  bf_move(&Q, &t);
+  double t1 = wall();
+  orc_bf_div(&u, &Q, &P, p, engine, threads);
+  bf_move(&P, &u);
+  double t2 = wall();
+  orc_bf_invsqrt(&Q, 10005, p, engine, threads);
+  double t3 = wall();
+  orc_bf_mul(&u, &P, &Q, p, engine, threads);
+  bf_move(&P, &u);
+  double t4 = wall();
+  long n = orc_bf_to_hex(&P, hex, out, cap);
+  double t5 = wall();
+  if (times) {
+    times[0] = t1 - t0;
+    times[1] = t2 - t1;
+    times[2] = t3 - t2;
+    times[3] = t4 - t3;
+    times[4] = t5 - t4;
+  }
+  orc_bf_free(&P); orc_bf_free(&Q); orc_bf_free(&R); orc_bf_free(&t); orc_bf_free(&u);
+  return n;
+}

@@ -1,0 +1,217 @@
+// CRT reconstruction and exact carry propagation (device side).
+//
+// After the inverse transform each point i holds the convolution coefficient
+// c_i modulo both primes; c_i < p0 * p1 (~2^94), so CRT gives c_i exactly as
+// a 96-bit integer (w0, w1, w2) of 32-bit words.  The product is
+//     sum_i c_i 2^{32 i},
+// so limb k receives s_k = w0_k + w1_{k-1} + w2_{k-2} < 3 * 2^32 plus the
+// carry from limb k-1.  With carries below 2^32, limb k maps a carry-in c to
+//     carry_out = hi(s_k) + [c >= 2^32 - lo(s_k)]
+// i.e. a function f(c) = base + [c >= T].  Such functions are closed under
+// composition, which makes carry propagation an associative scan:
+//     (o1,T1) then (o2,T2) = (o2+1, inf)  if o1 >= T2
+//                            (o2,  T1)   if o1 + 1 == T2
+//                            (o2,  inf)  otherwise.
+// The same form describes a whole block of limbs (T finite only when all but
+// its two lowest limbs are 0xFFFFFFFF), so blocks are resolved locally with
+// carry-in 0, block functions are scanned, and each block adds its carry-in.
+#pragma once
+#include "modfp64.hpp"
+
+#pragma clang fp contract(off)
+
+namespace pcg {
+
+constexpr int kCarryThreads = 256;
+constexpr int kCarryLPT = 4;                           // limbs per thread
+constexpr int kCarryBlock = kCarryThreads * kCarryLPT;  // limbs per block
+constexpr uint64_t kInf = ~0ull;
+
+struct CarryFn {
+  uint64_t base, T;  // T == 0 marks the identity function
+};
+
+__device__ __forceinline__ CarryFn cf_compose(CarryFn f1, CarryFn f2) {
+  if (f1.T == 0) return f2;
+  if (f2.T == 0) return f1;
+  CarryFn r;
+  if (f1.base >= f2.T) {
+    r.base = f2.base + 1;
+    r.T = kInf;
+  } else if (f1.base + 1 >= f2.T) {
+    r.base = f2.base;
+    r.T = f1.T;
+  } else {
+    r.base = f2.base;
+    r.T = kInf;
+  }
+  return r;
+}
+
+__device__ __forceinline__ uint64_t cf_eval0(CarryFn f) { return f.T == 0 ? 0 : f.base; }
+
+__device__ __forceinline__ CarryFn cf_shfl_up(CarryFn f, int d) {
+  CarryFn r;
+  r.base = __shfl_up(f.base, d, 64);
+  r.T = __shfl_up(f.T, d, 64);
+  return r;
+}
+
+// Exclusive scan of one CarryFn per thread across the workgroup (<= 1024
+// threads); `wave_tot` is LDS scratch of (threads / 64) entries.
+__device__ __forceinline__ CarryFn cf_block_exclusive(CarryFn f, CarryFn* wave_tot) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+  CarryFn incl = f;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    CarryFn o = cf_shfl_up(incl, d);
+    if (lane >= d) incl = cf_compose(o, incl);
+  }
+  CarryFn excl = cf_shfl_up(incl, 1);
+  if (lane == 0) excl.T = 0;
+  if (lane == 63) wave_tot[wv] = incl;
+  __syncthreads();
+  CarryFn pre;
+  pre.base = 0;
+  pre.T = 0;
+  for (int i = 0; i < wv; ++i) pre = cf_compose(pre, wave_tot[i]);
+  (void)nw;
+  __syncthreads();
+  return cf_compose(pre, excl);
+}
+
+struct CarryArgs {
+  const Prime* prm;  // the two primes (p, pinv)
+  const double* r;   // [2][N] inverse-transform outputs
+  uint64_t N;
+  uint64_t zL;       // output limbs
+  uint32_t* out;     // zL limbs
+  CarryFn* agg;      // per block
+  uint64_t* cin;     // per block carry-in
+  uint64_t nblk;
+  double2 crt;       // p0^{-1} mod p1 as (w, w/p1)
+  uint32_t* status;  // nonzero on overflow
+};
+
+__device__ __forceinline__ void crt_words(double r0, double r1, const Prime& P0, const Prime& P1,
+                                          double2 crt, uint32_t& w0, uint32_t& w1, uint32_t& w2) {
+  r0 = canon(r0, P0.p, P0.pinv);
+  r1 = canon(r1, P1.p, P1.pinv);
+  double t = mm_shoup(r1 - r0, crt.x, crt.y, P1.p);
+  t = canon(t, P1.p, P1.pinv);
+  const uint64_t tu = (uint64_t)t, r0u = (uint64_t)r0, p0u = (uint64_t)P0.p;
+  uint64_t lo = p0u * tu;
+  uint64_t hi = __umul64hi(p0u, tu);
+  lo += r0u;
+  hi += lo < r0u;
+  w0 = (uint32_t)lo;
+  w1 = (uint32_t)(lo >> 32);
+  w2 = (uint32_t)hi;
+}
+
+__global__ __launch_bounds__(kCarryThreads) void k_carry_local(CarryArgs a) {
+  __shared__ uint32_t sw1[kCarryBlock], sw2[kCarryBlock];
+  __shared__ CarryFn wtot[kCarryThreads / 64];
+  __shared__ uint32_t d01[2];
+  const int tid = threadIdx.x;
+  const uint64_t base = (uint64_t)blockIdx.x * kCarryBlock;
+  const Prime P0 = a.prm[0], P1 = a.prm[1];
+  uint32_t w0[kCarryLPT];
+#pragma unroll
+  for (int q = 0; q < kCarryLPT; ++q) {
+    const int li = tid * kCarryLPT + q;
+    const uint64_t k = base + li;
+    uint32_t x0 = 0, x1 = 0, x2 = 0;
+    if (k < a.zL && k < a.N) crt_words(a.r[k], a.r[a.N + k], P0, P1, a.crt, x0, x1, x2);
+    w0[q] = x0;
+    sw1[li] = x1;
+    sw2[li] = x2;
+  }
+  __syncthreads();
+  uint64_t s[kCarryLPT];
+  CarryFn f;
+  f.base = 0;
+  f.T = 0;
+#pragma unroll
+  for (int q = 0; q < kCarryLPT; ++q) {
+    const int li = tid * kCarryLPT + q;
+    uint64_t x = w0[q];
+    if (li >= 1) x += sw1[li - 1];
+    if (li >= 2) x += sw2[li - 2];
+    s[q] = x;
+    CarryFn g;
+    g.base = x >> 32;
+    g.T = (1ull << 32) - (x & 0xffffffffull);
+    f = cf_compose(f, g);
+  }
+  CarryFn pre = cf_block_exclusive(f, wtot);
+  uint64_t c = cf_eval0(pre);
+  uint32_t D[kCarryLPT];
+  bool ones = true;
+#pragma unroll
+  for (int q = 0; q < kCarryLPT; ++q) {
+    const int li = tid * kCarryLPT + q;
+    const uint64_t t = (s[q] & 0xffffffffull) + c;
+    D[q] = (uint32_t)t;
+    c = (s[q] >> 32) + (t >> 32);
+    const uint64_t k = base + li;
+    if (k < a.zL) a.out[k] = D[q];
+    if (li >= 2 && D[q] != 0xffffffffu) ones = false;
+    if (li < 2) d01[li] = D[q];
+  }
+  const int all_ones = __syncthreads_and(ones ? 1 : 0);
+  if (tid == kCarryThreads - 1) {
+    const int L = kCarryBlock;
+    uint64_t spill = (uint64_t)sw1[L - 1] + (uint64_t)sw2[L - 2] + ((uint64_t)sw2[L - 1] << 32);
+    CarryFn g;
+    g.base = spill + c;
+    g.T = kInf;
+    if (all_ones) {
+      const uint64_t low = (uint64_t)d01[0] | ((uint64_t)d01[1] << 32);
+      if (low) g.T = 0 - low;  // 2^64 - low
+    }
+    a.agg[blockIdx.x] = g;
+  }
+}
+
+// One workgroup of 1024 threads: exclusive scan of the block functions.
+__global__ __launch_bounds__(1024) void k_carry_scan(CarryArgs a) {
+  __shared__ CarryFn wtot[16];
+  const uint64_t n = a.nblk;
+  const uint64_t per = (n + blockDim.x - 1) / blockDim.x;
+  const uint64_t lo = (uint64_t)threadIdx.x * per;
+  const uint64_t hi = lo + per < n ? lo + per : n;
+  CarryFn f;
+  f.base = 0;
+  f.T = 0;
+  for (uint64_t i = lo; i < hi; ++i) f = cf_compose(f, a.agg[i]);
+  CarryFn run = cf_block_exclusive(f, wtot);
+  for (uint64_t i = lo; i < hi; ++i) {
+    a.cin[i] = cf_eval0(run);
+    run = cf_compose(run, a.agg[i]);
+  }
+  if (threadIdx.x == blockDim.x - 1) {
+    // carry out of the top block must vanish for an exact product
+    if (cf_eval0(run) != 0) *a.status = 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_carry_fix(CarryArgs a) {
+  const uint64_t blk = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (blk >= a.nblk) return;
+  uint64_t c = a.cin[blk];
+  if (c == 0) return;
+  uint64_t k = blk * kCarryBlock;
+  uint64_t end = k + kCarryBlock;
+  if (end > a.zL) end = a.zL;
+  while (c != 0 && k < end) {
+    const uint64_t t = (uint64_t)a.out[k] + (c & 0xffffffffull);
+    a.out[k] = (uint32_t)t;
+    c = (c >> 32) + (t >> 32);
+    ++k;
+  }
+  if (c != 0 && end < a.zL) *a.status = 2;
+}
+
+}  // namespace pcg

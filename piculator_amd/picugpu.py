@@ -1,0 +1,211 @@
+"""ctypes binding of the picugpu C ABI (include/picugpu.h).
+
+Tests and bench.py call the GPU engine through this module, i.e. through
+exactly the C entry points a C/C++ caller (piculator_amd/host/BigFloat.cpp)
+uses.  There is no fallback: if libpicugpu.so is missing or the GPU is not
+usable, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpicugpu.so")
+
+ENGINE_AUTO = 0
+ENGINE_NTT = 1
+ENGINE_FFT = 2
+SYNC = 0x100
+BATCH_PLAIN = 0x200
+PROFILE = 0x400
+
+STATUS = {
+    0: "PCG_OK", 1: "PCG_ERR_INVALID", 2: "PCG_ERR_NOT_INIT", 3: "PCG_ERR_NO_DEVICE",
+    4: "PCG_ERR_HIP", 5: "PCG_ERR_TOO_LARGE", 6: "PCG_ERR_OOM", 7: "PCG_ERR_INTERNAL",
+}
+
+# Every symbol include/picugpu.h declares.
+EXPORTS = [
+    "pcg_init", "pcg_shutdown", "pcg_last_error_string", "pcg_reserve", "pcg_mul",
+    "pcg_mul_dev", "pcg_mul_batch", "pcg_mul_batch_dev", "pcg_sync", "pcg_dev_alloc",
+    "pcg_dev_free", "pcg_memcpy_h2d", "pcg_memcpy_d2h", "pcg_fft_last_roundoff",
+    "pcg_last_profile", "pcg_ntt_plan",
+]
+
+
+class PcgError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("%s: %s" % (STATUS.get(code, code), msg))
+        self.code = code
+
+
+class MulDesc(C.Structure):
+    _fields_ = [("a", C.c_void_p), ("na", C.c_uint64), ("b", C.c_void_p), ("nb", C.c_uint64),
+                ("out", C.c_void_p)]
+
+
+_lib = None
+
+
+def load():
+    """Load libpicugpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            "%s is missing: build it with `make -C piculator_amd/csrc` "
+            "(or __graft_entry__.build())" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    u64 = C.c_uint64
+    vp = C.c_void_p
+    L.pcg_init.argtypes = [C.c_int]
+    L.pcg_shutdown.argtypes = []
+    L.pcg_shutdown.restype = None
+    L.pcg_last_error_string.restype = C.c_char_p
+    L.pcg_reserve.argtypes = [u64]
+    L.pcg_mul.argtypes = [vp, u64, vp, u64, vp, C.c_uint32]
+    L.pcg_mul_dev.argtypes = [vp, u64, vp, u64, vp, C.c_uint32]
+    L.pcg_mul_batch.argtypes = [C.POINTER(MulDesc), u64, C.c_uint32]
+    L.pcg_mul_batch_dev.argtypes = [C.POINTER(MulDesc), u64, C.c_uint32]
+    L.pcg_dev_alloc.argtypes = [C.POINTER(vp), u64]
+    L.pcg_dev_free.argtypes = [vp]
+    L.pcg_memcpy_h2d.argtypes = [vp, vp, u64]
+    L.pcg_memcpy_d2h.argtypes = [vp, vp, u64]
+    L.pcg_fft_last_roundoff.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.pcg_last_profile.argtypes = [C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
+    L.pcg_ntt_plan.argtypes = [u64, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise PcgError(rc, load().pcg_last_error_string().decode())
+
+
+def _u32(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.uint32))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+_inited = False
+
+
+def init(device: int = 0):
+    global _inited
+    _check(load().pcg_init(device))
+    _inited = True
+
+
+def shutdown():
+    global _inited
+    load().pcg_shutdown()
+    _inited = False
+
+
+def ensure_init():
+    if not _inited:
+        init(0)
+
+
+def mul(a, b, engine: int = ENGINE_AUTO, profile: bool = False) -> np.ndarray:
+    """Exact product of two limb vectors (host memory), na+nb limbs untrimmed."""
+    ensure_init()
+    a, b = _u32(a), _u32(b)
+    out = np.empty(a.size + b.size, dtype=np.uint32)
+    flags = engine | (PROFILE if profile else 0)
+    _check(load().pcg_mul(_ptr(a), a.size, _ptr(b), b.size, _ptr(out), flags))
+    return out
+
+
+def mul_batch(pairs, engine: int = ENGINE_AUTO, plain: bool = False):
+    """Products of a list of (a, b) limb vectors through pcg_mul_batch."""
+    ensure_init()
+    keep = []
+    descs = (MulDesc * max(1, len(pairs)))()
+    outs = []
+    for i, (a, b) in enumerate(pairs):
+        a, b = _u32(a), _u32(b)
+        o = np.empty(a.size + b.size, dtype=np.uint32)
+        keep += [a, b]
+        outs.append(o)
+        descs[i] = MulDesc(a.ctypes.data, a.size, b.ctypes.data, b.size, o.ctypes.data)
+    flags = engine | (BATCH_PLAIN if plain else 0)
+    _check(load().pcg_mul_batch(descs, len(pairs), flags))
+    return outs
+
+
+class DeviceBuffer:
+    """Device memory owned through pcg_dev_alloc / pcg_dev_free."""
+
+    def __init__(self, nbytes: int):
+        ensure_init()
+        p = C.c_void_p()
+        _check(load().pcg_dev_alloc(C.byref(p), nbytes))
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def upload(self, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _check(load().pcg_memcpy_h2d(self.ptr, _ptr(arr), arr.nbytes))
+
+    def download(self, arr: np.ndarray):
+        assert arr.nbytes <= self.nbytes and arr.flags["C_CONTIGUOUS"]
+        _check(load().pcg_memcpy_d2h(_ptr(arr), self.ptr, arr.nbytes))
+        return arr
+
+    def free(self):
+        if self.ptr:
+            load().pcg_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def mul_dev(d_a: int, na: int, d_b: int, nb: int, d_out: int, engine: int = ENGINE_AUTO,
+            sync: bool = False, profile: bool = False):
+    ensure_init()
+    flags = engine | (SYNC if sync else 0) | (PROFILE if profile else 0)
+    _check(load().pcg_mul_dev(d_a, na, d_b, nb, d_out, flags))
+
+
+def sync():
+    _check(load().pcg_sync())
+
+
+def last_profile():
+    st = (C.c_double * 3)()
+    tot = C.c_double()
+    _check(load().pcg_last_profile(st, 3, C.byref(tot)))
+    return list(st), tot.value
+
+
+def fft_last_roundoff():
+    d = C.c_double()
+    r = C.c_int()
+    _check(load().pcg_fft_last_roundoff(C.byref(d), C.byref(r)))
+    return d.value, bool(r.value)
+
+
+def ntt_plan(total_limbs: int):
+    lg = C.c_int()
+    ps = C.c_int()
+    _check(load().pcg_ntt_plan(total_limbs, C.byref(lg), C.byref(ps)))
+    return lg.value, ps.value
+
+
+def reserve(total_limbs: int):
+    ensure_init()
+    _check(load().pcg_reserve(total_limbs))

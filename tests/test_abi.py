@@ -1,0 +1,41 @@
+"""The C-ABI library loads and exports every entry point include/picugpu.h
+declares (no GPU needed); error paths work without a device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from piculator_amd import picugpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    with open(os.path.join(ROOT, "include", "picugpu.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(pcg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_matches_binding_list():
+    assert declared_functions() == sorted(picugpu.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(picugpu.LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_calls_before_init_fail_cleanly():
+    lib = picugpu.load()
+    assert lib.pcg_mul(None, 0, None, 0, None, 0) == 2  # PCG_ERR_NOT_INIT
+    assert b"pcg_init" in lib.pcg_last_error_string()
+
+
+def test_plan_limits():
+    lg, passes = picugpu.ntt_plan(2 ** 25)
+    assert lg == 25 and passes == 3
+    assert picugpu.ntt_plan(3)[0] == 8
+    with pytest.raises(picugpu.PcgError):
+        picugpu.ntt_plan(2 ** 31)

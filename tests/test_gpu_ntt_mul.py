@@ -1,0 +1,85 @@
+"""GPU parity of the NTT product (through the C ABI) against exact arithmetic
+and the oracle's restatement of the reference multiply (BigFloat.cpp:377-441)."""
+import random
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from piculator_amd import workload
+from tests.helpers import exact_product, residues_match
+
+pytestmark = pytest.mark.gpu
+
+M32 = 0xFFFFFFFF
+
+
+def rand_limbs(rng, n):
+    return np.array([rng.getrandbits(32) for _ in range(n)], dtype=np.uint32)
+
+
+@pytest.mark.parametrize("na,nb", [(1, 1), (1, 7), (3, 5), (17, 33), (100, 77), (255, 1),
+                                   (129, 127), (1000, 999), (2047, 2049), (2048, 2048),
+                                   (4095, 1), (3000, 5000), (12345, 6789)])
+def test_random_small(gpu, na, nb):
+    rng = random.Random(na * 7919 + nb)
+    a, b = rand_limbs(rng, na), rand_limbs(rng, nb)
+    got = gpu.mul(a, b, gpu.ENGINE_NTT)
+    assert np.array_equal(got, exact_product(a, b))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 64, 511, 4096, 70000])
+def test_all_ones_carry_chains(gpu, n):
+    # (2^(32n) - 1)^2 has the longest carry chains
+    a = np.full(n, M32, dtype=np.uint32)
+    got = gpu.mul(a, a, gpu.ENGINE_NTT)
+    assert np.array_equal(got, exact_product(a, a))
+
+
+def test_zero_operand(gpu):
+    a = np.zeros(0, dtype=np.uint32)
+    b = np.array([5, 6], dtype=np.uint32)
+    assert np.array_equal(gpu.mul(a, b), np.zeros(2, dtype=np.uint32))
+    z = np.zeros(50, dtype=np.uint32)
+    assert np.array_equal(gpu.mul(z, b), np.zeros(52, dtype=np.uint32))
+
+
+def test_sparse_and_power_of_two(gpu):
+    a = np.zeros(3000, dtype=np.uint32)
+    a[-1] = 1 << 31
+    b = np.zeros(2000, dtype=np.uint32)
+    b[0] = 1
+    b[-1] = M32
+    assert np.array_equal(gpu.mul(a, b), exact_product(a, b))
+
+
+@pytest.mark.parametrize("na,nb", [(4000, 3000), (20000, 9000)])
+def test_against_oracle_ntt(gpu, na, nb):
+    rng = random.Random(na)
+    a, b = rand_limbs(rng, na), rand_limbs(rng, nb)
+    ref = O.mul_limbs(a, b, O.ENGINE_NTT)
+    assert np.array_equal(gpu.mul(a, b, gpu.ENGINE_NTT), ref)
+
+
+@pytest.mark.parametrize("logn", [16, 20])
+@pytest.mark.parametrize("unbalanced", [False, True])
+def test_workload_exact(gpu, logn, unbalanced):
+    n = 1 << logn
+    a, b = workload.operand_pair(n, n // 2 if unbalanced else n)
+    got = gpu.mul(a, b, gpu.ENGINE_NTT)
+    if logn <= 16:
+        assert np.array_equal(got, exact_product(a, b))
+    else:
+        assert residues_match(a, b, got)
+        assert int(got[0]) == (int(a[0]) * int(b[0])) & 0xFFFFFFFF
+
+
+@pytest.mark.parametrize("logn", [24, 26])
+@pytest.mark.parametrize("unbalanced", [False, True])
+def test_workload_large_residues(gpu, logn, unbalanced):
+    n = 1 << logn
+    a, b = workload.operand_pair(n, n // 2 if unbalanced else n)
+    got = gpu.mul(a, b, gpu.ENGINE_NTT)
+    assert got.size == a.size + b.size
+    assert int(got[0]) == (int(a[0]) * int(b[0])) & 0xFFFFFFFF
+    assert residues_match(a, b, got)

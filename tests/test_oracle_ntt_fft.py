@@ -1,0 +1,161 @@
+"""Pins the oracle's transform engines against closed forms (no GPU).
+
+The reference has no tests or golden vectors, so each restated function is
+checked against what the mathematics fixes:
+  * prime table (NTT.cpp:125-158): primality, 2^k | p-1, generator order;
+  * NTT_fast (NTT.cpp:191-223): brute-force O(N^2) DFT mod p;
+  * iNTT (NTT.cpp:80-84): inverse o forward = identity, convolution theorem;
+  * fft_forward / fft_inverse (FFT.cpp:64-463): numpy.fft with the reference's
+    sign (+i) and bit-reversed order;
+  * conversions (NTT.cpp:330-369, FFT.cpp:485-571): digit closed forms.
+"""
+import random
+
+import numpy as np
+import pytest
+import sympy
+
+from oracle import oracle as O
+from tests.helpers import exact_product, to_int
+
+# Appendix-C style expectations: k -> (p, smallest primitive root), as the
+# generator search of NTT.cpp:104-123 finds them.
+EXPECTED_ROOTS = {11: 5, 12: 13, 14: 13, 15: 5, 16: 5, 17: 11, 18: 10, 20: 10, 21: 11, 22: 3,
+                  24: 3, 25: 3, 26: 10, 27: 10, 28: 3, 29: 3, 30: 3, 32: 3, 33: 13, 34: 5,
+                  36: 5, 37: 7, 41: 7, 42: 3}
+
+
+@pytest.mark.parametrize("k", range(11, 43))
+def test_prime_table(k):
+    p = O.ntt_prime(k)
+    assert sympy.isprime(p)
+    assert p < (1 << 63)
+    assert (p - 1) % (1 << k) == 0
+    # coefficient bound of 16-bit digits, as the reference relies on
+    if k <= 33:
+        assert (1 << k) // 4 * (2 ** 16 - 1) ** 2 < p
+
+
+@pytest.mark.parametrize("k", sorted(EXPECTED_ROOTS))
+def test_generator_is_smallest_primitive_root(k):
+    p = O.ntt_prime(k)
+    r = O.ntt_generator(p)
+    assert r == EXPECTED_ROOTS[k]
+    assert r == sympy.primitive_root(p)  # smallest primitive root by definition
+
+
+def test_small_k_uses_k11_prime():
+    for k in range(0, 11):
+        assert O.ntt_prime(k) == O.ntt_prime(11)
+
+
+def brute_dft(x, w, p):
+    n = len(x)
+    return [sum(int(x[j]) * pow(w, j * k, p) for j in range(n)) % p for k in range(n)]
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 8])
+def test_ntt_forward_is_dft(k):
+    n = 1 << k
+    p, W, iW, rN = O.ntt_roots(k)
+    assert pow(W, n, p) == 1 and (n == 1 or pow(W, n // 2, p) == p - 1)
+    assert W * iW % p == 1 and n * rN % p == 1
+    rng = random.Random(k)
+    x = np.array([rng.randrange(p) for _ in range(n)], dtype=np.uint64)
+    got = O.ntt_forward(x, k)
+    assert [int(v) for v in got] == brute_dft(x, W, p)
+
+
+@pytest.mark.parametrize("k", [4, 10, 14])
+def test_ntt_roundtrip_and_dc(k):
+    n = 1 << k
+    p, W, _, _ = O.ntt_roots(k)
+    rng = random.Random(100 + k)
+    x = np.array([rng.randrange(1 << 16) for _ in range(n)], dtype=np.uint64)
+    X = O.ntt_forward(x, k, threads=3)
+    assert int(X[0]) == int(sum(int(v) for v in x)) % p
+    assert np.array_equal(O.ntt_inverse(X, k, threads=2), x)
+
+
+def test_ntt_convolution_theorem():
+    k = 7
+    n = 1 << k
+    p, _, _, _ = O.ntt_roots(k)
+    rng = random.Random(7)
+    a = [rng.randrange(1 << 16) for _ in range(n // 2)] + [0] * (n // 2)
+    b = [rng.randrange(1 << 16) for _ in range(n // 2)] + [0] * (n // 2)
+    A = O.ntt_forward(np.array(a, dtype=np.uint64), k)
+    B = O.ntt_forward(np.array(b, dtype=np.uint64), k)
+    C = np.array([int(x) * int(y) % p for x, y in zip(A, B)], dtype=np.uint64)
+    c = O.ntt_inverse(C, k)
+    ref = np.convolve(np.array(a, dtype=object), np.array(b, dtype=object))[:n]
+    assert [int(v) for v in c] == [int(v) % p for v in ref]
+
+
+def test_int_ntt_conversions():
+    rng = random.Random(3)
+    A = np.array([rng.getrandbits(32) for _ in range(10)], dtype=np.uint32)
+    T = O.int_to_ntt(A, 5)
+    assert T.size == 32
+    assert all(int(T[2 * i]) == int(A[i]) & 0xFFFF and int(T[2 * i + 1]) == int(A[i]) >> 16
+               for i in range(10))
+    assert not T[20:].any()
+    assert np.array_equal(O.ntt_to_int(T, 5, 10), A)
+    # carry: digit-position coefficients larger than 2^16
+    T2 = np.zeros(8, dtype=np.uint64)
+    T2[0] = (1 << 40) + 7
+    T2[3] = 65535 * 3
+    val = sum(int(v) << (16 * i) for i, v in enumerate(T2))
+    out = O.ntt_to_int(T2, 3, 4)
+    assert to_int(out) == val % (1 << 128)
+    with pytest.raises(ValueError):
+        O.ntt_to_int(T2, 3, 5)  # "FFT length is too small."
+
+
+def bitrev(i, k):
+    return int(format(i, "0%db" % k)[::-1], 2) if k else 0
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 9, 12])
+def test_fft_forward_matches_numpy(k):
+    n = 1 << k
+    rng = np.random.default_rng(k)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    T = O.fft_forward(x, k)
+    X = np.fft.ifft(x) * n  # sum x_j e^{+2 pi i jk/N}
+    perm = [bitrev(i, k) for i in range(n)]
+    assert np.allclose(T[perm], X, rtol=0, atol=1e-9 * n)
+
+
+@pytest.mark.parametrize("k", [1, 3, 10])
+def test_fft_inverse_roundtrip(k):
+    n = 1 << k
+    rng = np.random.default_rng(50 + k)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    back = O.fft_inverse(O.fft_forward(x, k), k)
+    assert np.allclose(back, n * x, atol=1e-9 * n)
+
+
+@pytest.mark.parametrize("engine", [O.ENGINE_REFERENCE, O.ENGINE_NTT, O.ENGINE_FFT])
+@pytest.mark.parametrize("na,nb", [(1, 1), (2, 3), (50, 17), (777, 1000), (4096, 4096)])
+def test_mul_limbs_exact(engine, na, nb):
+    rng = random.Random(na * 31 + nb + engine)
+    a = np.array([rng.getrandbits(32) for _ in range(na)], dtype=np.uint32)
+    b = np.array([rng.getrandbits(32) for _ in range(nb)], dtype=np.uint32)
+    assert np.array_equal(O.mul_limbs(a, b, engine, threads=2), exact_product(a, b))
+
+
+@pytest.mark.parametrize("engine", [O.ENGINE_NTT, O.ENGINE_FFT])
+def test_mul_limbs_all_ones(engine):
+    a = np.full(3000, 0xFFFFFFFF, dtype=np.uint32)
+    assert np.array_equal(O.mul_limbs(a, a, engine), exact_product(a, a))
+
+
+def test_planted_mistake_is_caught():
+    """A wrong twiddle (a plausible indexing slip) must fail the DFT check."""
+    k = 4
+    n = 1 << k
+    p, W, _, _ = O.ntt_roots(k)
+    x = np.arange(1, n + 1, dtype=np.uint64)
+    wrong = brute_dft(x, pow(W, 3, p), p)  # W^3 instead of W
+    assert [int(v) for v in O.ntt_forward(x, k)] != wrong
