@@ -121,6 +121,11 @@ int pcg_fft_last_roundoff(double* max_dist, int* recomputed);
 #define PCG_PROFILE 0x400u
 int pcg_last_profile(double* stage_ms, int n_stage, double* total_ms);
 
+/* Elapsed time of the work enqueued on the library stream between the two
+ * calls, measured with HIP events recorded on that stream. */
+int pcg_timer_start(void);
+int pcg_timer_stop(double* ms);
+
 /* Static facts for reports: transform length (log2) chosen for na + nb
  * limbs by the NTT engine, and the number of passes. */
 int pcg_ntt_plan(uint64_t total_limbs, int* log2_points, int* passes);

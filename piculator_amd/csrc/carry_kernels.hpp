@@ -91,8 +91,41 @@ struct CarryArgs {
   uint64_t* cin;     // per block carry-in
   uint64_t nblk;
   double2 crt;       // p0^{-1} mod p1 as (w, w/p1)
-  uint32_t* status;  // nonzero on overflow
+  uint32_t* status;  // nonzero on overflow (null: no check)
+  // FFT engine: unweighted inverse-transform output and round-off maximum
+  const double2* z;
+  unsigned long long* maxdist;  // bits of a non-negative double
 };
+
+// Limb k of the FFT engine: digits 2k, 2k+1 are real parts of points 2k, 2k+1
+// (below N) or imaginary parts of points 2k-N, 2k+1-N.  The rounded digit
+// coefficients give s_k = c0 + 2^16 c1 (signed); the constant
+// K 2^32 - K (K 2^32 at k = 0) is added so every limb value is non-negative,
+// which adds K 2^(32 zL) to the total, above the product's limbs.
+__device__ __forceinline__ void fft_words(const double2* z, uint64_t N, uint64_t k, uint32_t& w0,
+                                          uint32_t& w1, uint32_t& w2, double& dist) {
+  const uint64_t d0 = 2 * k;
+  double x0, x1;
+  if (d0 < N) {
+    x0 = z[d0].x;
+    x1 = z[d0 + 1].x;
+  } else {
+    x0 = z[d0 - N].y;
+    x1 = z[d0 + 1 - N].y;
+  }
+  double r0 = __builtin_rint(x0), r1 = __builtin_rint(x1);
+  dist = fmax(fabs(x0 - r0), fabs(x1 - r1));
+  const double lim = 1152921504606846976.0;  // 2^60: garbage stays finite
+  r0 = fmin(fmax(r0, -lim), lim);
+  r1 = fmin(fmax(r1, -lim), lim);
+  const __int128 K = (__int128)1 << 40;
+  __int128 s = (__int128)(int64_t)r0 + ((__int128)(int64_t)r1 << 16) + (K << 32);
+  if (k > 0) s -= K;
+  const unsigned __int128 t = (unsigned __int128)s;
+  w0 = (uint32_t)t;
+  w1 = (uint32_t)(t >> 32);
+  w2 = (uint32_t)(t >> 64);
+}
 
 __device__ __forceinline__ void crt_words(double r0, double r1, const Prime& P0, const Prime& P1,
                                           double2 crt, uint32_t& w0, uint32_t& w1, uint32_t& w2) {
@@ -110,32 +143,33 @@ __device__ __forceinline__ void crt_words(double r0, double r1, const Prime& P0,
   w2 = (uint32_t)hi;
 }
 
-__global__ __launch_bounds__(kCarryThreads) void k_carry_local(CarryArgs a) {
-  __shared__ uint32_t sw1[kCarryBlock], sw2[kCarryBlock];
-  __shared__ CarryFn wtot[kCarryThreads / 64];
-  __shared__ uint32_t d01[2];
+// Resolves the carries of LPT*blockDim.x consecutive limbs held LPT per
+// thread (thread t holds limbs [LPT t, LPT t + LPT)), with carry-in 0.
+// Inputs are the CRT words (w0, w1, w2) of each limb position; sw1/sw2 are
+// LDS scratch of LPT*blockDim.x words, wtot LDS scratch of blockDim.x/64.
+// On return D holds the limbs, and for the last thread `cout` is the carry
+// out of the top limb and `spill` the value of the words that spill past
+// the top limb.
+template <int LPT>
+__device__ __forceinline__ void block_carry(const uint32_t* w0, const uint32_t* w1,
+                                            const uint32_t* w2, uint32_t* sw1, uint32_t* sw2,
+                                            CarryFn* wtot, uint32_t* D, uint64_t& cout,
+                                            uint64_t& spill) {
   const int tid = threadIdx.x;
-  const uint64_t base = (uint64_t)blockIdx.x * kCarryBlock;
-  const Prime P0 = a.prm[0], P1 = a.prm[1];
-  uint32_t w0[kCarryLPT];
+  const int total = LPT * blockDim.x;
 #pragma unroll
-  for (int q = 0; q < kCarryLPT; ++q) {
-    const int li = tid * kCarryLPT + q;
-    const uint64_t k = base + li;
-    uint32_t x0 = 0, x1 = 0, x2 = 0;
-    if (k < a.zL && k < a.N) crt_words(a.r[k], a.r[a.N + k], P0, P1, a.crt, x0, x1, x2);
-    w0[q] = x0;
-    sw1[li] = x1;
-    sw2[li] = x2;
+  for (int q = 0; q < LPT; ++q) {
+    sw1[tid * LPT + q] = w1[q];
+    sw2[tid * LPT + q] = w2[q];
   }
   __syncthreads();
-  uint64_t s[kCarryLPT];
+  uint64_t s[LPT];
   CarryFn f;
   f.base = 0;
   f.T = 0;
 #pragma unroll
-  for (int q = 0; q < kCarryLPT; ++q) {
-    const int li = tid * kCarryLPT + q;
+  for (int q = 0; q < LPT; ++q) {
+    const int li = tid * LPT + q;
     uint64_t x = w0[q];
     if (li >= 1) x += sw1[li - 1];
     if (li >= 2) x += sw2[li - 2];
@@ -145,16 +179,56 @@ __global__ __launch_bounds__(kCarryThreads) void k_carry_local(CarryArgs a) {
     g.T = (1ull << 32) - (x & 0xffffffffull);
     f = cf_compose(f, g);
   }
-  CarryFn pre = cf_block_exclusive(f, wtot);
+  const CarryFn pre = cf_block_exclusive(f, wtot);
   uint64_t c = cf_eval0(pre);
-  uint32_t D[kCarryLPT];
+#pragma unroll
+  for (int q = 0; q < LPT; ++q) {
+    const uint64_t t = (s[q] & 0xffffffffull) + c;
+    D[q] = (uint32_t)t;
+    c = (s[q] >> 32) + (t >> 32);
+  }
+  cout = c;
+  spill = (uint64_t)sw1[total - 1] + (uint64_t)sw2[total - 2] + ((uint64_t)sw2[total - 1] << 32);
+}
+
+template <bool FFT>
+__global__ __launch_bounds__(kCarryThreads) void k_carry_local(CarryArgs a) {
+  __shared__ uint32_t sw1[kCarryBlock], sw2[kCarryBlock];
+  __shared__ CarryFn wtot[kCarryThreads / 64];
+  __shared__ uint32_t d01[2];
+  const int tid = threadIdx.x;
+  const uint64_t base = (uint64_t)blockIdx.x * kCarryBlock;
+  uint32_t w0[kCarryLPT], w1[kCarryLPT], w2[kCarryLPT], D[kCarryLPT];
+  if (FFT) {
+    double dmax = 0.0;
+#pragma unroll
+    for (int q = 0; q < kCarryLPT; ++q) {
+      const uint64_t k = base + tid * kCarryLPT + q;
+      w0[q] = w1[q] = w2[q] = 0;
+      if (k < a.zL) {
+        double dist;
+        fft_words(a.z, a.N, k, w0[q], w1[q], w2[q], dist);
+        dmax = fmax(dmax, dist);
+      }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, d, 64));
+    if ((tid & 63) == 0) atomicMax(a.maxdist, (unsigned long long)__double_as_longlong(dmax));
+  } else {
+    const Prime P0 = a.prm[0], P1 = a.prm[1];
+#pragma unroll
+    for (int q = 0; q < kCarryLPT; ++q) {
+      const uint64_t k = base + tid * kCarryLPT + q;
+      w0[q] = w1[q] = w2[q] = 0;
+      if (k < a.zL && k < a.N) crt_words(a.r[k], a.r[a.N + k], P0, P1, a.crt, w0[q], w1[q], w2[q]);
+    }
+  }
+  uint64_t c, spill;
+  block_carry<kCarryLPT>(w0, w1, w2, sw1, sw2, wtot, D, c, spill);
   bool ones = true;
 #pragma unroll
   for (int q = 0; q < kCarryLPT; ++q) {
     const int li = tid * kCarryLPT + q;
-    const uint64_t t = (s[q] & 0xffffffffull) + c;
-    D[q] = (uint32_t)t;
-    c = (s[q] >> 32) + (t >> 32);
     const uint64_t k = base + li;
     if (k < a.zL) a.out[k] = D[q];
     if (li >= 2 && D[q] != 0xffffffffu) ones = false;
@@ -162,8 +236,6 @@ __global__ __launch_bounds__(kCarryThreads) void k_carry_local(CarryArgs a) {
   }
   const int all_ones = __syncthreads_and(ones ? 1 : 0);
   if (tid == kCarryThreads - 1) {
-    const int L = kCarryBlock;
-    uint64_t spill = (uint64_t)sw1[L - 1] + (uint64_t)sw2[L - 2] + ((uint64_t)sw2[L - 1] << 32);
     CarryFn g;
     g.base = spill + c;
     g.T = kInf;
@@ -193,7 +265,7 @@ __global__ __launch_bounds__(1024) void k_carry_scan(CarryArgs a) {
   }
   if (threadIdx.x == blockDim.x - 1) {
     // carry out of the top block must vanish for an exact product
-    if (cf_eval0(run) != 0) *a.status = 1;
+    if (a.status && cf_eval0(run) != 0) *a.status = 1;
   }
 }
 
@@ -211,7 +283,7 @@ __global__ __launch_bounds__(256) void k_carry_fix(CarryArgs a) {
     c = (c >> 32) + (t >> 32);
     ++k;
   }
-  if (c != 0 && end < a.zL) *a.status = 2;
+  if (c != 0 && a.status && k < a.zL) *a.status = 2;
 }
 
 }  // namespace pcg

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -18,7 +19,9 @@
 #include <vector>
 
 #include "../../include/picugpu.h"
+#include "batch_kernels.hpp"
 #include "carry_kernels.hpp"
+#include "fft_kernels.hpp"
 #include "ntt_kernels.hpp"
 
 using namespace pcg;
@@ -71,11 +74,11 @@ struct Plan {
   std::vector<Pass> passes;
 };
 
-Plan make_plan(int logN) {
+Plan make_plan(int logN, int logEmax = kLogE) {
   Plan pl;
   pl.logN = logN;
-  pl.logE = logN < kLogE ? logN : kLogE;
-  int m = (logN + kLogE - 1) / kLogE;
+  pl.logE = logN < logEmax ? logN : logEmax;
+  int m = (logN + logEmax - 1) / logEmax;
   if (m < 1) m = 1;
   const int base = logN / m, extra = logN % m;
   int rem = logN;
@@ -117,13 +120,24 @@ struct Ctx {
   // staging for host-pointer calls
   uint32_t* d_stage = nullptr;
   size_t stage_limbs = 0;
+  // batch pools: pinned host mirror and device copy, plus descriptors
+  uint32_t* h_pool = nullptr;
+  uint32_t* d_pool = nullptr;
+  size_t pool_limbs = 0;
+  BatchDesc* h_desc = nullptr;
+  BatchDesc* d_desc = nullptr;
+  size_t desc_cap = 0;
   // profiling
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t tev[2] = {nullptr, nullptr};
   double prof_ms[3] = {0, 0, 0};
   double prof_total = 0;
   bool prof_valid = false;
   double fft_maxdist = 0;
   int fft_recomputed = 0;
+  // FFT engine tables and round-off slot
+  double2* ftab[3] = {nullptr, nullptr, nullptr};
+  unsigned long long* d_maxdist = nullptr;
 };
 
 Ctx g_ctx;
@@ -180,6 +194,22 @@ int build_tables(Ctx& c) {
       P.twLo = c.tw[pi][2];
     }
   }
+  // FFT engine: e^{2 pi i e / 4096}, e^{2 pi i e / 2^16}, e^{2 pi i e / 2^32}
+  {
+    const long double tau = 6.283185307179586476925286766559005768L;
+    const size_t sz[3] = {4096, 65536, 65536};
+    const long double den[3] = {4096.0L, 65536.0L, 4294967296.0L};
+    for (int t = 0; t < 3; ++t) {
+      std::vector<double2> v(sz[t]);
+      for (size_t e = 0; e < sz[t]; ++e) {
+        const long double ang = tau * (long double)e / den[t];
+        v[e] = make_double2((double)cosl(ang), (double)sinl(ang));
+      }
+      HIPCHK(hipMalloc(&c.ftab[t], v.size() * sizeof(double2)));
+      HIPCHK(hipMemcpy(c.ftab[t], v.data(), v.size() * sizeof(double2), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMalloc(&c.d_maxdist, sizeof(unsigned long long)));
+  }
   HIPCHK(hipMalloc(&c.d_prime, hp.size() * sizeof(Prime)));
   HIPCHK(hipMemcpy(c.d_prime, hp.data(), hp.size() * sizeof(Prime), hipMemcpyHostToDevice));
   const uint64_t inv = powmod_u64(kP0 % kP1, kP1 - 2, kP1);
@@ -190,6 +220,7 @@ int build_tables(Ctx& c) {
 int ensure_workspace(Ctx& c, int logN, uint64_t zL) {
   const size_t pts = (size_t)1 << logN;
   if (pts > c.spec_points) {
+    HIPCHK(hipStreamSynchronize(c.stream));  // earlier products may still use it
     if (c.spec) HIPCHK(hipFree(c.spec));
     c.spec = nullptr;
     c.spec_points = 0;
@@ -201,6 +232,7 @@ int ensure_workspace(Ctx& c, int logN, uint64_t zL) {
   }
   const size_t nblk = (zL + kCarryBlock - 1) / kCarryBlock;
   if (nblk > c.nblk_cap) {
+    HIPCHK(hipStreamSynchronize(c.stream));
     if (c.agg) HIPCHK(hipFree(c.agg));
     if (c.cin) HIPCHK(hipFree(c.cin));
     c.agg = nullptr;
@@ -215,6 +247,7 @@ int ensure_workspace(Ctx& c, int logN, uint64_t zL) {
 
 int ensure_stage(Ctx& c, size_t limbs) {
   if (limbs <= c.stage_limbs) return PCG_OK;
+  HIPCHK(hipStreamSynchronize(c.stream));
   if (c.d_stage) HIPCHK(hipFree(c.d_stage));
   c.d_stage = nullptr;
   c.stage_limbs = 0;
@@ -297,11 +330,84 @@ int ntt_mul_enqueue(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_
   ca.nblk = (zL + kCarryBlock - 1) / kCarryBlock;
   ca.crt = c.crt;
   ca.status = c.status;
-  hipLaunchKernelGGL(k_carry_local, dim3((unsigned)ca.nblk), dim3(kCarryThreads), 0, c.stream, ca);
+  ca.z = nullptr;
+  ca.maxdist = nullptr;
+  hipLaunchKernelGGL(k_carry_local<false>, dim3((unsigned)ca.nblk), dim3(kCarryThreads), 0, c.stream, ca);
   hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, c.stream, ca);
   hipLaunchKernelGGL(k_carry_fix, dim3((unsigned)((ca.nblk + 255) / 256)), dim3(256), 0, c.stream,
                      ca);
   if (profile) HIPCHK(hipEventRecord(c.ev[3], c.stream));
+  HIPCHK(hipGetLastError());
+  return PCG_OK;
+}
+
+
+void launch_fft_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double2* src,
+                     const uint32_t* limbs, uint64_t nlimbs, double2* dst, const double2* spec,
+                     bool unweight) {
+  FftPassArgs a;
+  a.tb.stg = c.ftab[0];
+  a.tb.hi = c.ftab[1];
+  a.tb.lo = c.ftab[2];
+  a.src = src;
+  a.limbs = limbs;
+  a.nlimbs = nlimbs;
+  a.dst = dst;
+  a.spec = spec;
+  a.N = 1ull << pl.logN;
+  a.logN = pl.logN;
+  a.logL = ps.logL;
+  a.logS = ps.logS;
+  a.logLines = ps.logLines;
+  a.logG = ps.logG;
+  a.logB = ps.logB;
+  a.unweight = unweight ? 1 : 0;
+  const int E = 1 << pl.logE;
+  dim3 grid((unsigned)(a.N / E)), block(E / kEPTc);
+  if (mode == kFwd)
+    hipLaunchKernelGGL(k_fft_pass<kFwd>, grid, block, 0, c.stream, a);
+  else if (mode == kInv)
+    hipLaunchKernelGGL(k_fft_pass<kInv>, grid, block, 0, c.stream, a);
+  else
+    hipLaunchKernelGGL(k_fft_pass<kFused>, grid, block, 0, c.stream, a);
+}
+
+// Enqueue the FFT-engine product; the round-off maximum lands in d_maxdist.
+int fft_mul_enqueue(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_b, uint64_t nb,
+                    uint32_t* d_out) {
+  const uint64_t zL = na + nb;
+  int logN = ceil_log2(zL);
+  if (logN < kMinLogN) logN = kMinLogN;
+  int rc = ensure_workspace(c, logN, zL);
+  if (rc) return rc;
+  const Plan pl = make_plan(logN, kFftLogE);
+  const uint64_t N = 1ull << logN;
+  double2* A = reinterpret_cast<double2*>(c.spec);
+  double2* B = A + N;
+  const size_t m = pl.passes.size();
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c.stream,
+                     reinterpret_cast<uint32_t*>(c.d_maxdist), (uint64_t)2);
+  for (size_t j = 0; j < m; ++j)
+    launch_fft_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : A, d_a, na, A, nullptr, false);
+  for (size_t j = 0; j + 1 < m; ++j)
+    launch_fft_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : B, d_b, nb, B, nullptr, false);
+  launch_fft_pass(c, kFused, pl, pl.passes[m - 1], m == 1 ? nullptr : B, d_b, nb, B, A, m == 1);
+  for (size_t j = m - 1; j-- > 0;)
+    launch_fft_pass(c, kInv, pl, pl.passes[j], B, nullptr, 0, B, nullptr, j == 0);
+  CarryArgs ca;
+  memset(&ca, 0, sizeof ca);
+  ca.z = B;
+  ca.N = N;
+  ca.zL = zL;
+  ca.out = d_out;
+  ca.agg = c.agg;
+  ca.cin = c.cin;
+  ca.nblk = (zL + kCarryBlock - 1) / kCarryBlock;
+  ca.status = nullptr;
+  ca.maxdist = c.d_maxdist;
+  hipLaunchKernelGGL(k_carry_local<true>, dim3((unsigned)ca.nblk), dim3(kCarryThreads), 0, c.stream, ca);
+  hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, c.stream, ca);
+  hipLaunchKernelGGL(k_carry_fix, dim3((unsigned)((ca.nblk + 255) / 256)), dim3(256), 0, c.stream, ca);
   HIPCHK(hipGetLastError());
   return PCG_OK;
 }
@@ -318,7 +424,20 @@ int mul_dev_locked(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_b
     return PCG_OK;
   }
   const uint32_t eng = flags & PCG_ENGINE_MASK;
-  if (eng == PCG_ENGINE_FFT) return fail(PCG_ERR_INVALID, "FFT engine not available in this build");
+  if (eng == 3) return fail(PCG_ERR_INVALID, "unknown engine flag");
+  if (eng == PCG_ENGINE_FFT) {
+    // FP64 FFT, then the round-off test decides whether the NTT recomputes.
+    rc = fft_mul_enqueue(c, d_a, na, d_b, nb, d_out);
+    if (rc) return rc;
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, c.d_maxdist, sizeof bits, hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    double dist;
+    memcpy(&dist, &bits, sizeof dist);
+    c.fft_maxdist = dist;
+    c.fft_recomputed = !(dist <= 0.25);
+    if (!c.fft_recomputed) return PCG_OK;
+  }
   const bool prof = (flags & PCG_PROFILE) != 0;
   rc = ntt_mul_enqueue(c, d_a, na, d_b, nb, d_out, prof);
   if (rc) return rc;
@@ -332,6 +451,161 @@ int mul_dev_locked(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_b
     HIPCHK(hipEventElapsedTime(&ms, c.ev[0], c.ev[3]));
     c.prof_total = ms;
     c.prof_valid = true;
+  }
+  return PCG_OK;
+}
+
+
+int ensure_pool(Ctx& c, size_t limbs, size_t ndesc) {
+  if (limbs > c.pool_limbs) {
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (c.h_pool) HIPCHK(hipHostFree(c.h_pool));
+    if (c.d_pool) HIPCHK(hipFree(c.d_pool));
+    c.h_pool = nullptr;
+    c.d_pool = nullptr;
+    c.pool_limbs = 0;
+    size_t want = limbs + limbs / 4;
+    if (hipHostMalloc((void**)&c.h_pool, want * 4, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&c.d_pool, want * 4) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PCG_ERR_OOM, "cannot allocate batch pools of %zu limbs", want);
+    }
+    c.pool_limbs = want;
+  }
+  if (ndesc > c.desc_cap) {
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (c.h_desc) HIPCHK(hipHostFree(c.h_desc));
+    if (c.d_desc) HIPCHK(hipFree(c.d_desc));
+    c.h_desc = nullptr;
+    c.d_desc = nullptr;
+    c.desc_cap = 0;
+    size_t want = ndesc + ndesc / 4 + 16;
+    if (hipHostMalloc((void**)&c.h_desc, want * sizeof(BatchDesc), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&c.d_desc, want * sizeof(BatchDesc)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PCG_ERR_OOM, "cannot allocate %zu batch descriptors", want);
+    }
+    c.desc_cap = want;
+  }
+  return PCG_OK;
+}
+
+// Size class of a product for the batched path: 0 schoolbook, 11/12 the
+// single-workgroup NTT of 2^11 / 2^12 points, -1 the multi-pass engine.
+int batch_class(uint64_t na, uint64_t nb) {
+  const uint64_t zL = na + nb;
+  if (zL <= (uint64_t)kSchoolMaxLimbs) return 0;
+  if (zL <= (1ull << kSmallNttMaxLog)) return ceil_log2(zL) < 11 ? 11 : ceil_log2(zL);
+  return -1;
+}
+
+// Launches the per-class kernels over h_desc[0..n) (already on the device
+// in d_desc, ordered by class: school, ntt 2^11, ntt 2^12).
+int launch_classes(Ctx& c, size_t n_school, size_t n11, size_t n12) {
+  if (n_school)
+    hipLaunchKernelGGL(k_batch_school, dim3((unsigned)n_school), dim3(256), 0, c.stream, c.d_desc);
+  if (n11)
+    hipLaunchKernelGGL(k_batch_ntt, dim3((unsigned)n11), dim3(128), 0, c.stream,
+                       c.d_desc + n_school, c.d_prime + 11 * 2, 11, c.crt);
+  if (n12)
+    hipLaunchKernelGGL(k_batch_ntt, dim3((unsigned)n12), dim3(256), 0, c.stream,
+                       c.d_desc + n_school + n11, c.d_prime + 12 * 2, 12, c.crt);
+  HIPCHK(hipGetLastError());
+  return PCG_OK;
+}
+
+int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, bool dev) {
+  std::vector<uint64_t> cls[3], large;
+  for (uint64_t i = 0; i < count; ++i) {
+    int rc = check_sizes(d[i].na, d[i].nb);
+    if (rc) return rc;
+    if (d[i].na + d[i].nb == 0) continue;
+    if ((d[i].na + d[i].nb) && !d[i].out) return fail(PCG_ERR_INVALID, "null output pointer");
+    if (d[i].na == 0 || d[i].nb == 0) {
+      if (dev) {
+        const uint64_t zL = d[i].na + d[i].nb;
+        hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((zL + 255) / 256)), dim3(256), 0, c.stream,
+                           d[i].out, zL);
+      } else {
+        memset(d[i].out, 0, (d[i].na + d[i].nb) * 4);
+      }
+      continue;
+    }
+    if (!d[i].a || !d[i].b) return fail(PCG_ERR_INVALID, "null operand pointer");
+    const int k = batch_class(d[i].na, d[i].nb);
+    if (k < 0)
+      large.push_back(i);
+    else
+      cls[k == 0 ? 0 : k - 10].push_back(i);
+  }
+  const size_t nsmall = cls[0].size() + cls[1].size() + cls[2].size();
+  if (nsmall) {
+    size_t in_limbs = 0, out_limbs = 0;
+    for (int k = 0; k < 3; ++k)
+      for (uint64_t i : cls[k]) {
+        in_limbs += d[i].na + d[i].nb;
+        out_limbs += d[i].na + d[i].nb;
+      }
+    int rc = ensure_pool(c, dev ? 0 : in_limbs + out_limbs, nsmall);
+    if (rc) return rc;
+    size_t pos = 0, opos = in_limbs, j = 0;
+    for (int k = 0; k < 3; ++k)
+      for (uint64_t i : cls[k]) {
+        BatchDesc& bd = c.h_desc[j++];
+        bd.na = (uint32_t)d[i].na;
+        bd.nb = (uint32_t)d[i].nb;
+        if (dev) {
+          bd.a = d[i].a;
+          bd.b = d[i].b;
+          bd.out = d[i].out;
+        } else {
+          memcpy(c.h_pool + pos, d[i].a, d[i].na * 4);
+          bd.a = c.d_pool + pos;
+          pos += d[i].na;
+          memcpy(c.h_pool + pos, d[i].b, d[i].nb * 4);
+          bd.b = c.d_pool + pos;
+          pos += d[i].nb;
+          bd.out = c.d_pool + opos;
+          opos += d[i].na + d[i].nb;
+        }
+      }
+    if (!dev)
+      HIPCHK(hipMemcpyAsync(c.d_pool, c.h_pool, in_limbs * 4, hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipMemcpyAsync(c.d_desc, c.h_desc, nsmall * sizeof(BatchDesc), hipMemcpyHostToDevice,
+                          c.stream));
+    rc = launch_classes(c, cls[0].size(), cls[1].size(), cls[2].size());
+    if (rc) return rc;
+    if (!dev)
+      HIPCHK(hipMemcpyAsync(c.h_pool + in_limbs, c.d_pool + in_limbs, out_limbs * 4,
+                            hipMemcpyDeviceToHost, c.stream));
+    // descriptors and pools are reused by the next call: finish before that
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (!dev) {
+      size_t o = in_limbs;
+      for (int k = 0; k < 3; ++k)
+        for (uint64_t i : cls[k]) {
+          const size_t zL = d[i].na + d[i].nb;
+          memcpy(d[i].out, c.h_pool + o, zL * 4);
+          o += zL;
+        }
+    }
+  }
+  for (uint64_t i : large) {
+    int rc;
+    if (dev) {
+      rc = mul_dev_locked(c, d[i].a, d[i].na, d[i].b, d[i].nb, d[i].out, flags & ~PCG_PROFILE);
+    } else {
+      const uint64_t na = d[i].na, nb = d[i].nb, zL = na + nb;
+      rc = ensure_stage(c, 2 * zL);
+      if (rc) return rc;
+      HIPCHK(hipMemcpyAsync(c.d_stage, d[i].a, na * 4, hipMemcpyHostToDevice, c.stream));
+      HIPCHK(hipMemcpyAsync(c.d_stage + na, d[i].b, nb * 4, hipMemcpyHostToDevice, c.stream));
+      rc = mul_dev_locked(c, c.d_stage, na, c.d_stage + na, nb, c.d_stage + zL, flags & ~PCG_PROFILE);
+      if (rc) return rc;
+      HIPCHK(hipMemcpyAsync(d[i].out, c.d_stage + zL, zL * 4, hipMemcpyDeviceToHost, c.stream));
+      HIPCHK(hipStreamSynchronize(c.stream));
+    }
+    if (rc) return rc;
   }
   return PCG_OK;
 }
@@ -372,6 +646,7 @@ int pcg_init(int device) {
   HIPCHK(hipMalloc(&c.status, sizeof(uint32_t)));
   HIPCHK(hipMemset(c.status, 0, sizeof(uint32_t)));
   for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&c.ev[i]));
+  for (int i = 0; i < 2; ++i) HIPCHK(hipEventCreate(&c.tev[i]));
   c.device = device;
   c.ready = true;
   return PCG_OK;
@@ -384,13 +659,20 @@ void pcg_shutdown(void) {
   (void)hipStreamSynchronize(c.stream);
   for (int pi = 0; pi < 2; ++pi)
     for (int t = 0; t < 3; ++t) (void)hipFree(c.tw[pi][t]);
+  for (int t = 0; t < 3; ++t) (void)hipFree(c.ftab[t]);
+  (void)hipFree(c.d_maxdist);
   (void)hipFree(c.d_prime);
   (void)hipFree(c.spec);
   (void)hipFree(c.agg);
   (void)hipFree(c.cin);
   (void)hipFree(c.status);
   (void)hipFree(c.d_stage);
+  (void)hipHostFree(c.h_pool);
+  (void)hipFree(c.d_pool);
+  (void)hipHostFree(c.h_desc);
+  (void)hipFree(c.d_desc);
   for (int i = 0; i < 4; ++i) (void)hipEventDestroy(c.ev[i]);
+  for (int i = 0; i < 2; ++i) (void)hipEventDestroy(c.tev[i]);
   (void)hipStreamDestroy(c.stream);
   Ctx fresh_defaults;
   c.ready = false;
@@ -404,6 +686,12 @@ void pcg_shutdown(void) {
   c.status = nullptr;
   c.d_stage = nullptr;
   c.stage_limbs = 0;
+  c.h_pool = nullptr;
+  c.d_pool = nullptr;
+  c.pool_limbs = 0;
+  c.h_desc = nullptr;
+  c.d_desc = nullptr;
+  c.desc_cap = 0;
   c.d_prime = nullptr;
   c.prof_valid = false;
   (void)fresh_defaults;
@@ -465,20 +753,33 @@ int pcg_mul_batch_dev(const pcg_mul_desc* d, uint64_t count, uint32_t flags) {
   std::lock_guard<std::mutex> lk(c.mu);
   if (!c.ready) return fail(PCG_ERR_NOT_INIT, "pcg_init has not been called");
   if (count && !d) return fail(PCG_ERR_INVALID, "null descriptor array");
-  for (uint64_t i = 0; i < count; ++i) {
-    int rc = mul_dev_locked(c, d[i].a, d[i].na, d[i].b, d[i].nb, d[i].out, flags & ~PCG_PROFILE);
-    if (rc) return rc;
+  if (flags & PCG_BATCH_PLAIN) {
+    for (uint64_t i = 0; i < count; ++i) {
+      int rc = mul_dev_locked(c, d[i].a, d[i].na, d[i].b, d[i].nb, d[i].out, flags & ~PCG_PROFILE);
+      if (rc) return rc;
+    }
+    return sync_locked(c);
   }
+  int rc = batch_locked(c, d, count, flags, true);
+  if (rc) return rc;
   return sync_locked(c);
 }
 
 int pcg_mul_batch(const pcg_mul_desc* d, uint64_t count, uint32_t flags) {
   if (count && !d) return fail(PCG_ERR_INVALID, "null descriptor array");
-  for (uint64_t i = 0; i < count; ++i) {
-    int rc = pcg_mul(d[i].a, d[i].na, d[i].b, d[i].nb, d[i].out, flags & ~PCG_PROFILE);
-    if (rc) return rc;
+  if (flags & PCG_BATCH_PLAIN) {
+    for (uint64_t i = 0; i < count; ++i) {
+      int rc = pcg_mul(d[i].a, d[i].na, d[i].b, d[i].nb, d[i].out, flags & ~PCG_PROFILE);
+      if (rc) return rc;
+    }
+    return PCG_OK;
   }
-  return PCG_OK;
+  Ctx& c = g_ctx;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (!c.ready) return fail(PCG_ERR_NOT_INIT, "pcg_init has not been called");
+  int rc = batch_locked(c, d, count, flags, false);
+  if (rc) return rc;
+  return sync_locked(c);
 }
 
 int pcg_sync(void) {
@@ -522,6 +823,27 @@ int pcg_last_profile(double* stage_ms, int n_stage, double* total_ms) {
   if (!c.prof_valid) return fail(PCG_ERR_INVALID, "no profiled product yet");
   for (int i = 0; i < n_stage && i < 3; ++i) stage_ms[i] = c.prof_ms[i];
   if (total_ms) *total_ms = c.prof_total;
+  return PCG_OK;
+}
+
+int pcg_timer_start(void) {
+  Ctx& c = g_ctx;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (!c.ready) return fail(PCG_ERR_NOT_INIT, "pcg_init has not been called");
+  HIPCHK(hipEventRecord(c.tev[0], c.stream));
+  return PCG_OK;
+}
+
+int pcg_timer_stop(double* ms) {
+  Ctx& c = g_ctx;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (!c.ready) return fail(PCG_ERR_NOT_INIT, "pcg_init has not been called");
+  if (!ms) return fail(PCG_ERR_INVALID, "null pointer");
+  HIPCHK(hipEventRecord(c.tev[1], c.stream));
+  HIPCHK(hipEventSynchronize(c.tev[1]));
+  float f = 0;
+  HIPCHK(hipEventElapsedTime(&f, c.tev[0], c.tev[1]));
+  *ms = f;
   return PCG_OK;
 }
 

@@ -1,0 +1,209 @@
+// Chudnovsky binary splitting on the GPU BigFloat (see Pi.h).
+#include "Pi.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "BigFloat.h"
+
+namespace piculator {
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct PQR {
+  BigFloat P, Q, R;
+};
+
+// Leaf terms of PI.h:67-88:
+//   P = -(13591409 + 545140134 b)(2b-1)(6b-5)(6b-1) for odd b (sign (-1)^b)
+//   Q = 10939058860032000 b^3 = b^3 * 26726400 * 409297880
+//   R = (2b-1)(6b-5)(6b-1)
+void leaf(uint32_t b, PQR& out) {
+  BigFloat P = BigFloat(b).mul(545140134u);
+  P = P.add(BigFloat(13591409u));
+  P = P.mul(2 * b - 1);
+  P = P.mul(6 * b - 5);
+  P = P.mul(6 * b - 1);
+  if (b % 2 == 1) P.negate();
+  BigFloat Q = BigFloat(b).mul(b).mul(b).mul(26726400u).mul(409297880u);
+  BigFloat R = BigFloat(2 * b - 1).mul(6 * b - 5).mul(6 * b - 1);
+  out.P = std::move(P);
+  out.Q = std::move(Q);
+  out.R = std::move(R);
+}
+
+// PI.h:64-112, depth first; each mul is its own GPU call.
+void bsr_plain(PQR& out, uint32_t a, uint32_t b, size_t p) {
+  if (b - a == 1) {
+    leaf(b, out);
+    return;
+  }
+  const uint32_t m = (a + b) / 2;
+  PQR L, R;
+  bsr_plain(L, a, m, p);
+  bsr_plain(R, m, b, p);
+  out.P = L.P.mul(R.Q, p).add(R.P.mul(L.R, p), p);
+  out.Q = L.Q.mul(R.Q, p);
+  out.R = L.R.mul(R.R, p);
+}
+
+// ---------------------------------------------------------------- batched
+struct Node {
+  uint32_t a, b;
+  int32_t left, right;  // child node indices, -1 for a leaf
+  int32_t height;       // 1 for leaves
+};
+
+// Explicit tree of [a, b) with the reference's midpoints (PI.h:90).
+int32_t build(std::vector<Node>& nodes, uint32_t a, uint32_t b) {
+  const int32_t id = (int32_t)nodes.size();
+  nodes.push_back(Node{a, b, -1, -1, 1});
+  if (b - a > 1) {
+    const uint32_t m = (a + b) / 2;
+    const int32_t l = build(nodes, a, m);
+    const int32_t r = build(nodes, m, b);
+    nodes[id].left = l;
+    nodes[id].right = r;
+    nodes[id].height = std::max(nodes[l].height, nodes[r].height) + 1;
+  }
+  return id;
+}
+
+// Largest subtree evaluated as one explicit forest at a time.
+constexpr uint32_t kChunkLeaves = 1u << 18;
+
+// Evaluates the subtree [a, b) bottom-up, batching every height.
+void bsr_subtree_batched(PQR& out, uint32_t a, uint32_t b, size_t p, bool verbose) {
+  std::vector<Node> nodes;
+  nodes.reserve(2 * (size_t)(b - a));
+  const int32_t root = build(nodes, a, b);
+  std::vector<PQR> val(nodes.size());
+  const int32_t H = nodes[root].height;
+  std::vector<std::vector<int32_t>> by_h(H + 1);
+  for (int32_t i = 0; i < (int32_t)nodes.size(); ++i) by_h[nodes[i].height].push_back(i);
+  for (int32_t i : by_h[1]) leaf(nodes[i].b, val[i]);
+  std::vector<BigFloat> x, y;
+  for (int32_t h = 2; h <= H; ++h) {
+    const std::vector<int32_t>& ids = by_h[h];
+    x.resize(ids.size());
+    y.resize(ids.size());
+    MulBatch mb;
+    for (size_t k = 0; k < ids.size(); ++k) {
+      const Node& n = nodes[ids[k]];
+      PQR& L = val[n.left];
+      PQR& R = val[n.right];
+      PQR& O = val[ids[k]];
+      mb.add(L.P, R.Q, p, &x[k]);
+      mb.add(R.P, L.R, p, &y[k]);
+      mb.add(L.Q, R.Q, p, &O.Q);
+      mb.add(L.R, R.R, p, &O.R);
+    }
+    const double t0 = now_s();
+    mb.run(false);
+    for (size_t k = 0; k < ids.size(); ++k) {
+      const Node& n = nodes[ids[k]];
+      val[ids[k]].P = x[k].add(y[k], p);
+      val[n.left] = PQR();
+      val[n.right] = PQR();
+    }
+    if (verbose)
+      std::fprintf(stderr, "  height %d: %zu nodes, %.3f s\n", h, ids.size(), now_s() - t0);
+  }
+  out = std::move(val[root]);
+}
+
+// Above the chunk size, recurse on the halves first (each half is batched
+// internally), then combine; the combine products of one node run together.
+void bsr_batched(PQR& out, uint32_t a, uint32_t b, size_t p, bool verbose) {
+  if (b - a <= kChunkLeaves) {
+    bsr_subtree_batched(out, a, b, p, verbose);
+    return;
+  }
+  const uint32_t m = (a + b) / 2;
+  PQR L, R;
+  bsr_batched(L, a, m, p, verbose);
+  bsr_batched(R, m, b, p, verbose);
+  BigFloat x, y;
+  MulBatch mb;
+  mb.add(L.P, R.Q, p, &x);
+  mb.add(R.P, L.R, p, &y);
+  mb.add(L.Q, R.Q, p, &out.Q);
+  mb.add(L.R, R.R, p, &out.R);
+  mb.run(false);
+  out.P = x.add(y, p);
+}
+
+}  // namespace
+
+void pi_sizes(size_t decimal_digits, size_t* hex_digits, size_t* p, size_t* terms) {
+  const size_t hex = (size_t)(decimal_digits * std::log(10) / std::log(16));
+  const size_t pp = (hex + 9) / 9;
+  const size_t p_dec = (decimal_digits + 9) / 9;
+  const size_t t = (size_t)(p_dec * 0.6346230241342037371474889163921741077188431452678) + 1;
+  if (hex_digits) *hex_digits = hex;
+  if (p) *p = pp;
+  if (terms) *terms = t;
+}
+
+std::string pi_hex(size_t decimal_digits, PiMode mode, PiTimes* times, bool verbose) {
+  size_t hex, p, terms;
+  pi_sizes(decimal_digits, &hex, &p, &terms);
+  if ((uint32_t)terms != terms) throw "Limit Exceeded";
+  const uint64_t m0 = big_mul_count();
+  const double t0 = now_s();
+  PQR s;
+  if (mode == PiMode::Plain)
+    bsr_plain(s, 0, (uint32_t)terms, p);
+  else
+    bsr_batched(s, 0, (uint32_t)terms, p, verbose);
+  BigFloat P = s.Q.mul(13591409u).add(s.P, p);
+  BigFloat Q = s.Q.mul(4270934400u);
+  const double t1 = now_s();
+  P = Q.div(P, p);
+  const double t2 = now_s();
+  Q = invsqrt(10005, p);
+  const double t3 = now_s();
+  P = P.mul(Q, p);
+  const double t4 = now_s();
+  std::string str = P.to_string_hex(hex);
+  const double t5 = now_s();
+  if (times) {
+    times->series = t1 - t0;
+    times->division = t2 - t1;
+    times->invsqrt = t3 - t2;
+    times->final_mul = t4 - t3;
+    times->conversion = t5 - t4;
+    times->total = t5 - t0;
+    times->big_muls = big_mul_count() - m0;
+  }
+  return str;
+}
+
+void Pi(size_t decimal_digits, size_t threads) {
+  size_t hex, p, terms;
+  pi_sizes(decimal_digits, &hex, &p, &terms);
+  std::printf("Constant:\tPi\nAlgorithm:\tChudnovsky (1988)\n\n");
+  std::printf("Decimal Digits:\t%zu\nHex Digits:\t%zu\n\n", decimal_digits, hex);
+  std::printf("Memory Mode:\tGPU (MI355X HBM)\n# Threads:\t%zu (host; products run on the GPU)\n\n",
+              threads);
+  std::printf("Summing Series...  %zu\n", terms);
+  PiTimes t;
+  const std::string str = pi_hex(decimal_digits, PiMode::Batched, &t);
+  std::printf("series %.3f s\nDivision... %.3f s\nInvSqrt... %.3f s\nFinal Multiply... %.3f s\n"
+              "Base Conversion... %.3f s\nTotal Time:\t%.3f s\n",
+              t.series, t.division, t.invsqrt, t.final_mul, t.conversion, t.total);
+  FILE* f = std::fopen("pi.txt", "wb");
+  if (!f) throw "Cannot Create File";
+  std::fwrite(str.data(), 1, str.size(), f);
+  std::fclose(f);
+}
+
+}  // namespace piculator

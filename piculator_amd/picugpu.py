@@ -32,7 +32,7 @@ EXPORTS = [
     "pcg_init", "pcg_shutdown", "pcg_last_error_string", "pcg_reserve", "pcg_mul",
     "pcg_mul_dev", "pcg_mul_batch", "pcg_mul_batch_dev", "pcg_sync", "pcg_dev_alloc",
     "pcg_dev_free", "pcg_memcpy_h2d", "pcg_memcpy_d2h", "pcg_fft_last_roundoff",
-    "pcg_last_profile", "pcg_ntt_plan",
+    "pcg_last_profile", "pcg_ntt_plan", "pcg_timer_start", "pcg_timer_stop",
 ]
 
 
@@ -78,6 +78,7 @@ def load():
     L.pcg_fft_last_roundoff.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.pcg_last_profile.argtypes = [C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
     L.pcg_ntt_plan.argtypes = [u64, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.pcg_timer_stop.argtypes = [C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -209,3 +210,13 @@ def ntt_plan(total_limbs: int):
 def reserve(total_limbs: int):
     ensure_init()
     _check(load().pcg_reserve(total_limbs))
+
+
+def timer_start():
+    _check(load().pcg_timer_start())
+
+
+def timer_stop() -> float:
+    ms = C.c_double()
+    _check(load().pcg_timer_stop(C.byref(ms)))
+    return ms.value

@@ -1,0 +1,84 @@
+"""Batched products (pcg_mul_batch / pcg_mul_batch_dev) against exact
+arithmetic: every size class (schoolbook, single-workgroup NTT, multi-pass
+NTT), uneven lengths, and identity with the plain one-launch-per-product path."""
+import random
+
+import numpy as np
+import pytest
+
+from piculator_amd import workload
+from tests.helpers import exact_product, residues_match
+
+pytestmark = pytest.mark.gpu
+
+M32 = 0xFFFFFFFF
+
+
+def rand_limbs(rng, n, ones=False):
+    if ones:
+        return np.full(n, M32, dtype=np.uint32)
+    return np.array([rng.getrandbits(32) for _ in range(n)], dtype=np.uint32)
+
+
+def test_every_class_exact(gpu):
+    rng = random.Random(11)
+    sizes = [(1, 1), (1, 1023), (512, 512), (3, 700), (1000, 24), (1025, 1), (600, 1000),
+             (2048, 2048), (1, 4095), (3000, 1096), (4097, 1), (5000, 5000), (20000, 3)]
+    pairs = []
+    for k, (na, nb) in enumerate(sizes):
+        ones = k % 4 == 3
+        pairs.append((rand_limbs(rng, na, ones), rand_limbs(rng, nb, ones)))
+    outs = gpu.mul_batch(pairs)
+    for (a, b), o in zip(pairs, outs):
+        assert np.array_equal(o, exact_product(a, b)), (a.size, b.size)
+
+
+def test_zero_length_operands_in_batch(gpu):
+    a = np.array([1, 2, 3], dtype=np.uint32)
+    e = np.zeros(0, dtype=np.uint32)
+    outs = gpu.mul_batch([(a, e), (e, e), (a, a)])
+    assert np.array_equal(outs[0], np.zeros(3, dtype=np.uint32))
+    assert outs[1].size == 0
+    assert np.array_equal(outs[2], exact_product(a, a))
+
+
+def test_batch_equals_plain_on_uneven_workload(gpu):
+    pairs = workload.batch_operands(512)
+    got = gpu.mul_batch(pairs)
+    plain = gpu.mul_batch(pairs, plain=True)
+    for (a, b), g, p in zip(pairs, got, plain):
+        assert np.array_equal(g, p)
+        if a.size + b.size <= 3000:
+            assert np.array_equal(g, exact_product(a, b))
+        else:
+            assert residues_match(a, b, g)
+
+
+def test_full_uneven_batch_4096(gpu):
+    pairs = workload.batch_operands(4096)
+    lens = workload.batch_lengths(4096)
+    assert all(16 <= na <= 16384 and 16 <= nb <= 16384 for na, nb in lens)
+    got = gpu.mul_batch(pairs)
+    for (a, b), g in zip(pairs, got):
+        assert g.size == a.size + b.size
+        assert residues_match(a, b, g)
+
+
+def test_batch_dev_pointers(gpu):
+    rng = random.Random(5)
+    pairs = [(rand_limbs(rng, rng.randint(1, 3000)), rand_limbs(rng, rng.randint(1, 3000)))
+             for _ in range(40)]
+    bufs = []
+    descs = (gpu.MulDesc * len(pairs))()
+    for i, (a, b) in enumerate(pairs):
+        da, db = gpu.DeviceBuffer(a.nbytes), gpu.DeviceBuffer(b.nbytes)
+        do = gpu.DeviceBuffer(a.nbytes + b.nbytes)
+        da.upload(a)
+        db.upload(b)
+        bufs.append((da, db, do))
+        descs[i] = gpu.MulDesc(da.ptr, a.size, db.ptr, b.size, do.ptr)
+    gpu._check(gpu.load().pcg_mul_batch_dev(descs, len(pairs), 0))
+    for (a, b), (_, _, do) in zip(pairs, bufs):
+        out = np.empty(a.size + b.size, dtype=np.uint32)
+        do.download(out)
+        assert np.array_equal(out, exact_product(a, b))

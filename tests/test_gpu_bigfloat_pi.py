@@ -1,0 +1,71 @@
+"""GPU BigFloat (piculator_amd/host) against the oracle's BigFloat, and pi.txt
+from the GPU pipeline against the oracle digests (reference semantics,
+PI.h:138-210).  Every big multiply runs through libpicugpu.so."""
+import hashlib
+import json
+import os
+import random
+
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def host(gpu):
+    from piculator_amd import pihost
+
+    pihost.load()
+    return pihost
+
+
+def rand_tuple(rng, n, exp=None, sign=None):
+    limbs = [rng.getrandbits(32) for _ in range(n)]
+    limbs[-1] |= 1
+    return (rng.random() < 0.5 if sign is None else sign,
+            rng.randint(-30, 30) if exp is None else exp, limbs)
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_mul_add_sub_match_oracle(host, seed):
+    rng = random.Random(seed)
+    a = rand_tuple(rng, rng.choice([1, 2, 7, 100, 3000]))
+    b = rand_tuple(rng, rng.choice([1, 3, 50, 2000, 5000]))
+    oa, ob = O.BF(*a), O.BF(*b)
+    for p in (0, 1, 10, 1000):
+        assert host.bf_op("mul", a, b, p) == oa.mul(ob, p).tuple()
+        assert host.bf_op("add", a, b, p) == oa.add(ob, p).tuple()
+        assert host.bf_op("sub", a, b, p) == oa.sub(ob, p).tuple()
+
+
+@pytest.mark.parametrize("p", [0, 1, 2, 3, 17, 500])
+def test_newton_match_oracle(host, p):
+    rng = random.Random(p)
+    a = rand_tuple(rng, 600, exp=-7, sign=False)
+    b = rand_tuple(rng, 400, exp=2, sign=True)
+    oa, ob = O.BF(*a), O.BF(*b)
+    assert host.bf_op("rcp", a, p=p) == oa.rcp(p).tuple()
+    assert host.bf_op("div", b, a, p=p) == ob.div(oa, p).tuple()
+    assert host.bf_op("invsqrt", (True, 0, [10005]), p=p) == O.invsqrt(10005, p).tuple()
+
+
+def test_errors_are_reference_literals(host):
+    with pytest.raises(RuntimeError, match="Divide by Zero"):
+        host.bf_op("rcp", (True, 0, []), p=3)
+
+
+@pytest.mark.parametrize("D", [10000, 100000, 1000000])
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+def test_pi_txt_matches_oracle(host, D, mode):
+    if mode == "plain" and D > 100000:
+        pytest.skip("plain mode at 1e6 is exercised by bench.py")
+    with open(os.path.join(HERE, "golden", "pi_digests.json")) as f:
+        fx = json.load(f)[str(D)]
+    s, t = host.pi_hex(D, host.MODE_PLAIN if mode == "plain" else host.MODE_BATCHED)
+    assert len(s) == fx["bytes"]
+    assert s[:66] == fx["head"]
+    assert hashlib.sha256(s.encode()).hexdigest() == fx["sha256"]

@@ -1,0 +1,56 @@
+"""FP64 FFT engine (PCG_ENGINE_FFT): 16-bit balanced digits, right-angle
+convolution, round-off report and NTT recompute above 0.25.  The product must
+be bit-exact against the NTT engine and exact arithmetic in every case
+(the reference's FFT.cpp path uses 8-bit digits and no check; FFT.cpp:485-571)."""
+import random
+
+import numpy as np
+import pytest
+
+from piculator_amd import workload
+from tests.helpers import exact_product, residues_match
+
+pytestmark = pytest.mark.gpu
+
+M32 = 0xFFFFFFFF
+
+
+@pytest.mark.parametrize("na,nb", [(1, 1), (5, 3), (300, 200), (1000, 1000), (4096, 4096),
+                                   (20000, 777), (50000, 50000)])
+def test_fft_exact_small(gpu, na, nb):
+    rng = random.Random(na + 3 * nb)
+    a = np.array([rng.getrandbits(32) for _ in range(na)], dtype=np.uint32)
+    b = np.array([rng.getrandbits(32) for _ in range(nb)], dtype=np.uint32)
+    got = gpu.mul(a, b, gpu.ENGINE_FFT)
+    dist, recomputed = gpu.fft_last_roundoff()
+    assert 0.0 <= dist
+    assert np.array_equal(got, exact_product(a, b)), (dist, recomputed)
+
+
+def test_fft_all_ones(gpu):
+    a = np.full(3000, M32, dtype=np.uint32)
+    got = gpu.mul(a, a, gpu.ENGINE_FFT)
+    assert np.array_equal(got, exact_product(a, a))
+
+
+@pytest.mark.parametrize("logn", [16, 20, 22])
+def test_fft_roundoff_report_and_exactness(gpu, logn):
+    n = 1 << logn
+    a, b = workload.operand_pair(n)
+    got = gpu.mul(a, b, gpu.ENGINE_FFT)
+    dist, recomputed = gpu.fft_last_roundoff()
+    assert recomputed == (dist > 0.25)
+    ref = gpu.mul(a, b, gpu.ENGINE_NTT)
+    assert np.array_equal(got, ref)
+    assert residues_match(a, b, got)
+
+
+def test_fft_recompute_path_is_exercised(gpu):
+    # 2^24-limb operands: coefficients exceed 2^53, so the FFT cannot be exact;
+    # the product must still be right through the NTT recompute.
+    n = 1 << 24
+    a, b = workload.operand_pair(n)
+    got = gpu.mul(a, b, gpu.ENGINE_FFT)
+    dist, recomputed = gpu.fft_last_roundoff()
+    assert recomputed and dist > 0.25
+    assert residues_match(a, b, got)
