@@ -23,7 +23,7 @@ struct BatchDesc {
 };
 
 constexpr int kSchoolMaxLimbs = 1024;  // zL bound of k_batch_school
-constexpr int kSmallNttMaxLog = 12;    // N bound of k_batch_ntt
+constexpr int kSmallNttMaxLog = 12;    // N bound of k_batch_ntt (rows padded by one)
 
 __global__ __launch_bounds__(256) void k_batch_school(const BatchDesc* descs) {
   __shared__ uint32_t sa[kSchoolMaxLimbs], sb[kSchoolMaxLimbs];
@@ -64,9 +64,12 @@ __global__ __launch_bounds__(256) void k_batch_school(const BatchDesc* descs) {
 
 // Block size N / 16 (N = 2^logN, 2^10 <= N <= 2^12); all products of one
 // launch share logN.  prm points at the constants for this N.
-__global__ __launch_bounds__(256) void k_batch_ntt(const BatchDesc* descs, const Prime* prm,
-                                                    int logN, double2 crt) {
-  __shared__ double lds[2 << kSmallNttMaxLog];
+template <int LOGN>
+__global__ __launch_bounds__((1 << LOGN) / kEPT) void k_batch_ntt(const BatchDesc* descs,
+                                                                  const Prime* prm, double2 crt) {
+  using GE = Geo<LOGN, 0, true>;
+  constexpr int logN = LOGN;
+  __shared__ double lds[2 << LOGN];
   __shared__ CarryFn wtot[4];
   const BatchDesc d = descs[blockIdx.x];
   const int tid = threadIdx.x, nth = blockDim.x;
@@ -80,14 +83,14 @@ __global__ __launch_bounds__(256) void k_batch_ntt(const BatchDesc* descs, const
       lds[N + i] = i < nb ? (double)d.b[i] : 0.0;
     }
     __syncthreads();
-    line_dft(lds, logN, 0, P, false);
-    line_dft(lds + N, logN, 0, P, false);
+    line_dft<GE>(lds, P, false);
+    line_dft<GE>(lds + N, P, false);
     for (int i = tid; i < N; i += nth) {
       const double x = mm_gen(lds[i], lds[N + i], P.p, P.pinv);
       lds[i] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
     }
     __syncthreads();
-    line_dft(lds, logN, 0, P, true);
+    line_dft<GE>(lds, P, true);
 #pragma unroll
     for (int q = 0; q < kEPT; ++q) r[pi][q] = lds[tid * kEPT + q];
     __syncthreads();

@@ -283,7 +283,9 @@ __global__ __launch_bounds__(256) void k_carry_fix(CarryArgs a) {
     c = (c >> 32) + (t >> 32);
     ++k;
   }
-  if (c != 0 && a.status && k < a.zL) *a.status = 2;
+  // A carry left at the block end is the block's overflow bit (0 or 1); the
+  // scan already added it to the next block's carry-in.
+  if (c > 1 && a.status) *a.status = 2;
 }
 
 }  // namespace pcg

@@ -102,31 +102,54 @@ __device__ __forceinline__ double2 root_pow(uint64_t e, int logM, bool inv, cons
   return make_double2(w, w * P.pinv);
 }
 
-// One Stockham stage of radix R over all lines held in LDS.
-// LDS layout: point n of line lam at lds[n * lines + lam].
-template <int R, int LOGR>
-__device__ __forceinline__ void stockham_stage(double* lds, int logL, int logLines, int logNs,
-                                               const Prime& P, bool inv) {
+// LDS layout of the lines staged by one workgroup.  Column passes (stride
+// s > 1): point n of line lam at n * LINES + lam, so consecutive threads touch
+// consecutive columns.  Row passes (s == 1): line-major with one pad double
+// per line, point n of line lam at lam * (L + 1) + n.
+template <int LOGL, int LOGLINES, bool ROW>
+struct Geo {
+  static constexpr int L = 1 << LOGL;
+  static constexpr int LINES = 1 << LOGLINES;
+  static constexpr int E = L * LINES;
+  static constexpr int NTH = E / kEPT;
+  static constexpr int LP = ROW ? L + 1 : L;
+  static constexpr int LDS = ROW ? LINES * LP : E;
+  __device__ static __forceinline__ int idx(int n, int lam) {
+    return ROW ? lam * LP + n : (n << LOGLINES) + lam;
+  }
+  // butterfly bf -> (line, j): the fast-varying index follows the layout
+  __device__ static __forceinline__ void split(int bf, int logLR, int& lam, int& j) {
+    if (ROW) {
+      lam = bf >> logLR;
+      j = bf & ((1 << logLR) - 1);
+    } else {
+      lam = bf & (LINES - 1);
+      j = bf >> LOGLINES;
+    }
+  }
+};
+
+// One Stockham stage of radix R (Ns = 2^logNs) over every line in LDS.
+template <class GE, int R, int LOGR, int LOGNS>
+__device__ __forceinline__ void stockham_stage(double* lds, const Prime& P, bool inv) {
   constexpr int NB = kEPT / R;
-  const int nth = blockDim.x, tid = threadIdx.x;
-  const int linesMask = (1 << logLines) - 1;
-  const int logLR = logL - LOGR;
-  const int NsMask = (1 << logNs) - 1;
-  const int sh = kLogE - logNs - LOGR;  // 4096 / (Ns * R) as a shift
+  constexpr int LOGLR = GE::L == 1 ? 0 : (__builtin_ctz(GE::L) - LOGR);
+  constexpr int NS = 1 << LOGNS;
+  constexpr int SH = kLogE - LOGNS - LOGR;
+  const int tid = threadIdx.x;
   double v[NB][R];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const int bf = tid + b * nth;
-    const int lam = bf & linesMask;
-    const int j = bf >> logLines;
+    int lam, j;
+    GE::split(tid + b * GE::NTH, LOGLR, lam, j);
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[b][r] = lds[((j + (r << logLR)) << logLines) + lam];
-    if (logNs > 0) {
-      const int jm = j & NsMask;
+    for (int r = 0; r < R; ++r) v[b][r] = lds[GE::idx(j + (r << LOGLR), lam)];
+    if (LOGNS > 0) {
+      const int jm = j & (NS - 1);
       v[b][0] = reduce(v[b][0], P.p, P.pinv);
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        int e = (jm * r) << sh;
+        int e = (jm * r) << SH;
         if (inv) e = (kE - e) & (kE - 1);
         const double2 t = P.twL[e];
         v[b][r] = mm_shoup(v[b][r], t.x, t.y, P.p);
@@ -137,103 +160,123 @@ __device__ __forceinline__ void stockham_stage(double* lds, int logL, int logLin
   __syncthreads();
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const int bf = tid + b * nth;
-    const int lam = bf & linesMask;
-    const int j = bf >> logLines;
-    const int jm = j & NsMask;
+    int lam, j;
+    GE::split(tid + b * GE::NTH, LOGLR, lam, j);
+    const int jm = j & (NS - 1);
     const int idxD = ((j - jm) << LOGR) + jm;
 #pragma unroll
-    for (int r = 0; r < R; ++r) lds[((idxD + (r << logNs)) << logLines) + lam] = v[b][r];
+    for (int r = 0; r < R; ++r) lds[GE::idx(idxD + (r << LOGNS), lam)] = v[b][r];
   }
   __syncthreads();
 }
 
-// Natural-order DFT of length 2^logL of every line in LDS.
-__device__ __forceinline__ void line_dft(double* lds, int logL, int logLines, const Prime& P,
-                                         bool inv) {
-  int logNs = 0;
-  const int first = logL % 3;
-  if (first == 1) {
-    stockham_stage<2, 1>(lds, logL, logLines, 0, P, inv);
-    logNs = 1;
-  } else if (first == 2) {
-    stockham_stage<4, 2>(lds, logL, logLines, 0, P, inv);
-    logNs = 2;
+template <class GE, int LOGNS>
+__device__ __forceinline__ void radix8_stages(double* lds, const Prime& P, bool inv) {
+  if constexpr (LOGNS < __builtin_ctz(GE::L)) {
+    stockham_stage<GE, 8, 3, LOGNS>(lds, P, inv);
+    radix8_stages<GE, LOGNS + 3>(lds, P, inv);
   }
-  for (; logNs < logL; logNs += 3) stockham_stage<8, 3>(lds, logL, logLines, logNs, P, inv);
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_ntt_pass(PassArgs a) {
-  __shared__ double lds[kE];
+// Natural-order DFT of length L of every line in LDS.
+template <class GE>
+__device__ __forceinline__ void line_dft(double* lds, const Prime& P, bool inv) {
+  constexpr int LOGL = __builtin_ctz(GE::L);
+  constexpr int FIRST = LOGL % 3;
+  if constexpr (FIRST == 1) {
+    stockham_stage<GE, 2, 1, 0>(lds, P, inv);
+    radix8_stages<GE, 1>(lds, P, inv);
+  } else if constexpr (FIRST == 2) {
+    stockham_stage<GE, 4, 2, 0>(lds, P, inv);
+    radix8_stages<GE, 2>(lds, P, inv);
+  } else {
+    radix8_stages<GE, 0>(lds, P, inv);
+  }
+}
+
+// One pass.  Column passes (ROW false) need s >= LINES (one block of L*s
+// points spans several workgroups, each taking LINES adjacent columns); row
+// passes have s == 1 and take LINES consecutive lines.
+template <int MODE, int LOGL, int LOGLINES, bool ROW>
+__global__ __launch_bounds__(1 << (LOGL + LOGLINES - 4)) void k_ntt_pass(PassArgs a) {
+  using GE = Geo<LOGL, LOGLINES, ROW>;
+  __shared__ double lds[GE::LDS];
   const int pi = blockIdx.y;
   const Prime P = a.prm[pi];
-  const int tid = threadIdx.x, nth = blockDim.x;
-  const int logL = a.logL, logS = a.logS, logLines = a.logLines, logG = a.logG;
-  const int Gmask = (1 << logG) - 1, Lmask = (1 << logL) - 1;
-  const uint64_t ncb = 1ull << (logS - logG);  // column groups per block of L*s points
+  const int tid = threadIdx.x;
+  const int logS = a.logS;
   const uint64_t w = blockIdx.x;
-  const uint64_t blk0 = (w >> (logS - logG)) << a.logB;
-  const uint64_t col0 = (w & (ncb - 1)) << logG;
+  // first point of this workgroup's lines and the global step between them
+  uint64_t base;
+  if (ROW) {
+    base = w << (LOGL + LOGLINES);
+  } else {
+    const int lgc = logS - LOGLINES;  // column groups per block, log2
+    base = ((w >> lgc) << (LOGL + logS)) + ((w & ((1ull << lgc) - 1)) << LOGLINES);
+  }
+  const uint64_t col0 = ROW ? 0 : (base & ((1ull << logS) - 1));
   const double* src = a.src ? a.src + pi * a.N : nullptr;
   double* dst = a.dst + pi * a.N;
-  const int logRoot = logL + logS;
+  const int logRoot = LOGL + logS;
+
+  // element i of this thread: LDS point (n, lam) and its global index
+#define PCG_ELEM(i, n, lam, g)                                        \
+  int n, lam;                                                         \
+  uint64_t g;                                                         \
+  {                                                                   \
+    const int f = tid + (i) * GE::NTH;                                \
+    if (ROW) {                                                        \
+      lam = f >> LOGL;                                                \
+      n = f & (GE::L - 1);                                            \
+      g = base + (uint64_t)f;                                         \
+    } else {                                                          \
+      lam = f & (GE::LINES - 1);                                      \
+      n = f >> LOGLINES;                                              \
+      g = base + ((uint64_t)n << logS) + lam;                         \
+    }                                                                 \
+  }
 
 #pragma unroll
   for (int i = 0; i < kEPT; ++i) {
-    const int f = tid + i * nth;
-    const int col = f & Gmask;
-    const int t = f >> logG;
-    const int n = t & Lmask;
-    const int b = t >> logL;
-    const uint64_t g = ((blk0 + b) << logRoot) + ((uint64_t)n << logS) + col0 + col;
+    PCG_ELEM(i, n, lam, g);
     double x;
     if (src)
       x = src[g];
     else
       x = g < a.nlimbs ? (double)a.limbs[g] : 0.0;
-    if (MODE == kInv && logS > 0) {
-      const double2 tw = root_pow((col0 + col) * (uint64_t)n, logRoot, true, P);
+    if (MODE == kInv && !ROW) {
+      const double2 tw = root_pow((col0 + lam) * (uint64_t)n, logRoot, true, P);
       x = mm_shoup(x, tw.x, tw.y, P.p);
     }
-    lds[(n << logLines) + (b << logG) + col] = x;
+    lds[GE::idx(n, lam)] = x;
   }
   __syncthreads();
-  line_dft(lds, logL, logLines, P, MODE == kInv);
+  line_dft<GE>(lds, P, MODE == kInv);
   if (MODE == kFused) {
     const double* spec = a.spec + pi * a.N;
 #pragma unroll
     for (int i = 0; i < kEPT; ++i) {
-      const int f = tid + i * nth;
-      const int col = f & Gmask;
-      const int t = f >> logG;
-      const int n = t & Lmask;
-      const int b = t >> logL;
-      const uint64_t g = ((blk0 + b) << logRoot) + ((uint64_t)n << logS) + col0 + col;
-      const int li = (n << logLines) + (b << logG) + col;
-      double x = mm_gen(lds[li], spec[g], P.p, P.pinv);
+      PCG_ELEM(i, n, lam, g);
+      const int li = GE::idx(n, lam);
+      const double x = mm_gen(lds[li], spec[g], P.p, P.pinv);
       lds[li] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
     }
     __syncthreads();
-    line_dft(lds, logL, logLines, P, true);
+    line_dft<GE>(lds, P, true);
   }
 #pragma unroll
   for (int i = 0; i < kEPT; ++i) {
-    const int f = tid + i * nth;
-    const int col = f & Gmask;
-    const int t = f >> logG;
-    const int n = t & Lmask;
-    const int b = t >> logL;
-    const uint64_t g = ((blk0 + b) << logRoot) + ((uint64_t)n << logS) + col0 + col;
-    double x = lds[(n << logLines) + (b << logG) + col];
-    if (MODE == kFwd && logS > 0) {
-      const double2 tw = root_pow((col0 + col) * (uint64_t)n, logRoot, false, P);
+    PCG_ELEM(i, n, lam, g);
+    double x = lds[GE::idx(n, lam)];
+    if (MODE == kFwd && !ROW) {
+      const double2 tw = root_pow((col0 + lam) * (uint64_t)n, logRoot, false, P);
       x = mm_shoup(x, tw.x, tw.y, P.p);
     } else {
       x = reduce(x, P.p, P.pinv);
     }
     dst[g] = x;
   }
+#undef PCG_ELEM
 }
 
 }  // namespace pcg

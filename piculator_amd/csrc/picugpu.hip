@@ -74,8 +74,42 @@ struct Plan {
   std::vector<Pass> passes;
 };
 
-Plan make_plan(int logN, int logEmax = kLogE) {
+// NTT pass plan: strided passes take at most 2^9-point lines, so a
+// workgroup's 4096 staged points cover >= 8 adjacent columns (64-byte runs);
+// the last pass (contiguous rows) takes up to 2^12.
+constexpr int kMaxLogCol = 9;
+
+Plan make_plan(int logN) {
   Plan pl;
+  pl.logN = logN;
+  pl.logE = logN < kLogE ? logN : kLogE;
+  int m = 1;
+  while (logN > kMaxLogCol * (m - 1) + kLogE) ++m;
+  int last = (logN + m - 1) / m;
+  if (last < logN - kMaxLogCol * (m - 1)) last = logN - kMaxLogCol * (m - 1);
+  if (last > kLogE) last = kLogE;
+  std::vector<int> ls;
+  const int rest = logN - last;
+  for (int i = 0; i < m - 1; ++i) ls.push_back(rest / (m - 1) + (i < rest % (m - 1) ? 1 : 0));
+  ls.push_back(last);
+  int rem = logN;
+  for (int i = 0; i < m; ++i) {
+    Pass ps;
+    ps.logL = ls[i];
+    rem -= ps.logL;
+    ps.logS = rem;
+    ps.logLines = pl.logE - ps.logL;
+    ps.logG = ps.logS >= ps.logLines ? ps.logLines : ps.logS;
+    ps.logB = ps.logLines - ps.logG;
+    pl.passes.push_back(ps);
+  }
+  return pl;
+}
+
+// The same decomposition for the FFT engine (2048 points per workgroup).
+Plan make_fft_plan(int logN) {
+  Plan pl;
+  const int logEmax = kFftLogE;
   pl.logN = logN;
   pl.logE = logN < logEmax ? logN : logEmax;
   int m = (logN + logEmax - 1) / logEmax;
@@ -88,17 +122,52 @@ Plan make_plan(int logN, int logEmax = kLogE) {
     rem -= ps.logL;
     ps.logS = rem;
     ps.logLines = pl.logE - ps.logL;
-    if (ps.logS >= ps.logLines) {
-      ps.logG = ps.logLines;
-      ps.logB = 0;
-    } else {
-      ps.logG = ps.logS;
-      ps.logB = ps.logLines - ps.logS;
-    }
+    ps.logG = ps.logS >= ps.logLines ? ps.logLines : ps.logS;
+    ps.logB = ps.logLines - ps.logG;
     pl.passes.push_back(ps);
   }
   return pl;
 }
+
+// Kernel instance of an NTT pass: column passes (stride > 1) for line
+// lengths 2^4..2^9 with 4096 staged points, row passes (stride 1) for
+// 2^6..2^12 with 4096 staged points or a single line of 2^8..2^11.
+using PassFn = void (*)(PassArgs);
+template <int MODE, int LOGL, int LOGLINES, bool ROW>
+PassFn pass_fn() {
+  return &k_ntt_pass<MODE, LOGL, LOGLINES, ROW>;
+}
+#define PCG_COL(M, L) \
+  case L: return pass_fn<M, L, kLogE - L, false>();
+#define PCG_ROW(M, L) \
+  case L: return pass_fn<M, L, kLogE - L, true>();
+template <int M>
+PassFn col_pass_fn(int logL) {
+  switch (logL) {
+    PCG_COL(M, 4) PCG_COL(M, 5) PCG_COL(M, 6) PCG_COL(M, 7) PCG_COL(M, 8) PCG_COL(M, 9)
+  }
+  return nullptr;
+}
+template <int M>
+PassFn row_pass_fn(int logL, int logLines) {
+  if (logLines == 0) {
+    switch (logL) {
+      case 8: return pass_fn<M, 8, 0, true>();
+      case 9: return pass_fn<M, 9, 0, true>();
+      case 10: return pass_fn<M, 10, 0, true>();
+      case 11: return pass_fn<M, 11, 0, true>();
+      case 12: return pass_fn<M, 12, 0, true>();
+    }
+    return nullptr;
+  }
+  if (logL + logLines != kLogE) return nullptr;
+  switch (logL) {
+    PCG_ROW(M, 6) PCG_ROW(M, 7) PCG_ROW(M, 8) PCG_ROW(M, 9) PCG_ROW(M, 10) PCG_ROW(M, 11)
+  }
+  return nullptr;
+}
+#undef PCG_COL
+#undef PCG_ROW
 
 constexpr int kMinLogN = 8;
 
@@ -272,8 +341,30 @@ int check_sizes(uint64_t na, uint64_t nb) {
   return PCG_OK;
 }
 
-void launch_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double* src,
-                 const uint32_t* limbs, uint64_t nlimbs, double* dst, const double* spec) {
+PassFn select_pass(int mode, const Plan& pl, const Pass& ps) {
+  if (ps.logS == 0) {
+    if (mode == kFwd) return row_pass_fn<kFwd>(ps.logL, ps.logLines);
+    if (mode == kFused) return row_pass_fn<kFused>(ps.logL, ps.logLines);
+  } else if (ps.logG == ps.logLines && pl.logE == kLogE) {
+    if (mode == kFwd) return col_pass_fn<kFwd>(ps.logL);
+    if (mode == kInv) return col_pass_fn<kInv>(ps.logL);
+  }
+  return nullptr;
+}
+
+// Every pass of the plan has a kernel instance for each mode it runs in.
+bool plan_supported(const Plan& pl) {
+  const size_t m = pl.passes.size();
+  for (size_t j = 0; j < m; ++j) {
+    if (!select_pass(kFwd, pl, pl.passes[j])) return false;
+    if (j + 1 < m && !select_pass(kInv, pl, pl.passes[j])) return false;
+    if (j + 1 == m && !select_pass(kFused, pl, pl.passes[j])) return false;
+  }
+  return true;
+}
+
+int launch_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double* src,
+                const uint32_t* limbs, uint64_t nlimbs, double* dst, const double* spec) {
   PassArgs a;
   a.prm = c.d_prime + pl.logN * 2;
   a.src = src;
@@ -287,14 +378,14 @@ void launch_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double*
   a.logLines = ps.logLines;
   a.logG = ps.logG;
   a.logB = ps.logB;
+  PassFn fn = select_pass(mode, pl, ps);
+  if (!fn)
+    return fail(PCG_ERR_INTERNAL, "no NTT pass kernel for logN=%d logL=%d logS=%d mode=%d",
+                pl.logN, ps.logL, ps.logS, mode);
   const int E = 1 << pl.logE;
   dim3 grid((unsigned)(a.N / E), 2), block(E / kEPT);
-  if (mode == kFwd)
-    hipLaunchKernelGGL(k_ntt_pass<kFwd>, grid, block, 0, c.stream, a);
-  else if (mode == kInv)
-    hipLaunchKernelGGL(k_ntt_pass<kInv>, grid, block, 0, c.stream, a);
-  else
-    hipLaunchKernelGGL(k_ntt_pass<kFused>, grid, block, 0, c.stream, a);
+  hipLaunchKernelGGL(fn, grid, block, 0, c.stream, a);
+  return PCG_OK;
 }
 
 // Enqueue the whole NTT product on the library stream (device pointers).
@@ -311,13 +402,21 @@ int ntt_mul_enqueue(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_
   double* B = c.spec + 2 * N;
   const size_t m = pl.passes.size();
   if (profile) HIPCHK(hipEventRecord(c.ev[0], c.stream));
-  for (size_t j = 0; j < m; ++j)
-    launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : A, d_a, na, A, nullptr);
+  for (size_t j = 0; j < m; ++j) {
+    rc = launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : A, d_a, na, A, nullptr);
+    if (rc) return rc;
+  }
   if (profile) HIPCHK(hipEventRecord(c.ev[1], c.stream));
-  for (size_t j = 0; j + 1 < m; ++j)
-    launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : B, d_b, nb, B, nullptr);
-  launch_pass(c, kFused, pl, pl.passes[m - 1], m == 1 ? nullptr : B, d_b, nb, B, A);
-  for (size_t j = m - 1; j-- > 0;) launch_pass(c, kInv, pl, pl.passes[j], B, nullptr, 0, B, nullptr);
+  for (size_t j = 0; j + 1 < m; ++j) {
+    rc = launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : B, d_b, nb, B, nullptr);
+    if (rc) return rc;
+  }
+  rc = launch_pass(c, kFused, pl, pl.passes[m - 1], m == 1 ? nullptr : B, d_b, nb, B, A);
+  if (rc) return rc;
+  for (size_t j = m - 1; j-- > 0;) {
+    rc = launch_pass(c, kInv, pl, pl.passes[j], B, nullptr, 0, B, nullptr);
+    if (rc) return rc;
+  }
   if (profile) HIPCHK(hipEventRecord(c.ev[2], c.stream));
   CarryArgs ca;
   ca.prm = c.d_prime + logN * 2;
@@ -380,7 +479,7 @@ int fft_mul_enqueue(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_
   if (logN < kMinLogN) logN = kMinLogN;
   int rc = ensure_workspace(c, logN, zL);
   if (rc) return rc;
-  const Plan pl = make_plan(logN, kFftLogE);
+  const Plan pl = make_fft_plan(logN);
   const uint64_t N = 1ull << logN;
   double2* A = reinterpret_cast<double2*>(c.spec);
   double2* B = A + N;
@@ -505,11 +604,11 @@ int launch_classes(Ctx& c, size_t n_school, size_t n11, size_t n12) {
   if (n_school)
     hipLaunchKernelGGL(k_batch_school, dim3((unsigned)n_school), dim3(256), 0, c.stream, c.d_desc);
   if (n11)
-    hipLaunchKernelGGL(k_batch_ntt, dim3((unsigned)n11), dim3(128), 0, c.stream,
-                       c.d_desc + n_school, c.d_prime + 11 * 2, 11, c.crt);
+    hipLaunchKernelGGL(k_batch_ntt<11>, dim3((unsigned)n11), dim3(128), 0, c.stream,
+                       c.d_desc + n_school, c.d_prime + 11 * 2, c.crt);
   if (n12)
-    hipLaunchKernelGGL(k_batch_ntt, dim3((unsigned)n12), dim3(256), 0, c.stream,
-                       c.d_desc + n_school + n11, c.d_prime + 12 * 2, 12, c.crt);
+    hipLaunchKernelGGL(k_batch_ntt<12>, dim3((unsigned)n12), dim3(256), 0, c.stream,
+                       c.d_desc + n_school + n11, c.d_prime + 12 * 2, c.crt);
   HIPCHK(hipGetLastError());
   return PCG_OK;
 }
@@ -851,8 +950,10 @@ int pcg_ntt_plan(uint64_t total_limbs, int* log2_points, int* passes) {
   if (total_limbs > PCG_MAX_TOTAL_LIMBS) return fail(PCG_ERR_TOO_LARGE, "FFT length is too small.");
   int logN = ceil_log2(total_limbs);
   if (logN < kMinLogN) logN = kMinLogN;
+  const Plan pl = make_plan(logN);
+  if (!plan_supported(pl)) return fail(PCG_ERR_INTERNAL, "no kernel instance for 2^%d points", logN);
   if (log2_points) *log2_points = logN;
-  if (passes) *passes = (int)make_plan(logN).passes.size();
+  if (passes) *passes = (int)pl.passes.size();
   return PCG_OK;
 }
 

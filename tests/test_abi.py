@@ -36,6 +36,9 @@ def test_calls_before_init_fail_cleanly():
 def test_plan_limits():
     lg, passes = picugpu.ntt_plan(2 ** 25)
     assert lg == 25 and passes == 3
+    # every transform length has a kernel instance for each of its passes
+    for lg in range(8, 31):
+        assert picugpu.ntt_plan(2 ** lg)[0] == lg
     assert picugpu.ntt_plan(3)[0] == 8
     with pytest.raises(picugpu.PcgError):
         picugpu.ntt_plan(2 ** 31)
