@@ -63,6 +63,54 @@ def cpu_baseline(threads: int, logn_sample: int = 22, logn_target: int = 24):
     }
 
 
+def batch_compare(count: int = 4096, reps: int = 3):
+    """Uneven batch (lengths log-uniform in 2^4..2^14 limbs, seed 0xC0DE),
+    operands resident in HBM: one pcg_mul_batch_dev call vs the plain form
+    (one launch sequence per product, PCG_BATCH_PLAIN)."""
+    import numpy as np
+
+    from piculator_amd import picugpu, workload
+    from tests.helpers import residues_match
+
+    pairs = workload.batch_operands(count)
+    total = sum(a.size + b.size for a, b in pairs)
+    pool = picugpu.DeviceBuffer(8 * total + 64)
+    descs = (picugpu.MulDesc * count)()
+    host = np.zeros(2 * total, dtype=np.uint32)
+    off = 0
+    outs = []
+    for i, (a, b) in enumerate(pairs):
+        pa, pb = off, off + a.size
+        host[pa:pa + a.size] = a
+        host[pb:pb + b.size] = b
+        po = total + off
+        descs[i] = picugpu.MulDesc(pool.ptr + 4 * pa, a.size, pool.ptr + 4 * pb, b.size,
+                                   pool.ptr + 4 * po)
+        outs.append((po, a.size + b.size))
+        off += a.size + b.size
+    pool.upload(host)
+    lib = picugpu.load()
+    res = {"products": count, "operand_limbs": int(total)}
+    for name, flags in (("batched", 0), ("plain", picugpu.BATCH_PLAIN)):
+        picugpu._check(lib.pcg_mul_batch_dev(descs, count, flags))  # warm-up
+        best = None
+        for _ in range(reps):
+            picugpu.timer_start()
+            picugpu._check(lib.pcg_mul_batch_dev(descs, count, flags))
+            ms = picugpu.timer_stop()
+            best = ms if best is None else min(best, ms)
+        res[name + "_ms"] = best
+    res["speedup"] = res["plain_ms"] / res["batched_ms"]
+    pool.download(host)
+    ok = True
+    for i in range(0, count, 97):
+        a, b = pairs[i]
+        po, n = outs[i]
+        ok &= bool(residues_match(a, b, host[po:po + n]))
+    res["checked_products_ok"] = ok
+    return res
+
+
 def pi_compare(digits: int, plain: bool):
     from piculator_amd import pihost
 
@@ -87,6 +135,48 @@ def pi_compare(digits: int, plain: bool):
     return res
 
 
+def sweep(steps: int):
+    """One JSON line per (size, shape, engine) of the single-multiply table:
+    n = 2^16, 2^20, 2^24, 2^26 limbs, n x n and n x n/2, NTT and FFT engines."""
+    import numpy as np
+
+    from piculator_amd import picugpu, workload
+    from tests.helpers import residues_match
+
+    picugpu.init(0)
+    for logn in (16, 20, 24, 26):
+        n = 1 << logn
+        for nb in (n, n // 2):
+            a, b = workload.operand_pair(n, nb)
+            da, db = picugpu.DeviceBuffer(a.nbytes), picugpu.DeviceBuffer(b.nbytes)
+            dout = picugpu.DeviceBuffer(a.nbytes + b.nbytes)
+            da.upload(a)
+            db.upload(b)
+            picugpu.reserve(n + nb)
+            for name, eng in (("ntt", picugpu.ENGINE_NTT), ("fft", picugpu.ENGINE_FFT)):
+                picugpu.mul_dev(da.ptr, n, db.ptr, nb, dout.ptr, eng, sync=True)
+                k = steps if logn < 26 else max(2, steps // 4)
+                picugpu.timer_start()
+                for _ in range(k):
+                    picugpu.mul_dev(da.ptr, n, db.ptr, nb, dout.ptr, eng, sync=(eng == picugpu.ENGINE_FFT))
+                ms = picugpu.timer_stop() / k
+                rec = {"na": n, "nb": nb, "engine": name, "ms": ms}
+                if eng == picugpu.ENGINE_FFT:
+                    dist, recomputed = picugpu.fft_last_roundoff()
+                    rec["max_roundoff"] = dist
+                    rec["ntt_recompute"] = recomputed
+                balg = workload.bytes_alg(n, nb)
+                rec["gbs_alg"] = balg / (ms / 1e3) / 1e9
+                rec["hbm_frac"] = rec["gbs_alg"] / HBM_PEAK_GBS
+                rec["mulmods_per_s"] = workload.mulmods_alg(n, nb) / (ms / 1e3)
+                out = np.empty(n + nb, dtype=np.uint32)
+                dout.download(out)
+                rec["exact"] = bool(residues_match(a, b, out))
+                print(json.dumps(rec), flush=True)
+            for buf in (da, db, dout):
+                buf.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,12 +187,18 @@ def main():
     ap.add_argument("--no-pi", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain pi schedule")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--sweep", action="store_true",
+                    help="print the single-multiply table (sizes x shapes x engines) instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.sweep:
+        sweep(args.steps)
+        return
 
     import numpy as np
 
@@ -171,6 +267,10 @@ def main():
         "stage_ms": {"forward_a": stages[0], "forward_b_pointwise_inverse": stages[1],
                      "crt_carry": stages[2], "total": total},
     }
+    if not args.no_batch:
+        log("uneven batch of 4096 products...")
+        res["batch"] = batch_compare()
+        log("batch:", json.dumps(res["batch"]))
     if not args.no_cpu:
         th = args.cpu_threads or min(16, os.cpu_count() or 1)
         log("cpu baseline (%d threads)..." % th)

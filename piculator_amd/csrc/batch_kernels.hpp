@@ -69,13 +69,13 @@ __global__ __launch_bounds__((1 << LOGN) / kEPT) void k_batch_ntt(const BatchDes
                                                                   const Prime* prm, double2 crt) {
   using GE = Geo<LOGN, 0, true>;
   constexpr int logN = LOGN;
-  __shared__ double lds[2 << LOGN];
+  // [0, 2N): the two operand lines; [2N, 3N): prime-0 results
+  __shared__ double lds[3 << LOGN];
   __shared__ CarryFn wtot[4];
   const BatchDesc d = descs[blockIdx.x];
   const int tid = threadIdx.x, nth = blockDim.x;
   const int N = 1 << logN;
   const int na = (int)d.na, nb = (int)d.nb, zL = na + nb;
-  double r[2][kEPT];
   for (int pi = 0; pi < 2; ++pi) {
     const Prime P = prm[pi];
     for (int i = tid; i < N; i += nth) {
@@ -91,9 +91,11 @@ __global__ __launch_bounds__((1 << LOGN) / kEPT) void k_batch_ntt(const BatchDes
     }
     __syncthreads();
     line_dft<GE>(lds, P, true);
+    if (pi == 0) {
 #pragma unroll
-    for (int q = 0; q < kEPT; ++q) r[pi][q] = lds[tid * kEPT + q];
-    __syncthreads();
+      for (int q = 0; q < kEPT; ++q) lds[2 * N + tid * kEPT + q] = lds[tid * kEPT + q];
+      __syncthreads();
+    }
   }
   const Prime P0 = prm[0], P1 = prm[1];
   uint32_t w0[kEPT], w1[kEPT], w2[kEPT], D[kEPT];
@@ -101,8 +103,10 @@ __global__ __launch_bounds__((1 << LOGN) / kEPT) void k_batch_ntt(const BatchDes
   for (int q = 0; q < kEPT; ++q) {
     const int k = tid * kEPT + q;
     w0[q] = w1[q] = w2[q] = 0;
-    if (k < zL) crt_words(r[0][q], r[1][q], P0, P1, crt, w0[q], w1[q], w2[q]);
+    if (k < zL)
+      crt_words(lds[2 * N + tid * kEPT + q], lds[tid * kEPT + q], P0, P1, crt, w0[q], w1[q], w2[q]);
   }
+  __syncthreads();  // sw1/sw2 below reuse the operand lines
   uint32_t* sw1 = reinterpret_cast<uint32_t*>(lds);
   uint32_t* sw2 = sw1 + N;
   uint64_t c, spill;

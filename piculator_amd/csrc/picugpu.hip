@@ -74,10 +74,11 @@ struct Plan {
   std::vector<Pass> passes;
 };
 
-// NTT pass plan: strided passes take at most 2^9-point lines, so a
-// workgroup's 4096 staged points cover >= 8 adjacent columns (64-byte runs);
-// the last pass (contiguous rows) takes up to 2^12.
-constexpr int kMaxLogCol = 9;
+// NTT pass plan: strided passes take at most 2^8-point lines, so a
+// workgroup's 4096 staged points cover >= 16 adjacent columns (128-byte
+// runs, which stream at ~80% of a contiguous copy on MI355X; 64-byte runs
+// reach ~65%); the last pass (contiguous rows) takes up to 2^12.
+constexpr int kMaxLogCol = 8;
 
 Plan make_plan(int logN) {
   Plan pl;

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "BigFloat.h"
@@ -77,6 +78,27 @@ int32_t build(std::vector<Node>& nodes, uint32_t a, uint32_t b) {
   return id;
 }
 
+// Runs body(i) for i in [0, n) on the host threads (independent items).
+template <class F>
+void parallel_for(size_t n, F body) {
+  unsigned T = std::thread::hardware_concurrency();
+  if (T > 16) T = 16;
+  if (T <= 1 || n < 256) {
+    for (size_t i = 0; i < n; ++i) body(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t per = (n + T - 1) / T;
+  for (unsigned t = 0; t < T; ++t) {
+    const size_t lo = t * per, hi = std::min(n, lo + per);
+    if (lo >= hi) break;
+    th.emplace_back([lo, hi, &body]() {
+      for (size_t i = lo; i < hi; ++i) body(i);
+    });
+  }
+  for (std::thread& x : th) x.join();
+}
+
 // Largest subtree evaluated as one explicit forest at a time.
 constexpr uint32_t kChunkLeaves = 1u << 18;
 
@@ -89,7 +111,10 @@ void bsr_subtree_batched(PQR& out, uint32_t a, uint32_t b, size_t p, bool verbos
   const int32_t H = nodes[root].height;
   std::vector<std::vector<int32_t>> by_h(H + 1);
   for (int32_t i = 0; i < (int32_t)nodes.size(); ++i) by_h[nodes[i].height].push_back(i);
-  for (int32_t i : by_h[1]) leaf(nodes[i].b, val[i]);
+  {
+    const std::vector<int32_t>& leaves = by_h[1];
+    parallel_for(leaves.size(), [&](size_t k) { leaf(nodes[leaves[k]].b, val[leaves[k]]); });
+  }
   std::vector<BigFloat> x, y;
   for (int32_t h = 2; h <= H; ++h) {
     const std::vector<int32_t>& ids = by_h[h];
@@ -108,12 +133,12 @@ void bsr_subtree_batched(PQR& out, uint32_t a, uint32_t b, size_t p, bool verbos
     }
     const double t0 = now_s();
     mb.run(false);
-    for (size_t k = 0; k < ids.size(); ++k) {
+    parallel_for(ids.size(), [&](size_t k) {
       const Node& n = nodes[ids[k]];
       val[ids[k]].P = x[k].add(y[k], p);
       val[n.left] = PQR();
       val[n.right] = PQR();
-    }
+    });
     if (verbose)
       std::fprintf(stderr, "  height %d: %zu nodes, %.3f s\n", h, ids.size(), now_s() - t0);
   }

@@ -46,11 +46,23 @@ def test_fft_roundoff_report_and_exactness(gpu, logn):
 
 
 def test_fft_recompute_path_is_exercised(gpu):
-    # 2^24-limb operands: coefficients exceed 2^53, so the FFT cannot be exact;
-    # the product must still be right through the NTT recompute.
+    # Every balanced digit 32767: coefficients reach 2^21 * 2^30 = 2^51, so the
+    # FP64 round-off exceeds 0.25 and the NTT must recompute the product.
+    n = 1 << 20
+    a = np.full(n, 0x7FFF7FFF, dtype=np.uint32)
+    got = gpu.mul(a, a, gpu.ENGINE_FFT)
+    dist, recomputed = gpu.fft_last_roundoff()
+    assert recomputed and dist > 0.25
+    assert np.array_equal(got, gpu.mul(a, a, gpu.ENGINE_NTT))
+    assert residues_match(a, a, got)
+
+
+def test_fft_random_2p24_reports_roundoff(gpu):
+    # balanced digits keep random 2^24-limb products inside the bound; either
+    # way the report is consistent and the product exact
     n = 1 << 24
     a, b = workload.operand_pair(n)
     got = gpu.mul(a, b, gpu.ENGINE_FFT)
     dist, recomputed = gpu.fft_last_roundoff()
-    assert recomputed and dist > 0.25
+    assert recomputed == (dist > 0.25)
     assert residues_match(a, b, got)
