@@ -189,6 +189,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first run one device copy of 512 MiB (counter calibration)")
     ap.add_argument("--sweep", action="store_true",
                     help="print the single-multiply table (sizes x shapes x engines) instead")
     args = ap.parse_args()
@@ -196,6 +198,17 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.calib_copy:
+        # a plain copy of known size (512 MiB read + 512 MiB written), so that
+        # FETCH_SIZE / WRITE_SIZE of the multiply kernels can be scaled by what
+        # the same counters read for it
+        import torch
+
+        src = torch.ones(1 << 27, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if args.sweep:
         sweep(args.steps)
         return

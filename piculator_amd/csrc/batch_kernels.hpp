@@ -15,13 +15,6 @@
 
 namespace pcg {
 
-struct BatchDesc {
-  const uint32_t* a;
-  const uint32_t* b;
-  uint32_t* out;
-  uint32_t na, nb;
-};
-
 constexpr int kSchoolMaxLimbs = 1024;  // zL bound of k_batch_school
 constexpr int kSmallNttMaxLog = 12;    // N bound of k_batch_ntt (rows padded by one)
 
@@ -68,32 +61,39 @@ template <int LOGN>
 __global__ __launch_bounds__((1 << LOGN) / kEPT) void k_batch_ntt(const BatchDesc* descs,
                                                                   const Prime* prm, double2 crt) {
   using GE = Geo<LOGN, 0, true>;
-  constexpr int logN = LOGN;
-  // [0, 2N): the two operand lines; [2N, 3N): prime-0 results
-  __shared__ double lds[3 << LOGN];
+  constexpr int N = 1 << LOGN;
+  constexpr int LP = GE::LP;  // padded line length
+  // [0, LP): operand a's line; [LP, 2 LP): operand b's; [2 LP, 2 LP + N):
+  // prime-0 results (natural order)
+  __shared__ double lds[2 * LP + N];
   __shared__ CarryFn wtot[4];
+  __shared__ double2 ltw[LOGN <= kLogLdsTw ? (1 << LOGN) : 1];
   const BatchDesc d = descs[blockIdx.x];
   const int tid = threadIdx.x, nth = blockDim.x;
-  const int N = 1 << logN;
   const int na = (int)d.na, nb = (int)d.nb, zL = na + nb;
+  double* la = lds;
+  double* lb = lds + LP;
+  double* r0 = lds + 2 * LP;
   for (int pi = 0; pi < 2; ++pi) {
     const Prime P = prm[pi];
+    const double2* tw = stage_twiddles<GE>(ltw, P);
     for (int i = tid; i < N; i += nth) {
-      lds[i] = i < na ? (double)d.a[i] : 0.0;
-      lds[N + i] = i < nb ? (double)d.b[i] : 0.0;
+      la[GE::idx(i, 0)] = i < na ? (double)d.a[i] : 0.0;
+      lb[GE::idx(i, 0)] = i < nb ? (double)d.b[i] : 0.0;
     }
     __syncthreads();
-    line_dft<GE>(lds, P, false);
-    line_dft<GE>(lds + N, P, false);
+    line_dft<GE>(la, tw, P, false);
+    line_dft<GE>(lb, tw, P, false);
     for (int i = tid; i < N; i += nth) {
-      const double x = mm_gen(lds[i], lds[N + i], P.p, P.pinv);
-      lds[i] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
+      const int li = GE::idx(i, 0);
+      const double x = mm_gen(la[li], lb[li], P.p, P.pinv);
+      la[li] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
     }
     __syncthreads();
-    line_dft<GE>(lds, P, true);
+    line_dft<GE>(la, tw, P, true);
     if (pi == 0) {
 #pragma unroll
-      for (int q = 0; q < kEPT; ++q) lds[2 * N + tid * kEPT + q] = lds[tid * kEPT + q];
+      for (int q = 0; q < kEPT; ++q) r0[tid * kEPT + q] = la[GE::idx(tid * kEPT + q, 0)];
       __syncthreads();
     }
   }
@@ -103,8 +103,7 @@ __global__ __launch_bounds__((1 << LOGN) / kEPT) void k_batch_ntt(const BatchDes
   for (int q = 0; q < kEPT; ++q) {
     const int k = tid * kEPT + q;
     w0[q] = w1[q] = w2[q] = 0;
-    if (k < zL)
-      crt_words(lds[2 * N + tid * kEPT + q], lds[tid * kEPT + q], P0, P1, crt, w0[q], w1[q], w2[q]);
+    if (k < zL) crt_words(r0[k], la[GE::idx(k, 0)], P0, P1, crt, w0[q], w1[q], w2[q]);
   }
   __syncthreads();  // sw1/sw2 below reuse the operand lines
   uint32_t* sw1 = reinterpret_cast<uint32_t*>(lds);
@@ -115,6 +114,44 @@ __global__ __launch_bounds__((1 << LOGN) / kEPT) void k_batch_ntt(const BatchDes
   for (int q = 0; q < kEPT; ++q) {
     const int k = tid * kEPT + q;
     if (k < zL) d.out[k] = D[q];
+  }
+}
+
+// CRT and carry of batched multi-pass products: one workgroup per product
+// walks its limbs in chunks of 1024, folding each chunk's carry-in into the
+// chunk's first limbs.  r: inverse-transform outputs of product z at
+// r + z * zstride, prime q at offset q * N.
+__global__ __launch_bounds__(256) void k_batch_carry(const BatchDesc* descs, const double* r,
+                                                      uint64_t zstride, uint64_t N,
+                                                      const Prime* prm, double2 crt) {
+  __shared__ uint32_t sw1[kCarryBlock], sw2[kCarryBlock];
+  __shared__ CarryFn wtot[4];
+  __shared__ uint64_t next_cin;
+  const BatchDesc d = descs[blockIdx.x];
+  const double* rz = r + (uint64_t)blockIdx.x * zstride;
+  const Prime P0 = prm[0], P1 = prm[1];
+  const uint64_t zL = (uint64_t)d.na + d.nb;
+  const int tid = threadIdx.x;
+  uint64_t cin = 0;
+  for (uint64_t base = 0; base < zL; base += kCarryBlock) {
+    uint32_t w0[kCarryLPT], w1[kCarryLPT], w2[kCarryLPT], D[kCarryLPT];
+#pragma unroll
+    for (int q = 0; q < kCarryLPT; ++q) {
+      const uint64_t k = base + tid * kCarryLPT + q;
+      w0[q] = w1[q] = w2[q] = 0;
+      if (k < zL) crt_words(rz[k], rz[N + k], P0, P1, crt, w0[q], w1[q], w2[q]);
+    }
+    uint64_t c, spill;
+    block_carry<kCarryLPT>(w0, w1, w2, sw1, sw2, wtot, D, c, spill, cin);
+#pragma unroll
+    for (int q = 0; q < kCarryLPT; ++q) {
+      const uint64_t k = base + tid * kCarryLPT + q;
+      if (k < zL) d.out[k] = D[q];
+    }
+    if (tid == kCarryThreads - 1) next_cin = spill + c;
+    __syncthreads();
+    cin = next_cin;
+    __syncthreads();
   }
 }
 

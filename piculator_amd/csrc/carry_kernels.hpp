@@ -150,11 +150,13 @@ __device__ __forceinline__ void crt_words(double r0, double r1, const Prime& P0,
 // On return D holds the limbs, and for the last thread `cout` is the carry
 // out of the top limb and `spill` the value of the words that spill past
 // the top limb.
+// A carry-in cin (< 2^64) may be folded into the first two limbs: their
+// sums stay below 3.5 * 2^32, so carries stay small.
 template <int LPT>
 __device__ __forceinline__ void block_carry(const uint32_t* w0, const uint32_t* w1,
                                             const uint32_t* w2, uint32_t* sw1, uint32_t* sw2,
                                             CarryFn* wtot, uint32_t* D, uint64_t& cout,
-                                            uint64_t& spill) {
+                                            uint64_t& spill, uint64_t cin = 0) {
   const int tid = threadIdx.x;
   const int total = LPT * blockDim.x;
 #pragma unroll
@@ -173,6 +175,8 @@ __device__ __forceinline__ void block_carry(const uint32_t* w0, const uint32_t* 
     uint64_t x = w0[q];
     if (li >= 1) x += sw1[li - 1];
     if (li >= 2) x += sw2[li - 2];
+    if (li == 0) x += cin & 0xffffffffull;
+    if (li == 1) x += cin >> 32;
     s[q] = x;
     CarryFn g;
     g.base = x >> 32;

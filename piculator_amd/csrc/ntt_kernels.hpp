@@ -30,6 +30,14 @@ constexpr int kEPT = 16;   // points per thread per stage
 
 enum PassMode { kFwd = 0, kInv = 1, kFused = 2 };
 
+// One product of a batched launch (device pointers).
+struct BatchDesc {
+  const uint32_t* a;
+  const uint32_t* b;
+  uint32_t* out;
+  uint32_t na, nb;
+};
+
 struct PassArgs {
   const Prime* prm;        // device: the two primes' constants for this N
   const double* src;       // [2][N] or null (then limbs)
@@ -39,7 +47,13 @@ struct PassArgs {
   const double* spec;      // fused: spectrum of operand a, [2][N]
   uint64_t N;
   int logL, logS, logLines, logG, logB;
+  // batched launches: product blockIdx.z uses descs[z], scratch offset
+  // z * zstride doubles, and operand a (0) or b (1) of its descriptor
+  const BatchDesc* descs;
+  int operand;
+  uint64_t zstride;
 };
+
 
 // -------------------------------------------------------------- radix cores
 template <int R>
@@ -104,18 +118,19 @@ __device__ __forceinline__ double2 root_pow(uint64_t e, int logM, bool inv, cons
 
 // LDS layout of the lines staged by one workgroup.  Column passes (stride
 // s > 1): point n of line lam at n * LINES + lam, so consecutive threads touch
-// consecutive columns.  Row passes (s == 1): line-major with one pad double
-// per line, point n of line lam at lam * (L + 1) + n.
+// consecutive columns.  Row passes (s == 1): line-major, with one pad double
+// after every 8 points (index n + n / 8): the first Stockham stages write at
+// a stride of 8 points, which would otherwise hit the same LDS banks.
 template <int LOGL, int LOGLINES, bool ROW>
 struct Geo {
   static constexpr int L = 1 << LOGL;
   static constexpr int LINES = 1 << LOGLINES;
   static constexpr int E = L * LINES;
   static constexpr int NTH = E / kEPT;
-  static constexpr int LP = ROW ? L + 1 : L;
+  static constexpr int LP = ROW ? L + L / 8 : L;
   static constexpr int LDS = ROW ? LINES * LP : E;
   __device__ static __forceinline__ int idx(int n, int lam) {
-    return ROW ? lam * LP + n : (n << LOGLINES) + lam;
+    return ROW ? lam * LP + n + (n >> 3) : (n << LOGLINES) + lam;
   }
   // butterfly bf -> (line, j): the fast-varying index follows the layout
   __device__ static __forceinline__ void split(int bf, int logLR, int& lam, int& j) {
@@ -129,13 +144,22 @@ struct Geo {
   }
 };
 
+// Stage twiddles omega_L^e (e < L) come from an LDS copy for lines up to
+// 2^10 points and from the global 4096-entry table otherwise.
+constexpr int kLogLdsTw = 10;
+
 // One Stockham stage of radix R (Ns = 2^logNs) over every line in LDS.
+// tw: the twiddle table, omega_{2^logTw}^e.
 template <class GE, int R, int LOGR, int LOGNS>
-__device__ __forceinline__ void stockham_stage(double* lds, const Prime& P, bool inv) {
+__device__ __forceinline__ void stockham_stage(double* lds, const double2* tw, const Prime& P,
+                                               bool inv) {
+  constexpr int LOGL = __builtin_ctz(GE::L);
+  constexpr int LOGTW = LOGL <= kLogLdsTw ? LOGL : kLogE;
+  constexpr int TWN = 1 << LOGTW;
   constexpr int NB = kEPT / R;
-  constexpr int LOGLR = GE::L == 1 ? 0 : (__builtin_ctz(GE::L) - LOGR);
+  constexpr int LOGLR = GE::L == 1 ? 0 : (LOGL - LOGR);
   constexpr int NS = 1 << LOGNS;
-  constexpr int SH = kLogE - LOGNS - LOGR;
+  constexpr int SH = LOGTW - LOGNS - LOGR;
   const int tid = threadIdx.x;
   double v[NB][R];
 #pragma unroll
@@ -150,8 +174,8 @@ __device__ __forceinline__ void stockham_stage(double* lds, const Prime& P, bool
 #pragma unroll
       for (int r = 1; r < R; ++r) {
         int e = (jm * r) << SH;
-        if (inv) e = (kE - e) & (kE - 1);
-        const double2 t = P.twL[e];
+        if (inv) e = (TWN - e) & (TWN - 1);
+        const double2 t = tw[e];
         v[b][r] = mm_shoup(v[b][r], t.x, t.y, P.p);
       }
     }
@@ -171,26 +195,40 @@ __device__ __forceinline__ void stockham_stage(double* lds, const Prime& P, bool
 }
 
 template <class GE, int LOGNS>
-__device__ __forceinline__ void radix8_stages(double* lds, const Prime& P, bool inv) {
+__device__ __forceinline__ void radix8_stages(double* lds, const double2* tw, const Prime& P,
+                                              bool inv) {
   if constexpr (LOGNS < __builtin_ctz(GE::L)) {
-    stockham_stage<GE, 8, 3, LOGNS>(lds, P, inv);
-    radix8_stages<GE, LOGNS + 3>(lds, P, inv);
+    stockham_stage<GE, 8, 3, LOGNS>(lds, tw, P, inv);
+    radix8_stages<GE, LOGNS + 3>(lds, tw, P, inv);
   }
 }
 
 // Natural-order DFT of length L of every line in LDS.
 template <class GE>
-__device__ __forceinline__ void line_dft(double* lds, const Prime& P, bool inv) {
+__device__ __forceinline__ void line_dft(double* lds, const double2* tw, const Prime& P, bool inv) {
   constexpr int LOGL = __builtin_ctz(GE::L);
   constexpr int FIRST = LOGL % 3;
   if constexpr (FIRST == 1) {
-    stockham_stage<GE, 2, 1, 0>(lds, P, inv);
-    radix8_stages<GE, 1>(lds, P, inv);
+    stockham_stage<GE, 2, 1, 0>(lds, tw, P, inv);
+    radix8_stages<GE, 1>(lds, tw, P, inv);
   } else if constexpr (FIRST == 2) {
-    stockham_stage<GE, 4, 2, 0>(lds, P, inv);
-    radix8_stages<GE, 2>(lds, P, inv);
+    stockham_stage<GE, 4, 2, 0>(lds, tw, P, inv);
+    radix8_stages<GE, 2>(lds, tw, P, inv);
   } else {
-    radix8_stages<GE, 0>(lds, P, inv);
+    radix8_stages<GE, 0>(lds, tw, P, inv);
+  }
+}
+
+// Stage-twiddle table of a line length: LDS copy (omega_L^e) when small
+// enough, else the global 4096-entry table.  Call before the first barrier.
+template <class GE>
+__device__ __forceinline__ const double2* stage_twiddles(double2* ltw, const Prime& P) {
+  constexpr int LOGL = __builtin_ctz(GE::L);
+  if constexpr (LOGL <= kLogLdsTw) {
+    for (int e = threadIdx.x; e < GE::L; e += blockDim.x) ltw[e] = P.twL[e << (kLogE - LOGL)];
+    return ltw;
+  } else {
+    return P.twL;
   }
 }
 
@@ -201,8 +239,10 @@ template <int MODE, int LOGL, int LOGLINES, bool ROW>
 __global__ __launch_bounds__(1 << (LOGL + LOGLINES - 4)) void k_ntt_pass(PassArgs a) {
   using GE = Geo<LOGL, LOGLINES, ROW>;
   __shared__ double lds[GE::LDS];
+  __shared__ double2 ltw[LOGL <= kLogLdsTw ? (1 << LOGL) : 1];
   const int pi = blockIdx.y;
   const Prime P = a.prm[pi];
+  const double2* tw = stage_twiddles<GE>(ltw, P);
   const int tid = threadIdx.x;
   const int logS = a.logS;
   const uint64_t w = blockIdx.x;
@@ -215,8 +255,17 @@ __global__ __launch_bounds__(1 << (LOGL + LOGLINES - 4)) void k_ntt_pass(PassArg
     base = ((w >> lgc) << (LOGL + logS)) + ((w & ((1ull << lgc) - 1)) << LOGLINES);
   }
   const uint64_t col0 = ROW ? 0 : (base & ((1ull << logS) - 1));
-  const double* src = a.src ? a.src + pi * a.N : nullptr;
-  double* dst = a.dst + pi * a.N;
+  uint64_t zoff = 0;
+  const uint32_t* limbs = a.limbs;
+  uint64_t nlimbs = a.nlimbs;
+  if (a.descs) {
+    const BatchDesc d = a.descs[blockIdx.z];
+    zoff = (uint64_t)blockIdx.z * a.zstride;
+    limbs = a.operand ? d.b : d.a;
+    nlimbs = a.operand ? d.nb : d.na;
+  }
+  const double* src = a.src ? a.src + zoff + pi * a.N : nullptr;
+  double* dst = a.dst + zoff + pi * a.N;
   const int logRoot = LOGL + logS;
 
   // element i of this thread: LDS point (n, lam) and its global index
@@ -236,6 +285,17 @@ __global__ __launch_bounds__(1 << (LOGL + LOGLINES - 4)) void k_ntt_pass(PassArg
     }                                                                 \
   }
 
+  // Inter-pass twiddles of a column pass: this thread's column c is fixed
+  // and its points are n0 + i * (NTH / LINES), so omega^{c n} follows from
+  // base = omega^{c n0} and step = omega^{c NTH / LINES} by exact modular
+  // products (no table loads per point).
+  double2 twc = make_double2(1.0, 0.0), tws = make_double2(1.0, 0.0);
+  if (!ROW && MODE != kFused) {
+    const uint64_t c = col0 + (tid & (GE::LINES - 1));
+    const uint64_t n0 = (uint64_t)(tid >> LOGLINES);
+    twc = root_pow(c * n0, logRoot, MODE == kInv, P);
+    tws = root_pow(c * (uint64_t)(GE::NTH >> LOGLINES), logRoot, MODE == kInv, P);
+  }
 #pragma unroll
   for (int i = 0; i < kEPT; ++i) {
     PCG_ELEM(i, n, lam, g);
@@ -243,17 +303,18 @@ __global__ __launch_bounds__(1 << (LOGL + LOGLINES - 4)) void k_ntt_pass(PassArg
     if (src)
       x = src[g];
     else
-      x = g < a.nlimbs ? (double)a.limbs[g] : 0.0;
+      x = g < nlimbs ? (double)limbs[g] : 0.0;
     if (MODE == kInv && !ROW) {
-      const double2 tw = root_pow((col0 + lam) * (uint64_t)n, logRoot, true, P);
-      x = mm_shoup(x, tw.x, tw.y, P.p);
+      x = mm_shoup(x, twc.x, twc.y, P.p);
+      const double w = mm_shoup(twc.x, tws.x, tws.y, P.p);
+      twc = make_double2(w, w * P.pinv);
     }
     lds[GE::idx(n, lam)] = x;
   }
   __syncthreads();
-  line_dft<GE>(lds, P, MODE == kInv);
+  line_dft<GE>(lds, tw, P, MODE == kInv);
   if (MODE == kFused) {
-    const double* spec = a.spec + pi * a.N;
+    const double* spec = a.spec + zoff + pi * a.N;
 #pragma unroll
     for (int i = 0; i < kEPT; ++i) {
       PCG_ELEM(i, n, lam, g);
@@ -262,15 +323,16 @@ __global__ __launch_bounds__(1 << (LOGL + LOGLINES - 4)) void k_ntt_pass(PassArg
       lds[li] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
     }
     __syncthreads();
-    line_dft<GE>(lds, P, true);
+    line_dft<GE>(lds, tw, P, true);
   }
 #pragma unroll
   for (int i = 0; i < kEPT; ++i) {
     PCG_ELEM(i, n, lam, g);
     double x = lds[GE::idx(n, lam)];
     if (MODE == kFwd && !ROW) {
-      const double2 tw = root_pow((col0 + lam) * (uint64_t)n, logRoot, false, P);
-      x = mm_shoup(x, tw.x, tw.y, P.p);
+      x = mm_shoup(x, twc.x, twc.y, P.p);
+      const double w = mm_shoup(twc.x, tws.x, tws.y, P.p);
+      twc = make_double2(w, w * P.pinv);
     } else {
       x = reduce(x, P.p, P.pinv);
     }

@@ -9,6 +9,7 @@
 // exact carry scan.  The product is the same exact integer.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
@@ -197,6 +198,9 @@ struct Ctx {
   BatchDesc* h_desc = nullptr;
   BatchDesc* d_desc = nullptr;
   size_t desc_cap = 0;
+  // scratch of batched multi-pass products
+  double* bspec = nullptr;
+  size_t bspec_doubles = 0;
   // profiling
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t tev[2] = {nullptr, nullptr};
@@ -365,8 +369,13 @@ bool plan_supported(const Plan& pl) {
 }
 
 int launch_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double* src,
-                const uint32_t* limbs, uint64_t nlimbs, double* dst, const double* spec) {
+                const uint32_t* limbs, uint64_t nlimbs, double* dst, const double* spec,
+                const BatchDesc* descs = nullptr, int operand = 0, uint64_t zstride = 0,
+                unsigned count = 1) {
   PassArgs a;
+  a.descs = descs;
+  a.operand = operand;
+  a.zstride = zstride;
   a.prm = c.d_prime + pl.logN * 2;
   a.src = src;
   a.limbs = limbs;
@@ -384,7 +393,7 @@ int launch_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double* 
     return fail(PCG_ERR_INTERNAL, "no NTT pass kernel for logN=%d logL=%d logS=%d mode=%d",
                 pl.logN, ps.logL, ps.logS, mode);
   const int E = 1 << pl.logE;
-  dim3 grid((unsigned)(a.N / E), 2), block(E / kEPT);
+  dim3 grid((unsigned)(a.N / E), 2, count), block(E / kEPT);
   hipLaunchKernelGGL(fn, grid, block, 0, c.stream, a);
   return PCG_OK;
 }
@@ -591,12 +600,66 @@ int ensure_pool(Ctx& c, size_t limbs, size_t ndesc) {
 }
 
 // Size class of a product for the batched path: 0 schoolbook, 11/12 the
-// single-workgroup NTT of 2^11 / 2^12 points, -1 the multi-pass engine.
+// single-workgroup NTT of 2^11 / 2^12 points, 13..17 the multi-pass NTT run
+// for the whole class at once, -1 one product at a time.
+constexpr int kBatchMaxLog = 17;
 int batch_class(uint64_t na, uint64_t nb) {
   const uint64_t zL = na + nb;
   if (zL <= (uint64_t)kSchoolMaxLimbs) return 0;
-  if (zL <= (1ull << kSmallNttMaxLog)) return ceil_log2(zL) < 11 ? 11 : ceil_log2(zL);
+  const int lg = ceil_log2(zL);
+  if (lg <= kSmallNttMaxLog) return lg < 11 ? 11 : lg;
+  if (lg <= kBatchMaxLog) return lg;
   return -1;
+}
+
+// Multi-pass NTT products of one transform length, all at once: pass
+// kernels carry a product grid dimension; scratch is 4N doubles per product.
+int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN) {
+  const uint64_t N = 1ull << logN;
+  const uint64_t per = 4 * N;
+  const size_t group_max = std::max<size_t>(1, ((size_t)1 << 28) / per);  // 2 GiB of scratch
+  const size_t want = std::min(count, group_max) * per;
+  if (want > c.bspec_doubles) {
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (c.bspec) HIPCHK(hipFree(c.bspec));
+    c.bspec = nullptr;
+    c.bspec_doubles = 0;
+    if (hipMalloc(&c.bspec, want * sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PCG_ERR_OOM, "cannot allocate batched transform scratch");
+    }
+    c.bspec_doubles = want;
+  }
+  const Plan pl = make_plan(logN);
+  const size_t m = pl.passes.size();
+  double* A = c.bspec;
+  double* B = c.bspec + 2 * N;
+  for (size_t g0 = 0; g0 < count; g0 += group_max) {
+    const unsigned k = (unsigned)std::min(group_max, count - g0);
+    const BatchDesc* dd = d_desc + g0;
+    int rc;
+    for (size_t j = 0; j < m; ++j) {
+      rc = launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : A, nullptr, 0, A, nullptr,
+                       dd, 0, per, k);
+      if (rc) return rc;
+    }
+    for (size_t j = 0; j + 1 < m; ++j) {
+      rc = launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : B, nullptr, 0, B, nullptr,
+                       dd, 1, per, k);
+      if (rc) return rc;
+    }
+    rc = launch_pass(c, kFused, pl, pl.passes[m - 1], m == 1 ? nullptr : B, nullptr, 0, B, A, dd, 1,
+                     per, k);
+    if (rc) return rc;
+    for (size_t j = m - 1; j-- > 0;) {
+      rc = launch_pass(c, kInv, pl, pl.passes[j], B, nullptr, 0, B, nullptr, dd, 1, per, k);
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_batch_carry, dim3(k), dim3(kCarryThreads), 0, c.stream, dd, B, per, N,
+                       c.d_prime + logN * 2, c.crt);
+  }
+  HIPCHK(hipGetLastError());
+  return PCG_OK;
 }
 
 // Launches the per-class kernels over h_desc[0..n) (already on the device
@@ -615,7 +678,10 @@ int launch_classes(Ctx& c, size_t n_school, size_t n11, size_t n12) {
 }
 
 int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, bool dev) {
-  std::vector<uint64_t> cls[3], large;
+  // class slots: 0 schoolbook, 1..2 single-workgroup NTT (2^11, 2^12),
+  // 3..7 batched multi-pass (2^13 .. 2^17)
+  constexpr int kSlots = 3 + (kBatchMaxLog - kSmallNttMaxLog);
+  std::vector<uint64_t> cls[kSlots], large;
   for (uint64_t i = 0; i < count; ++i) {
     int rc = check_sizes(d[i].na, d[i].nb);
     if (rc) return rc;
@@ -638,10 +704,11 @@ int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, 
     else
       cls[k == 0 ? 0 : k - 10].push_back(i);
   }
-  const size_t nsmall = cls[0].size() + cls[1].size() + cls[2].size();
+  size_t nsmall = 0;
+  for (int k = 0; k < kSlots; ++k) nsmall += cls[k].size();
   if (nsmall) {
     size_t in_limbs = 0, out_limbs = 0;
-    for (int k = 0; k < 3; ++k)
+    for (int k = 0; k < kSlots; ++k)
       for (uint64_t i : cls[k]) {
         in_limbs += d[i].na + d[i].nb;
         out_limbs += d[i].na + d[i].nb;
@@ -649,7 +716,7 @@ int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, 
     int rc = ensure_pool(c, dev ? 0 : in_limbs + out_limbs, nsmall);
     if (rc) return rc;
     size_t pos = 0, opos = in_limbs, j = 0;
-    for (int k = 0; k < 3; ++k)
+    for (int k = 0; k < kSlots; ++k)
       for (uint64_t i : cls[k]) {
         BatchDesc& bd = c.h_desc[j++];
         bd.na = (uint32_t)d[i].na;
@@ -675,6 +742,13 @@ int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, 
                           c.stream));
     rc = launch_classes(c, cls[0].size(), cls[1].size(), cls[2].size());
     if (rc) return rc;
+    size_t off = cls[0].size() + cls[1].size() + cls[2].size();
+    for (int k = 3; k < kSlots; ++k) {
+      if (cls[k].empty()) continue;
+      rc = batched_multipass(c, c.d_desc + off, cls[k].size(), kSmallNttMaxLog + k - 2);
+      if (rc) return rc;
+      off += cls[k].size();
+    }
     if (!dev)
       HIPCHK(hipMemcpyAsync(c.h_pool + in_limbs, c.d_pool + in_limbs, out_limbs * 4,
                             hipMemcpyDeviceToHost, c.stream));
@@ -682,7 +756,7 @@ int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, 
     HIPCHK(hipStreamSynchronize(c.stream));
     if (!dev) {
       size_t o = in_limbs;
-      for (int k = 0; k < 3; ++k)
+      for (int k = 0; k < kSlots; ++k)
         for (uint64_t i : cls[k]) {
           const size_t zL = d[i].na + d[i].nb;
           memcpy(d[i].out, c.h_pool + o, zL * 4);
@@ -771,6 +845,7 @@ void pcg_shutdown(void) {
   (void)hipFree(c.d_pool);
   (void)hipHostFree(c.h_desc);
   (void)hipFree(c.d_desc);
+  (void)hipFree(c.bspec);
   for (int i = 0; i < 4; ++i) (void)hipEventDestroy(c.ev[i]);
   for (int i = 0; i < 2; ++i) (void)hipEventDestroy(c.tev[i]);
   (void)hipStreamDestroy(c.stream);
@@ -792,6 +867,8 @@ void pcg_shutdown(void) {
   c.h_desc = nullptr;
   c.d_desc = nullptr;
   c.desc_cap = 0;
+  c.bspec = nullptr;
+  c.bspec_doubles = 0;
   c.d_prime = nullptr;
   c.prof_valid = false;
   (void)fresh_defaults;
