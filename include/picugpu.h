@@ -121,6 +121,26 @@ int pcg_fft_last_roundoff(double* max_dist, int* recomputed);
 #define PCG_PROFILE 0x400u
 int pcg_last_profile(double* stage_ms, int n_stage, double* total_ms);
 
+/* One value of a binary-splitting subtree, written into caller memory. */
+typedef struct pcg_bsr_value {
+  uint32_t* limbs;  /* caller buffer of `cap` limbs                         */
+  uint64_t cap;
+  uint64_t L;       /* limbs written (0 = zero)                              */
+  int32_t sign;     /* 1 positive or zero, 0 negative (BigFloat::sign)        */
+} pcg_bsr_value;
+
+/* P, Q, R of the Chudnovsky binary-splitting subtree [a, b) (leaves and
+ * combine of PI.h:64-112, midpoints m = (a + b) / 2) computed entirely on
+ * the GPU.  Valid when every value of the subtree has at most max_limbs =
+ * p + 2 limbs: BigFloat then never truncates, every value is an exact
+ * integer (exponent 0), and the results equal those of the BigFloat
+ * recursion bit for bit.  Returns PCG_ERR_TOO_LARGE when a length bound
+ * exceeds max_limbs or a buffer is too small (nothing is written then).
+ * pcg_bsr_bound gives the limb bound of P, Q and R for sizing buffers. */
+int pcg_bsr_exact(uint32_t a, uint32_t b, uint64_t max_limbs, pcg_bsr_value* P, pcg_bsr_value* Q,
+                  pcg_bsr_value* R);
+int pcg_bsr_bound(uint32_t a, uint32_t b, uint64_t* bound);
+
 /* Elapsed time of the work enqueued on the library stream between the two
  * calls, measured with HIP events recorded on that stream. */
 int pcg_timer_start(void);

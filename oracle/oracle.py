@@ -82,6 +82,7 @@ def lib():
         L.orc_pi_hex.argtypes = [sz, C.c_int, C.c_int, C.c_char_p, sz, C.POINTER(C.c_double)]
         L.orc_pi_hex.restype = C.c_long
         L.orc_mul_count.restype = C.c_uint64
+        L.orc_pi_bsr.argtypes = [bfp, bfp, bfp, C.c_uint32, C.c_uint32, sz, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -272,6 +273,13 @@ def pi_sizes(decimal_digits: int):
     h, p, t = C.c_size_t(), C.c_size_t(), C.c_size_t()
     lib().orc_pi_sizes(decimal_digits, C.byref(h), C.byref(p), C.byref(t))
     return int(h.value), int(p.value), int(t.value)
+
+
+def pi_bsr(a: int, b: int, p: int, engine: int = ENGINE_NTT, threads: int = 1):
+    """(P, Q, R) of the binary-splitting recursion over [a, b) (PI.h:64-112)."""
+    P, Q, R = BF(), BF(), BF()
+    lib().orc_pi_bsr(P.ref, Q.ref, R.ref, a, b, p, engine, threads)
+    return P, Q, R
 
 
 def pi_hex(decimal_digits: int, engine: int = ENGINE_REFERENCE, threads: int = 1):

@@ -150,8 +150,40 @@ __device__ __forceinline__ void crt_words(double r0, double r1, const Prime& P0,
 // On return D holds the limbs, and for the last thread `cout` is the carry
 // out of the top limb and `spill` the value of the words that spill past
 // the top limb.
-// A carry-in cin (< 2^64) may be folded into the first two limbs: their
-// sums stay below 3.5 * 2^32, so carries stay small.
+// Resolves the carries of LPT * blockDim.x consecutive limbs held LPT per
+// thread (thread t holds limbs [LPT t, LPT t + LPT)) given their sums
+// s < 4 * 2^32 (limb value plus everything added into that limb position)
+// and a carry-in of 0: D receives the limbs, cout (meaningful in the last
+// thread) the carry out of the top limb.
+template <int LPT>
+__device__ __forceinline__ void block_carry_sums(const uint64_t* s, CarryFn* wtot, uint32_t* D,
+                                                 uint64_t& cout) {
+  CarryFn f;
+  f.base = 0;
+  f.T = 0;
+#pragma unroll
+  for (int q = 0; q < LPT; ++q) {
+    CarryFn g;
+    g.base = s[q] >> 32;
+    g.T = (1ull << 32) - (s[q] & 0xffffffffull);
+    f = cf_compose(f, g);
+  }
+  const CarryFn pre = cf_block_exclusive(f, wtot);
+  uint64_t c = cf_eval0(pre);
+#pragma unroll
+  for (int q = 0; q < LPT; ++q) {
+    const uint64_t t = (s[q] & 0xffffffffull) + c;
+    D[q] = (uint32_t)t;
+    c = (s[q] >> 32) + (t >> 32);
+  }
+  cout = c;
+}
+
+// Same from CRT words: limb k receives s_k = w0_k + w1_{k-1} + w2_{k-2}.
+// sw1/sw2 are LDS scratch of LPT * blockDim.x words, wtot LDS scratch of
+// blockDim.x / 64.  A carry-in cin (< 2^64) may be folded into the first two
+// limbs: their sums stay below 3.5 * 2^32, so carries stay small.  For the
+// last thread `spill` is the value of the words that spill past the top limb.
 template <int LPT>
 __device__ __forceinline__ void block_carry(const uint32_t* w0, const uint32_t* w1,
                                             const uint32_t* w2, uint32_t* sw1, uint32_t* sw2,
@@ -166,9 +198,6 @@ __device__ __forceinline__ void block_carry(const uint32_t* w0, const uint32_t* 
   }
   __syncthreads();
   uint64_t s[LPT];
-  CarryFn f;
-  f.base = 0;
-  f.T = 0;
 #pragma unroll
   for (int q = 0; q < LPT; ++q) {
     const int li = tid * LPT + q;
@@ -178,20 +207,8 @@ __device__ __forceinline__ void block_carry(const uint32_t* w0, const uint32_t* 
     if (li == 0) x += cin & 0xffffffffull;
     if (li == 1) x += cin >> 32;
     s[q] = x;
-    CarryFn g;
-    g.base = x >> 32;
-    g.T = (1ull << 32) - (x & 0xffffffffull);
-    f = cf_compose(f, g);
   }
-  const CarryFn pre = cf_block_exclusive(f, wtot);
-  uint64_t c = cf_eval0(pre);
-#pragma unroll
-  for (int q = 0; q < LPT; ++q) {
-    const uint64_t t = (s[q] & 0xffffffffull) + c;
-    D[q] = (uint32_t)t;
-    c = (s[q] >> 32) + (t >> 32);
-  }
-  cout = c;
+  block_carry_sums<LPT>(s, wtot, D, cout);
   spill = (uint64_t)sw1[total - 1] + (uint64_t)sw2[total - 2] + ((uint64_t)sw2[total - 1] << 32);
 }
 

@@ -21,6 +21,7 @@
 
 #include "../../include/picugpu.h"
 #include "batch_kernels.hpp"
+#include "bsr_kernels.hpp"
 #include "carry_kernels.hpp"
 #include "fft_kernels.hpp"
 #include "ntt_kernels.hpp"
@@ -612,6 +613,14 @@ int batch_class(uint64_t na, uint64_t nb) {
   return -1;
 }
 
+// Class of a product of at most zL limbs on the device-resident path:
+// 0 schoolbook, 11..30 transform length (log2).
+int bsr_class(uint64_t zL) {
+  if (zL <= (uint64_t)kSchoolMaxLimbs) return 0;
+  const int lg = ceil_log2(zL);
+  return lg < 11 ? 11 : (lg < kMinLogN ? kMinLogN : lg);
+}
+
 // Multi-pass NTT products of one transform length, all at once: pass
 // kernels carry a product grid dimension; scratch is 4N doubles per product.
 int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN) {
@@ -664,15 +673,15 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN) {
 
 // Launches the per-class kernels over h_desc[0..n) (already on the device
 // in d_desc, ordered by class: school, ntt 2^11, ntt 2^12).
-int launch_classes(Ctx& c, size_t n_school, size_t n11, size_t n12) {
+int launch_classes(Ctx& c, const BatchDesc* base, size_t n_school, size_t n11, size_t n12) {
   if (n_school)
-    hipLaunchKernelGGL(k_batch_school, dim3((unsigned)n_school), dim3(256), 0, c.stream, c.d_desc);
+    hipLaunchKernelGGL(k_batch_school, dim3((unsigned)n_school), dim3(256), 0, c.stream, base);
   if (n11)
     hipLaunchKernelGGL(k_batch_ntt<11>, dim3((unsigned)n11), dim3(128), 0, c.stream,
-                       c.d_desc + n_school, c.d_prime + 11 * 2, c.crt);
+                       base + n_school, c.d_prime + 11 * 2, c.crt);
   if (n12)
     hipLaunchKernelGGL(k_batch_ntt<12>, dim3((unsigned)n12), dim3(256), 0, c.stream,
-                       c.d_desc + n_school + n11, c.d_prime + 12 * 2, c.crt);
+                       base + n_school + n11, c.d_prime + 12 * 2, c.crt);
   HIPCHK(hipGetLastError());
   return PCG_OK;
 }
@@ -740,7 +749,7 @@ int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, 
       HIPCHK(hipMemcpyAsync(c.d_pool, c.h_pool, in_limbs * 4, hipMemcpyHostToDevice, c.stream));
     HIPCHK(hipMemcpyAsync(c.d_desc, c.h_desc, nsmall * sizeof(BatchDesc), hipMemcpyHostToDevice,
                           c.stream));
-    rc = launch_classes(c, cls[0].size(), cls[1].size(), cls[2].size());
+    rc = launch_classes(c, c.d_desc, cls[0].size(), cls[1].size(), cls[2].size());
     if (rc) return rc;
     size_t off = cls[0].size() + cls[1].size() + cls[2].size();
     for (int k = 3; k < kSlots; ++k) {
@@ -781,6 +790,259 @@ int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, 
     }
     if (rc) return rc;
   }
+  return PCG_OK;
+}
+
+
+// ------------------------------------------------------------------------
+// Device-resident binary splitting (see bsr_kernels.hpp)
+struct BsrNode {
+  uint32_t a, b;
+  int32_t left, right, height;
+};
+
+int32_t bsr_build(std::vector<BsrNode>& nodes, uint32_t a, uint32_t b) {
+  const int32_t id = (int32_t)nodes.size();
+  nodes.push_back(BsrNode{a, b, -1, -1, 1});
+  if (b - a > 1) {
+    const uint32_t m = (a + b) / 2;  // PI.h:90
+    const int32_t l = bsr_build(nodes, a, m);
+    const int32_t r = bsr_build(nodes, m, b);
+    nodes[id].left = l;
+    nodes[id].right = r;
+    nodes[id].height = std::max(nodes[l].height, nodes[r].height) + 1;
+  }
+  return id;
+}
+
+constexpr uint64_t kLeafBound = 5;  // leaf terms are below 2^160
+
+// Limb bounds of P, Q, R of every node (bottom-up), max over the subtree.
+uint64_t bsr_bounds(const std::vector<BsrNode>& nodes, std::vector<uint64_t>& bnd) {
+  bnd.assign(nodes.size() * 5, 0);
+  uint64_t mx = 0;
+  for (size_t i = nodes.size(); i-- > 0;) {
+    const BsrNode& n = nodes[i];
+    uint64_t* v = &bnd[i * 5];  // P Q R x y
+    if (n.left < 0) {
+      v[0] = v[1] = v[2] = kLeafBound;
+    } else {
+      const uint64_t* l = &bnd[n.left * 5];
+      const uint64_t* r = &bnd[n.right * 5];
+      v[3] = l[0] + r[1];
+      v[4] = r[0] + l[2];
+      v[0] = std::max(v[3], v[4]) + 1;
+      v[1] = l[1] + r[1];
+      v[2] = l[2] + r[2];
+    }
+    for (int j = 0; j < 5; ++j) mx = std::max(mx, v[j]);
+  }
+  return mx;
+}
+
+template <class T>
+T* carve(char*& p, size_t n) {
+  T* r = reinterpret_cast<T*>(p);
+  p += ((n * sizeof(T) + 255) / 256) * 256;
+  return r;
+}
+
+// Plan of a subtree of d leaves: tree shape, value bounds and arena offsets
+// (they depend on d only, since m - a = floor(d / 2)), per-height product
+// lists sorted by size class, and the device copies of all of it.
+struct BsrHeight {
+  size_t mul0, nmul, add0, nsmall, nlarge;
+  std::vector<std::pair<int, size_t>> classes;  // (class, count) in launch order
+};
+struct BsrPlan {
+  uint32_t d = 0;
+  uint64_t maxb = 0, arena_limbs = 0;
+  uint64_t root_bound[3] = {0, 0, 0};
+  uint64_t root_off[3] = {0, 0, 0};
+  int32_t H = 0;
+  std::vector<BsrHeight> hp;
+  uint64_t nleaves = 0;
+  // device
+  void* buf = nullptr;
+  uint32_t* arena = nullptr;
+  BsrVal* meta = nullptr;
+  uint32_t* leaf_rel = nullptr;  // b - a of each leaf
+  uint32_t* leaf_ids = nullptr;
+  BsrMul* muls = nullptr;
+  BsrAdd* adds = nullptr;
+  BatchDesc* descs = nullptr;
+};
+
+void bsr_plan_free(BsrPlan& P) {
+  (void)hipFree(P.buf);
+  (void)hipFree(P.arena);
+  P = BsrPlan();
+}
+
+int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
+  std::vector<BsrNode> nodes;
+  nodes.reserve(2 * (size_t)d);
+  bsr_build(nodes, 0, d);
+  std::vector<uint64_t> bnd;
+  P.d = d;
+  P.maxb = bsr_bounds(nodes, bnd);
+  const size_t nv = nodes.size() * 5;
+  std::vector<BsrVal> meta(nv);
+  uint64_t off = 0;
+  for (size_t v = 0; v < nv; ++v) {
+    meta[v].off = off;
+    meta[v].L = 0;
+    meta[v].sign = 1;
+    off += bnd[v] + 1;  // one spare limb for an addition's carry
+  }
+  P.arena_limbs = off;
+  for (int j = 0; j < 3; ++j) {
+    P.root_bound[j] = bnd[j];
+    P.root_off[j] = meta[j].off;
+  }
+  P.H = nodes[0].height;
+  std::vector<uint32_t> leaf_rel, leaf_ids;
+  std::vector<std::vector<int32_t>> by_h(P.H + 1);
+  for (int32_t i = 0; i < (int32_t)nodes.size(); ++i) {
+    by_h[nodes[i].height].push_back(i);
+    if (nodes[i].left < 0) {
+      leaf_rel.push_back(nodes[i].b);
+      for (int j = 0; j < 3; ++j) leaf_ids.push_back((uint32_t)(5 * i + j));
+    }
+  }
+  P.nleaves = leaf_rel.size();
+  P.hp.assign(P.H + 1, BsrHeight());
+  std::vector<BsrMul> muls;
+  std::vector<BsrAdd> adds;
+  for (int32_t h = 2; h <= P.H; ++h) {
+    std::vector<std::pair<int, BsrMul>> ms;
+    std::vector<BsrAdd> small, large;
+    for (int32_t i : by_h[h]) {
+      const BsrNode& n = nodes[i];
+      const uint32_t I = 5 * i, Lf = 5 * n.left, Rt = 5 * n.right;
+      // x = P0 Q1, y = P1 R0, Q = Q0 Q1, R = R0 R1 (PI.h:110-112)
+      const BsrMul m4[4] = {{Lf + 0, Rt + 1, I + 3}, {Rt + 0, Lf + 2, I + 4},
+                            {Lf + 1, Rt + 1, I + 1}, {Lf + 2, Rt + 2, I + 2}};
+      for (const BsrMul& m : m4) ms.push_back({bsr_class(bnd[m.a] + bnd[m.b]), m});
+      const BsrAdd ad{I + 3, I + 4, I + 0};  // P = x.add(y)
+      if (bnd[I] <= 1024)
+        small.push_back(ad);
+      else
+        large.push_back(ad);
+    }
+    std::stable_sort(ms.begin(), ms.end(),
+                     [](const std::pair<int, BsrMul>& x, const std::pair<int, BsrMul>& y) {
+                       return x.first < y.first;
+                     });
+    BsrHeight& hh = P.hp[h];
+    hh.mul0 = muls.size();
+    hh.nmul = ms.size();
+    for (size_t k = 0; k < ms.size(); ++k) {
+      if (hh.classes.empty() || hh.classes.back().first != ms[k].first)
+        hh.classes.push_back({ms[k].first, 0});
+      hh.classes.back().second++;
+      muls.push_back(ms[k].second);
+    }
+    hh.add0 = adds.size();
+    hh.nsmall = small.size();
+    hh.nlarge = large.size();
+    adds.insert(adds.end(), small.begin(), small.end());
+    adds.insert(adds.end(), large.begin(), large.end());
+  }
+  const size_t need = nv * sizeof(BsrVal) + leaf_rel.size() * 4 + leaf_ids.size() * 4 +
+                      muls.size() * sizeof(BsrMul) + adds.size() * sizeof(BsrAdd) +
+                      muls.size() * sizeof(BatchDesc) + 8 * 256;
+  if (hipMalloc(&P.buf, need) != hipSuccess || hipMalloc(&P.arena, P.arena_limbs * 4) != hipSuccess) {
+    (void)hipGetLastError();
+    bsr_plan_free(P);
+    return fail(PCG_ERR_OOM, "cannot allocate the binary-splitting plan (%llu arena limbs)",
+                (unsigned long long)off);
+  }
+  char* p = static_cast<char*>(P.buf);
+  P.meta = carve<BsrVal>(p, nv);
+  P.leaf_rel = carve<uint32_t>(p, leaf_rel.size());
+  P.leaf_ids = carve<uint32_t>(p, leaf_ids.size());
+  P.muls = carve<BsrMul>(p, muls.size());
+  P.adds = carve<BsrAdd>(p, adds.size());
+  P.descs = carve<BatchDesc>(p, muls.size());
+  HIPCHK(hipMemcpyAsync(P.meta, meta.data(), nv * sizeof(BsrVal), hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipMemcpyAsync(P.leaf_rel, leaf_rel.data(), leaf_rel.size() * 4, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipMemcpyAsync(P.leaf_ids, leaf_ids.data(), leaf_ids.size() * 4, hipMemcpyHostToDevice, c.stream));
+  if (!muls.empty())
+    HIPCHK(hipMemcpyAsync(P.muls, muls.data(), muls.size() * sizeof(BsrMul), hipMemcpyHostToDevice, c.stream));
+  if (!adds.empty())
+    HIPCHK(hipMemcpyAsync(P.adds, adds.data(), adds.size() * sizeof(BsrAdd), hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));  // host vectors go out of scope
+  return PCG_OK;
+}
+
+// Two cached plans: the binary splitting produces at most two subtree sizes
+// at a given depth.
+BsrPlan g_bsr_plans[2];
+int g_bsr_next = 0;
+
+int bsr_exact_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, pcg_bsr_value* out[3]) {
+  if (b <= a) return fail(PCG_ERR_INVALID, "empty range");
+  const uint32_t d = b - a;
+  BsrPlan* P = nullptr;
+  for (BsrPlan& q : g_bsr_plans)
+    if (q.d == d && q.buf) P = &q;
+  if (!P) {
+    P = &g_bsr_plans[g_bsr_next];
+    g_bsr_next ^= 1;
+    HIPCHK(hipStreamSynchronize(c.stream));
+    bsr_plan_free(*P);
+    int rc = bsr_plan_build(c, d, *P);
+    if (rc) return rc;
+  }
+  if (P->maxb > max_limbs)
+    return fail(PCG_ERR_TOO_LARGE, "subtree values may reach %llu limbs > %llu",
+                (unsigned long long)P->maxb, (unsigned long long)max_limbs);
+  for (int j = 0; j < 3; ++j)
+    if (out[j]->cap < P->root_bound[j]) return fail(PCG_ERR_TOO_LARGE, "result buffer too small");
+  hipLaunchKernelGGL(k_bsr_leaves, dim3((unsigned)((P->nleaves + 255) / 256)), dim3(256), 0, c.stream,
+                     P->leaf_rel, a, P->leaf_ids, P->nleaves, P->meta, P->arena);
+  for (int32_t h = 2; h <= P->H; ++h) {
+    const BsrHeight& hh = P->hp[h];
+    const BsrMul* hm = P->muls + hh.mul0;
+    BatchDesc* hd = P->descs + hh.mul0;
+    hipLaunchKernelGGL(k_bsr_descs, dim3((unsigned)((hh.nmul + 255) / 256)), dim3(256), 0, c.stream, hm,
+                       (uint64_t)hh.nmul, P->meta, P->arena, hd);
+    size_t o = 0;
+    for (const auto& cl : hh.classes) {
+      int rc;
+      if (cl.first == 0)
+        rc = launch_classes(c, hd + o, cl.second, 0, 0);
+      else if (cl.first == 11)
+        rc = launch_classes(c, hd + o, 0, cl.second, 0);
+      else if (cl.first == 12)
+        rc = launch_classes(c, hd + o, 0, 0, cl.second);
+      else
+        rc = batched_multipass(c, hd + o, cl.second, cl.first);
+      if (rc) return rc;
+      o += cl.second;
+    }
+    hipLaunchKernelGGL(k_bsr_trim, dim3((unsigned)((hh.nmul + 255) / 256)), dim3(256), 0, c.stream, hm,
+                       (uint64_t)hh.nmul, P->meta, P->arena);
+    if (hh.nsmall)
+      hipLaunchKernelGGL(k_bsr_add<64>, dim3((unsigned)hh.nsmall), dim3(64), 0, c.stream,
+                         P->adds + hh.add0, P->meta, P->arena);
+    if (hh.nlarge)
+      hipLaunchKernelGGL(k_bsr_add<256>, dim3((unsigned)hh.nlarge), dim3(256), 0, c.stream,
+                         P->adds + hh.add0 + hh.nsmall, P->meta, P->arena);
+  }
+  HIPCHK(hipGetLastError());
+  BsrVal rv[3];
+  HIPCHK(hipMemcpyAsync(rv, P->meta, 3 * sizeof(BsrVal), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  for (int j = 0; j < 3; ++j) {
+    out[j]->L = rv[j].L;
+    out[j]->sign = rv[j].sign;
+    if (rv[j].L)
+      HIPCHK(hipMemcpyAsync(out[j]->limbs, P->arena + rv[j].off, rv[j].L * 4, hipMemcpyDeviceToHost,
+                            c.stream));
+  }
+  HIPCHK(hipStreamSynchronize(c.stream));
   return PCG_OK;
 }
 
@@ -846,6 +1108,7 @@ void pcg_shutdown(void) {
   (void)hipHostFree(c.h_desc);
   (void)hipFree(c.d_desc);
   (void)hipFree(c.bspec);
+  for (BsrPlan& q : g_bsr_plans) bsr_plan_free(q);
   for (int i = 0; i < 4; ++i) (void)hipEventDestroy(c.ev[i]);
   for (int i = 0; i < 2; ++i) (void)hipEventDestroy(c.tev[i]);
   (void)hipStreamDestroy(c.stream);
@@ -1000,6 +1263,26 @@ int pcg_last_profile(double* stage_ms, int n_stage, double* total_ms) {
   if (!c.prof_valid) return fail(PCG_ERR_INVALID, "no profiled product yet");
   for (int i = 0; i < n_stage && i < 3; ++i) stage_ms[i] = c.prof_ms[i];
   if (total_ms) *total_ms = c.prof_total;
+  return PCG_OK;
+}
+
+int pcg_bsr_exact(uint32_t a, uint32_t b, uint64_t max_limbs, pcg_bsr_value* P, pcg_bsr_value* Q,
+                  pcg_bsr_value* R) {
+  Ctx& c = g_ctx;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (!c.ready) return fail(PCG_ERR_NOT_INIT, "pcg_init has not been called");
+  if (!P || !Q || !R || !P->limbs || !Q->limbs || !R->limbs) return fail(PCG_ERR_INVALID, "null pointer");
+  pcg_bsr_value* out[3] = {P, Q, R};
+  return bsr_exact_locked(c, a, b, max_limbs, out);
+}
+
+int pcg_bsr_bound(uint32_t a, uint32_t b, uint64_t* bound) {
+  if (!bound || b <= a) return fail(PCG_ERR_INVALID, "bad arguments");
+  std::vector<BsrNode> nodes;
+  nodes.reserve(2 * (size_t)(b - a));
+  bsr_build(nodes, a, b);
+  std::vector<uint64_t> bnd;
+  *bound = bsr_bounds(nodes, bnd);
   return PCG_OK;
 }
 

@@ -9,6 +9,7 @@
 #include <thread>
 #include <vector>
 
+#include "../../include/picugpu.h"
 #include "BigFloat.h"
 
 namespace piculator {
@@ -111,15 +112,18 @@ void bsr_subtree_batched(PQR& out, uint32_t a, uint32_t b, size_t p, bool verbos
   const int32_t H = nodes[root].height;
   std::vector<std::vector<int32_t>> by_h(H + 1);
   for (int32_t i = 0; i < (int32_t)nodes.size(); ++i) by_h[nodes[i].height].push_back(i);
+  double tl = now_s();
   {
     const std::vector<int32_t>& leaves = by_h[1];
     parallel_for(leaves.size(), [&](size_t k) { leaf(nodes[leaves[k]].b, val[leaves[k]]); });
   }
+  if (verbose) std::fprintf(stderr, "  leaves: %zu, %.3f s\n", by_h[1].size(), now_s() - tl);
   std::vector<BigFloat> x, y;
   for (int32_t h = 2; h <= H; ++h) {
     const std::vector<int32_t>& ids = by_h[h];
     x.resize(ids.size());
     y.resize(ids.size());
+    const double ta = now_s();
     MulBatch mb;
     for (size_t k = 0; k < ids.size(); ++k) {
       const Node& n = nodes[ids[k]];
@@ -133,6 +137,7 @@ void bsr_subtree_batched(PQR& out, uint32_t a, uint32_t b, size_t p, bool verbos
     }
     const double t0 = now_s();
     mb.run(false);
+    const double t1 = now_s();
     parallel_for(ids.size(), [&](size_t k) {
       const Node& n = nodes[ids[k]];
       val[ids[k]].P = x[k].add(y[k], p);
@@ -140,15 +145,40 @@ void bsr_subtree_batched(PQR& out, uint32_t a, uint32_t b, size_t p, bool verbos
       val[n.right] = PQR();
     });
     if (verbose)
-      std::fprintf(stderr, "  height %d: %zu nodes, %.3f s\n", h, ids.size(), now_s() - t0);
+      std::fprintf(stderr, "  height %d: %zu nodes, queue %.3f s, products %.3f s, adds %.3f s\n", h,
+                   ids.size(), t0 - ta, t1 - t0, now_s() - t1);
   }
   out = std::move(val[root]);
 }
 
-// Above the chunk size, recurse on the halves first (each half is batched
-// internally), then combine; the combine products of one node run together.
+// Largest subtree handed to the GPU-resident evaluation at once.
+constexpr uint32_t kDeviceLeaves = 1u << 20;
+
+// Subtree [a, b) entirely on the GPU (pcg_bsr_exact) when no value of it can
+// exceed p + 2 limbs; false if it could (or the call fails for size).
+bool bsr_device(PQR& out, uint32_t a, uint32_t b, size_t p) {
+  uint64_t bound = 0;
+  if (pcg_bsr_bound(a, b, &bound) != PCG_OK || bound > p + YCL_BIGFLOAT_EXTRA_PRECISION) return false;
+  std::vector<uint32_t> buf(3 * (bound + 1));
+  pcg_bsr_value v[3];
+  for (int j = 0; j < 3; ++j) v[j] = pcg_bsr_value{buf.data() + j * (bound + 1), bound + 1, 0, 1};
+  int rc = pcg_bsr_exact(a, b, p + YCL_BIGFLOAT_EXTRA_PRECISION, &v[0], &v[1], &v[2]);
+  if (rc == PCG_ERR_NOT_INIT && pcg_init(0) == PCG_OK)
+    rc = pcg_bsr_exact(a, b, p + YCL_BIGFLOAT_EXTRA_PRECISION, &v[0], &v[1], &v[2]);
+  if (rc == PCG_ERR_TOO_LARGE) return false;
+  if (rc != PCG_OK) throw pcg_last_error_string();
+  out.P = BigFloat::from_parts(v[0].sign != 0, 0, v[0].limbs, v[0].L);
+  out.Q = BigFloat::from_parts(v[1].sign != 0, 0, v[1].limbs, v[1].L);
+  out.R = BigFloat::from_parts(v[2].sign != 0, 0, v[2].limbs, v[2].L);
+  return true;
+}
+
+// Subtrees small enough for the GPU evaluation run there whole; above that,
+// recurse on the halves first, then combine on the host BigFloat (with its
+// truncation rules), the four combine products of a node as one batch.
 void bsr_batched(PQR& out, uint32_t a, uint32_t b, size_t p, bool verbose) {
-  if (b - a <= kChunkLeaves) {
+  if (b - a <= kDeviceLeaves && bsr_device(out, a, b, p)) return;
+  if (b - a <= 64) {
     bsr_subtree_batched(out, a, b, p, verbose);
     return;
   }

@@ -33,6 +33,7 @@ EXPORTS = [
     "pcg_mul_dev", "pcg_mul_batch", "pcg_mul_batch_dev", "pcg_sync", "pcg_dev_alloc",
     "pcg_dev_free", "pcg_memcpy_h2d", "pcg_memcpy_d2h", "pcg_fft_last_roundoff",
     "pcg_last_profile", "pcg_ntt_plan", "pcg_timer_start", "pcg_timer_stop",
+    "pcg_bsr_exact", "pcg_bsr_bound",
 ]
 
 
@@ -40,6 +41,10 @@ class PcgError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__("%s: %s" % (STATUS.get(code, code), msg))
         self.code = code
+
+
+class BsrValue(C.Structure):
+    _fields_ = [("limbs", C.c_void_p), ("cap", C.c_uint64), ("L", C.c_uint64), ("sign", C.c_int32)]
 
 
 class MulDesc(C.Structure):
@@ -79,6 +84,8 @@ def load():
     L.pcg_last_profile.argtypes = [C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
     L.pcg_ntt_plan.argtypes = [u64, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.pcg_timer_stop.argtypes = [C.POINTER(C.c_double)]
+    L.pcg_bsr_exact.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64] + [C.POINTER(BsrValue)] * 3
+    L.pcg_bsr_bound.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
     _lib = L
     return L
 
@@ -220,3 +227,21 @@ def timer_stop() -> float:
     ms = C.c_double()
     _check(load().pcg_timer_stop(C.byref(ms)))
     return ms.value
+
+
+def bsr_bound(a: int, b: int) -> int:
+    v = C.c_uint64()
+    _check(load().pcg_bsr_bound(a, b, C.byref(v)))
+    return v.value
+
+
+def bsr_exact(a: int, b: int, max_limbs: int):
+    """(sign, exp, limbs) tuples of P, Q, R of the subtree [a, b) computed on
+    the GPU (exact while no value exceeds max_limbs)."""
+    ensure_init()
+    bound = bsr_bound(a, b)
+    bufs = [np.zeros(bound + 1, dtype=np.uint32) for _ in range(3)]
+    vals = [BsrValue(buf.ctypes.data, buf.size, 0, 1) for buf in bufs]
+    _check(load().pcg_bsr_exact(a, b, max_limbs, C.byref(vals[0]), C.byref(vals[1]),
+                                C.byref(vals[2])))
+    return [(bool(v.sign), 0, buf[: v.L].tolist()) for v, buf in zip(vals, bufs)]

@@ -1,0 +1,34 @@
+"""Device-resident binary splitting (pcg_bsr_exact) against the oracle's
+restatement of Pi_BSR (PI.h:64-112): bit-identical P, Q, R for subtrees whose
+values fit the precision window, and the size error otherwise."""
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BIG_P = 1 << 28  # no truncation anywhere
+
+
+@pytest.mark.parametrize("a,b", [(0, 1), (0, 2), (1, 3), (0, 7), (0, 100), (37, 1000),
+                                 (5000, 13000), (0, 70515)])
+def test_subtree_matches_oracle(gpu, a, b):
+    got = gpu.bsr_exact(a, b, BIG_P)
+    P, Q, R = O.pi_bsr(a, b, BIG_P)
+    assert got[0] == P.tuple()
+    assert got[1] == Q.tuple()
+    assert got[2] == R.tuple()
+
+
+def test_large_leaf_indices(gpu):
+    # leaves near 7e7 (pi to 1e9 digits) exercise the uint32 products 6b-5, 6b-1
+    a, b = 70_000_000, 70_002_000
+    got = gpu.bsr_exact(a, b, BIG_P)
+    P, Q, R = O.pi_bsr(a, b, BIG_P)
+    assert got == [P.tuple(), Q.tuple(), R.tuple()]
+
+
+def test_window_too_small_is_refused(gpu):
+    with pytest.raises(gpu.PcgError) as e:
+        gpu.bsr_exact(0, 5000, 100)
+    assert e.value.code == 5  # PCG_ERR_TOO_LARGE
