@@ -1,18 +1,26 @@
-"""Benchmark of the big-number multiply hot path on one MI355X.
+"""Benchmark of pi to 10^9 decimal digits on one MI355X, with its big-number
+multiply hot path measured against the HBM roofline.
 
-Step = one exact 2^24 x 2^24-limb multiply (operands SplitMix64, radix 2^32,
-resident in HBM) through the C ABI (pcg_mul_dev), the size at which the
-measurement plan gates the NTT multiply against the HBM roofline.  The timed
-region is K back-to-back multiplies on the library stream, bracketed by HIP
-events recorded on that stream and by device synchronisation.
+Step = one computation of pi.txt for --digits decimal digits (default 10^9)
+through the batched schedule: Chudnovsky binary splitting with the lower tree
+evaluated on the GPU (pcg_bsr_exact), the top combines, division, 1/sqrt(10005)
+and the final product on the GPU BigFloat, and the hex conversion.  W untimed
+steps, then K timed steps bracketed by device synchronisation; `value` is the
+mean wall time of a step.  The result of the last step is checked: BBP digits
+at fixed positions and the prefix digest of the oracle's 10^8-digit pi.txt
+(and the full digest where the oracle fixture has this digit count).
 
 Also reported on the same JSON line:
-  roofline      algorithmic bytes of the multiply / time vs 8 TB/s
-  cpu_baseline  the oracle (CPU restatement of the reference NTT multiply,
-                NTT.cpp with its 128/64 divide) on a bounded 2^22 x 2^22
-                sample, scaled by N log N to the 2^24 workload
-  pi            pi to --pi-digits (default 10^6) on the GPU BigFloat, batched
-                and plain schedules, checked against the oracle digest
+  roofline      the 2^24 x 2^24-limb NTT multiply (operands resident in HBM):
+                algorithmic bytes / time vs 8 TB/s, time from HIP events on
+                the library stream; traffic from the committed counter profile
+  ntt_mul       that multiply's stage times and product check
+  batch         4096 uneven products: one batched call vs one launch sequence
+                per product
+  pi_small      pi to 10^6 digits, batched vs plain schedule, both checked
+                against the oracle digest
+  cpu_baseline  the oracle (CPU restatement of the reference, reference engine
+                choice) timed on pi to 10^7 digits and extrapolated to --digits
 Prints exactly one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -34,33 +42,88 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(threads: int, logn_sample: int = 22, logn_target: int = 24):
-    """Oracle NTT multiply of a 2^s x 2^s sample, scaled to 2^t x 2^t by N log N
-    of the reference transform length N = 2^ceil(log2(4 (na + nb)))."""
-    from oracle import oracle as O
-    from piculator_amd import workload
+def pi_cost(digits: int) -> float:
+    """Cost model of the binary-splitting pi computation, N log^2 N in the
+    hex digit count N (multiply M(N) ~ N log N, log N tree levels)."""
+    n = digits * math.log(10) / math.log(16)
+    return n * math.log2(n) ** 2
 
-    n = 1 << logn_sample
-    a, b = workload.operand_pair(n)
+
+def cpu_baseline(threads: int, digits: int, sample_digits: int = 10000000):
+    """The oracle's Pi() (reference engine crossover: FFT below k = 27, NTT
+    above, BigFloat.cpp:386) on a bounded sample, scaled to `digits`."""
+    from oracle import oracle as O
+
     t0 = time.perf_counter()
-    O.mul_limbs(a, b, O.ENGINE_NTT, threads=threads)
+    s, _ = O.pi_hex(sample_digits, O.ENGINE_REFERENCE, threads=threads)
     dt = time.perf_counter() - t0
-    ks = math.ceil(math.log2(4 * 2 * n))
-    kt = math.ceil(math.log2(4 * 2 * (1 << logn_target)))
-    scale = (2 ** kt * kt) / (2 ** ks * ks)
-    t_target = dt * scale
-    nt = 1 << logn_target
+    ok = None
+    gold = os.path.join(ROOT, "tests", "golden", "pi_digests.json")
+    with open(gold) as f:
+        fx = json.load(f)
+    if str(sample_digits) in fx:
+        ok = hashlib.sha256(s.encode()).hexdigest() == fx[str(sample_digits)]["sha256"]
+    scale = pi_cost(digits) / pi_cost(sample_digits)
     return {
-        "value": workload.bytes_alg(nt, nt) / t_target / 1e9,
-        "unit": "GB/s (algorithmic bytes of one 2^24 x 2^24 multiply / time)",
+        "value": dt * scale,
+        "unit": "s (pi to %d digits, extrapolated)" % digits,
         "cores": threads,
         "kind": "port",
-        "sample": "oracle NTT-forced multiply (63-bit prime, 128/64 divide, std::async-style "
-                  "thread split) of 2^%d x 2^%d SplitMix64 limbs: %.2f s, scaled x%.2f by "
-                  "N log N (N = 2^%d -> 2^%d) to %.1f s per 2^24 multiply"
-                  % (logn_sample, logn_sample, dt, scale, ks, kt, t_target),
-        "seconds_per_step": t_target,
+        "sample": "oracle Pi(%d) (reference engines, %d threads): %.2f s measured, digest %s; "
+                  "scaled x%.1f by N log^2 N in hex digits to %d digits"
+                  % (sample_digits, threads, dt, "ok" if ok else ("MISMATCH" if ok is False else "n/a"),
+                     scale, digits),
+        "sample_s": dt,
     }
+
+
+def ntt_mul(logn: int, steps: int, warmup: int):
+    """The 2^logn x 2^logn NTT multiply through pcg_mul_dev: mean time from HIP
+    events over `steps` back-to-back multiplies, stage profile, product check."""
+    import numpy as np
+
+    from piculator_amd import picugpu, workload
+    from tests.helpers import residues_match
+
+    n = 1 << logn
+    a, b = workload.operand_pair(n)
+    da, db = picugpu.DeviceBuffer(a.nbytes), picugpu.DeviceBuffer(b.nbytes)
+    dout = picugpu.DeviceBuffer(a.nbytes + b.nbytes)
+    da.upload(a)
+    db.upload(b)
+    picugpu.reserve(2 * n)
+    logN, passes = picugpu.ntt_plan(2 * n)
+    log("multiply %d x %d limbs: 2^%d points, %d passes" % (n, n, logN, passes))
+    for _ in range(warmup):
+        picugpu.mul_dev(da.ptr, n, db.ptr, n, dout.ptr, picugpu.ENGINE_NTT, sync=True)
+    picugpu.sync()
+    picugpu.timer_start()
+    for _ in range(steps):
+        picugpu.mul_dev(da.ptr, n, db.ptr, n, dout.ptr, picugpu.ENGINE_NTT)
+    ms = picugpu.timer_stop() / steps
+    picugpu.sync()
+    picugpu.mul_dev(da.ptr, n, db.ptr, n, dout.ptr, picugpu.ENGINE_NTT, sync=True, profile=True)
+    stages, total = picugpu.last_profile()
+    out = np.empty(2 * n, dtype=np.uint32)
+    dout.download(out)
+    ok = bool(residues_match(a, b, out)) and int(out[0]) == (int(a[0]) * int(b[0])) & 0xFFFFFFFF
+    for buf in (da, db, dout):
+        buf.free()
+    balg = workload.bytes_alg(n, n)
+    achieved = balg / (ms / 1e3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "ntt_mul_2p%d_traffic.json" % logn)
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_multiply")
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic}
+    info = {"na": n, "nb": n, "transform_points": 1 << logN, "passes": passes, "steps": steps,
+            "ms": ms, "algorithmic_bytes": balg, "product_check": ok,
+            "stage_ms": {"forward_a": stages[0], "forward_b_pointwise_inverse": stages[1],
+                         "crt_carry": stages[2], "total": total}}
+    log("multiply: %.3f ms, %.1f GB/s algorithmic, check %s" % (ms, achieved, ok))
+    return roofline, info
 
 
 def batch_compare(count: int = 4096, reps: int = 3):
@@ -180,17 +243,22 @@ def sweep(steps: int):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--logn", type=int, default=24, help="operand size 2^logn limbs")
-    ap.add_argument("--pi-digits", type=int, default=1000000)
-    ap.add_argument("--no-pi", action="store_true")
+    ap.add_argument("--steps", type=int, default=1, help="timed pi computations")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed pi computations")
+    ap.add_argument("--digits", type=int, default=1000000000)
+    ap.add_argument("--logn", type=int, default=24, help="roofline multiply size 2^logn limbs")
+    ap.add_argument("--mul-steps", type=int, default=20)
+    ap.add_argument("--small-digits", type=int, default=1000000)
+    ap.add_argument("--no-small", action="store_true", help="skip the batched-vs-plain pi run")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain pi schedule")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-sample-digits", type=int, default=10000000)
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run one device copy of 512 MiB (counter calibration)")
+    ap.add_argument("--mul-only", action="store_true",
+                    help="only the roofline multiply (for counter runs)")
     ap.add_argument("--sweep", action="store_true",
                     help="print the single-multiply table (sizes x shapes x engines) instead")
     args = ap.parse_args()
@@ -210,89 +278,87 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     if args.sweep:
-        sweep(args.steps)
+        sweep(args.mul_steps)
         return
 
-    import numpy as np
-
-    from piculator_amd import picugpu, workload
-    from tests.helpers import residues_match
+    from piculator_amd import picugpu, pihost
+    from tests.helpers import pi_spot_checks
 
     picugpu.init(0)
-    n = 1 << args.logn
-    a, b = workload.operand_pair(n)
-    da, db = picugpu.DeviceBuffer(a.nbytes), picugpu.DeviceBuffer(b.nbytes)
-    dout = picugpu.DeviceBuffer(a.nbytes + b.nbytes)
-    da.upload(a)
-    db.upload(b)
-    picugpu.reserve(2 * n)
-    logN, passes = picugpu.ntt_plan(2 * n)
-    log("multiply %d x %d limbs: 2^%d points, %d passes" % (n, n, logN, passes))
-    for _ in range(args.warmup):
-        picugpu.mul_dev(da.ptr, n, db.ptr, n, dout.ptr, picugpu.ENGINE_NTT, sync=True)
+    roofline, mul_info = ntt_mul(args.logn, args.mul_steps, 3)
+    if args.mul_only:
+        print(json.dumps({"roofline": roofline, "ntt_mul": mul_info}), flush=True)
+        return
+
+    D = args.digits
+    buf = pihost.pi_buffer(D)
+    for i in range(args.warmup):
+        log("warm-up pi %d digits (%d/%d)..." % (D, i + 1, args.warmup))
+        n, t = pihost.pi_hex_into(D, pihost.MODE_BATCHED, buf)
+        log("  %.3f s" % t["total"])
+    log("timed: %d x pi %d digits..." % (args.steps, D))
+    phases = []
     picugpu.sync()
     t0 = time.perf_counter()
-    picugpu.timer_start()
     for _ in range(args.steps):
-        picugpu.mul_dev(da.ptr, n, db.ptr, n, dout.ptr, picugpu.ENGINE_NTT)
-    ev_ms = picugpu.timer_stop()
+        n, t = pihost.pi_hex_into(D, pihost.MODE_BATCHED, buf)
+        phases.append(t)
     picugpu.sync()
-    wall_ms = (time.perf_counter() - t0) * 1e3
-    ms = ev_ms / args.steps
-    log("timed %d steps: %.3f ms/step (events), wall %.3f ms" % (args.steps, ms, wall_ms / args.steps))
-    picugpu.mul_dev(da.ptr, n, db.ptr, n, dout.ptr, picugpu.ENGINE_NTT, sync=True, profile=True)
-    stages, total = picugpu.last_profile()
-    out = np.empty(2 * n, dtype=np.uint32)
-    dout.download(out)
-    ok = bool(residues_match(a, b, out)) and int(out[0]) == (int(a[0]) * int(b[0])) & 0xFFFFFFFF
-    log("product check (residues mod 4 primes):", ok)
-
-    balg = workload.bytes_alg(n, n)
-    achieved = balg / (ms / 1e3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "ntt_mul_2p%d_traffic.json" % args.logn)
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_multiply")
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic}
-
+    wall = (time.perf_counter() - t0) / args.steps
+    log("  %.3f s per pi computation" % wall)
+    s = buf.raw[:n]
+    del buf
+    check = pi_spot_checks(s)
+    gold = os.path.join(ROOT, "tests", "golden", "pi_digests.json")
+    with open(gold) as f:
+        fx = json.load(f)
+    check["sha256"] = hashlib.sha256(s).hexdigest()
+    if str(D) in fx:
+        check["full_digest_ok"] = check["sha256"] == fx[str(D)]["sha256"]
+        check["ok"] = bool(check["full_digest_ok"])
+    check["head"] = s[:34].decode()
+    del s
+    log("pi check:", json.dumps(check))
+    hexd, p, terms = pihost.pi_sizes(D)
+    last = phases[-1]
     res = {
         "metric": METRIC,
-        "value": achieved,
-        "unit": "GB/s (algorithmic bytes 64 N + 8 (na + nb) of one 2^%d x 2^%d multiply / time)"
-                % (args.logn, args.logn),
+        "value": wall,
+        "unit": "s (wall time of one pi computation to %d decimal digits)" % D,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
-        "higher_is_better": True,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "single exact big-integer multiply, 2^%d x 2^%d limbs of radix 2^32 "
-                               "(SplitMix64 operands resident in HBM)" % (args.logn, args.logn),
-                   "na": n, "nb": n, "engine": "two-prime FP64 NTT + CRT + carry scan",
-                   "transform_points": 1 << logN, "passes": passes},
+        "config": {"workload": "pi to %d decimal digits (Chudnovsky binary splitting, batched "
+                               "schedule, every big multiply on the GPU)" % D,
+                   "digits": D, "hex_digits": hexd, "terms": terms, "precision_limbs": p,
+                   "roofline_kernel": "2^%d x 2^%d-limb two-prime FP64 NTT multiply"
+                                      % (args.logn, args.logn)},
         "roofline": roofline,
-        "product_check": ok,
-        "stage_ms": {"forward_a": stages[0], "forward_b_pointwise_inverse": stages[1],
-                     "crt_carry": stages[2], "total": total},
+        "phases_s": {k: last[k] for k in ("series", "division", "invsqrt", "final_mul",
+                                          "conversion", "total")},
+        "big_multiplies": int(last["big_muls"]),
+        "pi_check": check,
+        "ntt_mul": mul_info,
     }
     if not args.no_batch:
         log("uneven batch of 4096 products...")
         res["batch"] = batch_compare()
         log("batch:", json.dumps(res["batch"]))
+    if not args.no_small:
+        log("pi %d digits, batched vs plain..." % args.small_digits)
+        res["pi_small"] = pi_compare(args.small_digits, not args.no_plain)
+        log("pi_small:", json.dumps(res["pi_small"]))
     if not args.no_cpu:
         th = args.cpu_threads or min(16, os.cpu_count() or 1)
         log("cpu baseline (%d threads)..." % th)
-        res["cpu_baseline"] = cpu_baseline(th)
+        res["cpu_baseline"] = cpu_baseline(th, D, args.cpu_sample_digits)
         log("cpu baseline:", res["cpu_baseline"]["sample"])
-    if not args.no_pi:
-        log("pi %d digits..." % args.pi_digits)
-        res["pi"] = pi_compare(args.pi_digits, not args.no_plain)
-        log("pi:", json.dumps(res["pi"]))
     print(json.dumps(res), flush=True)
 
 

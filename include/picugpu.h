@@ -141,6 +141,56 @@ int pcg_bsr_exact(uint32_t a, uint32_t b, uint64_t max_limbs, pcg_bsr_value* P, 
                   pcg_bsr_value* R);
 int pcg_bsr_bound(uint32_t a, uint32_t b, uint64_t* bound);
 
+/* ---- Device-resident BigFloat values ----
+ * A BigFloat (BigFloat.h:33-36) whose limbs stay in device memory, so the
+ * top of the binary splitting, the Newton division and 1/sqrt(10005) of
+ * Pi() (PI.h:175-209) run without moving values through host memory.
+ * Value = sum limbs[i] 2^(32 (exp + i)); zero has L = 0.  Limb buffers come
+ * from the library's pool: start from a zeroed struct, let the calls below
+ * (re)fill it, release it with pcg_df_free before pcg_shutdown.  An output
+ * may be the same struct as an input.  Each call waits for its result (the
+ * new length depends on the limbs), so metadata is valid on return. */
+typedef struct pcg_dfloat {
+  uint32_t* limbs;  /* device memory owned by the library pool              */
+  uint64_t cap;     /* limbs allocated                                        */
+  uint64_t L;       /* limbs in use (0 = zero)                                */
+  int64_t exp;      /* exponent in limbs (BigFloat::exp)                      */
+  int32_t sign;     /* 1 positive or zero, 0 negative (BigFloat::sign)        */
+} pcg_dfloat;
+
+/* Returns x's buffer to the pool; x becomes zero. */
+int pcg_df_free(pcg_dfloat* x);
+/* x = (sign, exp, limbs[0..L)) from host memory (BigFloat(T, L, exp, sign)). */
+int pcg_df_set(pcg_dfloat* x, const uint32_t* limbs, uint64_t L, int64_t exp, int32_t sign);
+/* Copies limbs [first, first + count) of x to host memory. */
+int pcg_df_read(const pcg_dfloat* x, uint64_t first, uint64_t count, uint32_t* out);
+/* z = x (BigFloat copy constructor, BigFloat.cpp:24-33). */
+int pcg_df_copy(pcg_dfloat* z, const pcg_dfloat* x);
+/* BigFloat::ucmp (BigFloat.cpp:123-147): -1, 0, 1 for |a| <, =, > |b|. */
+int pcg_df_ucmp(const pcg_dfloat* a, const pcg_dfloat* b, int* cmp);
+/* z = a.add(b, p) / a.sub(b, p) (BigFloat.cpp:288-325 with uadd/usub
+ * 184-287): aligned by exponent, the bottom cut to p + 2 limbs (p = 0:
+ * exact), a carry limb appended, or every leading zero limb stripped. */
+int pcg_df_add(pcg_dfloat* z, const pcg_dfloat* a, const pcg_dfloat* b, uint64_t p);
+int pcg_df_sub(pcg_dfloat* z, const pcg_dfloat* a, const pcg_dfloat* b, uint64_t p);
+/* z = a.mul(b, p) (BigFloat.cpp:326-444): operands cut to their top p + 2
+ * limbs (p = 0: exact), exact product on the GPU engine chosen by `flags`
+ * (PCG_ENGINE_*), one zero top limb trimmed. */
+int pcg_df_mul(pcg_dfloat* z, const pcg_dfloat* a, const pcg_dfloat* b, uint64_t p, uint32_t flags);
+/* z = a.mul(x) (BigFloat.cpp:156-183). */
+int pcg_df_mul_u32(pcg_dfloat* z, const pcg_dfloat* a, uint32_t x);
+/* to_string_hex (BigFloat.cpp:95-106): the top limb in decimal, '.', then
+ * 8 hex digits per lower limb, cut to 2 + digits characters; NUL-terminated
+ * in out[0..cap); *written = characters before the NUL.  PCG_ERR_INVALID
+ * ("Limit Exceeded") when the value has too few limbs. */
+int pcg_df_to_hex(const pcg_dfloat* x, uint64_t digits, char* out, uint64_t cap, uint64_t* written);
+/* pcg_bsr_exact with the results left in device memory as values of
+ * exponent 0 (no copy through the host). */
+int pcg_df_bsr_exact(uint32_t a, uint32_t b, uint64_t max_limbs, pcg_dfloat* P, pcg_dfloat* Q,
+                     pcg_dfloat* R);
+/* Frees the pool's cached buffers (values in use are untouched). */
+int pcg_df_release_pool(void);
+
 /* Elapsed time of the work enqueued on the library stream between the two
  * calls, measured with HIP events recorded on that stream. */
 int pcg_timer_start(void);

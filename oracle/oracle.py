@@ -83,6 +83,8 @@ def lib():
         L.orc_pi_hex.restype = C.c_long
         L.orc_mul_count.restype = C.c_uint64
         L.orc_pi_bsr.argtypes = [bfp, bfp, bfp, C.c_uint32, C.c_uint32, sz, C.c_int, C.c_int]
+        L.orc_bbp_hex.argtypes = [C.c_uint64, C.c_int]
+        L.orc_bbp_hex.restype = C.c_uint32
         _lib = L
     return _lib
 
@@ -231,6 +233,10 @@ class BF:
         lib().orc_bf_mul_u32(z.ref, self.ref, x)
         return z
 
+    def ucmp(self, other: "BF") -> int:
+        """BigFloat::ucmp (BigFloat.cpp:123-147)."""
+        return int(lib().orc_bf_ucmp(self.ref, other.ref))
+
     def add(self, other: "BF", p: int = 0) -> "BF":
         z = BF()
         lib().orc_bf_add(z.ref, self.ref, other.ref, p)
@@ -291,3 +297,9 @@ def pi_hex(decimal_digits: int, engine: int = ENGINE_REFERENCE, threads: int = 1
     if n < 0:
         raise RuntimeError("oracle Pi failed")
     return buf.raw[:n].decode(), list(times)
+
+
+def bbp_hex(n: int, threads: int = 1, digits: int = 6) -> str:
+    """Hex digits of pi from digit n + 1 after the point (BBP extraction)."""
+    v = int(lib().orc_bbp_hex(n, threads))
+    return ("%08x" % v)[:digits]

@@ -953,3 +953,66 @@ long orc_pi_hex(size_t decimal_digits, int engine, int threads, char* out, size_
   orc_bf_free(&P); orc_bf_free(&Q); orc_bf_free(&R); orc_bf_free(&t); orc_bf_free(&u);
   return n;
 }
+
+/* ------------------------------------------------------------------ */
+/* BBP digit extraction (spot checks of pi output)                     */
+/* ------------------------------------------------------------------ */
+static uint64_t powmod16(uint64_t e, uint64_t m) {
+  return m == 1 ? 0 : orc_modpow(16 % m, e, m);
+}
+
+typedef struct {
+  uint64_t n, k0, k1;
+  long double s[4];
+} bbp_job;
+
+static const int bbp_j[4] = {1, 4, 5, 6};
+
+static void* bbp_thread(void* arg) {
+  bbp_job* J = (bbp_job*)arg;
+  for (int t = 0; t < 4; ++t) {
+    long double s = 0;
+    for (uint64_t k = J->k0; k < J->k1; ++k) {
+      const uint64_t m = 8 * k + bbp_j[t];
+      s += (long double)powmod16(J->n - k, m) / (long double)m;
+      s -= floorl(s);
+    }
+    J->s[t] = s;
+  }
+  return NULL;
+}
+
+uint32_t orc_bbp_hex(uint64_t n, int threads) {
+  if (threads < 1) threads = 1;
+  if (threads > 64) threads = 64;
+  bbp_job jobs[64];
+  pthread_t th[64];
+  const uint64_t total = n + 1;
+  for (int i = 0; i < threads; ++i) {
+    jobs[i].n = n;
+    jobs[i].k0 = total * i / threads;
+    jobs[i].k1 = total * (i + 1) / threads;
+    if (pthread_create(&th[i], NULL, bbp_thread, &jobs[i]) != 0) bbp_thread(&jobs[i]);
+    else continue;
+    th[i] = 0;
+  }
+  for (int i = 0; i < threads; ++i)
+    if (th[i]) pthread_join(th[i], NULL);
+  long double S[4] = {0, 0, 0, 0};
+  for (int t = 0; t < 4; ++t) {
+    for (int i = 0; i < threads; ++i) {
+      S[t] += jobs[i].s[t];
+      S[t] -= floorl(S[t]);
+    }
+    /* tail: k > n, terms 16^(n-k) / (8k + j) */
+    long double p = 1.0L / 16.0L;
+    for (uint64_t k = n + 1; k < n + 40; ++k) {
+      S[t] += p / (long double)(8 * k + bbp_j[t]);
+      p /= 16.0L;
+    }
+    S[t] -= floorl(S[t]);
+  }
+  long double x = 4 * S[0] - 2 * S[1] - S[2] - S[3];
+  x -= floorl(x);
+  return (uint32_t)(x * 16777216.0L) << 8;
+}

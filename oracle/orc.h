@@ -90,6 +90,13 @@ void orc_pi_bsr(orc_bf* P, orc_bf* Q, orc_bf* R, uint32_t a, uint32_t b, size_t 
 long orc_pi_hex(size_t decimal_digits, int engine, int threads, char* out, size_t cap,
                 double* times);
 
+/* Six hexadecimal digits of pi starting at digit n + 1 after the point, by
+ * Bailey-Borwein-Plouffe digit extraction (Bailey, Borwein & Plouffe 1995):
+ * frac(4 S1 - 2 S4 - S5 - S6) with S_j = sum_k 16^(n-k) / (8k + j).
+ * Independent of the binary splitting; used to spot-check pi output.
+ * Returns the digits as the top 24 bits of a 32-bit value. */
+uint32_t orc_bbp_hex(uint64_t n, int threads);
+
 /* Number of big (BigFloat x BigFloat) multiplies performed so far (statistics). */
 uint64_t orc_mul_count(void);
 

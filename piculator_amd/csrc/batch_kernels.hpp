@@ -117,41 +117,109 @@ __global__ __launch_bounds__((1 << LOGN) / kEPT) void k_batch_ntt(const BatchDes
   }
 }
 
-// CRT and carry of batched multi-pass products: one workgroup per product
-// walks its limbs in chunks of 1024, folding each chunk's carry-in into the
-// chunk's first limbs.  r: inverse-transform outputs of product z at
-// r + z * zstride, prime q at offset q * N.
-__global__ __launch_bounds__(256) void k_batch_carry(const BatchDesc* descs, const double* r,
-                                                      uint64_t zstride, uint64_t N,
-                                                      const Prime* prm, double2 crt) {
+// CRT and carry of batched multi-pass products, chunk-parallel: block
+// (chunk, z) resolves limbs [1024 chunk, 1024 chunk + 1024) of product z
+// with carry-in 0 and records the chunk's carry function (the form of
+// k_carry_local); k_batch_carry_scan scans each product's chunk functions
+// with one wave; k_batch_carry_fix adds the carry-ins.  r: inverse-transform
+// outputs of product z at r + z * zstride, prime q at offset q * N.
+// agg / cin: nch entries per product.
+__global__ __launch_bounds__(kCarryThreads) void k_batch_carry_local(
+    const BatchDesc* descs, const double* r, uint64_t zstride, uint64_t N, const Prime* prm,
+    double2 crt, CarryFn* agg, uint32_t nch) {
   __shared__ uint32_t sw1[kCarryBlock], sw2[kCarryBlock];
-  __shared__ CarryFn wtot[4];
-  __shared__ uint64_t next_cin;
-  const BatchDesc d = descs[blockIdx.x];
-  const double* rz = r + (uint64_t)blockIdx.x * zstride;
-  const Prime P0 = prm[0], P1 = prm[1];
+  __shared__ CarryFn wtot[kCarryThreads / 64];
+  __shared__ uint32_t d01[2];
+  const uint32_t z = blockIdx.y;
+  const BatchDesc d = descs[z];
   const uint64_t zL = (uint64_t)d.na + d.nb;
+  const uint64_t base = (uint64_t)blockIdx.x * kCarryBlock;
+  if (base >= zL) return;  // whole block: the chunk lies past this product
+  const double* rz = r + (uint64_t)z * zstride;
+  const Prime P0 = prm[0], P1 = prm[1];
   const int tid = threadIdx.x;
-  uint64_t cin = 0;
-  for (uint64_t base = 0; base < zL; base += kCarryBlock) {
-    uint32_t w0[kCarryLPT], w1[kCarryLPT], w2[kCarryLPT], D[kCarryLPT];
+  uint32_t w0[kCarryLPT], w1[kCarryLPT], w2[kCarryLPT], D[kCarryLPT];
 #pragma unroll
-    for (int q = 0; q < kCarryLPT; ++q) {
-      const uint64_t k = base + tid * kCarryLPT + q;
-      w0[q] = w1[q] = w2[q] = 0;
-      if (k < zL) crt_words(rz[k], rz[N + k], P0, P1, crt, w0[q], w1[q], w2[q]);
-    }
-    uint64_t c, spill;
-    block_carry<kCarryLPT>(w0, w1, w2, sw1, sw2, wtot, D, c, spill, cin);
+  for (int q = 0; q < kCarryLPT; ++q) {
+    const uint64_t k = base + tid * kCarryLPT + q;
+    w0[q] = w1[q] = w2[q] = 0;
+    if (k < zL) crt_words(rz[k], rz[N + k], P0, P1, crt, w0[q], w1[q], w2[q]);
+  }
+  uint64_t c, spill;
+  block_carry<kCarryLPT>(w0, w1, w2, sw1, sw2, wtot, D, c, spill);
+  bool ones = true;
 #pragma unroll
-    for (int q = 0; q < kCarryLPT; ++q) {
-      const uint64_t k = base + tid * kCarryLPT + q;
-      if (k < zL) d.out[k] = D[q];
+  for (int q = 0; q < kCarryLPT; ++q) {
+    const int li = tid * kCarryLPT + q;
+    const uint64_t k = base + li;
+    if (k < zL) d.out[k] = D[q];
+    if (li >= 2 && D[q] != 0xffffffffu) ones = false;
+    if (li < 2) d01[li] = D[q];
+  }
+  const int all_ones = __syncthreads_and(ones ? 1 : 0);
+  if (tid == kCarryThreads - 1) {
+    CarryFn g;
+    g.base = spill + c;
+    g.T = kInf;
+    if (all_ones) {
+      const uint64_t low = (uint64_t)d01[0] | ((uint64_t)d01[1] << 32);
+      if (low) g.T = 0 - low;  // 2^64 - low
     }
-    if (tid == kCarryThreads - 1) next_cin = spill + c;
-    __syncthreads();
-    cin = next_cin;
-    __syncthreads();
+    agg[(uint64_t)z * nch + blockIdx.x] = g;
+  }
+}
+
+// One wave per product: exclusive scan of its chunk functions, 64 at a time.
+__global__ __launch_bounds__(64) void k_batch_carry_scan(const BatchDesc* descs, const CarryFn* agg,
+                                                         uint64_t* cin, uint32_t nch) {
+  const uint32_t z = blockIdx.x;
+  const BatchDesc d = descs[z];
+  const uint64_t zL = (uint64_t)d.na + d.nb;
+  const uint32_t nc = (uint32_t)((zL + kCarryBlock - 1) / kCarryBlock);
+  const int lane = threadIdx.x;
+  CarryFn run;
+  run.base = 0;
+  run.T = 0;
+  for (uint32_t c0 = 0; c0 < nc; c0 += 64) {
+    const uint32_t idx = c0 + lane;
+    CarryFn f;
+    f.base = 0;
+    f.T = 0;
+    if (idx < nc) f = agg[(uint64_t)z * nch + idx];
+    CarryFn incl = f;
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+      const CarryFn o = cf_shfl_up(incl, s);
+      if (lane >= s) incl = cf_compose(o, incl);
+    }
+    CarryFn excl = cf_shfl_up(incl, 1);
+    if (lane == 0) excl.T = 0;
+    if (idx < nc) cin[(uint64_t)z * nch + idx] = cf_eval0(cf_compose(run, excl));
+    CarryFn tot;
+    tot.base = __shfl(incl.base, 63, 64);
+    tot.T = __shfl(incl.T, 63, 64);
+    run = cf_compose(run, tot);
+  }
+}
+
+// One thread per (product, chunk): ripple the chunk's carry-in.
+__global__ __launch_bounds__(256) void k_batch_carry_fix(const BatchDesc* descs, const uint64_t* cin,
+                                                         uint32_t nch, uint64_t count) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t z = t / nch, ch = t % nch;
+  if (z >= count) return;
+  const BatchDesc d = descs[z];
+  const uint64_t zL = (uint64_t)d.na + d.nb;
+  uint64_t k = ch * kCarryBlock;
+  if (k >= zL) return;
+  uint64_t c = cin[z * nch + ch];
+  uint64_t end = k + kCarryBlock;
+  if (end > zL) end = zL;
+  while (c != 0 && k < end) {
+    const uint64_t v = (uint64_t)d.out[k] + (c & 0xffffffffull);
+    d.out[k] = (uint32_t)v;
+    c = (c >> 32) + (v >> 32);
+    ++k;
   }
 }
 

@@ -257,4 +257,247 @@ __global__ __launch_bounds__(NT) void k_bsr_add(const BsrAdd* adds, BsrVal* meta
   }
 }
 
+// ------------------------------------------------ chunk-parallel large add
+// The same addition for values above 1024 limbs, with every 1024-limb chunk
+// of every node in its own workgroup: the comparison of equal-length
+// magnitudes (k_bsr_add_cmp), block-local sums with carry-in 0 and the
+// chunk carry functions (k_bsr_add_local), one wave per node scanning them
+// (k_bsr_add_scan), the carry-in ripple (k_bsr_add_fix), the top nonzero
+// limb of a difference (k_bsr_add_topnz) and the result length and sign
+// (k_bsr_add_final).  Decisions are recomputed from the metadata in each
+// kernel (bsr_add_plan), so they agree everywhere.
+struct BsrAddSlot {
+  unsigned long long cmp;  // 1 + highest index where equal-length magnitudes differ
+  unsigned long long top;  // 1 + highest nonzero limb of a difference
+  uint64_t carry;          // carry out of the top limb of a sum
+};
+
+struct BsrAddPlan {
+  const uint32_t* A;
+  const uint32_t* B;
+  uint32_t* O;
+  uint32_t LA, LB;
+  int sign;
+  bool sub, zero;
+};
+
+__device__ __forceinline__ BsrAddPlan bsr_add_plan(const BsrAdd op, const BsrVal* meta, uint32_t* arena,
+                                                   unsigned long long cmp_slot) {
+  const BsrVal mx = meta[op.x], my = meta[op.y];
+  BsrAddPlan p;
+  p.O = arena + meta[op.out].off;
+  const uint32_t* X = arena + mx.off;
+  const uint32_t* Y = arena + my.off;
+  p.A = X;
+  p.B = Y;
+  p.LA = mx.L;
+  p.LB = my.L;
+  p.sign = mx.sign;
+  p.sub = mx.sign != my.sign;
+  p.zero = false;
+  bool swap = false;
+  if (p.sub) {
+    int cmp;
+    if (mx.L != my.L) {
+      cmp = mx.L > my.L ? 1 : -1;
+    } else if (cmp_slot == 0) {
+      cmp = 0;
+    } else {
+      const uint64_t i = cmp_slot - 1;
+      cmp = X[i] > Y[i] ? 1 : -1;
+    }
+    if (cmp == 0) {
+      p.zero = true;
+      return p;
+    }
+    if (cmp < 0) {
+      swap = true;
+      p.sign = my.sign;
+    }
+  } else if (my.L > mx.L) {
+    swap = true;
+  }
+  if (swap) {
+    p.A = Y;
+    p.B = X;
+    p.LA = my.L;
+    p.LB = mx.L;
+  }
+  return p;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const unsigned long long o = __shfl_xor(v, d, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__global__ void k_bsr_add_prep(uint64_t n, BsrAddSlot* slots) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  slots[i].cmp = 0;
+  slots[i].top = 0;
+  slots[i].carry = 0;
+}
+
+__global__ __launch_bounds__(256) void k_bsr_add_cmp(const BsrAdd* adds, const BsrVal* meta,
+                                                     const uint32_t* arena, BsrAddSlot* slots) {
+  const BsrAdd op = adds[blockIdx.y];
+  const BsrVal mx = meta[op.x], my = meta[op.y];
+  if (mx.sign == my.sign || mx.L != my.L) return;
+  const uint64_t base = (uint64_t)blockIdx.x * kCarryBlock;
+  if (base >= mx.L) return;
+  const uint32_t* X = arena + mx.off;
+  const uint32_t* Y = arena + my.off;
+  unsigned long long hit = 0;
+#pragma unroll
+  for (int q = 0; q < kCarryLPT; ++q) {
+    const uint64_t k = base + (uint64_t)threadIdx.x * kCarryLPT + q;
+    if (k < mx.L && X[k] != Y[k]) hit = k + 1;
+  }
+  hit = wave_max_u64(hit);
+  if ((threadIdx.x & 63) == 0 && hit) atomicMax(&slots[blockIdx.y].cmp, hit);
+}
+
+__global__ __launch_bounds__(kCarryThreads) void k_bsr_add_local(const BsrAdd* adds, const BsrVal* meta,
+                                                                 uint32_t* arena, const BsrAddSlot* slots,
+                                                                 CarryFn* agg, uint32_t maxch) {
+  __shared__ CarryFn wtot[kCarryThreads / 64];
+  const uint32_t node = blockIdx.y;
+  const BsrAddPlan pl = bsr_add_plan(adds[node], meta, arena, slots[node].cmp);
+  const uint64_t base = (uint64_t)blockIdx.x * kCarryBlock;
+  if (pl.zero || base >= pl.LA) return;  // uniform over the block
+  const int tid = threadIdx.x;
+  uint64_t s[kCarryLPT];
+#pragma unroll
+  for (int q = 0; q < kCarryLPT; ++q) {
+    const uint64_t k = base + (uint64_t)tid * kCarryLPT + q;
+    uint64_t x = 0;
+    if (k < pl.LA) {
+      const uint32_t bk = k < pl.LB ? pl.B[k] : 0u;
+      x = (uint64_t)pl.A[k] + (pl.sub ? (uint32_t)~bk : bk);
+      if (pl.sub && k == 0) x += 1;
+    }
+    s[q] = x;
+  }
+  // Positions past the top pass the carry through unchanged (identity), so
+  // the block function ends at the top limb: its value is the carry out.
+  CarryFn f;
+  f.base = 0;
+  f.T = 0;
+#pragma unroll
+  for (int q = 0; q < kCarryLPT; ++q) {
+    if (base + (uint64_t)tid * kCarryLPT + q >= pl.LA) continue;
+    CarryFn g;
+    g.base = s[q] >> 32;
+    g.T = (1ull << 32) - (s[q] & 0xffffffffull);
+    f = cf_compose(f, g);
+  }
+  const CarryFn pre = cf_block_exclusive(f, wtot);
+  uint64_t c = cf_eval0(pre);
+#pragma unroll
+  for (int q = 0; q < kCarryLPT; ++q) {
+    const uint64_t k = base + (uint64_t)tid * kCarryLPT + q;
+    const uint64_t t = (s[q] & 0xffffffffull) + c;
+    if (k < pl.LA) pl.O[k] = (uint32_t)t;
+    c = (s[q] >> 32) + (t >> 32);
+  }
+  if (tid == kCarryThreads - 1) agg[(uint64_t)node * maxch + blockIdx.x] = cf_compose(pre, f);
+}
+
+// One wave per node: exclusive scan of its chunk functions; the total is
+// the carry out of the top limb.
+__global__ __launch_bounds__(64) void k_bsr_add_scan(const BsrAdd* adds, const BsrVal* meta, uint32_t* arena,
+                                                     BsrAddSlot* slots, const CarryFn* agg, uint64_t* cin,
+                                                     uint32_t maxch) {
+  const uint32_t node = blockIdx.x;
+  const BsrAddPlan pl = bsr_add_plan(adds[node], meta, arena, slots[node].cmp);
+  if (pl.zero) return;
+  const uint32_t nc = (uint32_t)((pl.LA + kCarryBlock - 1) / kCarryBlock);
+  const int lane = threadIdx.x;
+  CarryFn run;
+  run.base = 0;
+  run.T = 0;
+  for (uint32_t c0 = 0; c0 < nc; c0 += 64) {
+    const uint32_t idx = c0 + lane;
+    CarryFn f;
+    f.base = 0;
+    f.T = 0;
+    if (idx < nc) f = agg[(uint64_t)node * maxch + idx];
+    CarryFn incl = f;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const CarryFn o = cf_shfl_up(incl, d);
+      if (lane >= d) incl = cf_compose(o, incl);
+    }
+    CarryFn excl = cf_shfl_up(incl, 1);
+    if (lane == 0) excl.T = 0;
+    if (idx < nc) cin[(uint64_t)node * maxch + idx] = cf_eval0(cf_compose(run, excl));
+    CarryFn tot;
+    tot.base = __shfl(incl.base, 63, 64);
+    tot.T = __shfl(incl.T, 63, 64);
+    run = cf_compose(run, tot);
+  }
+  if (lane == 0) slots[node].carry = cf_eval0(run);
+}
+
+__global__ __launch_bounds__(256) void k_bsr_add_fix(const BsrAdd* adds, const BsrVal* meta, uint32_t* arena,
+                                                     const BsrAddSlot* slots, const uint64_t* cin,
+                                                     uint32_t maxch, uint64_t n) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t node = t / maxch, ch = t % maxch;
+  if (node >= n) return;
+  const BsrAddPlan pl = bsr_add_plan(adds[node], meta, arena, slots[node].cmp);
+  uint64_t k = ch * kCarryBlock;
+  if (pl.zero || k >= pl.LA) return;
+  uint64_t c = cin[node * maxch + ch];
+  uint64_t end = k + kCarryBlock;
+  if (end > pl.LA) end = pl.LA;
+  while (c != 0 && k < end) {
+    const uint64_t v = (uint64_t)pl.O[k] + (c & 0xffffffffull);
+    pl.O[k] = (uint32_t)v;
+    c = (c >> 32) + (v >> 32);
+    ++k;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bsr_add_topnz(const BsrAdd* adds, const BsrVal* meta,
+                                                       uint32_t* arena, BsrAddSlot* slots) {
+  const uint32_t node = blockIdx.y;
+  const BsrAddPlan pl = bsr_add_plan(adds[node], meta, arena, slots[node].cmp);
+  const uint64_t base = (uint64_t)blockIdx.x * kCarryBlock;
+  if (!pl.sub || pl.zero || base >= pl.LA) return;
+  unsigned long long hit = 0;
+#pragma unroll
+  for (int q = 0; q < kCarryLPT; ++q) {
+    const uint64_t k = base + (uint64_t)threadIdx.x * kCarryLPT + q;
+    if (k < pl.LA && pl.O[k] != 0) hit = k + 1;
+  }
+  hit = wave_max_u64(hit);
+  if ((threadIdx.x & 63) == 0 && hit) atomicMax(&slots[node].top, hit);
+}
+
+__global__ void k_bsr_add_final(const BsrAdd* adds, BsrVal* meta, uint32_t* arena, const BsrAddSlot* slots,
+                                uint64_t n) {
+  const uint64_t node = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= n) return;
+  const BsrAdd op = adds[node];
+  const BsrAddPlan pl = bsr_add_plan(op, meta, arena, slots[node].cmp);
+  uint32_t L;
+  if (pl.zero) {
+    L = 0;
+  } else if (pl.sub) {
+    L = (uint32_t)slots[node].top;  // every leading zero stripped (usub)
+  } else {
+    L = pl.LA;
+    if (slots[node].carry) pl.O[L++] = 1;  // carry limb (uadd)
+  }
+  BsrVal& mo = meta[op.out];
+  mo.L = L;
+  mo.sign = L ? pl.sign : 1;
+}
+
 }  // namespace pcg

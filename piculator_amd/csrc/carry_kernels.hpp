@@ -92,6 +92,7 @@ struct CarryArgs {
   uint64_t nblk;
   double2 crt;       // p0^{-1} mod p1 as (w, w/p1)
   uint32_t* status;  // nonzero on overflow (null: no check)
+  uint64_t* carry_out;  // carry out of the top block (null: not stored)
   // FFT engine: unweighted inverse-transform output and round-off maximum
   const double2* z;
   unsigned long long* maxdist;  // bits of a non-negative double
@@ -287,6 +288,7 @@ __global__ __launch_bounds__(1024) void k_carry_scan(CarryArgs a) {
   if (threadIdx.x == blockDim.x - 1) {
     // carry out of the top block must vanish for an exact product
     if (a.status && cf_eval0(run) != 0) *a.status = 1;
+    if (a.carry_out) *a.carry_out = cf_eval0(run);
   }
 }
 

@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -23,6 +24,7 @@
 #include "batch_kernels.hpp"
 #include "bsr_kernels.hpp"
 #include "carry_kernels.hpp"
+#include "df_kernels.hpp"
 #include "fft_kernels.hpp"
 #include "ntt_kernels.hpp"
 
@@ -202,6 +204,9 @@ struct Ctx {
   // scratch of batched multi-pass products
   double* bspec = nullptr;
   size_t bspec_doubles = 0;
+  CarryFn* bagg = nullptr;  // chunk carry functions of batched products
+  uint64_t* bcin = nullptr;
+  size_t bcarry_cap = 0;
   // profiling
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t tev[2] = {nullptr, nullptr};
@@ -213,6 +218,12 @@ struct Ctx {
   // FFT engine tables and round-off slot
   double2* ftab[3] = {nullptr, nullptr, nullptr};
   unsigned long long* d_maxdist = nullptr;
+  // device values: carry-scan blocks and readback slots
+  CarryFn* df_agg = nullptr;
+  uint64_t* df_cin = nullptr;
+  size_t df_nblk_cap = 0;
+  unsigned long long* df_slot = nullptr;  // [4] device
+  unsigned long long* h_slot = nullptr;   // [4] pinned host
 };
 
 Ctx g_ctx;
@@ -430,6 +441,7 @@ int ntt_mul_enqueue(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_
   }
   if (profile) HIPCHK(hipEventRecord(c.ev[2], c.stream));
   CarryArgs ca;
+  memset(&ca, 0, sizeof ca);
   ca.prm = c.d_prime + logN * 2;
   ca.r = B;
   ca.N = N;
@@ -628,6 +640,20 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN) {
   const uint64_t per = 4 * N;
   const size_t group_max = std::max<size_t>(1, ((size_t)1 << 28) / per);  // 2 GiB of scratch
   const size_t want = std::min(count, group_max) * per;
+  // chunk carry functions: (products per group) x (chunks per product)
+  const size_t carry_need = std::min(count, group_max) * (size_t)(N / kCarryBlock);
+  if (carry_need > c.bcarry_cap) {
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (c.bagg) HIPCHK(hipFree(c.bagg));
+    if (c.bcin) HIPCHK(hipFree(c.bcin));
+    c.bagg = nullptr;
+    c.bcin = nullptr;
+    c.bcarry_cap = 0;
+    const size_t cap = std::max<size_t>(carry_need, (size_t)1 << 16);
+    HIPCHK(hipMalloc(&c.bagg, cap * sizeof(CarryFn)));
+    HIPCHK(hipMalloc(&c.bcin, cap * sizeof(uint64_t)));
+    c.bcarry_cap = cap;
+  }
   if (want > c.bspec_doubles) {
     HIPCHK(hipStreamSynchronize(c.stream));
     if (c.bspec) HIPCHK(hipFree(c.bspec));
@@ -664,8 +690,13 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN) {
       rc = launch_pass(c, kInv, pl, pl.passes[j], B, nullptr, 0, B, nullptr, dd, 1, per, k);
       if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_batch_carry, dim3(k), dim3(kCarryThreads), 0, c.stream, dd, B, per, N,
-                       c.d_prime + logN * 2, c.crt);
+    const uint32_t nch = (uint32_t)(N / kCarryBlock);
+    if ((size_t)k * nch > c.bcarry_cap) return fail(PCG_ERR_INTERNAL, "batched carry blocks exceed the scratch");
+    hipLaunchKernelGGL(k_batch_carry_local, dim3(nch, k), dim3(kCarryThreads), 0, c.stream, dd, B, per,
+                       N, c.d_prime + logN * 2, c.crt, c.bagg, nch);
+    hipLaunchKernelGGL(k_batch_carry_scan, dim3(k), dim3(64), 0, c.stream, dd, c.bagg, c.bcin, nch);
+    hipLaunchKernelGGL(k_batch_carry_fix, dim3((unsigned)(((uint64_t)k * nch + 255) / 256)), dim3(256), 0,
+                       c.stream, dd, c.bcin, nch, (uint64_t)k);
   }
   HIPCHK(hipGetLastError());
   return PCG_OK;
@@ -852,6 +883,7 @@ T* carve(char*& p, size_t n) {
 // lists sorted by size class, and the device copies of all of it.
 struct BsrHeight {
   size_t mul0, nmul, add0, nsmall, nlarge;
+  uint32_t add_maxch;  // 1024-limb chunks of the largest large-add result
   std::vector<std::pair<int, size_t>> classes;  // (class, count) in launch order
 };
 struct BsrPlan {
@@ -871,6 +903,9 @@ struct BsrPlan {
   BsrMul* muls = nullptr;
   BsrAdd* adds = nullptr;
   BatchDesc* descs = nullptr;
+  BsrAddSlot* add_slots = nullptr;  // per large add of a height
+  CarryFn* add_agg = nullptr;       // per (large add, chunk)
+  uint64_t* add_cin = nullptr;
 };
 
 void bsr_plan_free(BsrPlan& P) {
@@ -914,9 +949,11 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
   P.hp.assign(P.H + 1, BsrHeight());
   std::vector<BsrMul> muls;
   std::vector<BsrAdd> adds;
+  size_t max_large = 0, max_large_chunks = 0;
   for (int32_t h = 2; h <= P.H; ++h) {
     std::vector<std::pair<int, BsrMul>> ms;
     std::vector<BsrAdd> small, large;
+    uint64_t large_max = 0;
     for (int32_t i : by_h[h]) {
       const BsrNode& n = nodes[i];
       const uint32_t I = 5 * i, Lf = 5 * n.left, Rt = 5 * n.right;
@@ -925,10 +962,12 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
                             {Lf + 1, Rt + 1, I + 1}, {Lf + 2, Rt + 2, I + 2}};
       for (const BsrMul& m : m4) ms.push_back({bsr_class(bnd[m.a] + bnd[m.b]), m});
       const BsrAdd ad{I + 3, I + 4, I + 0};  // P = x.add(y)
-      if (bnd[I] <= 1024)
+      if (bnd[I] <= 1024) {
         small.push_back(ad);
-      else
+      } else {
         large.push_back(ad);
+        large_max = std::max(large_max, bnd[I]);
+      }
     }
     std::stable_sort(ms.begin(), ms.end(),
                      [](const std::pair<int, BsrMul>& x, const std::pair<int, BsrMul>& y) {
@@ -946,12 +985,16 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
     hh.add0 = adds.size();
     hh.nsmall = small.size();
     hh.nlarge = large.size();
+    hh.add_maxch = (uint32_t)((large_max + kCarryBlock - 1) / kCarryBlock);
+    max_large = std::max(max_large, hh.nlarge);
+    max_large_chunks = std::max(max_large_chunks, hh.nlarge * hh.add_maxch);
     adds.insert(adds.end(), small.begin(), small.end());
     adds.insert(adds.end(), large.begin(), large.end());
   }
   const size_t need = nv * sizeof(BsrVal) + leaf_rel.size() * 4 + leaf_ids.size() * 4 +
                       muls.size() * sizeof(BsrMul) + adds.size() * sizeof(BsrAdd) +
-                      muls.size() * sizeof(BatchDesc) + 8 * 256;
+                      muls.size() * sizeof(BatchDesc) + max_large * sizeof(BsrAddSlot) +
+                      max_large_chunks * (sizeof(CarryFn) + sizeof(uint64_t)) + 11 * 256;
   if (hipMalloc(&P.buf, need) != hipSuccess || hipMalloc(&P.arena, P.arena_limbs * 4) != hipSuccess) {
     (void)hipGetLastError();
     bsr_plan_free(P);
@@ -965,6 +1008,9 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
   P.muls = carve<BsrMul>(p, muls.size());
   P.adds = carve<BsrAdd>(p, adds.size());
   P.descs = carve<BatchDesc>(p, muls.size());
+  P.add_slots = carve<BsrAddSlot>(p, max_large);
+  P.add_agg = carve<CarryFn>(p, max_large_chunks);
+  P.add_cin = carve<uint64_t>(p, max_large_chunks);
   HIPCHK(hipMemcpyAsync(P.meta, meta.data(), nv * sizeof(BsrVal), hipMemcpyHostToDevice, c.stream));
   HIPCHK(hipMemcpyAsync(P.leaf_rel, leaf_rel.data(), leaf_rel.size() * 4, hipMemcpyHostToDevice, c.stream));
   HIPCHK(hipMemcpyAsync(P.leaf_ids, leaf_ids.data(), leaf_ids.size() * 4, hipMemcpyHostToDevice, c.stream));
@@ -981,7 +1027,10 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
 BsrPlan g_bsr_plans[2];
 int g_bsr_next = 0;
 
-int bsr_exact_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, pcg_bsr_value* out[3]) {
+// Evaluates the subtree [a, b) into the plan's arena; rv receives the
+// metadata of P, Q, R (their limbs stay in (*Pout)->arena until the next call).
+int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const uint64_t* caps,
+                    BsrPlan** Pout, BsrVal rv[3]) {
   if (b <= a) return fail(PCG_ERR_INVALID, "empty range");
   const uint32_t d = b - a;
   BsrPlan* P = nullptr;
@@ -998,8 +1047,9 @@ int bsr_exact_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, pcg_bsr
   if (P->maxb > max_limbs)
     return fail(PCG_ERR_TOO_LARGE, "subtree values may reach %llu limbs > %llu",
                 (unsigned long long)P->maxb, (unsigned long long)max_limbs);
-  for (int j = 0; j < 3; ++j)
-    if (out[j]->cap < P->root_bound[j]) return fail(PCG_ERR_TOO_LARGE, "result buffer too small");
+  if (caps)
+    for (int j = 0; j < 3; ++j)
+      if (caps[j] < P->root_bound[j]) return fail(PCG_ERR_TOO_LARGE, "result buffer too small");
   hipLaunchKernelGGL(k_bsr_leaves, dim3((unsigned)((P->nleaves + 255) / 256)), dim3(256), 0, c.stream,
                      P->leaf_rel, a, P->leaf_ids, P->nleaves, P->meta, P->arena);
   for (int32_t h = 2; h <= P->H; ++h) {
@@ -1027,14 +1077,39 @@ int bsr_exact_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, pcg_bsr
     if (hh.nsmall)
       hipLaunchKernelGGL(k_bsr_add<64>, dim3((unsigned)hh.nsmall), dim3(64), 0, c.stream,
                          P->adds + hh.add0, P->meta, P->arena);
-    if (hh.nlarge)
-      hipLaunchKernelGGL(k_bsr_add<256>, dim3((unsigned)hh.nlarge), dim3(256), 0, c.stream,
-                         P->adds + hh.add0 + hh.nsmall, P->meta, P->arena);
+    if (hh.nlarge) {
+      const BsrAdd* la = P->adds + hh.add0 + hh.nsmall;
+      const uint64_t n = hh.nlarge;
+      const uint32_t mc = hh.add_maxch;
+      const dim3 grid(mc, (unsigned)n);
+      hipLaunchKernelGGL(k_bsr_add_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, n,
+                         P->add_slots);
+      hipLaunchKernelGGL(k_bsr_add_cmp, grid, dim3(256), 0, c.stream, la, P->meta, P->arena, P->add_slots);
+      hipLaunchKernelGGL(k_bsr_add_local, grid, dim3(kCarryThreads), 0, c.stream, la, P->meta, P->arena,
+                         P->add_slots, P->add_agg, mc);
+      hipLaunchKernelGGL(k_bsr_add_scan, dim3((unsigned)n), dim3(64), 0, c.stream, la, P->meta, P->arena,
+                         P->add_slots, P->add_agg, P->add_cin, mc);
+      hipLaunchKernelGGL(k_bsr_add_fix, dim3((unsigned)((n * mc + 255) / 256)), dim3(256), 0, c.stream, la,
+                         P->meta, P->arena, P->add_slots, P->add_cin, mc, n);
+      hipLaunchKernelGGL(k_bsr_add_topnz, grid, dim3(256), 0, c.stream, la, P->meta, P->arena,
+                         P->add_slots);
+      hipLaunchKernelGGL(k_bsr_add_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, la,
+                         P->meta, P->arena, P->add_slots, n);
+    }
   }
   HIPCHK(hipGetLastError());
-  BsrVal rv[3];
   HIPCHK(hipMemcpyAsync(rv, P->meta, 3 * sizeof(BsrVal), hipMemcpyDeviceToHost, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
+  *Pout = P;
+  return PCG_OK;
+}
+
+int bsr_exact_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, pcg_bsr_value* out[3]) {
+  const uint64_t caps[3] = {out[0]->cap, out[1]->cap, out[2]->cap};
+  BsrPlan* P = nullptr;
+  BsrVal rv[3];
+  int rc = bsr_eval_locked(c, a, b, max_limbs, caps, &P, rv);
+  if (rc) return rc;
   for (int j = 0; j < 3; ++j) {
     out[j]->L = rv[j].L;
     out[j]->sign = rv[j].sign;
@@ -1055,6 +1130,311 @@ int sync_locked(Ctx& c) {
     HIPCHK(hipMemcpy(c.status, &z, sizeof z, hipMemcpyHostToDevice));
     return fail(PCG_ERR_INTERNAL, "carry propagation overflowed (status %u)", st);
   }
+  return PCG_OK;
+}
+
+// ------------------------------------------------------------------------
+// Device-resident BigFloat values (see df_kernels.hpp)
+constexpr uint64_t kExtraPrecision = 2;  // BigFloat.h:6
+
+// Pool of limb buffers: capacities are 2^k or 3 * 2^(k-1) limbs (at least
+// 256), and released buffers are kept for reuse.  All work runs on the one
+// library stream, so a buffer may be handed out again as soon as its last
+// use is enqueued.
+std::multimap<uint64_t, uint32_t*> g_df_free;
+
+uint64_t df_round(uint64_t n) {
+  uint64_t c = 256;
+  while (c < n) {
+    if (c + c / 2 >= n) return c + c / 2;
+    c <<= 1;
+  }
+  return c;
+}
+
+int df_release_pool(Ctx& c) {
+  if (g_df_free.empty()) return PCG_OK;
+  HIPCHK(hipStreamSynchronize(c.stream));
+  for (auto& kv : g_df_free) HIPCHK(hipFree(kv.second));
+  g_df_free.clear();
+  return PCG_OK;
+}
+
+int df_alloc(Ctx& c, uint64_t n, uint32_t** p, uint64_t* cap) {
+  const uint64_t want = df_round(n);
+  auto it = g_df_free.find(want);
+  if (it != g_df_free.end()) {
+    *p = it->second;
+    *cap = want;
+    g_df_free.erase(it);
+    return PCG_OK;
+  }
+  if (hipMalloc(p, want * sizeof(uint32_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    int rc = df_release_pool(c);
+    if (rc) return rc;
+    if (hipMalloc(p, want * sizeof(uint32_t)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PCG_ERR_OOM, "cannot allocate a value of %llu limbs", (unsigned long long)want);
+    }
+  }
+  *cap = want;
+  return PCG_OK;
+}
+
+void df_put_back(uint32_t* p, uint64_t cap) {
+  if (p) g_df_free.emplace(cap, p);
+}
+
+// z takes the new buffer and metadata; its old buffer goes back to the pool.
+void df_assign(pcg_dfloat* z, uint32_t* buf, uint64_t cap, uint64_t L, int64_t exp, int32_t sign) {
+  df_put_back(z->limbs, z->cap);
+  z->limbs = buf;
+  z->cap = cap;
+  z->L = L;
+  z->exp = exp;
+  z->sign = sign;
+}
+
+void df_assign_zero(pcg_dfloat* z, int32_t sign = 1) {
+  df_put_back(z->limbs, z->cap);
+  z->limbs = nullptr;
+  z->cap = 0;
+  z->L = 0;
+  z->exp = 0;
+  z->sign = sign;
+}
+
+DfOperand df_opnd(const pcg_dfloat& x) {
+  DfOperand o;
+  o.t = x.limbs;
+  o.exp = x.exp;
+  o.L = x.L;
+  return o;
+}
+
+int df_ensure_blocks(Ctx& c, uint64_t nblk) {
+  if (nblk <= c.df_nblk_cap) return PCG_OK;
+  HIPCHK(hipStreamSynchronize(c.stream));
+  if (c.df_agg) HIPCHK(hipFree(c.df_agg));
+  if (c.df_cin) HIPCHK(hipFree(c.df_cin));
+  c.df_agg = nullptr;
+  c.df_cin = nullptr;
+  c.df_nblk_cap = 0;
+  HIPCHK(hipMalloc(&c.df_agg, nblk * sizeof(CarryFn)));
+  HIPCHK(hipMalloc(&c.df_cin, nblk * sizeof(uint64_t)));
+  c.df_nblk_cap = nblk;
+  return PCG_OK;
+}
+
+// Copies `n` slots to the pinned host mirror and waits for the stream.
+int df_read_slots(Ctx& c, int n) {
+  HIPCHK(hipMemcpyAsync(c.h_slot, c.df_slot, n * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return PCG_OK;
+}
+
+int df_read_limb(Ctx& c, const uint32_t* p, uint32_t* v) {
+  HIPCHK(hipMemcpyAsync(c.h_slot, p, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  memcpy(v, c.h_slot, sizeof(uint32_t));
+  return PCG_OK;
+}
+
+// Block pass OP over a.n limbs into a.out, block-function scan, fix-up;
+// the carry out of the top lands in df_slot[0].
+template <int OP>
+int df_carry_pass(Ctx& c, DfArgs a) {
+  const uint64_t nblk = (a.n + kCarryBlock - 1) / kCarryBlock;
+  if (nblk == 0) {
+    hipLaunchKernelGGL(k_df_put_u64, dim3(1), dim3(1), 0, c.stream, c.df_slot, 0ull);
+    HIPCHK(hipGetLastError());
+    return PCG_OK;
+  }
+  int rc = df_ensure_blocks(c, nblk);
+  if (rc) return rc;
+  a.agg = c.df_agg;
+  hipLaunchKernelGGL(k_df_local<OP>, dim3((unsigned)nblk), dim3(kCarryThreads), 0, c.stream, a);
+  CarryArgs ca;
+  memset(&ca, 0, sizeof ca);
+  ca.zL = a.n;
+  ca.out = a.out;
+  ca.agg = c.df_agg;
+  ca.cin = c.df_cin;
+  ca.nblk = nblk;
+  ca.carry_out = reinterpret_cast<uint64_t*>(c.df_slot);
+  hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, c.stream, ca);
+  hipLaunchKernelGGL(k_carry_fix, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, c.stream, ca);
+  HIPCHK(hipGetLastError());
+  return PCG_OK;
+}
+
+// BigFloat::ucmp (BigFloat.cpp:123-147): magnitudes exp + L first, then the
+// highest position where the aligned words differ.
+int df_ucmp_locked(Ctx& c, const pcg_dfloat& A, const pcg_dfloat& B, int* cmp) {
+  const int64_t magA = A.exp + (int64_t)A.L, magB = B.exp + (int64_t)B.L;
+  if (magA != magB) {
+    *cmp = magA > magB ? 1 : -1;
+    return PCG_OK;
+  }
+  const int64_t lo = A.exp < B.exp ? A.exp : B.exp;
+  if (magA <= lo) {
+    *cmp = 0;
+    return PCG_OK;
+  }
+  const uint64_t n = (uint64_t)(magA - lo);
+  hipLaunchKernelGGL(k_df_put_u64, dim3(1), dim3(1), 0, c.stream, c.df_slot + 2, 0ull);
+  hipLaunchKernelGGL(k_df_cmp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, df_opnd(A),
+                     df_opnd(B), lo, n, c.df_slot + 2);
+  hipLaunchKernelGGL(k_df_cmp_final, dim3(1), dim3(1), 0, c.stream, df_opnd(A), df_opnd(B), lo,
+                     c.df_slot + 2, reinterpret_cast<int64_t*>(c.df_slot + 3));
+  HIPCHK(hipGetLastError());
+  int rc = df_read_slots(c, 4);
+  if (rc) return rc;
+  *cmp = (int)(int64_t)c.h_slot[3];
+  return PCG_OK;
+}
+
+// BigFloat::uadd / usub (BigFloat.cpp:184-232, 233-287): positions
+// [bot, top) aligned by exponent, the bottom cut so at most p + 2 limbs
+// remain (p = 0: exact); uadd appends a carry limb, usub strips every
+// leading zero limb.  usub requires |A| >= |B|.
+int df_uaddsub_locked(Ctx& c, pcg_dfloat* z, const pcg_dfloat A, const pcg_dfloat B, uint64_t p,
+                      bool sub, int32_t sign) {
+  const int64_t magA = A.exp + (int64_t)A.L, magB = B.exp + (int64_t)B.L;
+  const int64_t top = magA > magB ? magA : magB;
+  int64_t bot = A.exp < B.exp ? A.exp : B.exp;
+  int64_t TL = top - bot;
+  const uint64_t pp = p == 0 ? (uint64_t)TL : p + kExtraPrecision;
+  if (TL > (int64_t)pp) {
+    bot = top - (int64_t)pp;
+    TL = (int64_t)pp;
+  }
+  uint32_t* buf = nullptr;
+  uint64_t cap = 0;
+  int rc = df_alloc(c, (uint64_t)TL + 1, &buf, &cap);
+  if (rc) return rc;
+  DfArgs a;
+  memset(&a, 0, sizeof a);
+  a.a = df_opnd(A);
+  a.b = df_opnd(B);
+  a.bot = bot;
+  a.n = (uint64_t)TL;
+  a.out = buf;
+  rc = sub ? df_carry_pass<kDfSub>(c, a) : df_carry_pass<kDfAdd>(c, a);
+  if (rc) {
+    df_put_back(buf, cap);
+    return rc;
+  }
+  uint64_t L = (uint64_t)TL;
+  int64_t exp = bot;
+  if (!sub) {
+    rc = df_read_slots(c, 1);
+    if (rc) return rc;
+    if (c.h_slot[0]) {
+      hipLaunchKernelGGL(k_df_put_u32, dim3(1), dim3(1), 0, c.stream, buf, L, 1u);
+      ++L;
+    }
+  } else {
+    hipLaunchKernelGGL(k_df_put_u64, dim3(1), dim3(1), 0, c.stream, c.df_slot + 1, 0ull);
+    if (L)
+      hipLaunchKernelGGL(k_df_topnz, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, c.stream, buf, L,
+                         c.df_slot + 1);
+    rc = df_read_slots(c, 2);
+    if (rc) return rc;
+    L = c.h_slot[1];
+    if (L == 0) {
+      exp = 0;
+      sign = 1;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  df_assign(z, buf, cap, L, exp, sign);
+  return PCG_OK;
+}
+
+// BigFloat::add / sub (BigFloat.cpp:288-325).
+int df_add_locked(Ctx& c, pcg_dfloat* z, const pcg_dfloat* pa, const pcg_dfloat* pb, uint64_t p,
+                  bool subtract) {
+  const pcg_dfloat A = *pa, B = *pb;
+  const bool same = (A.sign != 0) == (B.sign != 0);
+  if (same != subtract) return df_uaddsub_locked(c, z, A, B, p, false, A.sign);
+  int cmp = 0;
+  int rc = df_ucmp_locked(c, A, B, &cmp);
+  if (rc) return rc;
+  if (cmp > 0) return df_uaddsub_locked(c, z, A, B, p, true, A.sign);
+  rc = df_uaddsub_locked(c, z, B, A, p, true, B.sign);
+  if (rc) return rc;
+  if (subtract && z->L) z->sign = !z->sign;  // negate (BigFloat.cpp:150-155)
+  return PCG_OK;
+}
+
+// BigFloat::mul (BigFloat.cpp:326-444): operands cut to their top p + 2
+// limbs (p = 0: exact), full product, one zero top limb trimmed.
+int df_mul_locked(Ctx& c, pcg_dfloat* z, const pcg_dfloat* pa, const pcg_dfloat* pb, uint64_t p,
+                  uint32_t flags) {
+  const pcg_dfloat A = *pa, B = *pb;
+  if (A.L == 0 || B.L == 0) {
+    df_assign_zero(z);
+    return PCG_OK;
+  }
+  const uint64_t pp = p == 0 ? A.L + B.L : p + kExtraPrecision;
+  int64_t Aexp = A.exp, Bexp = B.exp;
+  uint64_t AL = A.L, BL = B.L;
+  const uint32_t* AT = A.limbs;
+  const uint32_t* BT = B.limbs;
+  if (AL > pp) {
+    AT += AL - pp;
+    Aexp += (int64_t)(AL - pp);
+    AL = pp;
+  }
+  if (BL > pp) {
+    BT += BL - pp;
+    Bexp += (int64_t)(BL - pp);
+    BL = pp;
+  }
+  uint32_t* buf = nullptr;
+  uint64_t cap = 0;
+  int rc = df_alloc(c, AL + BL, &buf, &cap);
+  if (rc) return rc;
+  rc = mul_dev_locked(c, AT, AL, BT, BL, buf, flags & PCG_ENGINE_MASK);
+  uint32_t topl = 0;
+  if (!rc) rc = df_read_limb(c, buf + AL + BL - 1, &topl);
+  if (!rc) rc = sync_locked(c);
+  if (rc) {
+    df_put_back(buf, cap);
+    return rc;
+  }
+  df_assign(z, buf, cap, AL + BL - (topl == 0 ? 1 : 0), Aexp + Bexp, (A.sign != 0) == (B.sign != 0));
+  return PCG_OK;
+}
+
+// BigFloat::mul(uint32_t) (BigFloat.cpp:156-183).
+int df_mul_u32_locked(Ctx& c, pcg_dfloat* z, const pcg_dfloat* pa, uint32_t x) {
+  const pcg_dfloat A = *pa;
+  if (A.L == 0 || x == 0) {
+    df_assign_zero(z);
+    return PCG_OK;
+  }
+  uint32_t* buf = nullptr;
+  uint64_t cap = 0;
+  int rc = df_alloc(c, A.L + 1, &buf, &cap);
+  if (rc) return rc;
+  DfArgs a;
+  memset(&a, 0, sizeof a);
+  a.a = df_opnd(A);
+  a.n = A.L + 1;
+  a.x = x;
+  a.out = buf;
+  rc = df_carry_pass<kDfMulU32>(c, a);
+  uint32_t topl = 0;
+  if (!rc) rc = df_read_limb(c, buf + A.L, &topl);
+  if (rc) {
+    df_put_back(buf, cap);
+    return rc;
+  }
+  df_assign(z, buf, cap, A.L + (topl ? 1 : 0), A.exp, A.sign);
   return PCG_OK;
 }
 
@@ -1081,6 +1461,8 @@ int pcg_init(int device) {
   if (rc) return rc;
   HIPCHK(hipMalloc(&c.status, sizeof(uint32_t)));
   HIPCHK(hipMemset(c.status, 0, sizeof(uint32_t)));
+  HIPCHK(hipMalloc(&c.df_slot, 4 * sizeof(unsigned long long)));
+  HIPCHK(hipHostMalloc(&c.h_slot, 4 * sizeof(unsigned long long), hipHostMallocDefault));
   for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&c.ev[i]));
   for (int i = 0; i < 2; ++i) HIPCHK(hipEventCreate(&c.tev[i]));
   c.device = device;
@@ -1108,7 +1490,14 @@ void pcg_shutdown(void) {
   (void)hipHostFree(c.h_desc);
   (void)hipFree(c.d_desc);
   (void)hipFree(c.bspec);
+  (void)hipFree(c.bagg);
+  (void)hipFree(c.bcin);
   for (BsrPlan& q : g_bsr_plans) bsr_plan_free(q);
+  (void)df_release_pool(c);
+  (void)hipFree(c.df_agg);
+  (void)hipFree(c.df_cin);
+  (void)hipFree(c.df_slot);
+  (void)hipHostFree(c.h_slot);
   for (int i = 0; i < 4; ++i) (void)hipEventDestroy(c.ev[i]);
   for (int i = 0; i < 2; ++i) (void)hipEventDestroy(c.tev[i]);
   (void)hipStreamDestroy(c.stream);
@@ -1132,8 +1521,16 @@ void pcg_shutdown(void) {
   c.desc_cap = 0;
   c.bspec = nullptr;
   c.bspec_doubles = 0;
+  c.bagg = nullptr;
+  c.bcin = nullptr;
+  c.bcarry_cap = 0;
   c.d_prime = nullptr;
   c.prof_valid = false;
+  c.df_agg = nullptr;
+  c.df_cin = nullptr;
+  c.df_nblk_cap = 0;
+  c.df_slot = nullptr;
+  c.h_slot = nullptr;
   (void)fresh_defaults;
 }
 
@@ -1316,6 +1713,164 @@ int pcg_ntt_plan(uint64_t total_limbs, int* log2_points, int* passes) {
   if (log2_points) *log2_points = logN;
   if (passes) *passes = (int)pl.passes.size();
   return PCG_OK;
+}
+
+// ---------------------------------------------------------------- device values
+#define DF_ENTER()                                                        \
+  Ctx& c = g_ctx;                                                         \
+  std::lock_guard<std::mutex> lk(c.mu);                                   \
+  if (!c.ready) return fail(PCG_ERR_NOT_INIT, "pcg_init has not been called")
+
+int pcg_df_free(pcg_dfloat* x) {
+  if (!x) return fail(PCG_ERR_INVALID, "null pointer");
+  DF_ENTER();
+  df_assign_zero(x);
+  return PCG_OK;
+}
+
+int pcg_df_set(pcg_dfloat* x, const uint32_t* limbs, uint64_t L, int64_t exp, int32_t sign) {
+  if (!x || (L && !limbs)) return fail(PCG_ERR_INVALID, "null pointer");
+  DF_ENTER();
+  if (L == 0) {
+    df_assign_zero(x, sign ? 1 : 0);
+    x->exp = exp;
+    return PCG_OK;
+  }
+  uint32_t* buf = nullptr;
+  uint64_t cap = 0;
+  int rc = df_alloc(c, L, &buf, &cap);
+  if (rc) return rc;
+  if (hipMemcpyAsync(buf, limbs, L * 4, hipMemcpyHostToDevice, c.stream) != hipSuccess ||
+      hipStreamSynchronize(c.stream) != hipSuccess) {
+    df_put_back(buf, cap);
+    return fail(PCG_ERR_HIP, "value upload failed: %s", hipGetErrorString(hipGetLastError()));
+  }
+  df_assign(x, buf, cap, L, exp, sign ? 1 : 0);
+  return PCG_OK;
+}
+
+int pcg_df_read(const pcg_dfloat* x, uint64_t first, uint64_t count, uint32_t* out) {
+  if (!x || (count && !out)) return fail(PCG_ERR_INVALID, "null pointer");
+  if (first > x->L || count > x->L - first) return fail(PCG_ERR_INVALID, "range beyond the value");
+  DF_ENTER();
+  if (count == 0) return PCG_OK;
+  HIPCHK(hipMemcpyAsync(out, x->limbs + first, count * 4, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return PCG_OK;
+}
+
+int pcg_df_copy(pcg_dfloat* z, const pcg_dfloat* x) {
+  if (!z || !x) return fail(PCG_ERR_INVALID, "null pointer");
+  if (z == x) return PCG_OK;
+  DF_ENTER();
+  const pcg_dfloat X = *x;
+  if (X.L == 0) {
+    df_assign_zero(z, X.sign);
+    z->exp = X.exp;
+    return PCG_OK;
+  }
+  uint32_t* buf = nullptr;
+  uint64_t cap = 0;
+  int rc = df_alloc(c, X.L, &buf, &cap);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(buf, X.limbs, X.L * 4, hipMemcpyDeviceToDevice, c.stream));
+  df_assign(z, buf, cap, X.L, X.exp, X.sign);
+  return PCG_OK;
+}
+
+int pcg_df_ucmp(const pcg_dfloat* a, const pcg_dfloat* b, int* cmp) {
+  if (!a || !b || !cmp) return fail(PCG_ERR_INVALID, "null pointer");
+  DF_ENTER();
+  return df_ucmp_locked(c, *a, *b, cmp);
+}
+
+int pcg_df_add(pcg_dfloat* z, const pcg_dfloat* a, const pcg_dfloat* b, uint64_t p) {
+  if (!z || !a || !b) return fail(PCG_ERR_INVALID, "null pointer");
+  DF_ENTER();
+  return df_add_locked(c, z, a, b, p, false);
+}
+
+int pcg_df_sub(pcg_dfloat* z, const pcg_dfloat* a, const pcg_dfloat* b, uint64_t p) {
+  if (!z || !a || !b) return fail(PCG_ERR_INVALID, "null pointer");
+  DF_ENTER();
+  return df_add_locked(c, z, a, b, p, true);
+}
+
+int pcg_df_mul(pcg_dfloat* z, const pcg_dfloat* a, const pcg_dfloat* b, uint64_t p, uint32_t flags) {
+  if (!z || !a || !b) return fail(PCG_ERR_INVALID, "null pointer");
+  DF_ENTER();
+  return df_mul_locked(c, z, a, b, p, flags);
+}
+
+int pcg_df_mul_u32(pcg_dfloat* z, const pcg_dfloat* a, uint32_t x) {
+  if (!z || !a) return fail(PCG_ERR_INVALID, "null pointer");
+  DF_ENTER();
+  return df_mul_u32_locked(c, z, a, x);
+}
+
+int pcg_df_to_hex(const pcg_dfloat* x, uint64_t digits, char* out, uint64_t cap, uint64_t* written) {
+  if (!x || !out || !written) return fail(PCG_ERR_INVALID, "null pointer");
+  DF_ENTER();
+  const pcg_dfloat X = *x;
+  const uint64_t nl = (digits + 7) / 8;
+  if (X.L == 0 || nl > X.L - 1) return fail(PCG_ERR_INVALID, "Limit Exceeded");
+  uint32_t topl = 0;
+  int rc = df_read_limb(c, X.limbs + X.L - 1, &topl);
+  if (rc) return rc;
+  char head[16];
+  const int hn = snprintf(head, sizeof head, "%u.", topl);
+  const uint64_t total = (uint64_t)hn + 8 * nl;
+  const uint64_t want = 2 + digits < total ? 2 + digits : total;
+  if (cap < want + 1) return fail(PCG_ERR_TOO_LARGE, "output buffer too small");
+  const uint64_t nhex = want > (uint64_t)hn ? want - (uint64_t)hn : 0;
+  memcpy(out, head, want < (uint64_t)hn ? want : (uint64_t)hn);
+  if (nhex) {
+    uint32_t* buf = nullptr;
+    uint64_t bcap = 0;
+    rc = df_alloc(c, (nhex + 3) / 4, &buf, &bcap);
+    if (rc) return rc;
+    char* dch = reinterpret_cast<char*>(buf);
+    hipLaunchKernelGGL(k_df_hex, dim3((unsigned)((nhex + 255) / 256)), dim3(256), 0, c.stream, X.limbs,
+                       X.L, nhex, dch);
+    const bool ok = hipGetLastError() == hipSuccess &&
+                    hipMemcpyAsync(out + hn, dch, nhex, hipMemcpyDeviceToHost, c.stream) == hipSuccess &&
+                    hipStreamSynchronize(c.stream) == hipSuccess;
+    df_put_back(buf, bcap);
+    if (!ok) return fail(PCG_ERR_HIP, "hex conversion failed");
+  }
+  out[want] = 0;
+  *written = want;
+  return PCG_OK;
+}
+
+int pcg_df_bsr_exact(uint32_t a, uint32_t b, uint64_t max_limbs, pcg_dfloat* P, pcg_dfloat* Q,
+                     pcg_dfloat* R) {
+  if (!P || !Q || !R) return fail(PCG_ERR_INVALID, "null pointer");
+  DF_ENTER();
+  BsrPlan* plan = nullptr;
+  BsrVal rv[3];
+  int rc = bsr_eval_locked(c, a, b, max_limbs, nullptr, &plan, rv);
+  if (rc) return rc;
+  pcg_dfloat* out[3] = {P, Q, R};
+  for (int j = 0; j < 3; ++j) {
+    if (rv[j].L == 0) {
+      df_assign_zero(out[j]);
+      continue;
+    }
+    uint32_t* buf = nullptr;
+    uint64_t cap = 0;
+    rc = df_alloc(c, rv[j].L, &buf, &cap);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(buf, plan->arena + rv[j].off, rv[j].L * 4, hipMemcpyDeviceToDevice,
+                          c.stream));
+    df_assign(out[j], buf, cap, rv[j].L, 0, rv[j].sign);
+  }
+  return PCG_OK;
+}
+
+int pcg_df_release_pool(void) {
+  DF_ENTER();
+  return df_release_pool(c);
 }
 
 }  // extern "C"

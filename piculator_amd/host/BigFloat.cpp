@@ -11,6 +11,7 @@
 #include <sstream>
 
 #include "../../include/picugpu.h"
+#include "Newton.h"
 
 namespace piculator {
 
@@ -35,7 +36,9 @@ void gpu_product(const uint32_t* a, size_t na, const uint32_t* b, size_t nb, uin
 }  // namespace
 
 void set_mul_engine_flags(uint32_t flags) { g_flags = flags; }
+uint32_t mul_engine_flags() { return g_flags; }
 uint64_t big_mul_count() { return g_mul_count.load(); }
+void count_big_muls(uint64_t n) { g_mul_count.fetch_add(n, std::memory_order_relaxed); }
 
 // ------------------------------------------------ construction (BigFloat.cpp:14-79)
 BigFloat::BigFloat(BigFloat&& x) noexcept : sign(x.sign), exp(x.exp), L(x.L), T(std::move(x.T)) {
@@ -299,10 +302,9 @@ void MulBatch::run(bool plain) {
 }
 
 // ------------------------------------------------ Newton iterations (BigFloat.cpp:445-567)
-namespace {
 // Two-limb float seed (value 2^32 <= v < 2^64 scaled by exponent e), as the
 // p == 0 branches of rcp and invsqrt build it.
-BigFloat seed_from_double(double val, int64_t e, bool sign) {
+BigFloat BigFloat::seed(double val, int64_t e, bool sign) {
   while (val < 4294967296.) {
     val *= 4294967296.;
     --e;
@@ -311,52 +313,30 @@ BigFloat seed_from_double(double val, int64_t e, bool sign) {
   const uint32_t limbs[2] = {(uint32_t)v, (uint32_t)(v >> 32)};
   return BigFloat::from_parts(sign, e, limbs, 2);
 }
-size_t half_precision(size_t p) {
-  if (p == 1) return 0;
-  if (p == 2) return 1;
-  return p / 2 + 1;
-}
-}  // namespace
 
-BigFloat BigFloat::rcp(size_t p, int threads) const {
-  if (L == 0) throw "Divide by Zero";
-  if (p == 0) {
-    // top three limbs -> double -> 1/x
-    size_t al = L;
-    int64_t ae = exp;
-    const uint32_t* at = T.get();
-    if (al > 3) {
-      at += al - 3;
-      ae += (int64_t)(al - 3);
-      al = 3;
-    }
-    double val = at[0];
-    if (al >= 2) val += at[1] * 4294967296.;
-    if (al >= 3) val += at[2] * 4294967296. * 4294967296.;
-    val = 1. / val;
-    return seed_from_double(val, -ae, sign);
+// Top three limbs as a double, BigFloat.cpp:451-464.
+double BigFloat::top_double(int64_t* e) const {
+  size_t al = L;
+  int64_t ae = exp;
+  const uint32_t* at = T.get();
+  if (al > 3) {
+    at += al - 3;
+    ae += (int64_t)(al - 3);
+    al = 3;
   }
-  const BigFloat r0 = rcp(half_precision(p), threads);
-  // r1 = r0 - (r0 * x - 1) * r0
-  return r0.sub(this->mul(r0, p, threads).sub(BigFloat(1), p).mul(r0, p, threads), p);
+  double val = at[0];
+  if (al >= 2) val += at[1] * 4294967296.;
+  if (al >= 3) val += at[2] * 4294967296. * 4294967296.;
+  *e = ae;
+  return val;
 }
+
+BigFloat BigFloat::rcp(size_t p, int threads) const { return newton_rcp(*this, p); }
 
 BigFloat BigFloat::div(const BigFloat& x, size_t p, int threads) const {
   return this->mul(x.rcp(p, threads), p, threads);
 }
 
-BigFloat invsqrt(uint32_t x, size_t p, int threads) {
-  if (x == 0) throw "Divide by Zero";
-  if (p == 0) return seed_from_double(1. / std::sqrt((double)x), 0, true);
-  const BigFloat r0 = invsqrt(x, half_precision(p), threads);
-  // r1 = r0 - (r0^2 * x - 1) / 2 * r0
-  BigFloat t = r0.mul(r0, p, threads);
-  t = t.mul(BigFloat(x), 0, threads);
-  t = t.sub(BigFloat(1), p);
-  t = t.mul(2147483648u);
-  t.exp--;
-  t = t.mul(r0, p, threads);
-  return r0.sub(t, p);
-}
+BigFloat invsqrt(uint32_t x, size_t p, int threads) { return newton_invsqrt<BigFloat>(x, p); }
 
 }  // namespace piculator

@@ -20,7 +20,9 @@ namespace piculator {
 
 // Which engine BigFloat::mul uses.  GPU: the C ABI (pcg_mul); the default.
 void set_mul_engine_flags(uint32_t flags);
+uint32_t mul_engine_flags();
 uint64_t big_mul_count();
+void count_big_muls(uint64_t n);
 
 class MulBatch;
 
@@ -52,6 +54,12 @@ class BigFloat {
   bool get_sign() const { return sign; }
   const uint32_t* limbs() const { return T.get(); }
   static BigFloat from_parts(bool sign, int64_t exp, const uint32_t* limbs, size_t n);
+
+  // Hooks of the Newton iterations (Newton.h): exponent shift, the top three
+  // limbs as a double (e: exponent of the lowest of them), two-limb seed.
+  void shift_exp(int64_t d) { exp += d; }
+  double top_double(int64_t* e) const;
+  static BigFloat seed(double val, int64_t e, bool sign);
 
  private:
   bool sign;     // true = positive or zero

@@ -11,6 +11,7 @@
 
 #include "../../include/picugpu.h"
 #include "BigFloat.h"
+#include "DeviceFloat.h"
 
 namespace piculator {
 
@@ -154,46 +155,59 @@ void bsr_subtree_batched(PQR& out, uint32_t a, uint32_t b, size_t p, bool verbos
 // Largest subtree handed to the GPU-resident evaluation at once.
 constexpr uint32_t kDeviceLeaves = 1u << 20;
 
-// Subtree [a, b) entirely on the GPU (pcg_bsr_exact) when no value of it can
-// exceed p + 2 limbs; false if it could (or the call fails for size).
-bool bsr_device(PQR& out, uint32_t a, uint32_t b, size_t p) {
+// Values of a node kept in device memory.
+struct DPQR {
+  DeviceFloat P, Q, R;
+};
+
+// Subtree [a, b) evaluated on the GPU, results left in device memory; false
+// when a value of it could exceed p + 2 limbs.
+bool bsr_device_resident(DPQR& out, uint32_t a, uint32_t b, size_t p) {
   uint64_t bound = 0;
   if (pcg_bsr_bound(a, b, &bound) != PCG_OK || bound > p + YCL_BIGFLOAT_EXTRA_PRECISION) return false;
-  std::vector<uint32_t> buf(3 * (bound + 1));
-  pcg_bsr_value v[3];
-  for (int j = 0; j < 3; ++j) v[j] = pcg_bsr_value{buf.data() + j * (bound + 1), bound + 1, 0, 1};
-  int rc = pcg_bsr_exact(a, b, p + YCL_BIGFLOAT_EXTRA_PRECISION, &v[0], &v[1], &v[2]);
+  const uint64_t mx = p + YCL_BIGFLOAT_EXTRA_PRECISION;
+  int rc = pcg_df_bsr_exact(a, b, mx, &out.P.raw(), &out.Q.raw(), &out.R.raw());
   if (rc == PCG_ERR_NOT_INIT && pcg_init(0) == PCG_OK)
-    rc = pcg_bsr_exact(a, b, p + YCL_BIGFLOAT_EXTRA_PRECISION, &v[0], &v[1], &v[2]);
+    rc = pcg_df_bsr_exact(a, b, mx, &out.P.raw(), &out.Q.raw(), &out.R.raw());
   if (rc == PCG_ERR_TOO_LARGE) return false;
   if (rc != PCG_OK) throw pcg_last_error_string();
-  out.P = BigFloat::from_parts(v[0].sign != 0, 0, v[0].limbs, v[0].L);
-  out.Q = BigFloat::from_parts(v[1].sign != 0, 0, v[1].limbs, v[1].L);
-  out.R = BigFloat::from_parts(v[2].sign != 0, 0, v[2].limbs, v[2].L);
   return true;
 }
 
-// Subtrees small enough for the GPU evaluation run there whole; above that,
-// recurse on the halves first, then combine on the host BigFloat (with its
-// truncation rules), the four combine products of a node as one batch.
-void bsr_batched(PQR& out, uint32_t a, uint32_t b, size_t p, bool verbose) {
-  if (b - a <= kDeviceLeaves && bsr_device(out, a, b, p)) return;
+// The series with every value resident in device memory: GPU subtrees
+// below, the node combine of PI.h:95-110 on DeviceFloat above (the same
+// truncation as BigFloat, so the same values).
+void bsr_resident(DPQR& out, uint32_t a, uint32_t b, size_t p, bool verbose) {
+  const double t0 = verbose ? now_s() : 0.0;
+  if (b - a <= kDeviceLeaves && bsr_device_resident(out, a, b, p)) {
+    if (verbose)
+      std::fprintf(stderr, "  device [%u, %u): %.3f s, |P| %zu |Q| %zu |R| %zu limbs\n", a, b,
+                   now_s() - t0, out.P.get_precision(), out.Q.get_precision(),
+                   out.R.get_precision());
+    return;
+  }
   if (b - a <= 64) {
-    bsr_subtree_batched(out, a, b, p, verbose);
+    PQR h;
+    bsr_subtree_batched(h, a, b, p, false);
+    out.P = DeviceFloat(h.P);
+    out.Q = DeviceFloat(h.Q);
+    out.R = DeviceFloat(h.R);
     return;
   }
   const uint32_t m = (a + b) / 2;
-  PQR L, R;
-  bsr_batched(L, a, m, p, verbose);
-  bsr_batched(R, m, b, p, verbose);
-  BigFloat x, y;
-  MulBatch mb;
-  mb.add(L.P, R.Q, p, &x);
-  mb.add(R.P, L.R, p, &y);
-  mb.add(L.Q, R.Q, p, &out.Q);
-  mb.add(L.R, R.R, p, &out.R);
-  mb.run(false);
+  DPQR L, R;
+  bsr_resident(L, a, m, p, verbose);
+  bsr_resident(R, m, b, p, verbose);
+  const double t1 = verbose ? now_s() : 0.0;
+  const DeviceFloat x = L.P.mul(R.Q, p);
+  const DeviceFloat y = R.P.mul(L.R, p);
+  out.Q = L.Q.mul(R.Q, p);
+  out.R = L.R.mul(R.R, p);
+  const double t2 = verbose ? now_s() : 0.0;
   out.P = x.add(y, p);
+  if (verbose)
+    std::fprintf(stderr, "  combine [%u, %u): products %.3f s, add %.3f s, |Q| %zu limbs\n", a, b,
+                 t2 - t1, now_s() - t2, out.Q.get_precision());
 }
 
 }  // namespace
@@ -208,27 +222,46 @@ void pi_sizes(size_t decimal_digits, size_t* hex_digits, size_t* p, size_t* term
   if (terms) *terms = t;
 }
 
-std::string pi_hex(size_t decimal_digits, PiMode mode, PiTimes* times, bool verbose) {
+size_t pi_hex_into(size_t decimal_digits, PiMode mode, char* out, size_t cap, PiTimes* times,
+                   bool verbose) {
   size_t hex, p, terms;
   pi_sizes(decimal_digits, &hex, &p, &terms);
   if ((uint32_t)terms != terms) throw "Limit Exceeded";
   const uint64_t m0 = big_mul_count();
   const double t0 = now_s();
-  PQR s;
-  if (mode == PiMode::Plain)
+  double t1, t2, t3, t4;
+  size_t n = 0;
+  if (mode == PiMode::Plain) {
+    PQR s;
     bsr_plain(s, 0, (uint32_t)terms, p);
-  else
-    bsr_batched(s, 0, (uint32_t)terms, p, verbose);
-  BigFloat P = s.Q.mul(13591409u).add(s.P, p);
-  BigFloat Q = s.Q.mul(4270934400u);
-  const double t1 = now_s();
-  P = Q.div(P, p);
-  const double t2 = now_s();
-  Q = invsqrt(10005, p);
-  const double t3 = now_s();
-  P = P.mul(Q, p);
-  const double t4 = now_s();
-  std::string str = P.to_string_hex(hex);
+    BigFloat P = s.Q.mul(13591409u).add(s.P, p);
+    BigFloat Q = s.Q.mul(4270934400u);
+    t1 = now_s();
+    P = Q.div(P, p);
+    t2 = now_s();
+    Q = invsqrt(10005, p);
+    t3 = now_s();
+    P = P.mul(Q, p);
+    t4 = now_s();
+    const std::string str = P.to_string_hex(hex);
+    if (str.size() + 1 > cap) throw "output capacity too small";
+    std::memcpy(out, str.c_str(), str.size() + 1);
+    n = str.size();
+  } else {
+    DPQR s;
+    bsr_resident(s, 0, (uint32_t)terms, p, verbose);
+    DeviceFloat P = s.Q.mul(13591409u).add(s.P, p);
+    DeviceFloat Q = s.Q.mul(4270934400u);
+    s = DPQR();
+    t1 = now_s();
+    P = Q.div(P, p);
+    t2 = now_s();
+    Q = invsqrt_device(10005, p);
+    t3 = now_s();
+    P = P.mul(Q, p);
+    t4 = now_s();
+    n = P.to_hex_into(hex, out, cap);
+  }
   const double t5 = now_s();
   if (times) {
     times->series = t1 - t0;
@@ -239,7 +272,15 @@ std::string pi_hex(size_t decimal_digits, PiMode mode, PiTimes* times, bool verb
     times->total = t5 - t0;
     times->big_muls = big_mul_count() - m0;
   }
-  return str;
+  return n;
+}
+
+std::string pi_hex(size_t decimal_digits, PiMode mode, PiTimes* times, bool verbose) {
+  size_t hex, p, terms;
+  pi_sizes(decimal_digits, &hex, &p, &terms);
+  std::string s(hex + 32, '\0');
+  s.resize(pi_hex_into(decimal_digits, mode, &s[0], s.size(), times, verbose));
+  return s;
 }
 
 void Pi(size_t decimal_digits, size_t threads) {

@@ -7,6 +7,7 @@
 
 #include "../../include/picugpu.h"
 #include "BigFloat.h"
+#include "DeviceFloat.h"
 #include "Pi.h"
 
 using piculator::BigFloat;
@@ -39,11 +40,21 @@ void pch_set_engine(uint32_t flags) { piculator::set_mul_engine_flags(flags); }
 
 // "3." + hex digits of Pi(digits) (PI.h:138-210); mode 0 plain, 1 batched.
 // times[7]: series, division, invsqrt, final multiply, conversion, total, big multiplies.
+// Hex digits, precision in limbs and series terms of Pi(digits) (PI.h:141-144).
+void pch_pi_sizes(uint64_t digits, uint64_t* hex_digits, uint64_t* p, uint64_t* terms) {
+  size_t h, pp, t;
+  piculator::pi_sizes(digits, &h, &pp, &t);
+  *hex_digits = h;
+  *p = pp;
+  *terms = t;
+}
+
 long pch_pi_hex(uint64_t digits, int mode, char* out, uint64_t cap, double* times, int verbose) {
   try {
     piculator::PiTimes t;
-    const std::string s = piculator::pi_hex(
-        digits, mode == 0 ? piculator::PiMode::Plain : piculator::PiMode::Batched, &t, verbose != 0);
+    const size_t n = piculator::pi_hex_into(
+        digits, mode == 0 ? piculator::PiMode::Plain : piculator::PiMode::Batched, out, cap, &t,
+        verbose != 0);
     if (times) {
       times[0] = t.series;
       times[1] = t.division;
@@ -53,9 +64,7 @@ long pch_pi_hex(uint64_t digits, int mode, char* out, uint64_t cap, double* time
       times[5] = t.total;
       times[6] = (double)t.big_muls;
     }
-    if (s.size() + 1 > cap) return catch_all("output capacity too small");
-    std::memcpy(out, s.c_str(), s.size() + 1);
-    return (long)s.size();
+    return (long)n;
   } catch (const char* e) {
     return catch_all(e);
   }
@@ -81,6 +90,40 @@ int pch_bf_op(int op, int sa, int64_t ea, const uint32_t* la, uint64_t na, int s
       default: return catch_all("bad op");
     }
     return export_bf(z, sz, ez, lz, cap, nz);
+  } catch (const char* e) {
+    return catch_all(e);
+  }
+}
+
+// The same operations on DeviceFloat (values resident in device memory;
+// uploaded, computed on the GPU, downloaded).  op 7: ucmp(a, b) into *sz.
+int pch_df_op(int op, int sa, int64_t ea, const uint32_t* la, uint64_t na, int sb, int64_t eb,
+              const uint32_t* lb, uint64_t nb, uint64_t p, int* sz, int64_t* ez, uint32_t* lz,
+              uint64_t cap, uint64_t* nz) {
+  try {
+    using piculator::DeviceFloat;
+    const DeviceFloat a(BigFloat::from_parts(sa != 0, ea, la, na));
+    const DeviceFloat b(BigFloat::from_parts(sb != 0, eb, lb, nb));
+    DeviceFloat z;
+    switch (op) {
+      case 0: z = a.mul(b, p); break;
+      case 1: z = a.add(b, p); break;
+      case 2: z = a.sub(b, p); break;
+      case 3: z = a.rcp(p); break;
+      case 4: z = a.div(b, p); break;
+      case 5: z = piculator::invsqrt_device(na ? la[0] : 0, p); break;
+      case 6: z = a.mul(nb ? lb[0] : 0u); break;
+      case 7: {
+        int cmp = 0;
+        if (pcg_df_ucmp(&a.raw(), &b.raw(), &cmp) != PCG_OK) return catch_all(pcg_last_error_string());
+        *sz = cmp;
+        *ez = 0;
+        *nz = 0;
+        return 0;
+      }
+      default: return catch_all("bad op");
+    }
+    return export_bf(z.to_host(), sz, ez, lz, cap, nz);
   } catch (const char* e) {
     return catch_all(e);
   }

@@ -33,7 +33,9 @@ EXPORTS = [
     "pcg_mul_dev", "pcg_mul_batch", "pcg_mul_batch_dev", "pcg_sync", "pcg_dev_alloc",
     "pcg_dev_free", "pcg_memcpy_h2d", "pcg_memcpy_d2h", "pcg_fft_last_roundoff",
     "pcg_last_profile", "pcg_ntt_plan", "pcg_timer_start", "pcg_timer_stop",
-    "pcg_bsr_exact", "pcg_bsr_bound",
+    "pcg_bsr_exact", "pcg_bsr_bound", "pcg_df_free", "pcg_df_set", "pcg_df_read", "pcg_df_copy",
+    "pcg_df_ucmp", "pcg_df_add", "pcg_df_sub", "pcg_df_mul", "pcg_df_mul_u32", "pcg_df_to_hex",
+    "pcg_df_bsr_exact", "pcg_df_release_pool",
 ]
 
 
@@ -45,6 +47,12 @@ class PcgError(RuntimeError):
 
 class BsrValue(C.Structure):
     _fields_ = [("limbs", C.c_void_p), ("cap", C.c_uint64), ("L", C.c_uint64), ("sign", C.c_int32)]
+
+
+class DFloat(C.Structure):
+    """pcg_dfloat: a BigFloat whose limbs live in the library's device pool."""
+    _fields_ = [("limbs", C.c_void_p), ("cap", C.c_uint64), ("L", C.c_uint64), ("exp", C.c_int64),
+                ("sign", C.c_int32)]
 
 
 class MulDesc(C.Structure):
@@ -86,6 +94,18 @@ def load():
     L.pcg_timer_stop.argtypes = [C.POINTER(C.c_double)]
     L.pcg_bsr_exact.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64] + [C.POINTER(BsrValue)] * 3
     L.pcg_bsr_bound.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
+    dfp = C.POINTER(DFloat)
+    L.pcg_df_free.argtypes = [dfp]
+    L.pcg_df_set.argtypes = [dfp, C.c_void_p, C.c_uint64, C.c_int64, C.c_int32]
+    L.pcg_df_read.argtypes = [dfp, C.c_uint64, C.c_uint64, C.c_void_p]
+    L.pcg_df_copy.argtypes = [dfp, dfp]
+    L.pcg_df_ucmp.argtypes = [dfp, dfp, C.POINTER(C.c_int)]
+    L.pcg_df_add.argtypes = [dfp, dfp, dfp, C.c_uint64]
+    L.pcg_df_sub.argtypes = [dfp, dfp, dfp, C.c_uint64]
+    L.pcg_df_mul.argtypes = [dfp, dfp, dfp, C.c_uint64, C.c_uint32]
+    L.pcg_df_mul_u32.argtypes = [dfp, dfp, C.c_uint32]
+    L.pcg_df_to_hex.argtypes = [dfp, C.c_uint64, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]
+    L.pcg_df_bsr_exact.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, dfp, dfp, dfp]
     _lib = L
     return L
 
@@ -245,3 +265,25 @@ def bsr_exact(a: int, b: int, max_limbs: int):
     _check(load().pcg_bsr_exact(a, b, max_limbs, C.byref(vals[0]), C.byref(vals[1]),
                                 C.byref(vals[2])))
     return [(bool(v.sign), 0, buf[: v.L].tolist()) for v, buf in zip(vals, bufs)]
+
+
+def df_to_tuple(x: DFloat):
+    """(sign, exp, limbs) of a device value (copied to the host)."""
+    n = int(x.L)
+    out = np.zeros(max(n, 1), dtype=np.uint32)
+    if n:
+        _check(load().pcg_df_read(C.byref(x), 0, n, out.ctypes.data))
+    return (bool(x.sign), int(x.exp), out[:n].tolist())
+
+
+def df_bsr_exact(a: int, b: int, max_limbs: int):
+    """(sign, exp, limbs) of P, Q, R of the subtree [a, b) through
+    pcg_df_bsr_exact (results left in device memory, then read back)."""
+    ensure_init()
+    vals = [DFloat() for _ in range(3)]
+    try:
+        _check(load().pcg_df_bsr_exact(a, b, max_limbs, *[C.byref(v) for v in vals]))
+        return [df_to_tuple(v) for v in vals]
+    finally:
+        for v in vals:
+            load().pcg_df_free(C.byref(v))

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libpiculator.so")
 MODE_PLAIN = 0
 MODE_BATCHED = 1
 
-OPS = {"mul": 0, "add": 1, "sub": 2, "rcp": 3, "div": 4, "invsqrt": 5, "mul_u32": 6}
+OPS = {"mul": 0, "add": 1, "sub": 2, "rcp": 3, "div": 4, "invsqrt": 5, "mul_u32": 6, "ucmp": 7}
 
 _lib = None
 
@@ -34,6 +34,12 @@ def load():
         L.pch_pi_hex.argtypes = [C.c_uint64, C.c_int, C.c_char_p, C.c_uint64,
                                  C.POINTER(C.c_double), C.c_int]
         L.pch_pi_hex.restype = C.c_long
+        L.pch_pi_sizes.argtypes = [C.c_uint64] + [C.POINTER(C.c_uint64)] * 3
+        L.pch_pi_sizes.restype = None
+        L.pch_df_op.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_void_p, C.c_uint64,
+                                C.c_int, C.c_int64, C.c_void_p, C.c_uint64, C.c_uint64,
+                                C.POINTER(C.c_int), C.POINTER(C.c_int64), C.c_void_p,
+                                C.c_uint64, C.POINTER(C.c_uint64)]
         L.pch_bf_op.argtypes = [C.c_int, C.c_int, C.c_int64, C.c_void_p, C.c_uint64,
                                 C.c_int, C.c_int64, C.c_void_p, C.c_uint64, C.c_uint64,
                                 C.POINTER(C.c_int), C.POINTER(C.c_int64), C.c_void_p,
@@ -46,32 +52,58 @@ def set_engine(flags: int):
     load().pch_set_engine(flags)
 
 
-def pi_hex(digits: int, mode: int = MODE_BATCHED, verbose: bool = False):
-    """(pi.txt content, phase times dict) computed with GPU products."""
+PHASES = ["series", "division", "invsqrt", "final_mul", "conversion", "total", "big_muls"]
+
+
+def pi_sizes(digits: int):
+    """(hex digits, precision limbs, series terms) of Pi(digits)."""
+    h, p, t = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    load().pch_pi_sizes(digits, C.byref(h), C.byref(p), C.byref(t))
+    return int(h.value), int(p.value), int(t.value)
+
+
+def pi_buffer(digits: int):
+    """Output buffer large enough for pi.txt of `digits` decimal digits."""
+    return C.create_string_buffer(int(digits * np.log(10) / np.log(16)) + 128)
+
+
+def pi_hex_into(digits: int, mode: int, buf, verbose: bool = False):
+    """Computes pi.txt into `buf` (from pi_buffer); returns (bytes written,
+    phase times dict).  Allocates no result memory on the Python side."""
     picugpu.ensure_init()
-    hexd = int(digits * np.log(10) / np.log(16)) + 64
-    buf = C.create_string_buffer(hexd + 64)
     t = (C.c_double * 7)()
     n = load().pch_pi_hex(digits, mode, buf, len(buf), t, 1 if verbose else 0)
     if n < 0:
         raise RuntimeError(load().pch_last_error().decode())
-    names = ["series", "division", "invsqrt", "final_mul", "conversion", "total", "big_muls"]
-    return buf.raw[:n].decode(), dict(zip(names, list(t)))
+    return n, dict(zip(PHASES, list(t)))
 
 
-def bf_op(op: str, a, b=(True, 0, []), p: int = 0):
-    """One BigFloat operation on (sign, exp, limbs) tuples; returns a tuple."""
+def pi_hex(digits: int, mode: int = MODE_BATCHED, verbose: bool = False):
+    """(pi.txt content, phase times dict) computed with GPU products."""
+    buf = pi_buffer(digits)
+    n, t = pi_hex_into(digits, mode, buf, verbose)
+    return buf.raw[:n].decode(), t
+
+
+def bf_op(op: str, a, b=(True, 0, []), p: int = 0, device: bool = False):
+    """One BigFloat operation on (sign, exp, limbs) tuples; returns a tuple.
+    device=True runs it on DeviceFloat (values resident in GPU memory);
+    op "ucmp" (device only) returns -1, 0 or 1."""
     picugpu.ensure_init()
     sa, ea, la = a
     sb, eb, lb = b
     la = np.ascontiguousarray(np.asarray(la, dtype=np.uint32))
     lb = np.ascontiguousarray(np.asarray(lb, dtype=np.uint32))
     cap = la.size + lb.size + 64 + (p if op in ("rcp", "div", "invsqrt") else 0) * 2
+    if op in ("add", "sub"):
+        cap += abs(int(ea) - int(eb))  # exponent gap filled by aligned zeros
     out = np.zeros(cap, dtype=np.uint32)
     sz, ez, nz = C.c_int(), C.c_int64(), C.c_uint64()
-    rc = load().pch_bf_op(OPS[op], int(sa), ea, la.ctypes.data, la.size, int(sb), eb,
-                          lb.ctypes.data, lb.size, p, C.byref(sz), C.byref(ez), out.ctypes.data,
-                          cap, C.byref(nz))
+    fn = load().pch_df_op if device else load().pch_bf_op
+    rc = fn(OPS[op], int(sa), ea, la.ctypes.data, la.size, int(sb), eb, lb.ctypes.data, lb.size, p,
+            C.byref(sz), C.byref(ez), out.ctypes.data, cap, C.byref(nz))
     if rc != 0:
         raise RuntimeError(load().pch_last_error().decode())
+    if op == "ucmp":
+        return int(sz.value)
     return (bool(sz.value), int(ez.value), out[: nz.value].tolist())

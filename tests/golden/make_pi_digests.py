@@ -2,7 +2,10 @@
 oracle (CPU restatement of the reference, reference engine crossover)
 produces it for Pi(D) (PI.h:138-210).  Run from the repository root:
     python tests/golden/make_pi_digests.py [D ...]
-The GPU pipeline must reproduce these bytes exactly."""
+The GPU pipeline must reproduce these bytes exactly.  prefix_sha256 covers
+"3." and the first 8 (p - 4) hex digits, the part that is digits of pi
+(the computation carries p = (hex + 9) / 9 limbs, PI.h:141-144), so a run
+at more digits must reproduce it too."""
 import hashlib
 import json
 import os
@@ -29,6 +32,7 @@ def main(digits):
             "sha256": hashlib.sha256(s.encode()).hexdigest(),
             "head": s[:66],
             "tail": s[-32:],
+            "prefix_sha256": hashlib.sha256(s[:2 + 8 * (p - 4)].encode()).hexdigest(),
             "bytes": len(s),
             "hex_digits": hexd,
             "p_limbs": p,

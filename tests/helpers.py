@@ -46,3 +46,42 @@ def residues_match(a, b, c) -> bool:
         if residue(c, q) != residue(a, q) * residue(b, q) % q:
             return False
     return True
+
+
+def pi_correct_digits(hex_digits: int) -> int:
+    """Hex digits of Pi(D)'s output that are digits of pi: the computation
+    carries p = (hex + 9) / 9 limbs (PI.h:141-144), 8 hex digits each, of
+    which the last few absorb the truncation error."""
+    return 8 * ((hex_digits + 9) // 9 - 4)
+
+
+def pi_spot_checks(s: bytes | str) -> dict:
+    """Checks of a pi.txt ("3." + hex digits) that need no run at the same
+    size: BBP digits at the fixture positions inside the correct region,
+    and the digest of the correct region of every oracle pi.txt fixture
+    shorter than s."""
+    import hashlib
+    import json
+    import os
+
+    if isinstance(s, str):
+        s = s.encode()
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    with open(os.path.join(gold, "bbp_spots.json")) as f:
+        spots = json.load(f)
+    with open(os.path.join(gold, "pi_digests.json")) as f:
+        digests = json.load(f)
+    good = pi_correct_digits(len(s) - 2)
+    res = {"bbp_positions": [], "bbp_ok": True, "prefix_digits": [], "prefix_ok": True}
+    for k, v in sorted(spots.items(), key=lambda kv: int(kv[0])):
+        n = int(k)
+        if n + 6 <= good:
+            res["bbp_positions"].append(n)
+            res["bbp_ok"] &= s[2 + n:2 + n + 6].decode() == v
+    for k, fx in sorted(digests.items(), key=lambda kv: int(kv[0])):
+        if "prefix_sha256" in fx and fx["bytes"] < len(s):
+            res["prefix_digits"].append(int(k))
+            pre = s[:2 + pi_correct_digits(fx["hex_digits"])]
+            res["prefix_ok"] &= hashlib.sha256(pre).hexdigest() == fx["prefix_sha256"]
+    res["ok"] = bool(res["bbp_ok"] and res["prefix_ok"] and res["bbp_positions"])
+    return res

@@ -58,7 +58,7 @@ def test_errors_are_reference_literals(host):
         host.bf_op("rcp", (True, 0, []), p=3)
 
 
-@pytest.mark.parametrize("D", [10000, 100000, 1000000, 10000000])
+@pytest.mark.parametrize("D", [10000, 100000, 1000000, 10000000, 100000000])
 @pytest.mark.parametrize("mode", ["plain", "batched"])
 def test_pi_txt_matches_oracle(host, D, mode):
     if mode == "plain" and D > 100000:
@@ -69,3 +69,21 @@ def test_pi_txt_matches_oracle(host, D, mode):
     assert len(s) == fx["bytes"]
     assert s[:66] == fx["head"]
     assert hashlib.sha256(s.encode()).hexdigest() == fx["sha256"]
+
+
+def test_pi_1e9_spot_checks(host):
+    """Pi to 10^9 digits (batched, every value resident in GPU memory): no CPU
+    run of this size is affordable, so the output is checked by BBP digits
+    at positions across its 738206216 correct hex digits and by the digest of
+    the correct region of the oracle's 10^8-digit pi.txt."""
+    from tests.helpers import pi_spot_checks
+
+    D = 1000000000
+    hexd, _, _ = host.pi_sizes(D)
+    buf = host.pi_buffer(D)
+    n, t = host.pi_hex_into(D, host.MODE_BATCHED, buf)
+    assert n == 2 + hexd
+    res = pi_spot_checks(buf.raw[:n])
+    assert res["bbp_positions"][-1] == 738206000
+    assert 100000000 in res["prefix_digits"]
+    assert res["ok"], res

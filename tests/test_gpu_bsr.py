@@ -32,3 +32,11 @@ def test_window_too_small_is_refused(gpu):
     with pytest.raises(gpu.PcgError) as e:
         gpu.bsr_exact(0, 5000, 100)
     assert e.value.code == 5  # PCG_ERR_TOO_LARGE
+
+
+@pytest.mark.parametrize("a,b", [(0, 1), (0, 100), (5000, 13000)])
+def test_device_resident_results(gpu, a, b):
+    """pcg_df_bsr_exact leaves the same values in device memory (exponent 0)."""
+    got = gpu.df_bsr_exact(a, b, BIG_P)
+    P, Q, R = O.pi_bsr(a, b, BIG_P)
+    assert got == [P.tuple(), Q.tuple(), R.tuple()]

@@ -1,0 +1,166 @@
+"""DeviceFloat (values resident in GPU memory, pcg_df_* entry points)
+against the oracle's BigFloat: mul with its precision chop, add/sub with
+exponent alignment, bottom cut and leading-zero strip, mul by a word,
+ucmp, and the Newton reciprocal, division and 1/sqrt (reference
+BigFloat.cpp:123-567).  Bit-exact on every (sign, exp, limbs)."""
+import random
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def host(gpu):
+    from piculator_amd import pihost
+
+    pihost.load()
+    return pihost
+
+
+def rand_val(rng, n, exp=None, sign=None, top_nonzero=True):
+    limbs = [rng.getrandbits(32) for _ in range(n)]
+    if n and top_nonzero:
+        limbs[-1] |= 1
+    return (rng.random() < 0.5 if sign is None else sign,
+            rng.randint(-40, 40) if exp is None else exp, limbs)
+
+
+def same(got, want):
+    """Equality of (sign, exp, limbs) with a short description of the first
+    difference (lengths, index, values) for the assertion message."""
+    if got == want:
+        return True, ""
+    gl, wl = got[2], want[2]
+    i = next((k for k in range(min(len(gl), len(wl))) if gl[k] != wl[k]), None)
+    return False, "sign %s/%s exp %s/%s len %d/%d first diff %s%s" % (
+        got[0], want[0], got[1], want[1], len(gl), len(wl), i,
+        "" if i is None else " got %#x want %#x" % (gl[i], wl[i]))
+
+
+def check(got, want):
+    ok, msg = same(got, want)
+    assert ok, msg
+
+
+def oracle_op(op, a, b, p):
+    oa, ob = O.BF(*a), O.BF(*b)
+    if op == "mul":
+        return oa.mul(ob, p).tuple()
+    if op == "add":
+        return oa.add(ob, p).tuple()
+    if op == "sub":
+        return oa.sub(ob, p).tuple()
+    if op == "mul_u32":
+        return oa.mul_u32(b[2][0] if b[2] else 0).tuple()
+    raise ValueError(op)
+
+
+SIZES = [1, 2, 3, 7, 100, 1023, 1024, 1025, 4097, 70001]
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_ops_match_oracle(host, seed):
+    rng = random.Random(1000 + seed)
+    a = rand_val(rng, rng.choice(SIZES))
+    b = rand_val(rng, rng.choice(SIZES))
+    for p in (0, 1, 10, 1000, 50000):
+        for op in ("mul", "add", "sub"):
+            got = host.bf_op(op, a, b, p, device=True)
+            assert got == oracle_op(op, a, b, p), (op, p, len(a[2]), len(b[2]))
+    x = (True, 0, [rng.getrandbits(32) | 1])
+    assert host.bf_op("mul_u32", a, x, device=True) == oracle_op("mul_u32", a, x, 0)
+
+
+def test_carry_and_borrow_chains(host):
+    n = 5000
+    ones = (True, 0, [0xFFFFFFFF] * n)
+    one = (True, 0, [1])
+    # carry through every limb and out of the top (within one block of
+    # 1024 limbs, then across blocks)
+    for m in (3, 1024, 1500, n):
+        x = (True, 0, [0xFFFFFFFF] * m)
+        check(host.bf_op("add", x, one, 0, device=True), oracle_op("add", x, one, 0))
+    # borrow through every limb, then the strip of all leading zeros
+    big = (True, 0, [0] * n + [1])
+    check(host.bf_op("sub", big, one, 0, device=True), oracle_op("sub", big, one, 0))
+    # equal magnitudes: zero with exponent 0, positive
+    a = (False, 3, [5, 6, 7])
+    b = (False, 3, [5, 6, 7])
+    check(host.bf_op("sub", a, b, 0, device=True), oracle_op("sub", a, b, 0))
+    check(host.bf_op("add", a, (True, 3, [5, 6, 7]), 0, device=True),
+          oracle_op("add", a, (True, 3, [5, 6, 7]), 0))
+    # multiply by a word with maximal carries
+    w = (True, 0, [0xFFFFFFFF])
+    check(host.bf_op("mul_u32", ones, w, device=True), oracle_op("mul_u32", ones, w, 0))
+    # one-limb trim of a product
+    s = (True, 0, [0, 0, 1])
+    check(host.bf_op("mul", s, s, 0, device=True), oracle_op("mul", s, s, 0))
+
+
+def test_alignment_and_cut(host):
+    rng = random.Random(7)
+    for _ in range(30):
+        a = rand_val(rng, rng.randint(1, 3000), exp=rng.randint(-5000, 5000))
+        b = rand_val(rng, rng.randint(1, 3000), exp=rng.randint(-5000, 5000))
+        p = rng.choice([0, 1, 5, 100, 2500])
+        for op in ("add", "sub"):
+            assert host.bf_op(op, a, b, p, device=True) == oracle_op(op, a, b, p), (op, p)
+
+
+def test_ucmp(host):
+    rng = random.Random(11)
+    cases = []
+    for _ in range(20):
+        a = rand_val(rng, rng.randint(1, 2000), exp=0)
+        b = (a[0], 0, list(a[2]))
+        i = rng.randrange(len(b[2]))
+        b[2][i] ^= 1 << rng.randrange(32)
+        if b[2][-1] == 0:
+            b[2][-1] = 1
+        cases.append((a, b))
+    cases.append(((True, 2, [1, 2]), (True, 0, [0, 0, 1, 2])))   # same value, other exponent
+    cases.append(((True, 2, [1, 2]), (True, 0, [5, 0, 1, 2])))
+    cases.append(((True, 0, []), (True, 0, [])))
+    for a, b in cases:
+        assert host.bf_op("ucmp", a, b, device=True) == O.BF(*a).ucmp(O.BF(*b))
+
+
+@pytest.mark.parametrize("p", [0, 1, 2, 3, 17, 500, 5000])
+def test_newton_match_oracle(host, p):
+    rng = random.Random(p)
+    a = rand_val(rng, 600, exp=-7, sign=False)
+    b = rand_val(rng, 400, exp=2, sign=True)
+    oa, ob = O.BF(*a), O.BF(*b)
+    assert host.bf_op("rcp", a, p=p, device=True) == oa.rcp(p).tuple()
+    assert host.bf_op("div", b, a, p=p, device=True) == ob.div(oa, p).tuple()
+    assert host.bf_op("invsqrt", (True, 0, [10005]), p=p, device=True) == O.invsqrt(10005, p).tuple()
+
+
+def test_errors_are_reference_literals(host):
+    with pytest.raises(RuntimeError, match="Divide by Zero"):
+        host.bf_op("rcp", (True, 0, []), p=3, device=True)
+
+
+def test_large_values_residue(host):
+    """Multi-million-limb add and product: size-independent residue checks."""
+    from tests.helpers import residue, CHECK_PRIMES
+
+    rng = np.random.default_rng(5)
+    n = 3_000_000
+    a = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    b = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    a[-1] |= 1
+    b[-1] |= 1
+    s = host.bf_op("add", (True, 0, a), (True, 0, b), 0, device=True)
+    z = np.asarray(s[2], dtype=np.uint32)
+    for q in CHECK_PRIMES[:2]:
+        assert residue(z, q) == (residue(a, q) + residue(b, q)) % q
+    m = host.bf_op("mul", (True, 0, a), (True, 0, b), 0, device=True)
+    z = np.asarray(m[2], dtype=np.uint32)
+    assert m[1] == 0
+    for q in CHECK_PRIMES[:2]:
+        assert residue(z, q) == residue(a, q) * residue(b, q) % q

@@ -196,3 +196,50 @@ def test_planted_truncation_mistake_is_caught():
     bl, be = chop(b.limbs, b.exp, 6)
     wrong = Fraction(to_int(al) * to_int(bl)) * Fraction(2) ** (32 * (ae + be))
     assert val(z) != wrong
+
+
+def test_bbp_extraction_matches_mpmath_and_python():
+    """The C BBP extraction (oracle.bbp_hex) against mpmath's hex expansion of
+    pi and the pure-Python digit extraction above."""
+    with mpmath.workdps(3000):
+        x = mpmath.mp.pi - 3
+        s = format(int(mpmath.floor(x * mpmath.mpf(16) ** 2400)), "02400x")
+    for n in (0, 1, 7, 100, 999, 2000, 2390):
+        assert O.bbp_hex(n, threads=2) == s[n:n + 6]
+    for n in (5000, 123457):
+        assert O.bbp_hex(n, threads=3)[0] == bbp_digit(n)
+
+
+def test_bbp_fixture_and_spot_check_helper():
+    """tests/golden/bbp_spots.json agrees with the extraction (small positions)
+    and with the oracle's pi.txt at 10^6 digits; pi_spot_checks accepts that
+    pi.txt and rejects planted single-digit mistakes."""
+    import json
+    import os
+
+    from tests.helpers import pi_correct_digits, pi_spot_checks
+
+    path = os.path.join(os.path.dirname(__file__), "golden", "bbp_spots.json")
+    with open(path) as f:
+        spots = json.load(f)
+    for k in ("0", "1000", "99990"):
+        assert O.bbp_hex(int(k), threads=2) == spots[k]
+    s, _ = O.pi_hex(1000000, O.ENGINE_REFERENCE, threads=os.cpu_count() or 1)
+    good = pi_correct_digits(len(s) - 2)
+    assert good == 738176
+    for k, v in spots.items():
+        n = int(k)
+        if n + 6 <= good:
+            assert s[2 + n:2 + n + 6] == v, n
+    res = pi_spot_checks(s)
+    assert res["ok"] and res["bbp_positions"] == [0, 1000, 99990, 738100]
+    assert res["prefix_digits"] == [10000, 100000]
+
+    def flip(text, i):
+        c = "0" if text[i] != "0" else "1"
+        return text[:i] + c + text[i + 1:]
+
+    bad = pi_spot_checks(flip(s, 2 + 738103))
+    assert not bad["bbp_ok"] and not bad["ok"]
+    bad = pi_spot_checks(flip(s, 2 + 5000))
+    assert bad["bbp_ok"] and not bad["prefix_ok"] and not bad["ok"]
