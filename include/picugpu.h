@@ -191,6 +191,18 @@ int pcg_df_bsr_exact(uint32_t a, uint32_t b, uint64_t max_limbs, pcg_dfloat* P, 
 /* Frees the pool's cached buffers (values in use are untouched). */
 int pcg_df_release_pool(void);
 
+/* Size classes of batched products (pcg_mul_batch*, pcg_bsr_exact): a
+ * product of zL = na + nb limbs with 2^(b-1) < zL <= 2^b, b <= 12, runs in
+ * class table[b]: 0 the schoolbook kernel (zL <= 1024), 10..12 the
+ * single-workgroup NTT of 2^k points.  The crossover is the autotuner's
+ * (timed on the device at the first batched call, or by pcg_autotune,
+ * which returns the table); pcg_batch_table reads it (set = 0) or installs
+ * one (set = 1, every entry valid for its bucket).  Products are exact in
+ * every class; the table only moves work between kernels.  The reference
+ * picks its engine by size alone (BigFloat.cpp:386). */
+int pcg_autotune(int* table, int n);
+int pcg_batch_table(int* table, int n, int set);
+
 /* Elapsed time of the work enqueued on the library stream between the two
  * calls, measured with HIP events recorded on that stream. */
 int pcg_timer_start(void);

@@ -19,38 +19,62 @@ constexpr int kSchoolMaxLimbs = 1024;  // zL bound of k_batch_school
 constexpr int kSmallNttMaxLog = 12;    // N bound of k_batch_ntt (rows padded by one)
 
 __global__ __launch_bounds__(256) void k_batch_school(const BatchDesc* descs) {
-  __shared__ uint32_t sa[kSchoolMaxLimbs], sb[kSchoolMaxLimbs];
+  // sb is stored 3 limbs to the right with zeros around it, so the window
+  // b[k0 + q - i] (q = 0..3) is readable for every i of the loop below.
+  __shared__ uint32_t sa[kSchoolMaxLimbs], sbp[kSchoolMaxLimbs + 8];
   __shared__ uint32_t sw1[kSchoolMaxLimbs], sw2[kSchoolMaxLimbs];
   __shared__ CarryFn wtot[4];
   const BatchDesc d = descs[blockIdx.x];
   const int tid = threadIdx.x;
   const int na = (int)d.na, nb = (int)d.nb, zL = na + nb;
   for (int i = tid; i < na; i += 256) sa[i] = d.a[i];
-  for (int i = tid; i < nb; i += 256) sb[i] = d.b[i];
+  for (int i = tid; i < kSchoolMaxLimbs + 8; i += 256) {
+    const int j = i - 3;
+    sbp[i] = (j >= 0 && j < nb) ? d.b[j] : 0u;
+  }
   __syncthreads();
+  // Columns k0 .. k0 + 3 as 96-bit sums (lo 64 bits + hi word).  Each step
+  // of i reads one limb of a and slides the window of b by one limb.
+  const int k0 = 4 * tid;
+  uint64_t lo[4] = {0, 0, 0, 0};
+  uint32_t hi[4] = {0, 0, 0, 0};
+  int ilo = k0 - nb + 1;
+  if (ilo < 0) ilo = 0;
+  int ihi = k0 + 3 < na - 1 ? k0 + 3 : na - 1;
+  if (k0 < zL && ilo <= ihi) {
+    // window: w[q] = b[k0 + q - i] = sbp[3 + k0 + q - i]
+    uint32_t w0 = sbp[3 + k0 - ilo], w1 = sbp[4 + k0 - ilo], w2 = sbp[5 + k0 - ilo],
+             w3 = sbp[6 + k0 - ilo];
+    for (int i = ilo; i <= ihi; ++i) {
+      const uint64_t ai = sa[i];
+      const uint64_t p0 = ai * w0, p1 = ai * w1, p2 = ai * w2, p3 = ai * w3;
+      lo[0] += p0;
+      hi[0] += lo[0] < p0;
+      lo[1] += p1;
+      hi[1] += lo[1] < p1;
+      lo[2] += p2;
+      hi[2] += lo[2] < p2;
+      lo[3] += p3;
+      hi[3] += lo[3] < p3;
+      w3 = w2;
+      w2 = w1;
+      w1 = w0;
+      const int nxt = 2 + k0 - i;  // b[k0 - (i + 1)], needed while i + 1 <= k0 + 3
+      w0 = nxt >= 0 ? sbp[nxt] : 0u;
+    }
+  }
   uint32_t w0[4], w1[4], w2[4], D[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int k = 4 * tid + q;
-    int i0 = k - nb + 1;
-    if (i0 < 0) i0 = 0;
-    int i1 = k < na - 1 ? k : na - 1;
-    uint64_t lo = 0;
-    uint32_t hi = 0;
-    for (int i = i0; i <= i1; ++i) {
-      const uint64_t pr = (uint64_t)sa[i] * sb[k - i];
-      lo += pr;
-      hi += lo < pr;
-    }
-    w0[q] = (uint32_t)lo;
-    w1[q] = (uint32_t)(lo >> 32);
-    w2[q] = hi;
+    w0[q] = (uint32_t)lo[q];
+    w1[q] = (uint32_t)(lo[q] >> 32);
+    w2[q] = hi[q];
   }
   uint64_t c, spill;
   block_carry<4>(w0, w1, w2, sw1, sw2, wtot, D, c, spill);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int k = 4 * tid + q;
+    const int k = k0 + q;
     if (k < zL) d.out[k] = D[q];
   }
 }

@@ -89,19 +89,43 @@ __global__ __launch_bounds__(kCarryThreads) void k_df_local(DfArgs a) {
   if (tid == kCarryThreads - 1) a.agg[blockIdx.x] = cf_compose(pre, f);
 }
 
-// Highest position in [lo, lo + n) where the two operands differ, as
-// offset + 1 (0: equal everywhere), by atomicMax into *slot (zeroed first).
-__global__ __launch_bounds__(256) void k_df_cmp(DfOperand a, DfOperand b, int64_t lo, uint64_t n,
-                                                unsigned long long* slot) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long hit = 0;
-  if (i < n && df_word(a, lo + (int64_t)i) != df_word(b, lo + (int64_t)i)) hit = i + 1;
+// Block-wide maximum, one atomicMax per workgroup (256 threads).
+__device__ __forceinline__ void block_max_to(unsigned long long v, unsigned long long* slot) {
+  __shared__ unsigned long long wmax[4];
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
-    const unsigned long long o = __shfl_xor(hit, d, 64);
-    hit = o > hit ? o : hit;
+    const unsigned long long o = __shfl_xor(v, d, 64);
+    v = o > v ? o : v;
   }
-  if ((threadIdx.x & 63) == 0 && hit) atomicMax(slot, hit);
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = wmax[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = wmax[w] > m ? wmax[w] : m;
+    if (m) atomicMax(slot, m);
+  }
+}
+
+// The searches below walk down from the top: thread j of the grid visits
+// n - 1 - j, n - 1 - j - G, ... (G = grid threads) and stops at its first
+// hit, which is the highest hit of its residue class; the block maximum of
+// those goes to *slot (zeroed first) as index + 1 (0: no hit).  The work is
+// the length of the run without hits at the top plus one pass of the grid.
+
+// Highest position in [lo, lo + n) where the two operands differ.
+__global__ __launch_bounds__(256) void k_df_cmp(DfOperand a, DfOperand b, int64_t lo, uint64_t n,
+                                                unsigned long long* slot) {
+  const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long hit = 0;
+  for (uint64_t i = n - 1 - j; j < n; i -= G) {
+    if (df_word(a, lo + (int64_t)i) != df_word(b, lo + (int64_t)i)) {
+      hit = i + 1;
+      break;
+    }
+    if (i < G) break;
+  }
+  block_max_to(hit, slot);
 }
 
 // Turns the difference position into the comparison result (-1, 0, 1).
@@ -119,14 +143,17 @@ __global__ void k_df_cmp_final(DfOperand a, DfOperand b, int64_t lo,
 // Index + 1 of the highest nonzero limb of t[0, n) (0: all zero).
 __global__ __launch_bounds__(256) void k_df_topnz(const uint32_t* t, uint64_t n,
                                                   unsigned long long* slot) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long hit = (i < n && t[i] != 0) ? i + 1 : 0;
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    const unsigned long long o = __shfl_xor(hit, d, 64);
-    hit = o > hit ? o : hit;
+  const uint64_t G = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long hit = 0;
+  for (uint64_t i = n - 1 - j; j < n; i -= G) {
+    if (t[i] != 0) {
+      hit = i + 1;
+      break;
+    }
+    if (i < G) break;
   }
-  if ((threadIdx.x & 63) == 0 && hit) atomicMax(slot, hit);
+  block_max_to(hit, slot);
 }
 
 __global__ void k_df_put_u64(unsigned long long* p, unsigned long long v) { *p = v; }

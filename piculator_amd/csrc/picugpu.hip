@@ -612,25 +612,41 @@ int ensure_pool(Ctx& c, size_t limbs, size_t ndesc) {
   return PCG_OK;
 }
 
-// Size class of a product for the batched path: 0 schoolbook, 11/12 the
-// single-workgroup NTT of 2^11 / 2^12 points, 13..17 the multi-pass NTT run
-// for the whole class at once, -1 one product at a time.
+// Size classes of the batched paths.  Small products (zL = na + nb in
+// (2^(b-1), 2^b], b <= 12) go to the class g_small_class[b]: 0 the
+// schoolbook kernel (zL <= 1024), 10..12 the single-workgroup NTT of 2^k
+// points; the autotuner (tune_small) picks between them per bucket on the
+// device at first use.  Classes 13..17 are the multi-pass NTT run for the
+// whole class at once, -1 one product at a time.
 constexpr int kBatchMaxLog = 17;
+constexpr int kSmallNttMinLog = 10;
+int g_small_class[kSmallNttMaxLog + 1];
+bool g_small_ready = false;
+
+int small_class_default(int b) {
+  if ((1 << b) <= kSchoolMaxLimbs) return 0;
+  return b < kSmallNttMinLog ? kSmallNttMinLog : b;
+}
+
+bool small_class_valid(int b, int cls) {
+  if (cls == 0) return (1 << b) <= kSchoolMaxLimbs;
+  return cls >= kSmallNttMinLog && cls <= kSmallNttMaxLog && cls >= b;
+}
+
 int batch_class(uint64_t na, uint64_t nb) {
   const uint64_t zL = na + nb;
-  if (zL <= (uint64_t)kSchoolMaxLimbs) return 0;
   const int lg = ceil_log2(zL);
-  if (lg <= kSmallNttMaxLog) return lg < 11 ? 11 : lg;
+  if (lg <= kSmallNttMaxLog) return g_small_class[lg];
   if (lg <= kBatchMaxLog) return lg;
   return -1;
 }
 
 // Class of a product of at most zL limbs on the device-resident path:
-// 0 schoolbook, 11..30 transform length (log2).
+// the small classes above, else the transform length (log2).
 int bsr_class(uint64_t zL) {
-  if (zL <= (uint64_t)kSchoolMaxLimbs) return 0;
   const int lg = ceil_log2(zL);
-  return lg < 11 ? 11 : (lg < kMinLogN ? kMinLogN : lg);
+  if (lg <= kSmallNttMaxLog) return g_small_class[lg];
+  return lg < kMinLogN ? kMinLogN : lg;
 }
 
 // Multi-pass NTT products of one transform length, all at once: pass
@@ -702,25 +718,114 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN) {
   return PCG_OK;
 }
 
-// Launches the per-class kernels over h_desc[0..n) (already on the device
-// in d_desc, ordered by class: school, ntt 2^11, ntt 2^12).
-int launch_classes(Ctx& c, const BatchDesc* base, size_t n_school, size_t n11, size_t n12) {
+// Launches the single-workgroup classes over consecutive descriptors
+// (already on the device): schoolbook, then NTT 2^10, 2^11, 2^12.
+int launch_classes(Ctx& c, const BatchDesc* base, size_t n_school, size_t n10, size_t n11, size_t n12) {
   if (n_school)
     hipLaunchKernelGGL(k_batch_school, dim3((unsigned)n_school), dim3(256), 0, c.stream, base);
+  base += n_school;
+  if (n10)
+    hipLaunchKernelGGL(k_batch_ntt<10>, dim3((unsigned)n10), dim3(64), 0, c.stream, base,
+                       c.d_prime + 10 * 2, c.crt);
+  base += n10;
   if (n11)
-    hipLaunchKernelGGL(k_batch_ntt<11>, dim3((unsigned)n11), dim3(128), 0, c.stream,
-                       base + n_school, c.d_prime + 11 * 2, c.crt);
+    hipLaunchKernelGGL(k_batch_ntt<11>, dim3((unsigned)n11), dim3(128), 0, c.stream, base,
+                       c.d_prime + 11 * 2, c.crt);
+  base += n11;
   if (n12)
-    hipLaunchKernelGGL(k_batch_ntt<12>, dim3((unsigned)n12), dim3(256), 0, c.stream,
-                       base + n_school + n11, c.d_prime + 12 * 2, c.crt);
+    hipLaunchKernelGGL(k_batch_ntt<12>, dim3((unsigned)n12), dim3(256), 0, c.stream, base,
+                       c.d_prime + 12 * 2, c.crt);
   HIPCHK(hipGetLastError());
   return PCG_OK;
 }
 
+// Launches one small class (0 or 10..12) over n descriptors.
+int launch_small_class(Ctx& c, int cls, const BatchDesc* base, size_t n) {
+  switch (cls) {
+    case 0: return launch_classes(c, base, n, 0, 0, 0);
+    case 10: return launch_classes(c, base, 0, n, 0, 0);
+    case 11: return launch_classes(c, base, 0, 0, n, 0);
+    case 12: return launch_classes(c, base, 0, 0, 0, n);
+    default: return fail(PCG_ERR_INTERNAL, "bad small class %d", cls);
+  }
+}
+
+__global__ void k_fill_pattern(uint32_t* p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (uint32_t)(i * 2654435761u) ^ 0x9e3779b9u;
+}
+
+// Autotuner of the small-product classes: for each bucket b = 4..10 it
+// times 2048 products of na = nb = 3 * 2^(b-3) limbs (zL = 3/4 of the
+// bucket's top) through the schoolbook kernel and through the 2^10-point
+// NTT (best of three launches, HIP events on the library stream) and keeps
+// the faster; smaller buckets stay schoolbook, 11 and 12 are NTT only.
+// Results are exact either way: the choice only moves the crossover.
+int tune_small(Ctx& c) {
+  for (int b = 0; b <= kSmallNttMaxLog; ++b) g_small_class[b] = small_class_default(b);
+  constexpr int kCount = 2048;
+  constexpr int kMaxHalf = 3 << 7;  // na at b = 10
+  uint32_t* in = nullptr;
+  uint32_t* out = nullptr;
+  BatchDesc* dd = nullptr;
+  const size_t in_limbs = (size_t)kCount * 2 * kMaxHalf, out_limbs = (size_t)kCount * 2 * kMaxHalf;
+  if (hipMalloc(&in, in_limbs * 4) != hipSuccess || hipMalloc(&out, out_limbs * 4) != hipSuccess ||
+      hipMalloc(&dd, kCount * sizeof(BatchDesc)) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(in);
+    (void)hipFree(out);
+    return fail(PCG_ERR_OOM, "cannot allocate the autotuner's test batch");
+  }
+  hipLaunchKernelGGL(k_fill_pattern, dim3((unsigned)((in_limbs + 255) / 256)), dim3(256), 0, c.stream, in,
+                     (uint64_t)in_limbs);
+  std::vector<BatchDesc> h(kCount);
+  int rc = PCG_OK;
+  for (int b = 4; b <= kSmallNttMinLog && rc == PCG_OK; ++b) {
+    const uint32_t n = 3u << (b - 3);
+    for (int i = 0; i < kCount; ++i) {
+      h[i].a = in + (size_t)i * 2 * n;
+      h[i].b = h[i].a + n;
+      h[i].out = out + (size_t)i * 2 * n;
+      h[i].na = n;
+      h[i].nb = n;
+    }
+    if (hipMemcpyAsync(dd, h.data(), kCount * sizeof(BatchDesc), hipMemcpyHostToDevice, c.stream) !=
+        hipSuccess) {
+      rc = fail(PCG_ERR_HIP, "autotuner upload failed");
+      break;
+    }
+    float best[2] = {1e30f, 1e30f};
+    const int cls[2] = {0, kSmallNttMinLog};
+    for (int m = 0; m < 2 && rc == PCG_OK; ++m)
+      for (int rep = 0; rep < 4 && rc == PCG_OK; ++rep) {
+        if (hipEventRecord(c.tev[0], c.stream) != hipSuccess) rc = fail(PCG_ERR_HIP, "event record failed");
+        if (!rc) rc = launch_small_class(c, cls[m], dd, kCount);
+        float ms = 0;
+        if (!rc && (hipEventRecord(c.tev[1], c.stream) != hipSuccess ||
+                    hipEventSynchronize(c.tev[1]) != hipSuccess ||
+                    hipEventElapsedTime(&ms, c.tev[0], c.tev[1]) != hipSuccess))
+          rc = fail(PCG_ERR_HIP, "autotuner timing failed");
+        if (!rc && rep > 0) best[m] = std::min(best[m], ms);
+      }
+    if (!rc) g_small_class[b] = best[0] <= best[1] ? 0 : kSmallNttMinLog;
+  }
+  (void)hipStreamSynchronize(c.stream);
+  (void)hipFree(in);
+  (void)hipFree(out);
+  (void)hipFree(dd);
+  if (rc) return rc;
+  g_small_ready = true;
+  return PCG_OK;
+}
+
+int ensure_tuned(Ctx& c) { return g_small_ready ? PCG_OK : tune_small(c); }
+
 int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, bool dev) {
-  // class slots: 0 schoolbook, 1..2 single-workgroup NTT (2^11, 2^12),
-  // 3..7 batched multi-pass (2^13 .. 2^17)
-  constexpr int kSlots = 3 + (kBatchMaxLog - kSmallNttMaxLog);
+  // class slots: 0 schoolbook, 1..3 single-workgroup NTT (2^10 .. 2^12),
+  // 4..8 batched multi-pass (2^13 .. 2^17)
+  constexpr int kSlots = 4 + (kBatchMaxLog - kSmallNttMaxLog);
+  int rc0 = ensure_tuned(c);
+  if (rc0) return rc0;
   std::vector<uint64_t> cls[kSlots], large;
   for (uint64_t i = 0; i < count; ++i) {
     int rc = check_sizes(d[i].na, d[i].nb);
@@ -742,7 +847,7 @@ int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, 
     if (k < 0)
       large.push_back(i);
     else
-      cls[k == 0 ? 0 : k - 10].push_back(i);
+      cls[k == 0 ? 0 : k - kSmallNttMinLog + 1].push_back(i);
   }
   size_t nsmall = 0;
   for (int k = 0; k < kSlots; ++k) nsmall += cls[k].size();
@@ -780,12 +885,12 @@ int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, 
       HIPCHK(hipMemcpyAsync(c.d_pool, c.h_pool, in_limbs * 4, hipMemcpyHostToDevice, c.stream));
     HIPCHK(hipMemcpyAsync(c.d_desc, c.h_desc, nsmall * sizeof(BatchDesc), hipMemcpyHostToDevice,
                           c.stream));
-    rc = launch_classes(c, c.d_desc, cls[0].size(), cls[1].size(), cls[2].size());
+    rc = launch_classes(c, c.d_desc, cls[0].size(), cls[1].size(), cls[2].size(), cls[3].size());
     if (rc) return rc;
-    size_t off = cls[0].size() + cls[1].size() + cls[2].size();
-    for (int k = 3; k < kSlots; ++k) {
+    size_t off = cls[0].size() + cls[1].size() + cls[2].size() + cls[3].size();
+    for (int k = 4; k < kSlots; ++k) {
       if (cls[k].empty()) continue;
-      rc = batched_multipass(c, c.d_desc + off, cls[k].size(), kSmallNttMaxLog + k - 2);
+      rc = batched_multipass(c, c.d_desc + off, cls[k].size(), kSmallNttMaxLog + k - 3);
       if (rc) return rc;
       off += cls[k].size();
     }
@@ -869,6 +974,32 @@ uint64_t bsr_bounds(const std::vector<BsrNode>& nodes, std::vector<uint64_t>& bn
     for (int j = 0; j < 5; ++j) mx = std::max(mx, v[j]);
   }
   return mx;
+}
+
+// The same bounds for the root of a subtree of d leaves, by recursion on the
+// sizes (left d / 2, right d - d / 2): O(log d) distinct sizes, memoised.
+struct BsrRootBound {
+  uint64_t P, Q, R, mx;
+};
+BsrRootBound bsr_bound_of(uint32_t d) {
+  static std::mutex mu;
+  static std::map<uint32_t, BsrRootBound> memo;
+  if (d == 1) return BsrRootBound{kLeafBound, kLeafBound, kLeafBound, kLeafBound};
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = memo.find(d);
+    if (it != memo.end()) return it->second;
+  }
+  const BsrRootBound l = bsr_bound_of(d / 2), r = bsr_bound_of(d - d / 2);
+  BsrRootBound v;
+  const uint64_t x = l.P + r.Q, y = r.P + l.R;
+  v.P = std::max(x, y) + 1;
+  v.Q = l.Q + r.Q;
+  v.R = l.R + r.R;
+  v.mx = std::max({l.mx, r.mx, x, y, v.P, v.Q, v.R});
+  std::lock_guard<std::mutex> lk(mu);
+  memo[d] = v;
+  return v;
 }
 
 template <class T>
@@ -1032,6 +1163,8 @@ int g_bsr_next = 0;
 int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const uint64_t* caps,
                     BsrPlan** Pout, BsrVal rv[3]) {
   if (b <= a) return fail(PCG_ERR_INVALID, "empty range");
+  int rct = ensure_tuned(c);  // plans record the small-product classes
+  if (rct) return rct;
   const uint32_t d = b - a;
   BsrPlan* P = nullptr;
   for (BsrPlan& q : g_bsr_plans)
@@ -1061,12 +1194,8 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
     size_t o = 0;
     for (const auto& cl : hh.classes) {
       int rc;
-      if (cl.first == 0)
-        rc = launch_classes(c, hd + o, cl.second, 0, 0);
-      else if (cl.first == 11)
-        rc = launch_classes(c, hd + o, 0, cl.second, 0);
-      else if (cl.first == 12)
-        rc = launch_classes(c, hd + o, 0, 0, cl.second);
+      if (cl.first <= kSmallNttMaxLog)
+        rc = launch_small_class(c, cl.first, hd + o, cl.second);
       else
         rc = batched_multipass(c, hd + o, cl.second, cl.first);
       if (rc) return rc;
@@ -1270,6 +1399,12 @@ int df_carry_pass(Ctx& c, DfArgs a) {
   return PCG_OK;
 }
 
+// Grid of the top-down searches: at most 1024 workgroups of 256.
+unsigned df_search_blocks(uint64_t n) {
+  const uint64_t b = (n + 255) / 256;
+  return (unsigned)(b < 1024 ? (b ? b : 1) : 1024);
+}
+
 // BigFloat::ucmp (BigFloat.cpp:123-147): magnitudes exp + L first, then the
 // highest position where the aligned words differ.
 int df_ucmp_locked(Ctx& c, const pcg_dfloat& A, const pcg_dfloat& B, int* cmp) {
@@ -1285,7 +1420,7 @@ int df_ucmp_locked(Ctx& c, const pcg_dfloat& A, const pcg_dfloat& B, int* cmp) {
   }
   const uint64_t n = (uint64_t)(magA - lo);
   hipLaunchKernelGGL(k_df_put_u64, dim3(1), dim3(1), 0, c.stream, c.df_slot + 2, 0ull);
-  hipLaunchKernelGGL(k_df_cmp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, df_opnd(A),
+  hipLaunchKernelGGL(k_df_cmp, dim3(df_search_blocks(n)), dim3(256), 0, c.stream, df_opnd(A),
                      df_opnd(B), lo, n, c.df_slot + 2);
   hipLaunchKernelGGL(k_df_cmp_final, dim3(1), dim3(1), 0, c.stream, df_opnd(A), df_opnd(B), lo,
                      c.df_slot + 2, reinterpret_cast<int64_t*>(c.df_slot + 3));
@@ -1339,7 +1474,7 @@ int df_uaddsub_locked(Ctx& c, pcg_dfloat* z, const pcg_dfloat A, const pcg_dfloa
   } else {
     hipLaunchKernelGGL(k_df_put_u64, dim3(1), dim3(1), 0, c.stream, c.df_slot + 1, 0ull);
     if (L)
-      hipLaunchKernelGGL(k_df_topnz, dim3((unsigned)((L + 255) / 256)), dim3(256), 0, c.stream, buf, L,
+      hipLaunchKernelGGL(k_df_topnz, dim3(df_search_blocks(L)), dim3(256), 0, c.stream, buf, L,
                          c.df_slot + 1);
     rc = df_read_slots(c, 2);
     if (rc) return rc;
@@ -1465,6 +1600,8 @@ int pcg_init(int device) {
   HIPCHK(hipHostMalloc(&c.h_slot, 4 * sizeof(unsigned long long), hipHostMallocDefault));
   for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&c.ev[i]));
   for (int i = 0; i < 2; ++i) HIPCHK(hipEventCreate(&c.tev[i]));
+  for (int b = 0; b <= kSmallNttMaxLog; ++b) g_small_class[b] = small_class_default(b);
+  g_small_ready = false;
   c.device = device;
   c.ready = true;
   return PCG_OK;
@@ -1675,11 +1812,7 @@ int pcg_bsr_exact(uint32_t a, uint32_t b, uint64_t max_limbs, pcg_bsr_value* P, 
 
 int pcg_bsr_bound(uint32_t a, uint32_t b, uint64_t* bound) {
   if (!bound || b <= a) return fail(PCG_ERR_INVALID, "bad arguments");
-  std::vector<BsrNode> nodes;
-  nodes.reserve(2 * (size_t)(b - a));
-  bsr_build(nodes, a, b);
-  std::vector<uint64_t> bnd;
-  *bound = bsr_bounds(nodes, bnd);
+  *bound = bsr_bound_of(b - a).mx;
   return PCG_OK;
 }
 
@@ -1865,6 +1998,33 @@ int pcg_df_bsr_exact(uint32_t a, uint32_t b, uint64_t max_limbs, pcg_dfloat* P, 
                           c.stream));
     df_assign(out[j], buf, cap, rv[j].L, 0, rv[j].sign);
   }
+  return PCG_OK;
+}
+
+int pcg_autotune(int* table, int n) {
+  DF_ENTER();
+  HIPCHK(hipStreamSynchronize(c.stream));
+  for (BsrPlan& q : g_bsr_plans) bsr_plan_free(q);  // plans record the classes
+  int rc = tune_small(c);
+  if (rc) return rc;
+  for (int b = 0; b < n && b <= kSmallNttMaxLog; ++b) table[b] = g_small_class[b];
+  return PCG_OK;
+}
+
+int pcg_batch_table(int* table, int n, int set) {
+  if (!table && n > 0) return fail(PCG_ERR_INVALID, "null pointer");
+  DF_ENTER();
+  if (set) {
+    for (int b = 0; b < n && b <= kSmallNttMaxLog; ++b)
+      if (!small_class_valid(b, table[b]))
+        return fail(PCG_ERR_INVALID, "class %d is not valid for bucket %d", table[b], b);
+    HIPCHK(hipStreamSynchronize(c.stream));
+    for (BsrPlan& q : g_bsr_plans) bsr_plan_free(q);
+    for (int b = 0; b < n && b <= kSmallNttMaxLog; ++b) g_small_class[b] = table[b];
+    g_small_ready = true;
+    return PCG_OK;
+  }
+  for (int b = 0; b < n && b <= kSmallNttMaxLog; ++b) table[b] = g_small_class[b];
   return PCG_OK;
 }
 

@@ -35,7 +35,7 @@ EXPORTS = [
     "pcg_last_profile", "pcg_ntt_plan", "pcg_timer_start", "pcg_timer_stop",
     "pcg_bsr_exact", "pcg_bsr_bound", "pcg_df_free", "pcg_df_set", "pcg_df_read", "pcg_df_copy",
     "pcg_df_ucmp", "pcg_df_add", "pcg_df_sub", "pcg_df_mul", "pcg_df_mul_u32", "pcg_df_to_hex",
-    "pcg_df_bsr_exact", "pcg_df_release_pool",
+    "pcg_df_bsr_exact", "pcg_df_release_pool", "pcg_autotune", "pcg_batch_table",
 ]
 
 
@@ -94,6 +94,8 @@ def load():
     L.pcg_timer_stop.argtypes = [C.POINTER(C.c_double)]
     L.pcg_bsr_exact.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64] + [C.POINTER(BsrValue)] * 3
     L.pcg_bsr_bound.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
+    L.pcg_autotune.argtypes = [C.POINTER(C.c_int), C.c_int]
+    L.pcg_batch_table.argtypes = [C.POINTER(C.c_int), C.c_int, C.c_int]
     dfp = C.POINTER(DFloat)
     L.pcg_df_free.argtypes = [dfp]
     L.pcg_df_set.argtypes = [dfp, C.c_void_p, C.c_uint64, C.c_int64, C.c_int32]
@@ -287,3 +289,22 @@ def df_bsr_exact(a: int, b: int, max_limbs: int):
     finally:
         for v in vals:
             load().pcg_df_free(C.byref(v))
+
+
+SMALL_BUCKETS = 13  # b = 0..12
+
+
+def autotune():
+    """Re-runs the small-product autotuner; returns the class per bucket."""
+    ensure_init()
+    t = (C.c_int * SMALL_BUCKETS)()
+    _check(load().pcg_autotune(t, SMALL_BUCKETS))
+    return list(t)
+
+
+def batch_table(table=None):
+    """Reads the small-product class table, or installs `table`."""
+    ensure_init()
+    t = (C.c_int * SMALL_BUCKETS)(*(table or [0] * SMALL_BUCKETS))
+    _check(load().pcg_batch_table(t, SMALL_BUCKETS, 1 if table is not None else 0))
+    return list(t)

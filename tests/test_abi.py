@@ -42,3 +42,33 @@ def test_plan_limits():
     assert picugpu.ntt_plan(3)[0] == 8
     with pytest.raises(picugpu.PcgError):
         picugpu.ntt_plan(2 ** 31)
+
+
+def _tree_bound(d: int) -> int:
+    """Limb bound of every value of a subtree of d leaves, node by node over
+    the explicit tree (midpoints m = (a + b) / 2 of PI.h:90): leaves 5 limbs,
+    x = P_L + Q_R, y = P_R + R_L, P = max(x, y) + 1, Q and R sums."""
+    mx = 0
+
+    def rec(a, b):
+        nonlocal mx
+        if b - a == 1:
+            mx = max(mx, 5)
+            return 5, 5, 5
+        m = (a + b) // 2
+        lp, lq, lr = rec(a, m)
+        rp, rq, rr = rec(m, b)
+        x, y = lp + rq, rp + lr
+        v = (max(x, y) + 1, lq + rq, lr + rr)
+        mx = max(mx, x, y, *v)
+        return v
+
+    rec(1000, 1000 + d)
+    return mx
+
+
+def test_bsr_bound_matches_tree_walk():
+    from piculator_amd import picugpu
+
+    for d in (1, 2, 3, 5, 64, 100, 1000, 4097, 70515):
+        assert picugpu.bsr_bound(1000, 1000 + d) == _tree_bound(d), d
