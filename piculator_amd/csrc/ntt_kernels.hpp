@@ -52,6 +52,7 @@ struct PassArgs {
   const BatchDesc* descs;
   int operand;
   uint64_t zstride;
+  uint64_t ntiles;  // tiles of E points per prime and product
 };
 
 
@@ -235,6 +236,11 @@ __device__ __forceinline__ const double2* stage_twiddles(double2* ltw, const Pri
 // One pass.  Column passes (ROW false) need s >= LINES (one block of L*s
 // points spans several workgroups, each taking LINES adjacent columns); row
 // passes have s == 1 and take LINES consecutive lines.
+//
+// A workgroup walks tiles w = blockIdx.x, + gridDim.x, ... (a.ntiles per
+// prime and product).  The next tile's points are loaded into registers
+// right after the current tile is in LDS, so its HBM reads are in flight
+// while the current tile's DFT stages and stores run.
 template <int MODE, int LOGL, int LOGLINES, bool ROW>
 __global__ __launch_bounds__(1 << (LOGL + LOGLINES - 4)) void k_ntt_pass(PassArgs a) {
   using GE = Geo<LOGL, LOGLINES, ROW>;
@@ -245,16 +251,6 @@ __global__ __launch_bounds__(1 << (LOGL + LOGLINES - 4)) void k_ntt_pass(PassArg
   const double2* tw = stage_twiddles<GE>(ltw, P);
   const int tid = threadIdx.x;
   const int logS = a.logS;
-  const uint64_t w = blockIdx.x;
-  // first point of this workgroup's lines and the global step between them
-  uint64_t base;
-  if (ROW) {
-    base = w << (LOGL + LOGLINES);
-  } else {
-    const int lgc = logS - LOGLINES;  // column groups per block, log2
-    base = ((w >> lgc) << (LOGL + logS)) + ((w & ((1ull << lgc) - 1)) << LOGLINES);
-  }
-  const uint64_t col0 = ROW ? 0 : (base & ((1ull << logS) - 1));
   uint64_t zoff = 0;
   const uint32_t* limbs = a.limbs;
   uint64_t nlimbs = a.nlimbs;
@@ -267,9 +263,17 @@ __global__ __launch_bounds__(1 << (LOGL + LOGLINES - 4)) void k_ntt_pass(PassArg
   const double* src = a.src ? a.src + zoff + pi * a.N : nullptr;
   double* dst = a.dst + zoff + pi * a.N;
   const int logRoot = LOGL + logS;
+  const uint64_t ntiles = a.ntiles;
 
+  // first point of tile t's lines (the global step between them is implied
+  // by the layout)
+  auto tile_base = [&](uint64_t t) -> uint64_t {
+    if (ROW) return t << (LOGL + LOGLINES);
+    const int lgc = logS - LOGLINES;  // column groups per block, log2
+    return ((t >> lgc) << (LOGL + logS)) + ((t & ((1ull << lgc) - 1)) << LOGLINES);
+  };
   // element i of this thread: LDS point (n, lam) and its global index
-#define PCG_ELEM(i, n, lam, g)                                        \
+#define PCG_ELEM(i, base, n, lam, g)                                  \
   int n, lam;                                                         \
   uint64_t g;                                                         \
   {                                                                   \
@@ -277,66 +281,94 @@ __global__ __launch_bounds__(1 << (LOGL + LOGLINES - 4)) void k_ntt_pass(PassArg
     if (ROW) {                                                        \
       lam = f >> LOGL;                                                \
       n = f & (GE::L - 1);                                            \
-      g = base + (uint64_t)f;                                         \
+      g = (base) + (uint64_t)f;                                       \
     } else {                                                          \
       lam = f & (GE::LINES - 1);                                      \
       n = f >> LOGLINES;                                              \
-      g = base + ((uint64_t)n << logS) + lam;                         \
+      g = (base) + ((uint64_t)n << logS) + lam;                       \
     }                                                                 \
   }
 
-  // Inter-pass twiddles of a column pass: this thread's column c is fixed
-  // and its points are n0 + i * (NTH / LINES), so omega^{c n} follows from
-  // base = omega^{c n0} and step = omega^{c NTH / LINES} by exact modular
-  // products (no table loads per point).
-  double2 twc = make_double2(1.0, 0.0), tws = make_double2(1.0, 0.0);
-  if (!ROW && MODE != kFused) {
-    const uint64_t c = col0 + (tid & (GE::LINES - 1));
-    const uint64_t n0 = (uint64_t)(tid >> LOGLINES);
-    twc = root_pow(c * n0, logRoot, MODE == kInv, P);
-    tws = root_pow(c * (uint64_t)(GE::NTH >> LOGLINES), logRoot, MODE == kInv, P);
-  }
-#pragma unroll
-  for (int i = 0; i < kEPT; ++i) {
-    PCG_ELEM(i, n, lam, g);
-    double x;
-    if (src)
-      x = src[g];
-    else
-      x = g < nlimbs ? (double)limbs[g] : 0.0;
-    if (MODE == kInv && !ROW) {
-      x = mm_shoup(x, twc.x, twc.y, P.p);
-      const double w = mm_shoup(twc.x, tws.x, tws.y, P.p);
-      twc = make_double2(w, w * P.pinv);
-    }
-    lds[GE::idx(n, lam)] = x;
-  }
-  __syncthreads();
-  line_dft<GE>(lds, tw, P, MODE == kInv);
-  if (MODE == kFused) {
-    const double* spec = a.spec + zoff + pi * a.N;
+  uint64_t w = blockIdx.x;
+  if (w >= ntiles) return;
+  double xr[kEPT];
+  {
+    const uint64_t b0 = tile_base(w);
 #pragma unroll
     for (int i = 0; i < kEPT; ++i) {
-      PCG_ELEM(i, n, lam, g);
-      const int li = GE::idx(n, lam);
-      const double x = mm_gen(lds[li], spec[g], P.p, P.pinv);
-      lds[li] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
+      PCG_ELEM(i, b0, n, lam, g);
+      (void)n;
+      (void)lam;
+      xr[i] = src ? src[g] : (g < nlimbs ? (double)limbs[g] : 0.0);
+    }
+  }
+  for (;;) {
+    const uint64_t base = tile_base(w);
+    const uint64_t col0 = ROW ? 0 : (base & ((1ull << logS) - 1));
+    // Inter-pass twiddles of a column pass: this thread's column c is fixed
+    // and its points are n0 + i * (NTH / LINES), so omega^{c n} follows from
+    // omega^{c n0} and the step omega^{c NTH / LINES} by exact modular
+    // products (no table loads per point).
+    double2 twc = make_double2(1.0, 0.0), tws = make_double2(1.0, 0.0);
+    if (!ROW && MODE != kFused) {
+      const uint64_t c = col0 + (tid & (GE::LINES - 1));
+      const uint64_t n0 = (uint64_t)(tid >> LOGLINES);
+      twc = root_pow(c * n0, logRoot, MODE == kInv, P);
+      tws = root_pow(c * (uint64_t)(GE::NTH >> LOGLINES), logRoot, MODE == kInv, P);
+    }
+#pragma unroll
+    for (int i = 0; i < kEPT; ++i) {
+      PCG_ELEM(i, base, n, lam, g);
+      (void)g;
+      double x = xr[i];
+      if (MODE == kInv && !ROW) {
+        x = mm_shoup(x, twc.x, twc.y, P.p);
+        const double wv = mm_shoup(twc.x, tws.x, tws.y, P.p);
+        twc = make_double2(wv, wv * P.pinv);
+      }
+      lds[GE::idx(n, lam)] = x;
     }
     __syncthreads();
-    line_dft<GE>(lds, tw, P, true);
-  }
+    const uint64_t wn = w + gridDim.x;
+    if (wn < ntiles) {
+      const uint64_t bn = tile_base(wn);
 #pragma unroll
-  for (int i = 0; i < kEPT; ++i) {
-    PCG_ELEM(i, n, lam, g);
-    double x = lds[GE::idx(n, lam)];
-    if (MODE == kFwd && !ROW) {
-      x = mm_shoup(x, twc.x, twc.y, P.p);
-      const double w = mm_shoup(twc.x, tws.x, tws.y, P.p);
-      twc = make_double2(w, w * P.pinv);
-    } else {
-      x = reduce(x, P.p, P.pinv);
+      for (int i = 0; i < kEPT; ++i) {
+        PCG_ELEM(i, bn, n, lam, g);
+        (void)n;
+        (void)lam;
+        xr[i] = src ? src[g] : (g < nlimbs ? (double)limbs[g] : 0.0);
+      }
     }
-    dst[g] = x;
+    line_dft<GE>(lds, tw, P, MODE == kInv);
+    if (MODE == kFused) {
+      const double* spec = a.spec + zoff + pi * a.N;
+#pragma unroll
+      for (int i = 0; i < kEPT; ++i) {
+        PCG_ELEM(i, base, n, lam, g);
+        const int li = GE::idx(n, lam);
+        const double x = mm_gen(lds[li], spec[g], P.p, P.pinv);
+        lds[li] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
+      }
+      __syncthreads();
+      line_dft<GE>(lds, tw, P, true);
+    }
+#pragma unroll
+    for (int i = 0; i < kEPT; ++i) {
+      PCG_ELEM(i, base, n, lam, g);
+      double x = lds[GE::idx(n, lam)];
+      if (MODE == kFwd && !ROW) {
+        x = mm_shoup(x, twc.x, twc.y, P.p);
+        const double wv = mm_shoup(twc.x, tws.x, tws.y, P.p);
+        twc = make_double2(wv, wv * P.pinv);
+      } else {
+        x = reduce(x, P.p, P.pinv);
+      }
+      dst[g] = x;
+    }
+    if (wn >= ntiles) break;
+    __syncthreads();  // the next tile overwrites LDS
+    w = wn;
   }
 #undef PCG_ELEM
 }

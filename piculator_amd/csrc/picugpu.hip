@@ -218,6 +218,7 @@ struct Ctx {
   // FFT engine tables and round-off slot
   double2* ftab[3] = {nullptr, nullptr, nullptr};
   unsigned long long* d_maxdist = nullptr;
+  int num_cu = 256;
   // device values: carry-scan blocks and readback slots
   CarryFn* df_agg = nullptr;
   uint64_t* df_cin = nullptr;
@@ -380,6 +381,24 @@ bool plan_supported(const Plan& pl) {
   return true;
 }
 
+// Resident workgroups per compute unit of a pass kernel instance (cached).
+int pass_occupancy(PassFn fn, int threads) {
+  static std::mutex mu;
+  static std::map<PassFn, int> memo;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = memo.find(fn);
+  if (it != memo.end()) return it->second;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(fn), threads, 0) !=
+          hipSuccess ||
+      nb < 1) {
+    (void)hipGetLastError();
+    nb = 2;
+  }
+  memo[fn] = nb;
+  return nb;
+}
+
 int launch_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double* src,
                 const uint32_t* limbs, uint64_t nlimbs, double* dst, const double* spec,
                 const BatchDesc* descs = nullptr, int operand = 0, uint64_t zstride = 0,
@@ -405,7 +424,14 @@ int launch_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double* 
     return fail(PCG_ERR_INTERNAL, "no NTT pass kernel for logN=%d logL=%d logS=%d mode=%d",
                 pl.logN, ps.logL, ps.logS, mode);
   const int E = 1 << pl.logE;
-  dim3 grid((unsigned)(a.N / E), 2, count), block(E / kEPT);
+  a.ntiles = a.N / E;
+  // Persistent tiles: as many workgroups as stay resident (occupancy of
+  // this instance x compute units), spread over the primes and products.
+  int per_cu = pass_occupancy(fn, E / kEPT);
+  uint64_t gx = ((uint64_t)per_cu * (uint64_t)c.num_cu) / (2ull * count);
+  if (gx < 1) gx = 1;
+  if (gx > a.ntiles) gx = a.ntiles;
+  dim3 grid((unsigned)gx, 2, count), block(E / kEPT);
   hipLaunchKernelGGL(fn, grid, block, 0, c.stream, a);
   return PCG_OK;
 }
@@ -1591,6 +1617,11 @@ int pcg_init(int device) {
   }
   if (device < 0 || device >= n) return fail(PCG_ERR_NO_DEVICE, "device %d out of range (%d devices)", device, n);
   HIPCHK(hipSetDevice(device));
+  {
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    c.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  }
   HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   int rc = build_tables(c);
   if (rc) return rc;
