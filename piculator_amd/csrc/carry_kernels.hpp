@@ -93,6 +93,7 @@ struct CarryArgs {
   double2 crt;       // p0^{-1} mod p1 as (w, w/p1)
   uint32_t* status;  // nonzero on overflow (null: no check)
   uint64_t* carry_out;  // carry out of the top block (null: not stored)
+  CarryFn* tot;         // scan scratch: one entry per 1024 blocks
   // FFT engine: unweighted inverse-transform output and round-off maximum
   const double2* z;
   unsigned long long* maxdist;  // bits of a non-negative double
@@ -269,27 +270,61 @@ __global__ __launch_bounds__(kCarryThreads) void k_carry_local(CarryArgs a) {
   }
 }
 
-// One workgroup of 1024 threads: exclusive scan of the block functions.
-__global__ __launch_bounds__(1024) void k_carry_scan(CarryArgs a) {
-  __shared__ CarryFn wtot[16];
-  const uint64_t n = a.nblk;
-  const uint64_t per = (n + blockDim.x - 1) / blockDim.x;
-  const uint64_t lo = (uint64_t)threadIdx.x * per;
-  const uint64_t hi = lo + per < n ? lo + per : n;
+// Exclusive scan of the block functions agg[0 .. nblk) in three launches:
+// k_scan_partial reduces each run of 1024 functions, k_scan_totals scans
+// those totals (one workgroup), k_scan_apply forms every block's prefix.
+// Every thread handles one function, so the loads are coalesced and none
+// waits on another.
+constexpr int kScanChunk = 1024;
+
+__global__ __launch_bounds__(kScanChunk) void k_scan_partial(CarryArgs a) {
+  __shared__ CarryFn wtot[kScanChunk / 64];
+  const uint64_t i = (uint64_t)blockIdx.x * kScanChunk + threadIdx.x;
   CarryFn f;
   f.base = 0;
   f.T = 0;
-  for (uint64_t i = lo; i < hi; ++i) f = cf_compose(f, a.agg[i]);
-  CarryFn run = cf_block_exclusive(f, wtot);
-  for (uint64_t i = lo; i < hi; ++i) {
-    a.cin[i] = cf_eval0(run);
-    run = cf_compose(run, a.agg[i]);
+  if (i < a.nblk) f = a.agg[i];
+  const CarryFn ex = cf_block_exclusive(f, wtot);
+  if (threadIdx.x == kScanChunk - 1) a.tot[blockIdx.x] = cf_compose(ex, f);
+}
+
+// One workgroup: tot[c] becomes the composition of all chunks before c;
+// the total is the carry out of the top block.
+__global__ __launch_bounds__(kScanChunk) void k_scan_totals(CarryArgs a, uint64_t ntot) {
+  __shared__ CarryFn wtot[kScanChunk / 64];
+  __shared__ CarryFn chunk_tot;
+  CarryFn run;
+  run.base = 0;
+  run.T = 0;
+  for (uint64_t c0 = 0; c0 < ntot; c0 += kScanChunk) {
+    const uint64_t i = c0 + threadIdx.x;
+    CarryFn f;
+    f.base = 0;
+    f.T = 0;
+    if (i < ntot) f = a.tot[i];
+    const CarryFn ex = cf_block_exclusive(f, wtot);
+    if (threadIdx.x == kScanChunk - 1) chunk_tot = cf_compose(ex, f);
+    if (i < ntot) a.tot[i] = cf_compose(run, ex);
+    __syncthreads();
+    run = cf_compose(run, chunk_tot);
+    __syncthreads();
   }
-  if (threadIdx.x == blockDim.x - 1) {
+  if (threadIdx.x == 0) {
     // carry out of the top block must vanish for an exact product
     if (a.status && cf_eval0(run) != 0) *a.status = 1;
     if (a.carry_out) *a.carry_out = cf_eval0(run);
   }
+}
+
+__global__ __launch_bounds__(kScanChunk) void k_scan_apply(CarryArgs a) {
+  __shared__ CarryFn wtot[kScanChunk / 64];
+  const uint64_t i = (uint64_t)blockIdx.x * kScanChunk + threadIdx.x;
+  CarryFn f;
+  f.base = 0;
+  f.T = 0;
+  if (i < a.nblk) f = a.agg[i];
+  const CarryFn ex = cf_block_exclusive(f, wtot);
+  if (i < a.nblk) a.cin[i] = cf_eval0(cf_compose(a.tot[blockIdx.x], ex));
 }
 
 __global__ __launch_bounds__(256) void k_carry_fix(CarryArgs a) {

@@ -325,7 +325,7 @@ int ensure_workspace(Ctx& c, int logN, uint64_t zL) {
     c.agg = nullptr;
     c.cin = nullptr;
     c.nblk_cap = 0;
-    HIPCHK(hipMalloc(&c.agg, nblk * sizeof(CarryFn)));
+    HIPCHK(hipMalloc(&c.agg, (nblk + nblk / kScanChunk + 1) * sizeof(CarryFn)));  // + scan totals
     HIPCHK(hipMalloc(&c.cin, nblk * sizeof(uint64_t)));
     c.nblk_cap = nblk;
   }
@@ -379,6 +379,15 @@ bool plan_supported(const Plan& pl) {
     if (j + 1 == m && !select_pass(kFused, pl, pl.passes[j])) return false;
   }
   return true;
+}
+
+// Scan of the block carry functions (three launches, see carry_kernels.hpp);
+// a.tot needs (nblk + 1023) / 1024 entries.
+void launch_carry_scan(Ctx& c, const CarryArgs& a) {
+  const uint64_t ntot = (a.nblk + kScanChunk - 1) / kScanChunk;
+  hipLaunchKernelGGL(k_scan_partial, dim3((unsigned)ntot), dim3(kScanChunk), 0, c.stream, a);
+  hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanChunk), 0, c.stream, a, ntot);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntot), dim3(kScanChunk), 0, c.stream, a);
 }
 
 // Resident workgroups per compute unit of a pass kernel instance (cached).
@@ -476,12 +485,13 @@ int ntt_mul_enqueue(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_
   ca.agg = c.agg;
   ca.cin = c.cin;
   ca.nblk = (zL + kCarryBlock - 1) / kCarryBlock;
+  ca.tot = c.agg + c.nblk_cap;
   ca.crt = c.crt;
   ca.status = c.status;
   ca.z = nullptr;
   ca.maxdist = nullptr;
   hipLaunchKernelGGL(k_carry_local<false>, dim3((unsigned)ca.nblk), dim3(kCarryThreads), 0, c.stream, ca);
-  hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, c.stream, ca);
+  launch_carry_scan(c, ca);
   hipLaunchKernelGGL(k_carry_fix, dim3((unsigned)((ca.nblk + 255) / 256)), dim3(256), 0, c.stream,
                      ca);
   if (profile) HIPCHK(hipEventRecord(c.ev[3], c.stream));
@@ -551,10 +561,11 @@ int fft_mul_enqueue(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_
   ca.agg = c.agg;
   ca.cin = c.cin;
   ca.nblk = (zL + kCarryBlock - 1) / kCarryBlock;
+  ca.tot = c.agg + c.nblk_cap;
   ca.status = nullptr;
   ca.maxdist = c.d_maxdist;
   hipLaunchKernelGGL(k_carry_local<true>, dim3((unsigned)ca.nblk), dim3(kCarryThreads), 0, c.stream, ca);
-  hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, c.stream, ca);
+  launch_carry_scan(c, ca);
   hipLaunchKernelGGL(k_carry_fix, dim3((unsigned)((ca.nblk + 255) / 256)), dim3(256), 0, c.stream, ca);
   HIPCHK(hipGetLastError());
   return PCG_OK;
@@ -1376,7 +1387,7 @@ int df_ensure_blocks(Ctx& c, uint64_t nblk) {
   c.df_agg = nullptr;
   c.df_cin = nullptr;
   c.df_nblk_cap = 0;
-  HIPCHK(hipMalloc(&c.df_agg, nblk * sizeof(CarryFn)));
+  HIPCHK(hipMalloc(&c.df_agg, (nblk + nblk / kScanChunk + 1) * sizeof(CarryFn)));  // + scan totals
   HIPCHK(hipMalloc(&c.df_cin, nblk * sizeof(uint64_t)));
   c.df_nblk_cap = nblk;
   return PCG_OK;
@@ -1418,8 +1429,9 @@ int df_carry_pass(Ctx& c, DfArgs a) {
   ca.agg = c.df_agg;
   ca.cin = c.df_cin;
   ca.nblk = nblk;
+  ca.tot = c.df_agg + c.df_nblk_cap;
   ca.carry_out = reinterpret_cast<uint64_t*>(c.df_slot);
-  hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, c.stream, ca);
+  launch_carry_scan(c, ca);
   hipLaunchKernelGGL(k_carry_fix, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, c.stream, ca);
   HIPCHK(hipGetLastError());
   return PCG_OK;
