@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void k_batch_school(const BatchDesc* descs) {
   }
 }
 
-// Block size N / 16 (N = 2^logN, 2^10 <= N <= 2^12); all products of one
+// Block size N / kEPT (N = 2^logN, 2^10 <= N <= 2^12); all products of one
 // launch share logN.  prm points at the constants for this N.
 template <int LOGN>
 __global__ __launch_bounds__((1 << LOGN) / kEPT) void k_batch_ntt(const BatchDesc* descs,
@@ -90,7 +90,7 @@ __global__ __launch_bounds__((1 << LOGN) / kEPT) void k_batch_ntt(const BatchDes
   // [0, LP): operand a's line; [LP, 2 LP): operand b's; [2 LP, 2 LP + N):
   // prime-0 results (natural order)
   __shared__ double lds[2 * LP + N];
-  __shared__ CarryFn wtot[4];
+  __shared__ CarryFn wtot[(N / kEPT + 63) / 64];
   __shared__ double2 ltw[LOGN <= kLogLdsTw ? (1 << LOGN) : 1];
   const BatchDesc d = descs[blockIdx.x];
   const int tid = threadIdx.x, nth = blockDim.x;

@@ -16,7 +16,7 @@
 // several adjacent columns for strided passes so that global accesses are
 // contiguous runs -- and runs Stockham autosort radix-8 stages (one radix-2
 // or radix-4 stage when l_j is not a multiple of 3) with each thread holding
-// 16 points in registers per stage.
+// kEPT (8) points in registers per stage.
 #pragma once
 #include "modfp64.hpp"
 
@@ -26,7 +26,7 @@ namespace pcg {
 
 constexpr int kLogE = 12;  // points staged per workgroup (max)
 constexpr int kE = 1 << kLogE;
-constexpr int kEPT = 16;   // points per thread per stage
+constexpr int kEPT = 8;    // points per thread per stage
 
 enum PassMode { kFwd = 0, kInv = 1, kFused = 2 };
 
@@ -242,7 +242,7 @@ __device__ __forceinline__ const double2* stage_twiddles(double2* ltw, const Pri
 // right after the current tile is in LDS, so its HBM reads are in flight
 // while the current tile's DFT stages and stores run.
 template <int MODE, int LOGL, int LOGLINES, bool ROW>
-__global__ __launch_bounds__(1 << (LOGL + LOGLINES - 4)) void k_ntt_pass(PassArgs a) {
+__global__ __launch_bounds__((1 << (LOGL + LOGLINES)) / kEPT) void k_ntt_pass(PassArgs a) {
   using GE = Geo<LOGL, LOGLINES, ROW>;
   __shared__ double lds[GE::LDS];
   __shared__ double2 ltw[LOGL <= kLogLdsTw ? (1 << LOGL) : 1];

@@ -762,15 +762,15 @@ int launch_classes(Ctx& c, const BatchDesc* base, size_t n_school, size_t n10, s
     hipLaunchKernelGGL(k_batch_school, dim3((unsigned)n_school), dim3(256), 0, c.stream, base);
   base += n_school;
   if (n10)
-    hipLaunchKernelGGL(k_batch_ntt<10>, dim3((unsigned)n10), dim3(64), 0, c.stream, base,
+    hipLaunchKernelGGL(k_batch_ntt<10>, dim3((unsigned)n10), dim3((1 << 10) / kEPT), 0, c.stream, base,
                        c.d_prime + 10 * 2, c.crt);
   base += n10;
   if (n11)
-    hipLaunchKernelGGL(k_batch_ntt<11>, dim3((unsigned)n11), dim3(128), 0, c.stream, base,
+    hipLaunchKernelGGL(k_batch_ntt<11>, dim3((unsigned)n11), dim3((1 << 11) / kEPT), 0, c.stream, base,
                        c.d_prime + 11 * 2, c.crt);
   base += n11;
   if (n12)
-    hipLaunchKernelGGL(k_batch_ntt<12>, dim3((unsigned)n12), dim3(256), 0, c.stream, base,
+    hipLaunchKernelGGL(k_batch_ntt<12>, dim3((unsigned)n12), dim3((1 << 12) / kEPT), 0, c.stream, base,
                        c.d_prime + 12 * 2, c.crt);
   HIPCHK(hipGetLastError());
   return PCG_OK;
