@@ -240,6 +240,57 @@ def sweep(steps: int):
                 buf.free()
 
 
+def _rocpd(dirname: str):
+    """sqlite connection to the rocpd database rocprofv3 wrote under dirname."""
+    import glob
+    import sqlite3
+
+    dbs = glob.glob(os.path.join(dirname, "**", "*.db"), recursive=True)
+    if not dbs:
+        raise SystemExit("no rocpd database under %s" % dirname)
+    return sqlite3.connect(dbs[0])
+
+
+def summarize_trace(dirname: str):
+    """Markdown table of kernel durations from a rocprofv3 --kernel-trace run."""
+    con = _rocpd(dirname)
+    rows = con.execute("select name, count(*), avg(duration), min(duration), max(duration), "
+                       "sum(duration) from kernels group by name order by sum(duration) desc").fetchall()
+    tot = sum(r[5] for r in rows)
+    print("| kernel | calls | avg us | min us | max us | total % |")
+    print("|---|---|---|---|---|---|")
+    for n, c, a, mn, mx, sm in rows:
+        n = n.replace("(pcg::PassArgs)", "").replace("void ", "")
+        print("| `%s` | %d | %.1f | %.1f | %.1f | %.1f |" % (n, c, a / 1e3, mn / 1e3, mx / 1e3,
+                                                          100.0 * sm / tot))
+    print("total kernel time %.3f ms" % (tot / 1e6))
+
+
+def summarize_traffic(fetch_dir: str, write_dir: str):
+    """HBM bytes per multiply from two rocprofv3 --pmc runs (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch) of `bench.py --mul-only --calib-copy`, each
+    scaled by what the same counter reads for the 512 MiB calibration copy."""
+    out = {}
+    for key, d, counter in (("read", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
+        con = _rocpd(d)
+        rows = con.execute("select kernel_name, sum(value) from counters_collection where counter_name = ? "
+                           "group by dispatch_id, kernel_name", (counter,)).fetchall()
+        copies = [v for n, v in rows if "copyBuffer" in n]
+        calib = max(copies) if copies else None
+        scale = (512 * 1024) / calib if calib else 1.0
+        mults = sum(1 for n, _ in rows if "k_carry_fix" in n)
+        per_kernel = {}
+        for n, v in rows:
+            if "pcg::" not in n:  # the calibration copy and its setup
+                continue
+            name = n.split("(")[0].replace("void ", "")
+            per_kernel[name] = per_kernel.get(name, 0.0) + v * 1024 * scale / max(mults, 1)
+        out[key] = {"scale": scale, "multiplies": mults, "bytes_per_multiply": sum(per_kernel.values()),
+                    "per_kernel_bytes": per_kernel}
+    out["hbm_bytes_per_multiply"] = out["read"]["bytes_per_multiply"] + out["write"]["bytes_per_multiply"]
+    print(json.dumps(out, indent=1, sort_keys=True))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,7 +312,17 @@ def main():
                     help="only the roofline multiply (for counter runs)")
     ap.add_argument("--sweep", action="store_true",
                     help="print the single-multiply table (sizes x shapes x engines) instead")
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="print the kernel table of a rocprofv3 --kernel-trace run (no GPU)")
+    ap.add_argument("--summarize-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="print HBM bytes per multiply from two rocprofv3 --pmc runs (no GPU)")
     args = ap.parse_args()
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace)
+        return
+    if args.summarize_traffic:
+        summarize_traffic(*args.summarize_traffic)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)

@@ -91,7 +91,7 @@ __global__ __launch_bounds__((1 << LOGN) / kEPT) void k_batch_ntt(const BatchDes
   // prime-0 results (natural order)
   __shared__ double lds[2 * LP + N];
   __shared__ CarryFn wtot[(N / kEPT + 63) / 64];
-  __shared__ double2 ltw[LOGN <= kLogLdsTw ? (1 << LOGN) : 1];
+  __shared__ double2 ltw[lds_tw_size<LOGN>()];
   const BatchDesc d = descs[blockIdx.x];
   const int tid = threadIdx.x, nth = blockDim.x;
   const int na = (int)d.na, nb = (int)d.nb, zL = na + nb;

@@ -145,9 +145,30 @@ struct Geo {
   }
 };
 
-// Stage twiddles omega_L^e (e < L) come from an LDS copy for lines up to
-// 2^10 points and from the global 4096-entry table otherwise.
+// Stage twiddles omega_L^e (e < L) come from an LDS copy of the table for
+// lines up to 2^10 points; longer lines keep two 64-entry LDS tables,
+// omega_4096^(64 h) and omega_4096^l, and form omega_4096^(64 h + l) with
+// one modular product (no global loads inside the stages).
 constexpr int kLogLdsTw = 10;
+constexpr int kTwSplit = 64;
+constexpr int kLdsTwEntries = 1 << kLogLdsTw;
+
+template <int LOGL>
+constexpr int lds_tw_size() {
+  return LOGL <= kLogLdsTw ? (1 << LOGL) : 2 * kTwSplit;
+}
+
+template <int LOGL>
+__device__ __forceinline__ double2 stage_tw(const double2* tw, int e, const Prime& P) {
+  if constexpr (LOGL <= kLogLdsTw) {
+    return tw[e];
+  } else {
+    const double2 lo = tw[e & (kTwSplit - 1)];
+    const double2 hi = tw[kTwSplit + (e >> 6)];
+    const double w = mm_shoup(hi.x, lo.x, lo.y, P.p);
+    return make_double2(w, w * P.pinv);
+  }
+}
 
 // One Stockham stage of radix R (Ns = 2^logNs) over every line in LDS.
 // tw: the twiddle table, omega_{2^logTw}^e.
@@ -176,7 +197,7 @@ __device__ __forceinline__ void stockham_stage(double* lds, const double2* tw, c
       for (int r = 1; r < R; ++r) {
         int e = (jm * r) << SH;
         if (inv) e = (TWN - e) & (TWN - 1);
-        const double2 t = tw[e];
+        const double2 t = stage_tw<LOGL>(tw, e, P);
         v[b][r] = mm_shoup(v[b][r], t.x, t.y, P.p);
       }
     }
@@ -227,10 +248,12 @@ __device__ __forceinline__ const double2* stage_twiddles(double2* ltw, const Pri
   constexpr int LOGL = __builtin_ctz(GE::L);
   if constexpr (LOGL <= kLogLdsTw) {
     for (int e = threadIdx.x; e < GE::L; e += blockDim.x) ltw[e] = P.twL[e << (kLogE - LOGL)];
-    return ltw;
   } else {
-    return P.twL;
+    // exponents of the 4096-entry table (LOGTW = kLogE): e = 64 h + l
+    for (int e = threadIdx.x; e < 2 * kTwSplit; e += blockDim.x)
+      ltw[e] = e < kTwSplit ? P.twL[e] : P.twL[(e - kTwSplit) * kTwSplit];
   }
+  return ltw;
 }
 
 // One pass.  Column passes (ROW false) need s >= LINES (one block of L*s
@@ -245,7 +268,7 @@ template <int MODE, int LOGL, int LOGLINES, bool ROW>
 __global__ __launch_bounds__((1 << (LOGL + LOGLINES)) / kEPT) void k_ntt_pass(PassArgs a) {
   using GE = Geo<LOGL, LOGLINES, ROW>;
   __shared__ double lds[GE::LDS];
-  __shared__ double2 ltw[LOGL <= kLogLdsTw ? (1 << LOGL) : 1];
+  __shared__ double2 ltw[lds_tw_size<LOGL>()];
   const int pi = blockIdx.y;
   const Prime P = a.prm[pi];
   const double2* tw = stage_twiddles<GE>(ltw, P);
