@@ -79,6 +79,110 @@ __global__ __launch_bounds__(256) void k_batch_school(const BatchDesc* descs) {
   }
 }
 
+// Schoolbook products of zL <= 4 G limbs with G threads per product (G = 16
+// or 64, a part of one wave) and 256 / G products per workgroup, for the many
+// small products of the lower tree levels.  The column sums are those of
+// k_batch_school; the carries are resolved inside the group with shuffles:
+// each thread composes the carry functions of its four limbs, the group
+// scans them (width-G shuffles), and each thread adds its carry-in.
+__device__ __forceinline__ CarryFn cf_shfl_up_w(CarryFn f, int d, int width) {
+  CarryFn r;
+  r.base = __shfl_up(f.base, d, width);
+  r.T = __shfl_up(f.T, d, width);
+  return r;
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void k_batch_school_g(const BatchDesc* descs, uint32_t count) {
+  constexpr int PPB = 256 / G;  // products per workgroup
+  constexpr int MAXL = 4 * G;   // zL bound
+  __shared__ uint32_t sa[PPB][MAXL], sbp[PPB][MAXL + 8];
+  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+  const uint32_t z = blockIdx.x * PPB + grp;
+  BatchDesc d;
+  d.a = nullptr;
+  d.b = nullptr;
+  d.out = nullptr;
+  d.na = 0;
+  d.nb = 0;
+  if (z < count) d = descs[z];
+  const int na = (int)d.na, nb = (int)d.nb, zL = na + nb;
+  for (int i = lane; i < na; i += G) sa[grp][i] = d.a[i];
+  for (int i = lane; i < MAXL + 8; i += G) {
+    const int j = i - 3;
+    sbp[grp][i] = (j >= 0 && j < nb) ? d.b[j] : 0u;
+  }
+  __syncthreads();
+  const int k0 = 4 * lane;
+  uint64_t lo[4] = {0, 0, 0, 0};
+  uint32_t hi[4] = {0, 0, 0, 0};
+  int ilo = k0 - nb + 1;
+  if (ilo < 0) ilo = 0;
+  const int ihi = k0 + 3 < na - 1 ? k0 + 3 : na - 1;
+  if (k0 < zL && ilo <= ihi) {
+    const uint32_t* sb = sbp[grp];
+    uint32_t w0 = sb[3 + k0 - ilo], w1 = sb[4 + k0 - ilo], w2 = sb[5 + k0 - ilo], w3 = sb[6 + k0 - ilo];
+    for (int i = ilo; i <= ihi; ++i) {
+      const uint64_t ai = sa[grp][i];
+      const uint64_t p0 = ai * w0, p1 = ai * w1, p2 = ai * w2, p3 = ai * w3;
+      lo[0] += p0;
+      hi[0] += lo[0] < p0;
+      lo[1] += p1;
+      hi[1] += lo[1] < p1;
+      lo[2] += p2;
+      hi[2] += lo[2] < p2;
+      lo[3] += p3;
+      hi[3] += lo[3] < p3;
+      w3 = w2;
+      w2 = w1;
+      w1 = w0;
+      const int nxt = 2 + k0 - i;  // b[k0 - (i + 1)], needed while i + 1 <= k0 + 3
+      w0 = nxt >= 0 ? sb[nxt] : 0u;
+    }
+  }
+  // limb k receives w0_k + w1_{k-1} + w2_{k-2}; the previous thread's top
+  // words come by shuffle (zero for the group's first thread)
+  uint32_t x0[4], x1[4], x2[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    x0[q] = (uint32_t)lo[q];
+    x1[q] = (uint32_t)(lo[q] >> 32);
+    x2[q] = hi[q];
+  }
+  uint32_t p1 = __shfl_up(x1[3], 1, G), p22 = __shfl_up(x2[2], 1, G), p23 = __shfl_up(x2[3], 1, G);
+  if (lane == 0) p1 = p22 = p23 = 0;
+  uint64_t s[4];
+  s[0] = (uint64_t)x0[0] + p1 + p22;
+  s[1] = (uint64_t)x0[1] + x1[0] + p23;
+  s[2] = (uint64_t)x0[2] + x1[1] + x2[0];
+  s[3] = (uint64_t)x0[3] + x1[2] + x2[1];
+  CarryFn f;
+  f.base = 0;
+  f.T = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    CarryFn g;
+    g.base = s[q] >> 32;
+    g.T = (1ull << 32) - (s[q] & 0xffffffffull);
+    f = cf_compose(f, g);
+  }
+  CarryFn incl = f;
+#pragma unroll
+  for (int dd = 1; dd < G; dd <<= 1) {
+    const CarryFn o = cf_shfl_up_w(incl, dd, G);
+    if (lane >= dd) incl = cf_compose(o, incl);
+  }
+  CarryFn excl = cf_shfl_up_w(incl, 1, G);
+  if (lane == 0) excl.T = 0;
+  uint64_t c = cf_eval0(excl);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint64_t t = (s[q] & 0xffffffffull) + c;
+    if (k0 + q < zL) d.out[k0 + q] = (uint32_t)t;
+    c = (s[q] >> 32) + (t >> 32);
+  }
+}
+
 // Block size N / kEPT (N = 2^logN, 2^10 <= N <= 2^12); all products of one
 // launch share logN.  prm points at the constants for this N.
 template <int LOGN>

@@ -651,22 +651,28 @@ int ensure_pool(Ctx& c, size_t limbs, size_t ndesc) {
 
 // Size classes of the batched paths.  Small products (zL = na + nb in
 // (2^(b-1), 2^b], b <= 12) go to the class g_small_class[b]: 0 the
-// schoolbook kernel (zL <= 1024), 10..12 the single-workgroup NTT of 2^k
-// points; the autotuner (tune_small) picks between them per bucket on the
-// device at first use.  Classes 13..17 are the multi-pass NTT run for the
-// whole class at once, -1 one product at a time.
+// 256-thread schoolbook (zL <= 1024), 1 and 2 the grouped schoolbook with 16
+// or 64 threads per product (zL <= 64, <= 256), 10..12 the single-workgroup
+// NTT of 2^k points; the autotuner (tune_small) picks among them per bucket
+// on the device at first use.  Classes 13..17 are the multi-pass NTT run for
+// the whole class at once, -1 one product at a time.
 constexpr int kBatchMaxLog = 17;
 constexpr int kSmallNttMinLog = 10;
 int g_small_class[kSmallNttMaxLog + 1];
 bool g_small_ready = false;
 
+// Largest zL of a schoolbook class.
+int school_max(int cls) { return cls == 1 ? 64 : cls == 2 ? 256 : kSchoolMaxLimbs; }
+
 int small_class_default(int b) {
+  if ((1 << b) <= 64) return 1;
+  if ((1 << b) <= 256) return 2;
   if ((1 << b) <= kSchoolMaxLimbs) return 0;
   return b < kSmallNttMinLog ? kSmallNttMinLog : b;
 }
 
 bool small_class_valid(int b, int cls) {
-  if (cls == 0) return (1 << b) <= kSchoolMaxLimbs;
+  if (cls >= 0 && cls <= 2) return (1 << b) <= school_max(cls);
   return cls >= kSmallNttMinLog && cls <= kSmallNttMaxLog && cls >= b;
 }
 
@@ -755,36 +761,39 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN) {
   return PCG_OK;
 }
 
-// Launches the single-workgroup classes over consecutive descriptors
-// (already on the device): schoolbook, then NTT 2^10, 2^11, 2^12.
-int launch_classes(Ctx& c, const BatchDesc* base, size_t n_school, size_t n10, size_t n11, size_t n12) {
-  if (n_school)
-    hipLaunchKernelGGL(k_batch_school, dim3((unsigned)n_school), dim3(256), 0, c.stream, base);
-  base += n_school;
-  if (n10)
-    hipLaunchKernelGGL(k_batch_ntt<10>, dim3((unsigned)n10), dim3((1 << 10) / kEPT), 0, c.stream, base,
-                       c.d_prime + 10 * 2, c.crt);
-  base += n10;
-  if (n11)
-    hipLaunchKernelGGL(k_batch_ntt<11>, dim3((unsigned)n11), dim3((1 << 11) / kEPT), 0, c.stream, base,
-                       c.d_prime + 11 * 2, c.crt);
-  base += n11;
-  if (n12)
-    hipLaunchKernelGGL(k_batch_ntt<12>, dim3((unsigned)n12), dim3((1 << 12) / kEPT), 0, c.stream, base,
-                       c.d_prime + 12 * 2, c.crt);
+// Launches one small class (0..2 schoolbook, 10..12 single-workgroup NTT)
+// over n consecutive descriptors (already on the device).
+int launch_small_class(Ctx& c, int cls, const BatchDesc* base, size_t n) {
+  if (n == 0) return PCG_OK;
+  switch (cls) {
+    case 0:
+      hipLaunchKernelGGL(k_batch_school, dim3((unsigned)n), dim3(256), 0, c.stream, base);
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_batch_school_g<16>, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, c.stream,
+                         base, (uint32_t)n);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_batch_school_g<64>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.stream,
+                         base, (uint32_t)n);
+      break;
+    case 10:
+      hipLaunchKernelGGL(k_batch_ntt<10>, dim3((unsigned)n), dim3((1 << 10) / kEPT), 0, c.stream, base,
+                         c.d_prime + 10 * 2, c.crt);
+      break;
+    case 11:
+      hipLaunchKernelGGL(k_batch_ntt<11>, dim3((unsigned)n), dim3((1 << 11) / kEPT), 0, c.stream, base,
+                         c.d_prime + 11 * 2, c.crt);
+      break;
+    case 12:
+      hipLaunchKernelGGL(k_batch_ntt<12>, dim3((unsigned)n), dim3((1 << 12) / kEPT), 0, c.stream, base,
+                         c.d_prime + 12 * 2, c.crt);
+      break;
+    default:
+      return fail(PCG_ERR_INTERNAL, "bad small class %d", cls);
+  }
   HIPCHK(hipGetLastError());
   return PCG_OK;
-}
-
-// Launches one small class (0 or 10..12) over n descriptors.
-int launch_small_class(Ctx& c, int cls, const BatchDesc* base, size_t n) {
-  switch (cls) {
-    case 0: return launch_classes(c, base, n, 0, 0, 0);
-    case 10: return launch_classes(c, base, 0, n, 0, 0);
-    case 11: return launch_classes(c, base, 0, 0, n, 0);
-    case 12: return launch_classes(c, base, 0, 0, 0, n);
-    default: return fail(PCG_ERR_INTERNAL, "bad small class %d", cls);
-  }
 }
 
 __global__ void k_fill_pattern(uint32_t* p, uint64_t n) {
@@ -794,10 +803,11 @@ __global__ void k_fill_pattern(uint32_t* p, uint64_t n) {
 
 // Autotuner of the small-product classes: for each bucket b = 4..10 it
 // times 2048 products of na = nb = 3 * 2^(b-3) limbs (zL = 3/4 of the
-// bucket's top) through the schoolbook kernel and through the 2^10-point
-// NTT (best of three launches, HIP events on the library stream) and keeps
-// the faster; smaller buckets stay schoolbook, 11 and 12 are NTT only.
-// Results are exact either way: the choice only moves the crossover.
+// bucket's top) through every class valid for the bucket (the schoolbook
+// kernels and the 2^10-point NTT; best of three launches, HIP events on the
+// library stream) and keeps the fastest; smaller buckets take the smallest
+// schoolbook, 11 and 12 are NTT only.  Results are exact in every class:
+// the choice only moves the crossovers.
 int tune_small(Ctx& c) {
   for (int b = 0; b <= kSmallNttMaxLog; ++b) g_small_class[b] = small_class_default(b);
   constexpr int kCount = 2048;
@@ -831,9 +841,12 @@ int tune_small(Ctx& c) {
       rc = fail(PCG_ERR_HIP, "autotuner upload failed");
       break;
     }
-    float best[2] = {1e30f, 1e30f};
-    const int cls[2] = {0, kSmallNttMinLog};
-    for (int m = 0; m < 2 && rc == PCG_OK; ++m)
+    const int cls[4] = {1, 2, 0, kSmallNttMinLog};
+    float best_ms = 1e30f;
+    int best_cls = small_class_default(b);
+    for (int m = 0; m < 4 && rc == PCG_OK; ++m) {
+      if (!small_class_valid(b, cls[m])) continue;
+      float best = 1e30f;
       for (int rep = 0; rep < 4 && rc == PCG_OK; ++rep) {
         if (hipEventRecord(c.tev[0], c.stream) != hipSuccess) rc = fail(PCG_ERR_HIP, "event record failed");
         if (!rc) rc = launch_small_class(c, cls[m], dd, kCount);
@@ -842,9 +855,14 @@ int tune_small(Ctx& c) {
                     hipEventSynchronize(c.tev[1]) != hipSuccess ||
                     hipEventElapsedTime(&ms, c.tev[0], c.tev[1]) != hipSuccess))
           rc = fail(PCG_ERR_HIP, "autotuner timing failed");
-        if (!rc && rep > 0) best[m] = std::min(best[m], ms);
+        if (!rc && rep > 0) best = std::min(best, ms);
       }
-    if (!rc) g_small_class[b] = best[0] <= best[1] ? 0 : kSmallNttMinLog;
+      if (!rc && best < best_ms) {
+        best_ms = best;
+        best_cls = cls[m];
+      }
+    }
+    if (!rc) g_small_class[b] = best_cls;
   }
   (void)hipStreamSynchronize(c.stream);
   (void)hipFree(in);
@@ -858,9 +876,9 @@ int tune_small(Ctx& c) {
 int ensure_tuned(Ctx& c) { return g_small_ready ? PCG_OK : tune_small(c); }
 
 int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, bool dev) {
-  // class slots: 0 schoolbook, 1..3 single-workgroup NTT (2^10 .. 2^12),
-  // 4..8 batched multi-pass (2^13 .. 2^17)
-  constexpr int kSlots = 4 + (kBatchMaxLog - kSmallNttMaxLog);
+  // class slots: 0..2 schoolbook classes 0..2, 3..5 single-workgroup NTT
+  // (2^10 .. 2^12), 6..10 batched multi-pass (2^13 .. 2^17)
+  constexpr int kSlots = 6 + (kBatchMaxLog - kSmallNttMaxLog);
   int rc0 = ensure_tuned(c);
   if (rc0) return rc0;
   std::vector<uint64_t> cls[kSlots], large;
@@ -884,7 +902,7 @@ int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, 
     if (k < 0)
       large.push_back(i);
     else
-      cls[k == 0 ? 0 : k - kSmallNttMinLog + 1].push_back(i);
+      cls[k <= 2 ? k : k - kSmallNttMinLog + 3].push_back(i);
   }
   size_t nsmall = 0;
   for (int k = 0; k < kSlots; ++k) nsmall += cls[k].size();
@@ -922,12 +940,15 @@ int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, 
       HIPCHK(hipMemcpyAsync(c.d_pool, c.h_pool, in_limbs * 4, hipMemcpyHostToDevice, c.stream));
     HIPCHK(hipMemcpyAsync(c.d_desc, c.h_desc, nsmall * sizeof(BatchDesc), hipMemcpyHostToDevice,
                           c.stream));
-    rc = launch_classes(c, c.d_desc, cls[0].size(), cls[1].size(), cls[2].size(), cls[3].size());
-    if (rc) return rc;
-    size_t off = cls[0].size() + cls[1].size() + cls[2].size() + cls[3].size();
-    for (int k = 4; k < kSlots; ++k) {
+    size_t off = 0;
+    for (int k = 0; k < 6; ++k) {
+      rc = launch_small_class(c, k <= 2 ? k : kSmallNttMinLog + k - 3, c.d_desc + off, cls[k].size());
+      if (rc) return rc;
+      off += cls[k].size();
+    }
+    for (int k = 6; k < kSlots; ++k) {
       if (cls[k].empty()) continue;
-      rc = batched_multipass(c, c.d_desc + off, cls[k].size(), kSmallNttMaxLog + k - 3);
+      rc = batched_multipass(c, c.d_desc + off, cls[k].size(), kSmallNttMaxLog + k - 5);
       if (rc) return rc;
       off += cls[k].size();
     }

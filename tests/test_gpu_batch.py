@@ -84,36 +84,44 @@ def test_batch_dev_pointers(gpu):
         assert np.array_equal(out, exact_product(a, b))
 
 
+SCHOOL_MAX = {0: 1024, 1: 64, 2: 256}
+
+
+def class_valid(b, c):
+    if c in SCHOOL_MAX:
+        return (1 << b) <= SCHOOL_MAX[c]
+    return 10 <= c <= 12 and c >= b
+
+
 def test_autotune_table_is_valid(gpu):
     t = gpu.autotune()
     assert len(t) == 13
     for b, c in enumerate(t):
+        assert class_valid(b, c), (b, c)
         if (1 << b) > 1024:
             assert c == b if b >= 10 else c == 10
-        else:
-            assert c in (0, 10)
     assert gpu.batch_table() == t
 
 
-@pytest.mark.parametrize("forced", ["school", "ntt"])
+@pytest.mark.parametrize("forced", [0, 1, 2, 10])
 def test_forced_small_classes_exact(gpu, forced):
-    """Both kernels of the tuned crossover (schoolbook, 2^10-point NTT) give
-    the exact product for every bucket up to 1024 limbs, including the
-    all-ones carry chains and unbalanced shapes."""
+    """Every kernel of the tuned crossover (schoolbook with 256 threads, with
+    16 or 64 threads per product, 2^10-point NTT) gives the exact product in
+    every bucket it may serve, including all-ones carry chains and unbalanced
+    shapes."""
     saved = gpu.batch_table()
-    table = list(saved)
-    for b in range(11):
-        table[b] = 0 if forced == "school" else 10
+    table = [forced if class_valid(b, forced) else c for b, c in enumerate(saved)]
     gpu.batch_table(table)
     try:
-        rng = random.Random(5 if forced == "school" else 6)
+        rng = random.Random(5 + forced)
         sizes = [(1, 1), (2, 1), (3, 4), (7, 9), (16, 16), (33, 31), (64, 64), (100, 27),
                  (128, 128), (255, 257), (300, 212), (384, 384), (511, 513), (1, 1023),
-                 (1023, 1), (700, 324)]
+                 (1023, 1), (700, 324), (1, 63), (31, 33), (40, 24), (1, 255), (128, 127)]
         pairs = []
         for k, (na, nb) in enumerate(sizes):
             ones = k % 3 == 2
             pairs.append((rand_limbs(rng, na, ones), rand_limbs(rng, nb, ones)))
+        pairs = pairs * 3  # several products per launch of each class
         outs = gpu.mul_batch(pairs)
         for (a, b), o in zip(pairs, outs):
             assert np.array_equal(o, exact_product(a, b)), (forced, a.size, b.size)
