@@ -241,6 +241,109 @@ __device__ __forceinline__ void line_dft(double* lds, const double2* tw, const P
   }
 }
 
+// Pieces of a Stockham stage for the pass kernel, whose first stage takes
+// its inputs from registers (loaded straight from HBM) and whose last stage
+// leaves its outputs in registers (stored straight to HBM): two LDS round
+// trips and four barriers fewer per tile than a line DFT done wholly in LDS.
+// v[b][r] is input r of butterfly bf = tid + b * NTH; in the last stage
+// (Ns = L / R) every output r of a butterfly lands where its input r came
+// from: point j + r (L / R) of line lam.
+template <class GE, int LOGR>
+__device__ __forceinline__ void stage_pos(int b, int r, int& lam, int& n) {
+  constexpr int LOGLR = __builtin_ctz(GE::L) - LOGR;
+  int j;
+  GE::split(threadIdx.x + b * GE::NTH, LOGLR, lam, j);
+  n = j + (r << LOGLR);
+}
+
+template <class GE, int R, int LOGR, int LOGNS>
+__device__ __forceinline__ void stage_compute(double (&v)[kEPT / R][R], const double2* tw,
+                                              const Prime& P, bool inv) {
+  constexpr int LOGL = __builtin_ctz(GE::L);
+  constexpr int LOGTW = LOGL <= kLogLdsTw ? LOGL : kLogE;
+  constexpr int TWN = 1 << LOGTW;
+  constexpr int LOGLR = LOGL - LOGR;
+  constexpr int NS = 1 << LOGNS;
+  constexpr int SH = LOGTW - LOGNS - LOGR;
+#pragma unroll
+  for (int b = 0; b < kEPT / R; ++b) {
+    if (LOGNS > 0) {
+      int lam, j;
+      GE::split(threadIdx.x + b * GE::NTH, LOGLR, lam, j);
+      const int jm = j & (NS - 1);
+      v[b][0] = reduce(v[b][0], P.p, P.pinv);
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        int e = (jm * r) << SH;
+        if (inv) e = (TWN - e) & (TWN - 1);
+        const double2 t = stage_tw<LOGL>(tw, e, P);
+        v[b][r] = mm_shoup(v[b][r], t.x, t.y, P.p);
+      }
+    }
+    dft_core<R>(v[b], P, inv);
+  }
+}
+
+template <class GE, int R, int LOGR, int LOGNS>
+__device__ __forceinline__ void stage_write(const double (&v)[kEPT / R][R], double* lds) {
+  constexpr int LOGLR = __builtin_ctz(GE::L) - LOGR;
+  constexpr int NS = 1 << LOGNS;
+#pragma unroll
+  for (int b = 0; b < kEPT / R; ++b) {
+    int lam, j;
+    GE::split(threadIdx.x + b * GE::NTH, LOGLR, lam, j);
+    const int jm = j & (NS - 1);
+    const int idxD = ((j - jm) << LOGR) + jm;
+#pragma unroll
+    for (int r = 0; r < R; ++r) lds[GE::idx(idxD + (r << LOGNS), lam)] = v[b][r];
+  }
+}
+
+// Radix-8 stages with Ns = 2^LOGNS .. L / 64 (all but the last), in LDS.
+template <class GE, int LOGNS>
+__device__ __forceinline__ void middle_stages(double* lds, const double2* tw, const Prime& P,
+                                              bool inv) {
+  if constexpr (LOGNS + 3 < __builtin_ctz(GE::L)) {
+    stockham_stage<GE, 8, 3, LOGNS>(lds, tw, P, inv);
+    middle_stages<GE, LOGNS + 3>(lds, tw, P, inv);
+  }
+}
+
+// The last (radix-8) stage: LDS -> registers, y[r] = output at stage_pos<GE,3>(0, r).
+template <class GE>
+__device__ __forceinline__ void last_stage_to_regs(const double* lds, double (&y)[1][8],
+                                                   const double2* tw, const Prime& P, bool inv) {
+  constexpr int LOGL = __builtin_ctz(GE::L);
+  static_assert(kEPT == 8, "one radix-8 butterfly per thread in the last stage");
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    int lam, n;
+    stage_pos<GE, 3>(0, r, lam, n);
+    y[0][r] = lds[GE::idx(n, lam)];
+  }
+  stage_compute<GE, 8, 3, LOGL - 3>(y, tw, P, inv);
+}
+
+// Line DFT whose inputs are the registers x (stage_pos<GE, LOGR1> order) and
+// whose outputs are returned in y (stage_pos<GE, 3>(0, r) order).
+template <class GE>
+__device__ __forceinline__ void line_dft_regs(double (&x)[kEPT], double (&y)[1][8], double* lds,
+                                              const double2* tw, const Prime& P, bool inv) {
+  constexpr int LOGL = __builtin_ctz(GE::L);
+  constexpr int LOGR1 = LOGL % 3 == 1 ? 1 : (LOGL % 3 == 2 ? 2 : 3);
+  constexpr int R1 = 1 << LOGR1;
+  double v[kEPT / R1][R1];
+#pragma unroll
+  for (int b = 0; b < kEPT / R1; ++b)
+#pragma unroll
+    for (int r = 0; r < R1; ++r) v[b][r] = x[b * R1 + r];
+  stage_compute<GE, R1, LOGR1, 0>(v, tw, P, inv);
+  stage_write<GE, R1, LOGR1, 0>(v, lds);
+  __syncthreads();
+  middle_stages<GE, LOGR1>(lds, tw, P, inv);
+  last_stage_to_regs<GE>(lds, y, tw, P, inv);
+}
+
 // Stage-twiddle table of a line length: LDS copy (omega_L^e) when small
 // enough, else the global 4096-entry table.  Call before the first barrier.
 template <class GE>
@@ -261,18 +364,22 @@ __device__ __forceinline__ const double2* stage_twiddles(double2* ltw, const Pri
 // passes have s == 1 and take LINES consecutive lines.
 //
 // A workgroup walks tiles w = blockIdx.x, + gridDim.x, ... (a.ntiles per
-// prime and product).  The next tile's points are loaded into registers
-// right after the current tile is in LDS, so its HBM reads are in flight
-// while the current tile's DFT stages and stores run.
+// prime and product).  Each thread loads the inputs of its first-stage
+// butterflies straight from HBM and stores the outputs of its last-stage
+// butterfly straight to HBM; the next tile's inputs are loaded right after
+// the current tile's first stage, so they are in flight during its
+// remaining stages.
 template <int MODE, int LOGL, int LOGLINES, bool ROW>
 __global__ __launch_bounds__((1 << (LOGL + LOGLINES)) / kEPT) void k_ntt_pass(PassArgs a) {
   using GE = Geo<LOGL, LOGLINES, ROW>;
+  constexpr int LOGR1 = LOGL % 3 == 1 ? 1 : (LOGL % 3 == 2 ? 2 : 3);
+  constexpr int R1 = 1 << LOGR1;
+  constexpr int NB1 = kEPT / R1;
   __shared__ double lds[GE::LDS];
   __shared__ double2 ltw[lds_tw_size<LOGL>()];
   const int pi = blockIdx.y;
   const Prime P = a.prm[pi];
   const double2* tw = stage_twiddles<GE>(ltw, P);
-  const int tid = threadIdx.x;
   const int logS = a.logS;
   uint64_t zoff = 0;
   const uint32_t* limbs = a.limbs;
@@ -289,111 +396,120 @@ __global__ __launch_bounds__((1 << (LOGL + LOGLINES)) / kEPT) void k_ntt_pass(Pa
   const uint64_t ntiles = a.ntiles;
 
   // first point of tile t's lines (the global step between them is implied
-  // by the layout)
+  // by the layout), and the global index of point n of line lam
   auto tile_base = [&](uint64_t t) -> uint64_t {
     if (ROW) return t << (LOGL + LOGLINES);
     const int lgc = logS - LOGLINES;  // column groups per block, log2
     return ((t >> lgc) << (LOGL + logS)) + ((t & ((1ull << lgc) - 1)) << LOGLINES);
   };
-  // element i of this thread: LDS point (n, lam) and its global index
-#define PCG_ELEM(i, base, n, lam, g)                                  \
-  int n, lam;                                                         \
-  uint64_t g;                                                         \
-  {                                                                   \
-    const int f = tid + (i) * GE::NTH;                                \
-    if (ROW) {                                                        \
-      lam = f >> LOGL;                                                \
-      n = f & (GE::L - 1);                                            \
-      g = (base) + (uint64_t)f;                                       \
-    } else {                                                          \
-      lam = f & (GE::LINES - 1);                                      \
-      n = f >> LOGLINES;                                              \
-      g = (base) + ((uint64_t)n << logS) + lam;                       \
-    }                                                                 \
-  }
+  auto gidx = [&](uint64_t base, int n, int lam) -> uint64_t {
+    return ROW ? base + ((uint64_t)lam << LOGL) + (uint64_t)n : base + ((uint64_t)n << logS) + lam;
+  };
+  // inputs of this thread's first-stage butterflies: x[b R1 + r]
+  auto load_tile = [&](uint64_t t, double (&x)[kEPT]) {
+    const uint64_t b0 = tile_base(t);
+#pragma unroll
+    for (int b = 0; b < NB1; ++b)
+#pragma unroll
+      for (int r = 0; r < R1; ++r) {
+        int lam, n;
+        stage_pos<GE, LOGR1>(b, r, lam, n);
+        const uint64_t g = gidx(b0, n, lam);
+        x[b * R1 + r] = src ? src[g] : (g < nlimbs ? (double)limbs[g] : 0.0);
+      }
+  };
+  // Inter-pass twiddle omega^{c n} of a column pass, c = this thread's
+  // column (fixed: NTH is a multiple of LINES), along the thread's points
+  // n = n0 + k * step: base omega^{c n0} times powers of omega^{c step}.
+  auto col_twiddle = [&](uint64_t base, int lam, uint64_t n0, uint64_t step, double2& cur,
+                         double2& stp) {
+    const uint64_t c = (base & ((1ull << logS) - 1)) + (uint64_t)lam;
+    cur = root_pow(c * n0, logRoot, MODE == kInv, P);
+    stp = root_pow(c * step, logRoot, MODE == kInv, P);
+  };
 
   uint64_t w = blockIdx.x;
   if (w >= ntiles) return;
   double xr[kEPT];
-  {
-    const uint64_t b0 = tile_base(w);
-#pragma unroll
-    for (int i = 0; i < kEPT; ++i) {
-      PCG_ELEM(i, b0, n, lam, g);
-      (void)n;
-      (void)lam;
-      xr[i] = src ? src[g] : (g < nlimbs ? (double)limbs[g] : 0.0);
-    }
-  }
+  load_tile(w, xr);
   for (;;) {
     const uint64_t base = tile_base(w);
-    const uint64_t col0 = ROW ? 0 : (base & ((1ull << logS) - 1));
-    // Inter-pass twiddles of a column pass: this thread's column c is fixed
-    // and its points are n0 + i * (NTH / LINES), so omega^{c n} follows from
-    // omega^{c n0} and the step omega^{c NTH / LINES} by exact modular
-    // products (no table loads per point).
-    double2 twc = make_double2(1.0, 0.0), tws = make_double2(1.0, 0.0);
-    if (!ROW && MODE != kFused) {
-      const uint64_t c = col0 + (tid & (GE::LINES - 1));
-      const uint64_t n0 = (uint64_t)(tid >> LOGLINES);
-      twc = root_pow(c * n0, logRoot, MODE == kInv, P);
-      tws = root_pow(c * (uint64_t)(GE::NTH >> LOGLINES), logRoot, MODE == kInv, P);
-    }
+    if (MODE == kInv && !ROW) {
+      // pre-twiddle: butterfly b's points are j_b + r L / R1
 #pragma unroll
-    for (int i = 0; i < kEPT; ++i) {
-      PCG_ELEM(i, base, n, lam, g);
-      (void)g;
-      double x = xr[i];
-      if (MODE == kInv && !ROW) {
-        x = mm_shoup(x, twc.x, twc.y, P.p);
-        const double wv = mm_shoup(twc.x, tws.x, tws.y, P.p);
-        twc = make_double2(wv, wv * P.pinv);
+      for (int b = 0; b < NB1; ++b) {
+        int lam, n0;
+        stage_pos<GE, LOGR1>(b, 0, lam, n0);
+        double2 cur, stp;
+        col_twiddle(base, lam, (uint64_t)n0, (uint64_t)(GE::L >> LOGR1), cur, stp);
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+          double& x = xr[b * R1 + r];
+          x = mm_shoup(x, cur.x, cur.y, P.p);
+          const double wv = mm_shoup(cur.x, stp.x, stp.y, P.p);
+          cur = make_double2(wv, wv * P.pinv);
+        }
       }
-      lds[GE::idx(n, lam)] = x;
     }
+    // first stage from registers
+    double v1[NB1][R1];
+#pragma unroll
+    for (int b = 0; b < NB1; ++b)
+#pragma unroll
+      for (int r = 0; r < R1; ++r) v1[b][r] = xr[b * R1 + r];
+    stage_compute<GE, R1, LOGR1, 0>(v1, tw, P, MODE == kInv);
+    stage_write<GE, R1, LOGR1, 0>(v1, lds);
     __syncthreads();
     const uint64_t wn = w + gridDim.x;
-    if (wn < ntiles) {
-      const uint64_t bn = tile_base(wn);
-#pragma unroll
-      for (int i = 0; i < kEPT; ++i) {
-        PCG_ELEM(i, bn, n, lam, g);
-        (void)n;
-        (void)lam;
-        xr[i] = src ? src[g] : (g < nlimbs ? (double)limbs[g] : 0.0);
-      }
-    }
-    line_dft<GE>(lds, tw, P, MODE == kInv);
+    if (wn < ntiles) load_tile(wn, xr);
+    middle_stages<GE, LOGR1>(lds, tw, P, MODE == kInv);
+    double y[1][8];
+    last_stage_to_regs<GE>(lds, y, tw, P, MODE == kInv);
+    int lam_o, n_o0;
+    stage_pos<GE, 3>(0, 0, lam_o, n_o0);
     if (MODE == kFused) {
+      // pointwise with a's spectrum and N^-1, then the inverse DFT in LDS
       const double* spec = a.spec + zoff + pi * a.N;
 #pragma unroll
-      for (int i = 0; i < kEPT; ++i) {
-        PCG_ELEM(i, base, n, lam, g);
-        const int li = GE::idx(n, lam);
-        const double x = mm_gen(lds[li], spec[g], P.p, P.pinv);
-        lds[li] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
+      for (int r = 0; r < 8; ++r) {
+        const uint64_t g = gidx(base, n_o0 + r * (GE::L >> 3), lam_o);
+        const double x = mm_gen(y[0][r], spec[g], P.p, P.pinv);
+        y[0][r] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
       }
-      __syncthreads();
-      line_dft<GE>(lds, tw, P, true);
-    }
+      __syncthreads();  // every last-stage read of LDS is done
 #pragma unroll
-    for (int i = 0; i < kEPT; ++i) {
-      PCG_ELEM(i, base, n, lam, g);
-      double x = lds[GE::idx(n, lam)];
+      for (int r = 0; r < 8; ++r) lds[GE::idx(n_o0 + r * (GE::L >> 3), lam_o)] = y[0][r];
+      __syncthreads();
+      if constexpr (LOGR1 == 1) {
+        stockham_stage<GE, 2, 1, 0>(lds, tw, P, true);
+      } else if constexpr (LOGR1 == 2) {
+        stockham_stage<GE, 4, 2, 0>(lds, tw, P, true);
+      } else {
+        stockham_stage<GE, 8, 3, 0>(lds, tw, P, true);
+      }
+      middle_stages<GE, LOGR1>(lds, tw, P, true);
+      last_stage_to_regs<GE>(lds, y, tw, P, true);
+    }
+    // last-stage outputs straight to HBM: point n_o0 + r L / 8 of line lam_o
+    double2 cur = make_double2(1.0, 0.0), stp = make_double2(1.0, 0.0);
+    if (MODE == kFwd && !ROW)
+      col_twiddle(base, lam_o, (uint64_t)n_o0, (uint64_t)(GE::L >> 3), cur, stp);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      double x = y[0][r];
       if (MODE == kFwd && !ROW) {
-        x = mm_shoup(x, twc.x, twc.y, P.p);
-        const double wv = mm_shoup(twc.x, tws.x, tws.y, P.p);
-        twc = make_double2(wv, wv * P.pinv);
+        x = mm_shoup(x, cur.x, cur.y, P.p);
+        const double wv = mm_shoup(cur.x, stp.x, stp.y, P.p);
+        cur = make_double2(wv, wv * P.pinv);
       } else {
         x = reduce(x, P.p, P.pinv);
       }
-      dst[g] = x;
+      dst[gidx(base, n_o0 + r * (GE::L >> 3), lam_o)] = x;
     }
     if (wn >= ntiles) break;
-    __syncthreads();  // the next tile overwrites LDS
+    __syncthreads();  // the next tile's first stage overwrites LDS
     w = wn;
   }
-#undef PCG_ELEM
 }
 
 }  // namespace pcg
