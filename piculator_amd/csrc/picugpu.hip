@@ -653,9 +653,10 @@ int ensure_pool(Ctx& c, size_t limbs, size_t ndesc) {
 // (2^(b-1), 2^b], b <= 12) go to the class g_small_class[b]: 0 the
 // 256-thread schoolbook (zL <= 1024), 1 and 2 the grouped schoolbook with 16
 // or 64 threads per product (zL <= 64, <= 256), 10..12 the single-workgroup
-// NTT of 2^k points; the autotuner (tune_small) picks among them per bucket
-// on the device at first use.  Classes 13..17 are the multi-pass NTT run for
-// the whole class at once, -1 one product at a time.
+// NTT of 2^k points, 110..112 the batched pass kernels at 2^(k - 100)
+// points (spectra through HBM); the autotuner (tune_small) picks among them
+// per bucket on the device at first use.  Classes 13..17 are the multi-pass
+// NTT run for the whole class at once, -1 one product at a time.
 constexpr int kBatchMaxLog = 17;
 constexpr int kSmallNttMinLog = 10;
 int g_small_class[kSmallNttMaxLog + 1];
@@ -671,9 +672,24 @@ int small_class_default(int b) {
   return b < kSmallNttMinLog ? kSmallNttMinLog : b;
 }
 
+constexpr int kPassClass = 100;  // + log2 N: batched pass kernels
+
 bool small_class_valid(int b, int cls) {
   if (cls >= 0 && cls <= 2) return (1 << b) <= school_max(cls);
+  if (cls >= kPassClass + kSmallNttMinLog && cls <= kPassClass + kSmallNttMaxLog) return cls - kPassClass >= b;
   return cls >= kSmallNttMinLog && cls <= kSmallNttMaxLog && cls >= b;
+}
+
+// Slot of a small class in launch order (0..8).
+int small_slot(int cls) {
+  if (cls <= 2) return cls;
+  if (cls < kPassClass) return cls - kSmallNttMinLog + 3;
+  return cls - kPassClass - kSmallNttMinLog + 6;
+}
+int small_slot_class(int slot) {
+  if (slot <= 2) return slot;
+  if (slot <= 5) return slot - 3 + kSmallNttMinLog;
+  return slot - 6 + kSmallNttMinLog + kPassClass;
 }
 
 int batch_class(uint64_t na, uint64_t nb) {
@@ -790,6 +806,8 @@ int launch_small_class(Ctx& c, int cls, const BatchDesc* base, size_t n) {
                          c.d_prime + 12 * 2, c.crt);
       break;
     default:
+      if (cls >= kPassClass + kSmallNttMinLog && cls <= kPassClass + kSmallNttMaxLog)
+        return batched_multipass(c, base, n, cls - kPassClass);
       return fail(PCG_ERR_INTERNAL, "bad small class %d", cls);
   }
   HIPCHK(hipGetLastError());
@@ -801,17 +819,18 @@ __global__ void k_fill_pattern(uint32_t* p, uint64_t n) {
   if (i < n) p[i] = (uint32_t)(i * 2654435761u) ^ 0x9e3779b9u;
 }
 
-// Autotuner of the small-product classes: for each bucket b = 4..10 it
+// Autotuner of the small-product classes: for each bucket b = 4..12 it
 // times 2048 products of na = nb = 3 * 2^(b-3) limbs (zL = 3/4 of the
 // bucket's top) through every class valid for the bucket (the schoolbook
-// kernels and the 2^10-point NTT; best of three launches, HIP events on the
-// library stream) and keeps the fastest; smaller buckets take the smallest
-// schoolbook, 11 and 12 are NTT only.  Results are exact in every class:
-// the choice only moves the crossovers.
+// kernels, the single-workgroup NTT and the batched pass kernels at
+// 2^max(b, 10) points; best of three launches, HIP events on the library
+// stream) and keeps the fastest; smaller buckets take the smallest
+// schoolbook.  Results are exact in every class: the choice only moves the
+// crossovers.
 int tune_small(Ctx& c) {
   for (int b = 0; b <= kSmallNttMaxLog; ++b) g_small_class[b] = small_class_default(b);
   constexpr int kCount = 2048;
-  constexpr int kMaxHalf = 3 << 7;  // na at b = 10
+  constexpr int kMaxHalf = 3 << 9;  // na at b = 12
   uint32_t* in = nullptr;
   uint32_t* out = nullptr;
   BatchDesc* dd = nullptr;
@@ -827,7 +846,7 @@ int tune_small(Ctx& c) {
                      (uint64_t)in_limbs);
   std::vector<BatchDesc> h(kCount);
   int rc = PCG_OK;
-  for (int b = 4; b <= kSmallNttMinLog && rc == PCG_OK; ++b) {
+  for (int b = 4; b <= kSmallNttMaxLog && rc == PCG_OK; ++b) {
     const uint32_t n = 3u << (b - 3);
     for (int i = 0; i < kCount; ++i) {
       h[i].a = in + (size_t)i * 2 * n;
@@ -841,10 +860,11 @@ int tune_small(Ctx& c) {
       rc = fail(PCG_ERR_HIP, "autotuner upload failed");
       break;
     }
-    const int cls[4] = {1, 2, 0, kSmallNttMinLog};
+    const int lg = b < kSmallNttMinLog ? kSmallNttMinLog : b;
+    const int cls[5] = {1, 2, 0, lg, kPassClass + lg};
     float best_ms = 1e30f;
     int best_cls = small_class_default(b);
-    for (int m = 0; m < 4 && rc == PCG_OK; ++m) {
+    for (int m = 0; m < 5 && rc == PCG_OK; ++m) {
       if (!small_class_valid(b, cls[m])) continue;
       float best = 1e30f;
       for (int rep = 0; rep < 4 && rc == PCG_OK; ++rep) {
@@ -877,8 +897,10 @@ int ensure_tuned(Ctx& c) { return g_small_ready ? PCG_OK : tune_small(c); }
 
 int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, bool dev) {
   // class slots: 0..2 schoolbook classes 0..2, 3..5 single-workgroup NTT
-  // (2^10 .. 2^12), 6..10 batched multi-pass (2^13 .. 2^17)
-  constexpr int kSlots = 6 + (kBatchMaxLog - kSmallNttMaxLog);
+  // (2^10 .. 2^12), 6..8 batched passes at 2^10 .. 2^12, 9..13 batched
+  // multi-pass (2^13 .. 2^17)
+  constexpr int kSmallSlots = 9;
+  constexpr int kSlots = kSmallSlots + (kBatchMaxLog - kSmallNttMaxLog);
   int rc0 = ensure_tuned(c);
   if (rc0) return rc0;
   std::vector<uint64_t> cls[kSlots], large;
@@ -902,7 +924,8 @@ int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, 
     if (k < 0)
       large.push_back(i);
     else
-      cls[k <= 2 ? k : k - kSmallNttMinLog + 3].push_back(i);
+      cls[k <= kSmallNttMaxLog || k >= kPassClass ? small_slot(k) : k - kSmallNttMaxLog - 1 + kSmallSlots]
+          .push_back(i);
   }
   size_t nsmall = 0;
   for (int k = 0; k < kSlots; ++k) nsmall += cls[k].size();
@@ -941,14 +964,14 @@ int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, 
     HIPCHK(hipMemcpyAsync(c.d_desc, c.h_desc, nsmall * sizeof(BatchDesc), hipMemcpyHostToDevice,
                           c.stream));
     size_t off = 0;
-    for (int k = 0; k < 6; ++k) {
-      rc = launch_small_class(c, k <= 2 ? k : kSmallNttMinLog + k - 3, c.d_desc + off, cls[k].size());
+    for (int k = 0; k < kSmallSlots; ++k) {
+      rc = launch_small_class(c, small_slot_class(k), c.d_desc + off, cls[k].size());
       if (rc) return rc;
       off += cls[k].size();
     }
-    for (int k = 6; k < kSlots; ++k) {
+    for (int k = kSmallSlots; k < kSlots; ++k) {
       if (cls[k].empty()) continue;
-      rc = batched_multipass(c, c.d_desc + off, cls[k].size(), kSmallNttMaxLog + k - 5);
+      rc = batched_multipass(c, c.d_desc + off, cls[k].size(), kSmallNttMaxLog + 1 + k - kSmallSlots);
       if (rc) return rc;
       off += cls[k].size();
     }
@@ -1252,7 +1275,7 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
     size_t o = 0;
     for (const auto& cl : hh.classes) {
       int rc;
-      if (cl.first <= kSmallNttMaxLog)
+      if (cl.first <= kSmallNttMaxLog || cl.first >= kPassClass)
         rc = launch_small_class(c, cl.first, hd + o, cl.second);
       else
         rc = batched_multipass(c, hd + o, cl.second, cl.first);

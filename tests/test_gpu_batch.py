@@ -90,6 +90,8 @@ SCHOOL_MAX = {0: 1024, 1: 64, 2: 256}
 def class_valid(b, c):
     if c in SCHOOL_MAX:
         return (1 << b) <= SCHOOL_MAX[c]
+    if 110 <= c <= 112:  # batched pass kernels at 2^(c - 100) points
+        return c - 100 >= b
     return 10 <= c <= 12 and c >= b
 
 
@@ -99,16 +101,16 @@ def test_autotune_table_is_valid(gpu):
     for b, c in enumerate(t):
         assert class_valid(b, c), (b, c)
         if (1 << b) > 1024:
-            assert c == b if b >= 10 else c == 10
+            assert c in (b, 100 + b)
     assert gpu.batch_table() == t
 
 
-@pytest.mark.parametrize("forced", [0, 1, 2, 10])
+@pytest.mark.parametrize("forced", [0, 1, 2, 10, 110])
 def test_forced_small_classes_exact(gpu, forced):
     """Every kernel of the tuned crossover (schoolbook with 256 threads, with
-    16 or 64 threads per product, 2^10-point NTT) gives the exact product in
-    every bucket it may serve, including all-ones carry chains and unbalanced
-    shapes."""
+    16 or 64 threads per product, 2^10-point single-workgroup NTT, batched
+    pass kernels at 2^10 points) gives the exact product in every bucket it
+    may serve, including all-ones carry chains and unbalanced shapes."""
     saved = gpu.batch_table()
     table = [forced if class_valid(b, forced) else c for b, c in enumerate(saved)]
     gpu.batch_table(table)
