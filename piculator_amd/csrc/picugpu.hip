@@ -1164,6 +1164,7 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
   size_t max_large = 0, max_large_chunks = 0;
   for (int32_t h = 2; h <= P.H; ++h) {
     std::vector<std::pair<int, BsrMul>> ms;
+    ms.reserve(4 * by_h[h].size());
     std::vector<BsrAdd> small, large;
     uint64_t large_max = 0;
     for (int32_t i : by_h[h]) {
@@ -1181,18 +1182,23 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
         large_max = std::max(large_max, bnd[I]);
       }
     }
-    std::stable_sort(ms.begin(), ms.end(),
-                     [](const std::pair<int, BsrMul>& x, const std::pair<int, BsrMul>& y) {
-                       return x.first < y.first;
-                     });
+    // group the height's products by class (a height has only a few
+    // distinct classes): one counting pass per class, in class order
+    std::vector<int> kinds;
+    for (const auto& e : ms)
+      if (std::find(kinds.begin(), kinds.end(), e.first) == kinds.end()) kinds.push_back(e.first);
+    std::sort(kinds.begin(), kinds.end());
     BsrHeight& hh = P.hp[h];
     hh.mul0 = muls.size();
     hh.nmul = ms.size();
-    for (size_t k = 0; k < ms.size(); ++k) {
-      if (hh.classes.empty() || hh.classes.back().first != ms[k].first)
-        hh.classes.push_back({ms[k].first, 0});
-      hh.classes.back().second++;
-      muls.push_back(ms[k].second);
+    for (int kind : kinds) {
+      size_t cnt = 0;
+      for (const auto& e : ms)
+        if (e.first == kind) {
+          muls.push_back(e.second);
+          ++cnt;
+        }
+      hh.classes.push_back({kind, cnt});
     }
     hh.add0 = adds.size();
     hh.nsmall = small.size();
