@@ -177,6 +177,15 @@ int pcg_df_sub(pcg_dfloat* z, const pcg_dfloat* a, const pcg_dfloat* b, uint64_t
  * limbs (p = 0: exact), exact product on the GPU engine chosen by `flags`
  * (PCG_ENGINE_*), one zero top limb trimmed. */
 int pcg_df_mul(pcg_dfloat* z, const pcg_dfloat* a, const pcg_dfloat* b, uint64_t p, uint32_t flags);
+/* z1 = x1.mul(s, p) and z2 = x2.mul(s, p), each exactly as pcg_df_mul, with
+ * the forward transform of the shared operand computed once when both
+ * products have the same transform length. Replaces the two products of
+ * one node combine that share an operand (PI.h:110-112: Q1 in P0*Q1 and
+ * Q0*Q1, R0 in P1*R0 and R0*R1); the reference transforms it twice
+ * (BigFloat.cpp:377-441 per call). z1 and z2 must be distinct; either may be
+ * one of the operands. */
+int pcg_df_mul_pair(pcg_dfloat* z1, const pcg_dfloat* x1, pcg_dfloat* z2, const pcg_dfloat* x2,
+                    const pcg_dfloat* s, uint64_t p, uint32_t flags);
 /* z = a.mul(x) (BigFloat.cpp:156-183). */
 int pcg_df_mul_u32(pcg_dfloat* z, const pcg_dfloat* a, uint32_t x);
 /* to_string_hex (BigFloat.cpp:95-106): the top limb in decimal, '.', then

@@ -47,10 +47,14 @@ struct PassArgs {
   const double* spec;      // fused: spectrum of operand a, [2][N]
   uint64_t N;
   int logL, logS, logLines, logG, logB;
-  // batched launches: product blockIdx.z uses descs[z], scratch offset
-  // z * zstride doubles, and operand a (0) or b (1) of its descriptor
+  // batched launches: product blockIdx.z uses descs[i] with i = z << zshift,
+  // scratch offset i * zstride doubles, and operand a (0) or b (1) of its
+  // descriptor; the fused pass reads a's spectrum at product
+  // (i >> sshift) << sshift (sshift 1: products 2k and 2k+1 share operand a,
+  // whose spectrum only product 2k computes)
   const BatchDesc* descs;
   int operand;
+  int zshift, sshift;
   uint64_t zstride;
   uint64_t ntiles;  // tiles of E points per prime and product
 };
@@ -381,12 +385,14 @@ __global__ __launch_bounds__((1 << (LOGL + LOGLINES)) / kEPT) void k_ntt_pass(Pa
   const Prime P = a.prm[pi];
   const double2* tw = stage_twiddles<GE>(ltw, P);
   const int logS = a.logS;
-  uint64_t zoff = 0;
+  uint64_t zoff = 0, soff = 0;
   const uint32_t* limbs = a.limbs;
   uint64_t nlimbs = a.nlimbs;
   if (a.descs) {
-    const BatchDesc d = a.descs[blockIdx.z];
-    zoff = (uint64_t)blockIdx.z * a.zstride;
+    const uint64_t zi = (uint64_t)blockIdx.z << a.zshift;
+    const BatchDesc d = a.descs[zi];
+    zoff = zi * a.zstride;
+    soff = ((zi >> a.sshift) << a.sshift) * a.zstride;
     limbs = a.operand ? d.b : d.a;
     nlimbs = a.operand ? d.nb : d.na;
   }
@@ -469,7 +475,7 @@ __global__ __launch_bounds__((1 << (LOGL + LOGLINES)) / kEPT) void k_ntt_pass(Pa
     stage_pos<GE, 3>(0, 0, lam_o, n_o0);
     if (MODE == kFused) {
       // pointwise with a's spectrum and N^-1, then the inverse DFT in LDS
-      const double* spec = a.spec + zoff + pi * a.N;
+      const double* spec = a.spec + soff + pi * a.N;
 #pragma unroll
       for (int r = 0; r < 8; ++r) {
         const uint64_t g = gidx(base, n_o0 + r * (GE::L >> 3), lam_o);

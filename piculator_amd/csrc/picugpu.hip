@@ -411,10 +411,12 @@ int pass_occupancy(PassFn fn, int threads) {
 int launch_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double* src,
                 const uint32_t* limbs, uint64_t nlimbs, double* dst, const double* spec,
                 const BatchDesc* descs = nullptr, int operand = 0, uint64_t zstride = 0,
-                unsigned count = 1) {
+                unsigned count = 1, int zshift = 0, int sshift = 0) {
   PassArgs a;
   a.descs = descs;
   a.operand = operand;
+  a.zshift = zshift;
+  a.sshift = sshift;
   a.zstride = zstride;
   a.prm = c.d_prime + pl.logN * 2;
   a.src = src;
@@ -445,25 +447,50 @@ int launch_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double* 
   return PCG_OK;
 }
 
+int ntt_log_points(uint64_t zL) {
+  const int logN = ceil_log2(zL);
+  return logN < kMinLogN ? kMinLogN : logN;
+}
+
+// Forward transform of operand a into the spectrum A = c.spec[0, 2N).
+int ntt_forward_a(Ctx& c, const Plan& pl, const uint32_t* d_a, uint64_t na) {
+  double* A = c.spec;
+  for (size_t j = 0; j < pl.passes.size(); ++j) {
+    const int rc = launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : A, d_a, na, A, nullptr);
+    if (rc) return rc;
+  }
+  return PCG_OK;
+}
+
+// The rest of a product with the spectrum of a already in A: forward
+// transform of b (its last pass fused with the pointwise product and the
+// first inverse pass), inverse passes, CRT and carry into d_out[0, zL).
+int ntt_mul_tail(Ctx& c, const Plan& pl, const uint32_t* d_b, uint64_t nb, uint64_t zL, uint32_t* d_out,
+                 bool profile);
+
 // Enqueue the whole NTT product on the library stream (device pointers).
 int ntt_mul_enqueue(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_b, uint64_t nb,
                     uint32_t* d_out, bool profile) {
   const uint64_t zL = na + nb;
-  int logN = ceil_log2(zL);
-  if (logN < kMinLogN) logN = kMinLogN;
+  const int logN = ntt_log_points(zL);
   int rc = ensure_workspace(c, logN, zL);
   if (rc) return rc;
   const Plan pl = make_plan(logN);
+  if (profile) HIPCHK(hipEventRecord(c.ev[0], c.stream));
+  rc = ntt_forward_a(c, pl, d_a, na);
+  if (rc) return rc;
+  if (profile) HIPCHK(hipEventRecord(c.ev[1], c.stream));
+  return ntt_mul_tail(c, pl, d_b, nb, zL, d_out, profile);
+}
+
+int ntt_mul_tail(Ctx& c, const Plan& pl, const uint32_t* d_b, uint64_t nb, uint64_t zL, uint32_t* d_out,
+                 bool profile) {
+  const int logN = pl.logN;
   const uint64_t N = 1ull << logN;
   double* A = c.spec;
   double* B = c.spec + 2 * N;
   const size_t m = pl.passes.size();
-  if (profile) HIPCHK(hipEventRecord(c.ev[0], c.stream));
-  for (size_t j = 0; j < m; ++j) {
-    rc = launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : A, d_a, na, A, nullptr);
-    if (rc) return rc;
-  }
-  if (profile) HIPCHK(hipEventRecord(c.ev[1], c.stream));
+  int rc;
   for (size_t j = 0; j + 1 < m; ++j) {
     rc = launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : B, d_b, nb, B, nullptr);
     if (rc) return rc;
@@ -708,12 +735,20 @@ int bsr_class(uint64_t zL) {
   return lg < kMinLogN ? kMinLogN : lg;
 }
 
+// Classes that run on the batched pass kernels (a transform spectrum in HBM
+// that two products can share).
+bool class_pairs(int cls) { return cls >= kPassClass || cls > kSmallNttMaxLog; }
+
 // Multi-pass NTT products of one transform length, all at once: pass
 // kernels carry a product grid dimension; scratch is 4N doubles per product.
-int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN) {
+// pairs: products 2k and 2k+1 have the same operand a, whose forward
+// transform runs once (count is even).
+int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, bool pairs = false) {
   const uint64_t N = 1ull << logN;
   const uint64_t per = 4 * N;
-  const size_t group_max = std::max<size_t>(1, ((size_t)1 << 28) / per);  // 2 GiB of scratch
+  if (pairs && (count & 1)) return fail(PCG_ERR_INTERNAL, "odd count of paired products");
+  size_t group_max = std::max<size_t>(1, ((size_t)1 << 28) / per);  // 2 GiB of scratch
+  if (pairs) group_max = std::max<size_t>(2, group_max & ~(size_t)1);
   const size_t want = std::min(count, group_max) * per;
   // chunk carry functions: (products per group) x (chunks per product)
   const size_t carry_need = std::min(count, group_max) * (size_t)(N / kCarryBlock);
@@ -748,9 +783,10 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN) {
     const unsigned k = (unsigned)std::min(group_max, count - g0);
     const BatchDesc* dd = d_desc + g0;
     int rc;
+    const int sh = pairs ? 1 : 0;
     for (size_t j = 0; j < m; ++j) {
       rc = launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : A, nullptr, 0, A, nullptr,
-                       dd, 0, per, k);
+                       dd, 0, per, k >> sh, sh, 0);
       if (rc) return rc;
     }
     for (size_t j = 0; j + 1 < m; ++j) {
@@ -759,7 +795,7 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN) {
       if (rc) return rc;
     }
     rc = launch_pass(c, kFused, pl, pl.passes[m - 1], m == 1 ? nullptr : B, nullptr, 0, B, A, dd, 1,
-                     per, k);
+                     per, k, 0, sh);
     if (rc) return rc;
     for (size_t j = m - 1; j-- > 0;) {
       rc = launch_pass(c, kInv, pl, pl.passes[j], B, nullptr, 0, B, nullptr, dd, 1, per, k);
@@ -1018,6 +1054,10 @@ struct BsrNode {
   int32_t left, right, height;
 };
 
+struct BsrMulPair {
+  BsrMul m[2];  // same operand a
+};
+
 int32_t bsr_build(std::vector<BsrNode>& nodes, uint32_t a, uint32_t b) {
   const int32_t id = (int32_t)nodes.size();
   nodes.push_back(BsrNode{a, b, -1, -1, 1});
@@ -1096,7 +1136,12 @@ T* carve(char*& p, size_t n) {
 struct BsrHeight {
   size_t mul0, nmul, add0, nsmall, nlarge;
   uint32_t add_maxch;  // 1024-limb chunks of the largest large-add result
-  std::vector<std::pair<int, size_t>> classes;  // (class, count) in launch order
+  struct Launch {
+    int cls;
+    size_t n;    // products
+    bool pairs;  // products 2k, 2k+1 share operand a (one forward transform)
+  };
+  std::vector<Launch> classes;  // in launch order
 };
 struct BsrPlan {
   uint32_t d = 0;
@@ -1164,16 +1209,29 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
   size_t max_large = 0, max_large_chunks = 0;
   for (int32_t h = 2; h <= P.H; ++h) {
     std::vector<std::pair<int, BsrMul>> ms;
+    std::vector<std::pair<int, BsrMulPair>> mp;
     ms.reserve(4 * by_h[h].size());
+    mp.reserve(2 * by_h[h].size());
     std::vector<BsrAdd> small, large;
     uint64_t large_max = 0;
     for (int32_t i : by_h[h]) {
       const BsrNode& n = nodes[i];
       const uint32_t I = 5 * i, Lf = 5 * n.left, Rt = 5 * n.right;
-      // x = P0 Q1, y = P1 R0, Q = Q0 Q1, R = R0 R1 (PI.h:110-112)
-      const BsrMul m4[4] = {{Lf + 0, Rt + 1, I + 3}, {Rt + 0, Lf + 2, I + 4},
-                            {Lf + 1, Rt + 1, I + 1}, {Lf + 2, Rt + 2, I + 2}};
-      for (const BsrMul& m : m4) ms.push_back({bsr_class(bnd[m.a] + bnd[m.b]), m});
+      // x = P0 Q1, Q = Q0 Q1, y = P1 R0, R = R0 R1 (PI.h:110-112), as two
+      // pairs around the operands used twice (products of exact integers
+      // commute, so the shared one is operand a of both)
+      const BsrMul m4[4] = {{Rt + 1, Lf + 0, I + 3}, {Rt + 1, Lf + 1, I + 1},
+                            {Lf + 2, Rt + 0, I + 4}, {Lf + 2, Rt + 2, I + 2}};
+      for (int q = 0; q < 4; q += 2) {
+        const int c0 = bsr_class(bnd[m4[q].a] + bnd[m4[q].b]);
+        const int c1 = bsr_class(bnd[m4[q + 1].a] + bnd[m4[q + 1].b]);
+        if (c0 == c1 && class_pairs(c0)) {
+          mp.push_back({c0, BsrMulPair{m4[q], m4[q + 1]}});
+        } else {
+          ms.push_back({c0, m4[q]});
+          ms.push_back({c1, m4[q + 1]});
+        }
+      }
       const BsrAdd ad{I + 3, I + 4, I + 0};  // P = x.add(y)
       if (bnd[I] <= 1024) {
         small.push_back(ad);
@@ -1183,22 +1241,33 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
       }
     }
     // group the height's products by class (a height has only a few
-    // distinct classes): one counting pass per class, in class order
+    // distinct classes): one counting pass per class, in class order, the
+    // paired products of a class before its single ones
     std::vector<int> kinds;
     for (const auto& e : ms)
+      if (std::find(kinds.begin(), kinds.end(), e.first) == kinds.end()) kinds.push_back(e.first);
+    for (const auto& e : mp)
       if (std::find(kinds.begin(), kinds.end(), e.first) == kinds.end()) kinds.push_back(e.first);
     std::sort(kinds.begin(), kinds.end());
     BsrHeight& hh = P.hp[h];
     hh.mul0 = muls.size();
-    hh.nmul = ms.size();
+    hh.nmul = ms.size() + 2 * mp.size();
     for (int kind : kinds) {
       size_t cnt = 0;
+      for (const auto& e : mp)
+        if (e.first == kind) {
+          muls.push_back(e.second.m[0]);
+          muls.push_back(e.second.m[1]);
+          cnt += 2;
+        }
+      if (cnt) hh.classes.push_back({kind, cnt, true});
+      cnt = 0;
       for (const auto& e : ms)
         if (e.first == kind) {
           muls.push_back(e.second);
           ++cnt;
         }
-      hh.classes.push_back({kind, cnt});
+      if (cnt) hh.classes.push_back({kind, cnt, false});
     }
     hh.add0 = adds.size();
     hh.nsmall = small.size();
@@ -1281,12 +1350,14 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
     size_t o = 0;
     for (const auto& cl : hh.classes) {
       int rc;
-      if (cl.first <= kSmallNttMaxLog || cl.first >= kPassClass)
-        rc = launch_small_class(c, cl.first, hd + o, cl.second);
+      if (cl.pairs)
+        rc = batched_multipass(c, hd + o, cl.n, cl.cls >= kPassClass ? cl.cls - kPassClass : cl.cls, true);
+      else if (cl.cls <= kSmallNttMaxLog || cl.cls >= kPassClass)
+        rc = launch_small_class(c, cl.cls, hd + o, cl.n);
       else
-        rc = batched_multipass(c, hd + o, cl.second, cl.first);
+        rc = batched_multipass(c, hd + o, cl.n, cl.cls);
       if (rc) return rc;
-      o += cl.second;
+      o += cl.n;
     }
     hipLaunchKernelGGL(k_bsr_trim, dim3((unsigned)((hh.nmul + 255) / 256)), dim3(256), 0, c.stream, hm,
                        (uint64_t)hh.nmul, P->meta, P->arena);
@@ -1593,6 +1664,28 @@ int df_add_locked(Ctx& c, pcg_dfloat* z, const pcg_dfloat* pa, const pcg_dfloat*
   return PCG_OK;
 }
 
+// Operand of BigFloat::mul after the precision cut (BigFloat.cpp:337-366):
+// the top pp limbs, the exponent raised by the limbs dropped.
+struct DfCut {
+  const uint32_t* t;
+  uint64_t L;
+  int64_t exp;
+};
+DfCut df_cut(const pcg_dfloat& x, uint64_t pp) {
+  DfCut o{x.limbs, x.L, x.exp};
+  if (o.L > pp) {
+    o.t += o.L - pp;
+    o.exp += (int64_t)(o.L - pp);
+    o.L = pp;
+  }
+  return o;
+}
+
+// Limbs kept by BigFloat::mul at precision p (p = 0: exact).
+uint64_t df_mul_keep(const pcg_dfloat& A, const pcg_dfloat& B, uint64_t p) {
+  return p == 0 ? A.L + B.L : p + kExtraPrecision;
+}
+
 // BigFloat::mul (BigFloat.cpp:326-444): operands cut to their top p + 2
 // limbs (p = 0: exact), full product, one zero top limb trimmed.
 int df_mul_locked(Ctx& c, pcg_dfloat* z, const pcg_dfloat* pa, const pcg_dfloat* pb, uint64_t p,
@@ -1602,34 +1695,84 @@ int df_mul_locked(Ctx& c, pcg_dfloat* z, const pcg_dfloat* pa, const pcg_dfloat*
     df_assign_zero(z);
     return PCG_OK;
   }
-  const uint64_t pp = p == 0 ? A.L + B.L : p + kExtraPrecision;
-  int64_t Aexp = A.exp, Bexp = B.exp;
-  uint64_t AL = A.L, BL = B.L;
-  const uint32_t* AT = A.limbs;
-  const uint32_t* BT = B.limbs;
-  if (AL > pp) {
-    AT += AL - pp;
-    Aexp += (int64_t)(AL - pp);
-    AL = pp;
-  }
-  if (BL > pp) {
-    BT += BL - pp;
-    Bexp += (int64_t)(BL - pp);
-    BL = pp;
-  }
+  const uint64_t pp = df_mul_keep(A, B, p);
+  const DfCut a = df_cut(A, pp), b = df_cut(B, pp);
   uint32_t* buf = nullptr;
   uint64_t cap = 0;
-  int rc = df_alloc(c, AL + BL, &buf, &cap);
+  int rc = df_alloc(c, a.L + b.L, &buf, &cap);
   if (rc) return rc;
-  rc = mul_dev_locked(c, AT, AL, BT, BL, buf, flags & PCG_ENGINE_MASK);
+  rc = mul_dev_locked(c, a.t, a.L, b.t, b.L, buf, flags & PCG_ENGINE_MASK);
   uint32_t topl = 0;
-  if (!rc) rc = df_read_limb(c, buf + AL + BL - 1, &topl);
+  if (!rc) rc = df_read_limb(c, buf + a.L + b.L - 1, &topl);
   if (!rc) rc = sync_locked(c);
   if (rc) {
     df_put_back(buf, cap);
     return rc;
   }
-  df_assign(z, buf, cap, AL + BL - (topl == 0 ? 1 : 0), Aexp + Bexp, (A.sign != 0) == (B.sign != 0));
+  df_assign(z, buf, cap, a.L + b.L - (topl == 0 ? 1 : 0), a.exp + b.exp, (A.sign != 0) == (B.sign != 0));
+  return PCG_OK;
+}
+
+// Two products with one shared operand, z1 = x1 s and z2 = x2 s, each with
+// the rules of BigFloat::mul.  Both cut s to the same limbs (its top p + 2,
+// or all of it when p = 0), so when the two products have the same
+// transform length the forward transform of s runs once: the node combine of
+// PI.h:110-112 multiplies Q1 and R0 twice each.  The results are assigned
+// after both products are complete, so z1 and z2 may alias any operand.
+int df_mul_pair_locked(Ctx& c, pcg_dfloat* z1, const pcg_dfloat* px1, pcg_dfloat* z2,
+                       const pcg_dfloat* px2, const pcg_dfloat* ps, uint64_t p, uint32_t flags) {
+  if (z1 == z2) return fail(PCG_ERR_INVALID, "the two results of a product pair must be distinct");
+  const pcg_dfloat X[2] = {*px1, *px2}, S = *ps;
+  const uint32_t eng = flags & PCG_ENGINE_MASK;
+  if (eng == 3) return fail(PCG_ERR_INVALID, "unknown engine flag");
+  DfCut x[2], sc[2];
+  uint64_t zL[2] = {0, 0};
+  for (int i = 0; i < 2; ++i) {
+    if (X[i].L == 0 || S.L == 0) continue;
+    const uint64_t pp = df_mul_keep(X[i], S, p);
+    x[i] = df_cut(X[i], pp);
+    sc[i] = df_cut(S, pp);
+    zL[i] = x[i].L + sc[i].L;
+    const int rc = check_sizes(x[i].L, sc[i].L);
+    if (rc) return rc;
+  }
+  const bool shared = zL[0] && zL[1] && eng != PCG_ENGINE_FFT && sc[0].L == sc[1].L &&
+                      ntt_log_points(zL[0]) == ntt_log_points(zL[1]);
+  uint32_t* buf[2] = {nullptr, nullptr};
+  uint64_t cap[2] = {0, 0};
+  int rc = PCG_OK;
+  for (int i = 0; i < 2 && !rc; ++i)
+    if (zL[i]) rc = df_alloc(c, zL[i], &buf[i], &cap[i]);
+  if (!rc && shared) {
+    const int logN = ntt_log_points(zL[0]);
+    rc = ensure_workspace(c, logN, std::max(zL[0], zL[1]));
+    const Plan pl = make_plan(logN);
+    if (!rc) rc = ntt_forward_a(c, pl, sc[0].t, sc[0].L);
+    for (int i = 0; i < 2 && !rc; ++i) rc = ntt_mul_tail(c, pl, x[i].t, x[i].L, zL[i], buf[i], false);
+  } else {
+    for (int i = 0; i < 2 && !rc; ++i)
+      if (zL[i]) rc = mul_dev_locked(c, x[i].t, x[i].L, sc[i].t, sc[i].L, buf[i], eng);
+  }
+  for (int i = 0; i < 2 && !rc; ++i)
+    if (zL[i])
+      HIPCHK(hipMemcpyAsync(reinterpret_cast<uint32_t*>(c.h_slot) + i, buf[i] + zL[i] - 1, sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, c.stream));
+  if (!rc) rc = sync_locked(c);
+  if (rc) {
+    for (int i = 0; i < 2; ++i) df_put_back(buf[i], cap[i]);
+    return rc;
+  }
+  uint32_t top[2];
+  memcpy(top, c.h_slot, sizeof top);
+  pcg_dfloat* z[2] = {z1, z2};
+  for (int i = 0; i < 2; ++i) {
+    if (!zL[i]) {
+      df_assign_zero(z[i]);
+      continue;
+    }
+    df_assign(z[i], buf[i], cap[i], zL[i] - (top[i] == 0 ? 1 : 0), x[i].exp + sc[i].exp,
+              (X[i].sign != 0) == (S.sign != 0));
+  }
   return PCG_OK;
 }
 
@@ -2026,6 +2169,13 @@ int pcg_df_mul(pcg_dfloat* z, const pcg_dfloat* a, const pcg_dfloat* b, uint64_t
   if (!z || !a || !b) return fail(PCG_ERR_INVALID, "null pointer");
   DF_ENTER();
   return df_mul_locked(c, z, a, b, p, flags);
+}
+
+int pcg_df_mul_pair(pcg_dfloat* z1, const pcg_dfloat* x1, pcg_dfloat* z2, const pcg_dfloat* x2,
+                    const pcg_dfloat* s, uint64_t p, uint32_t flags) {
+  if (!z1 || !x1 || !z2 || !x2 || !s) return fail(PCG_ERR_INVALID, "null pointer");
+  DF_ENTER();
+  return df_mul_pair_locked(c, z1, x1, z2, x2, s, p, flags);
 }
 
 int pcg_df_mul_u32(pcg_dfloat* z, const pcg_dfloat* a, uint32_t x) {

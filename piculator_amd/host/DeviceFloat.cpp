@@ -115,6 +115,14 @@ DeviceFloat DeviceFloat::mul(const DeviceFloat& x, size_t p) const {
   return z;
 }
 
+std::pair<DeviceFloat, DeviceFloat> DeviceFloat::mul_pair(const DeviceFloat& x1, const DeviceFloat& x2,
+                                                          size_t p) const {
+  std::pair<DeviceFloat, DeviceFloat> z;
+  call([&] { return pcg_df_mul_pair(&z.first.v_, &x1.v_, &z.second.v_, &x2.v_, &v_, p, mul_engine_flags()); });
+  if (v_.L) count_big_muls((x1.v_.L ? 1 : 0) + (x2.v_.L ? 1 : 0));
+  return z;
+}
+
 DeviceFloat DeviceFloat::rcp(size_t p) const { return newton_rcp(*this, p); }
 
 DeviceFloat DeviceFloat::div(const DeviceFloat& x, size_t p) const { return mul(x.rcp(p), p); }

@@ -10,6 +10,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <utility>
 
 #include "../../include/picugpu.h"
 #include "BigFloat.h"
@@ -37,6 +38,10 @@ class DeviceFloat {
   DeviceFloat add(const DeviceFloat& x, size_t p = 0) const;
   DeviceFloat sub(const DeviceFloat& x, size_t p = 0) const;
   DeviceFloat mul(const DeviceFloat& x, size_t p = 0) const;
+  // {x1.mul(*this, p), x2.mul(*this, p)}, this value's forward transform
+  // shared by the two products (pcg_df_mul_pair)
+  std::pair<DeviceFloat, DeviceFloat> mul_pair(const DeviceFloat& x1, const DeviceFloat& x2,
+                                               size_t p = 0) const;
   DeviceFloat rcp(size_t p) const;
   DeviceFloat div(const DeviceFloat& x, size_t p) const;
   std::string to_string_hex(size_t digits) const;

@@ -199,12 +199,13 @@ void bsr_resident(DPQR& out, uint32_t a, uint32_t b, size_t p, bool verbose) {
   bsr_resident(L, a, m, p, verbose);
   bsr_resident(R, m, b, p, verbose);
   const double t1 = verbose ? now_s() : 0.0;
-  const DeviceFloat x = L.P.mul(R.Q, p);
-  const DeviceFloat y = R.P.mul(L.R, p);
-  out.Q = L.Q.mul(R.Q, p);
-  out.R = L.R.mul(R.R, p);
+  // P0 Q1 and Q0 Q1 share the transform of Q1, P1 R0 and R0 R1 that of R0
+  auto xq = R.Q.mul_pair(L.P, L.Q, p);  // x = P0 Q1, Q = Q0 Q1
+  auto yr = L.R.mul_pair(R.P, R.R, p);  // y = P1 R0, R = R0 R1
+  out.Q = std::move(xq.second);
+  out.R = std::move(yr.second);
   const double t2 = verbose ? now_s() : 0.0;
-  out.P = x.add(y, p);
+  out.P = xq.first.add(yr.first, p);
   if (verbose)
     std::fprintf(stderr, "  combine [%u, %u): products %.3f s, add %.3f s, |Q| %zu limbs\n", a, b,
                  t2 - t1, now_s() - t2, out.Q.get_precision());

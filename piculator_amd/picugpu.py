@@ -34,7 +34,8 @@ EXPORTS = [
     "pcg_dev_free", "pcg_memcpy_h2d", "pcg_memcpy_d2h", "pcg_fft_last_roundoff",
     "pcg_last_profile", "pcg_ntt_plan", "pcg_timer_start", "pcg_timer_stop",
     "pcg_bsr_exact", "pcg_bsr_bound", "pcg_df_free", "pcg_df_set", "pcg_df_read", "pcg_df_copy",
-    "pcg_df_ucmp", "pcg_df_add", "pcg_df_sub", "pcg_df_mul", "pcg_df_mul_u32", "pcg_df_to_hex",
+    "pcg_df_ucmp", "pcg_df_add", "pcg_df_sub", "pcg_df_mul", "pcg_df_mul_pair", "pcg_df_mul_u32",
+    "pcg_df_to_hex",
     "pcg_df_bsr_exact", "pcg_df_release_pool", "pcg_autotune", "pcg_batch_table",
 ]
 
@@ -105,6 +106,7 @@ def load():
     L.pcg_df_add.argtypes = [dfp, dfp, dfp, C.c_uint64]
     L.pcg_df_sub.argtypes = [dfp, dfp, dfp, C.c_uint64]
     L.pcg_df_mul.argtypes = [dfp, dfp, dfp, C.c_uint64, C.c_uint32]
+    L.pcg_df_mul_pair.argtypes = [dfp, dfp, dfp, dfp, dfp, C.c_uint64, C.c_uint32]
     L.pcg_df_mul_u32.argtypes = [dfp, dfp, C.c_uint32]
     L.pcg_df_to_hex.argtypes = [dfp, C.c_uint64, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]
     L.pcg_df_bsr_exact.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, dfp, dfp, dfp]
@@ -276,6 +278,32 @@ def df_to_tuple(x: DFloat):
     if n:
         _check(load().pcg_df_read(C.byref(x), 0, n, out.ctypes.data))
     return (bool(x.sign), int(x.exp), out[:n].tolist())
+
+
+def df_from_tuple(v) -> DFloat:
+    """A device value holding the (sign, exp, limbs) tuple v."""
+    ensure_init()
+    sign, exp, limbs = v
+    arr = _u32(limbs)
+    x = DFloat()
+    _check(load().pcg_df_set(C.byref(x), arr.ctypes.data, arr.size, int(exp), int(bool(sign))))
+    return x
+
+
+def df_mul_pair(x1, x2, s, p: int = 0, flags: int = 0, alias: str = ""):
+    """(x1.mul(s, p), x2.mul(s, p)) through pcg_df_mul_pair on device values
+    made from (sign, exp, limbs) tuples.  alias "x1" / "s" writes the first
+    result over that operand (the entry point allows it)."""
+    vals = [df_from_tuple(v) for v in (x1, x2, s)]
+    z1, z2 = DFloat(), DFloat()
+    t1 = {"x1": vals[0], "s": vals[2]}.get(alias, z1)
+    try:
+        _check(load().pcg_df_mul_pair(C.byref(t1), C.byref(vals[0]), C.byref(z2), C.byref(vals[1]),
+                                      C.byref(vals[2]), p, flags))
+        return df_to_tuple(t1), df_to_tuple(z2)
+    finally:
+        for v in vals + [z1, z2]:
+            load().pcg_df_free(C.byref(v))
 
 
 def df_bsr_exact(a: int, b: int, max_limbs: int):

@@ -164,3 +164,61 @@ def test_large_values_residue(host):
     assert m[1] == 0
     for q in CHECK_PRIMES[:2]:
         assert residue(z, q) == residue(a, q) * residue(b, q) % q
+
+
+# (len x1, len x2, len s, p): one transform length for both products (the
+# shared forward transform), two lengths (two separate products), the
+# precision cut of every operand, one-limb operands and zeros
+PAIR_CASES = [
+    (100, 120, 110, 0),
+    (3000, 2900, 3100, 0),
+    (1000, 5000, 3000, 0),
+    (70001, 65000, 70001, 0),
+    (70001, 65000, 70001, 50000),
+    (70001, 3, 70001, 1000),
+    (1, 1, 1, 0),
+    (0, 50, 40, 0),
+    (50, 40, 0, 7),
+]
+
+
+@pytest.mark.parametrize("n1,n2,ns,p", PAIR_CASES)
+def test_mul_pair_matches_oracle(gpu, n1, n2, ns, p):
+    """pcg_df_mul_pair gives exactly x1.mul(s, p) and x2.mul(s, p)."""
+    rng = random.Random(n1 * 7 + n2 * 13 + ns * 3 + p)
+    x1, x2, s = rand_val(rng, n1), rand_val(rng, n2), rand_val(rng, ns)
+    want1 = O.BF(*x1).mul(O.BF(*s), p).tuple()
+    want2 = O.BF(*x2).mul(O.BF(*s), p).tuple()
+    got1, got2 = gpu.df_mul_pair(x1, x2, s, p)
+    check(got1, want1)
+    check(got2, want2)
+
+
+@pytest.mark.parametrize("alias", ["x1", "s"])
+def test_mul_pair_result_over_operand(gpu, alias):
+    """The first result may replace an operand: both products are complete
+    before either result is assigned."""
+    rng = random.Random(31)
+    x1, x2, s = rand_val(rng, 9000), rand_val(rng, 8500), rand_val(rng, 9100)
+    got1, got2 = gpu.df_mul_pair(x1, x2, s, 0, alias=alias)
+    check(got1, O.BF(*x1).mul(O.BF(*s), 0).tuple())
+    check(got2, O.BF(*x2).mul(O.BF(*s), 0).tuple())
+
+
+def test_mul_pair_fft_engine(gpu):
+    """With the FFT engine requested the pair runs as two products."""
+    rng = random.Random(32)
+    x1, x2, s = rand_val(rng, 4000), rand_val(rng, 4100), rand_val(rng, 3900)
+    got1, got2 = gpu.df_mul_pair(x1, x2, s, 0, flags=2)
+    check(got1, O.BF(*x1).mul(O.BF(*s), 0).tuple())
+    check(got2, O.BF(*x2).mul(O.BF(*s), 0).tuple())
+
+
+def test_mul_pair_rejects_one_result_twice(gpu):
+    x = gpu.df_from_tuple((True, 0, [1, 2, 3]))
+    r = gpu.C.byref
+    try:
+        assert gpu.load().pcg_df_mul_pair(r(x), r(x), r(x), r(x), r(x), 0, 0) == 1  # PCG_ERR_INVALID
+        assert gpu.df_to_tuple(x) == (True, 0, [1, 2, 3])
+    finally:
+        gpu.load().pcg_df_free(r(x))
