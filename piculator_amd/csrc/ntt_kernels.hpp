@@ -434,11 +434,10 @@ __global__ __launch_bounds__((1 << (LOGL + LOGLINES)) / kEPT) void k_ntt_pass(Pa
     stp = root_pow(c * step, logRoot, MODE == kInv, P);
   };
 
-  uint64_t w = blockIdx.x;
-  if (w >= ntiles) return;
-  double xr[kEPT];
-  load_tile(w, xr);
-  for (;;) {
+  // One tile: xr holds its first-stage inputs; once they are in LDS, xr
+  // receives the loads of tile wload, which stay in flight through this
+  // tile's remaining stages.
+  auto tile_step = [&](const uint64_t w, double (&xr)[kEPT], const uint64_t wload) {
     const uint64_t base = tile_base(w);
     if (MODE == kInv && !ROW) {
       // pre-twiddle: butterfly b's points are j_b + r L / R1
@@ -466,8 +465,7 @@ __global__ __launch_bounds__((1 << (LOGL + LOGLINES)) / kEPT) void k_ntt_pass(Pa
     stage_compute<GE, R1, LOGR1, 0>(v1, tw, P, MODE == kInv);
     stage_write<GE, R1, LOGR1, 0>(v1, lds);
     __syncthreads();
-    const uint64_t wn = w + gridDim.x;
-    if (wn < ntiles) load_tile(wn, xr);
+    if (wload < ntiles) load_tile(wload, xr);
     middle_stages<GE, LOGR1>(lds, tw, P, MODE == kInv);
     double y[1][8];
     last_stage_to_regs<GE>(lds, y, tw, P, MODE == kInv);
@@ -512,9 +510,34 @@ __global__ __launch_bounds__((1 << (LOGL + LOGLINES)) / kEPT) void k_ntt_pass(Pa
       }
       dst[gidx(base, n_o0 + r * (GE::L >> 3), lam_o)] = x;
     }
-    if (wn >= ntiles) break;
-    __syncthreads();  // the next tile's first stage overwrites LDS
-    w = wn;
+  };
+
+  uint64_t w = blockIdx.x;
+  if (w >= ntiles) return;
+  const uint64_t g = gridDim.x;
+  double xa[kEPT];
+  load_tile(w, xa);
+  if constexpr (!ROW) {
+    // column passes (at most 106 VGPRs) keep two tiles' loads in flight
+    double xb[kEPT];
+    if (w + g < ntiles) load_tile(w + g, xb);
+    for (;;) {
+      tile_step(w, xa, w + 2 * g);
+      w += g;
+      if (w >= ntiles) break;
+      __syncthreads();  // the next tile's first stage overwrites LDS
+      tile_step(w, xb, w + 2 * g);
+      w += g;
+      if (w >= ntiles) break;
+      __syncthreads();
+    }
+  } else {
+    for (;;) {
+      tile_step(w, xa, w + g);
+      w += g;
+      if (w >= ntiles) break;
+      __syncthreads();  // the next tile's first stage overwrites LDS
+    }
   }
 }
 
