@@ -81,17 +81,30 @@ struct Plan {
 // NTT pass plan: strided passes take at most 2^8-point lines, so a
 // workgroup's 4096 staged points cover >= 16 adjacent columns (128-byte
 // runs, which stream at ~80% of a contiguous copy on MI355X; 64-byte runs
-// reach ~65%); the last pass (contiguous rows) takes up to 2^12.
+// reach ~65%); the last pass (contiguous rows) takes up to 2^12.  Where
+// 2^9-point columns (8 adjacent, 64-byte runs) save a whole pass, the plan
+// takes them: one sweep fewer outweighs the slower strided pattern.
 constexpr int kMaxLogCol = 8;
+constexpr int kMaxLogColWide = 9;
+
+int plan_passes(int logN, int maxcol) {
+  int m = 1;
+  while (logN > maxcol * (m - 1) + kLogE) ++m;
+  return m;
+}
 
 Plan make_plan(int logN) {
   Plan pl;
   pl.logN = logN;
   pl.logE = logN < kLogE ? logN : kLogE;
-  int m = 1;
-  while (logN > kMaxLogCol * (m - 1) + kLogE) ++m;
+  int m = plan_passes(logN, kMaxLogCol);
+  int maxcol = kMaxLogCol;
+  if (plan_passes(logN, kMaxLogColWide) < m) {
+    maxcol = kMaxLogColWide;
+    m = plan_passes(logN, maxcol);
+  }
   int last = (logN + m - 1) / m;
-  if (last < logN - kMaxLogCol * (m - 1)) last = logN - kMaxLogCol * (m - 1);
+  if (last < logN - maxcol * (m - 1)) last = logN - maxcol * (m - 1);
   if (last > kLogE) last = kLogE;
   std::vector<int> ls;
   const int rest = logN - last;
