@@ -376,7 +376,7 @@ def main():
     check["sha256"] = hashlib.sha256(s).hexdigest()
     if str(D) in fx:
         check["full_digest_ok"] = check["sha256"] == fx[str(D)]["sha256"]
-        check["ok"] = bool(check["full_digest_ok"])
+        check["ok"] = bool(check["ok"] and check["full_digest_ok"])
     check["head"] = s[:34].decode()
     del s
     log("pi check:", json.dumps(check))

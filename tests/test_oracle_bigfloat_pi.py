@@ -185,6 +185,12 @@ def test_pi_digest_fixtures():
         s, _ = O.pi_hex(int(D), O.ENGINE_REFERENCE, threads=2)
         assert hashlib.sha256(s.encode()).hexdigest() == fx[D]["sha256"]
         assert s[:66] == fx[D]["head"]
+    # every entry (10^4 ... 10^9): sizes of Pi(D) and the first 64 digits of pi
+    head = pi_hex_mpmath(64)
+    for D, e in fx.items():
+        assert (e["hex_digits"], e["p_limbs"], e["terms"]) == O.pi_sizes(int(D))
+        assert e["bytes"] == 2 + e["hex_digits"]
+        assert e["head"] == head
 
 
 def test_planted_truncation_mistake_is_caught():
