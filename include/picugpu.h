@@ -202,8 +202,10 @@ int pcg_df_release_pool(void);
 
 /* Size classes of batched products (pcg_mul_batch*, pcg_bsr_exact): a
  * product of zL = na + nb limbs with 2^(b-1) < zL <= 2^b, b <= 12, runs in
- * class table[b]: 0 the schoolbook kernel (zL <= 1024), 10..12 the
- * single-workgroup NTT of 2^k points.  The crossover is the autotuner's
+ * class table[b]: 0 the schoolbook kernel (zL <= 1024), 1..3 the grouped
+ * schoolbook (16 threads x 4 columns, zL <= 64; 64 x 4, zL <= 256; 64 x 8,
+ * zL <= 512), 10..12 the single-workgroup NTT of 2^k points, 110..112 the
+ * batched pass kernels at 2^(k-100) points.  The crossover is the autotuner's
  * (timed on the device at the first batched call, or by pcg_autotune,
  * which returns the table); pcg_batch_table reads it (set = 0) or installs
  * one (set = 1, every entry valid for its bucket).  Products are exact in

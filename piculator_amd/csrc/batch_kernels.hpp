@@ -18,6 +18,38 @@ namespace pcg {
 constexpr int kSchoolMaxLimbs = 1024;  // zL bound of k_batch_school
 constexpr int kSmallNttMaxLog = 12;    // N bound of k_batch_ntt (rows padded by one)
 
+// 96-bit sums (lo 64 bits + carry count) of columns k0 .. k0 + C - 1 over
+// a's limbs i = ilo .. ihi: each step multiplies a_i by a window of C limbs
+// of b, w[q] = b[k0 + q - i] = sb[4 + k0 + q - i] (sb holds b after four
+// zero limbs and is zero past its end, for C + 3 more limbs), then slides
+// the window down one limb.  The next step's a limb and window limb are read
+// before this step's products, so their LDS latency overlaps the
+// multiply-adds; sa[ihi + 1] is read but not used (ihi + 1 <= na <= the
+// array's length - 1, as nb >= 1).
+template <int C>
+__device__ __forceinline__ void school_columns(const uint32_t* sa, const uint32_t* sb, int k0, int ilo,
+                                               int ihi, uint64_t (&lo)[C], uint32_t (&hi)[C]) {
+  uint32_t w[C];
+#pragma unroll
+  for (int q = 0; q < C; ++q) w[q] = sb[4 + k0 + q - ilo];
+  uint32_t an = sa[ilo];
+#pragma unroll 2
+  for (int i = ilo; i <= ihi; ++i) {
+    const uint64_t ai = an;
+    an = sa[i + 1];
+    const uint32_t bn = sb[3 + k0 - i];  // b[k0 - (i + 1)]
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const uint64_t pq = ai * w[q];
+      lo[q] += pq;
+      hi[q] += lo[q] < pq;
+    }
+#pragma unroll
+    for (int q = C - 1; q > 0; --q) w[q] = w[q - 1];
+    w[0] = bn;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_batch_school(const BatchDesc* descs) {
   // sb is stored 3 limbs to the right with zeros around it, so the window
   // b[k0 + q - i] (q = 0..3) is readable for every i of the loop below.
@@ -29,7 +61,7 @@ __global__ __launch_bounds__(256) void k_batch_school(const BatchDesc* descs) {
   const int na = (int)d.na, nb = (int)d.nb, zL = na + nb;
   for (int i = tid; i < na; i += 256) sa[i] = d.a[i];
   for (int i = tid; i < kSchoolMaxLimbs + 8; i += 256) {
-    const int j = i - 3;
+    const int j = i - 4;
     sbp[i] = (j >= 0 && j < nb) ? d.b[j] : 0u;
   }
   __syncthreads();
@@ -41,27 +73,8 @@ __global__ __launch_bounds__(256) void k_batch_school(const BatchDesc* descs) {
   int ilo = k0 - nb + 1;
   if (ilo < 0) ilo = 0;
   int ihi = k0 + 3 < na - 1 ? k0 + 3 : na - 1;
-  if (k0 < zL && ilo <= ihi) {
-    // window: w[q] = b[k0 + q - i] = sbp[3 + k0 + q - i]
-    uint32_t w0 = sbp[3 + k0 - ilo], w1 = sbp[4 + k0 - ilo], w2 = sbp[5 + k0 - ilo],
-             w3 = sbp[6 + k0 - ilo];
-    for (int i = ilo; i <= ihi; ++i) {
-      const uint64_t ai = sa[i];
-      const uint64_t p0 = ai * w0, p1 = ai * w1, p2 = ai * w2, p3 = ai * w3;
-      lo[0] += p0;
-      hi[0] += lo[0] < p0;
-      lo[1] += p1;
-      hi[1] += lo[1] < p1;
-      lo[2] += p2;
-      hi[2] += lo[2] < p2;
-      lo[3] += p3;
-      hi[3] += lo[3] < p3;
-      w3 = w2;
-      w2 = w1;
-      w1 = w0;
-      const int nxt = 2 + k0 - i;  // b[k0 - (i + 1)], needed while i + 1 <= k0 + 3
-      w0 = nxt >= 0 ? sbp[nxt] : 0u;
-    }
+  if (k0 < zL && ilo <= ihi && nb > 0) {
+    school_columns<4>(sa, sbp, k0, ilo, ihi, lo, hi);
   }
   uint32_t w0[4], w1[4], w2[4], D[4];
 #pragma unroll
@@ -79,12 +92,13 @@ __global__ __launch_bounds__(256) void k_batch_school(const BatchDesc* descs) {
   }
 }
 
-// Schoolbook products of zL <= 4 G limbs with G threads per product (G = 16
-// or 64, a part of one wave) and 256 / G products per workgroup, for the many
-// small products of the lower tree levels.  The column sums are those of
-// k_batch_school; the carries are resolved inside the group with shuffles:
-// each thread composes the carry functions of its four limbs, the group
-// scans them (width-G shuffles), and each thread adds its carry-in.
+// Schoolbook products of zL <= C G limbs with G threads per product (G = 16
+// or 64, a part of one wave), C adjacent columns per thread (4 or 8) and
+// 256 / G products per workgroup, for the many small products of the lower
+// tree levels.  The column sums are those of k_batch_school with a window of
+// C limbs; the carries are resolved inside the group with shuffles: each
+// thread composes the carry functions of its C limbs, the group scans them
+// (width-G shuffles), and each thread adds its carry-in.
 __device__ __forceinline__ CarryFn cf_shfl_up_w(CarryFn f, int d, int width) {
   CarryFn r;
   r.base = __shfl_up(f.base, d, width);
@@ -92,11 +106,12 @@ __device__ __forceinline__ CarryFn cf_shfl_up_w(CarryFn f, int d, int width) {
   return r;
 }
 
-template <int G>
+template <int G, int C>
 __global__ __launch_bounds__(256) void k_batch_school_g(const BatchDesc* descs, uint32_t count) {
   constexpr int PPB = 256 / G;  // products per workgroup
-  constexpr int MAXL = 4 * G;   // zL bound
-  __shared__ uint32_t sa[PPB][MAXL], sbp[PPB][MAXL + 8];
+  constexpr int MAXL = C * G;   // zL bound
+  constexpr int BPAD = MAXL + C + 4;  // b after four zeros, zeros to the window's reach
+  __shared__ uint32_t sa[PPB][MAXL], sbp[PPB][BPAD];
   const int grp = threadIdx.x / G, lane = threadIdx.x % G;
   const uint32_t z = blockIdx.x * PPB + grp;
   BatchDesc d;
@@ -108,59 +123,47 @@ __global__ __launch_bounds__(256) void k_batch_school_g(const BatchDesc* descs, 
   if (z < count) d = descs[z];
   const int na = (int)d.na, nb = (int)d.nb, zL = na + nb;
   for (int i = lane; i < na; i += G) sa[grp][i] = d.a[i];
-  for (int i = lane; i < MAXL + 8; i += G) {
-    const int j = i - 3;
+  for (int i = lane; i < BPAD; i += G) {
+    const int j = i - 4;
     sbp[grp][i] = (j >= 0 && j < nb) ? d.b[j] : 0u;
   }
   __syncthreads();
-  const int k0 = 4 * lane;
-  uint64_t lo[4] = {0, 0, 0, 0};
-  uint32_t hi[4] = {0, 0, 0, 0};
+  const int k0 = C * lane;
+  uint64_t lo[C];
+  uint32_t hi[C];
+#pragma unroll
+  for (int q = 0; q < C; ++q) {
+    lo[q] = 0;
+    hi[q] = 0;
+  }
   int ilo = k0 - nb + 1;
   if (ilo < 0) ilo = 0;
-  const int ihi = k0 + 3 < na - 1 ? k0 + 3 : na - 1;
-  if (k0 < zL && ilo <= ihi) {
-    const uint32_t* sb = sbp[grp];
-    uint32_t w0 = sb[3 + k0 - ilo], w1 = sb[4 + k0 - ilo], w2 = sb[5 + k0 - ilo], w3 = sb[6 + k0 - ilo];
-    for (int i = ilo; i <= ihi; ++i) {
-      const uint64_t ai = sa[grp][i];
-      const uint64_t p0 = ai * w0, p1 = ai * w1, p2 = ai * w2, p3 = ai * w3;
-      lo[0] += p0;
-      hi[0] += lo[0] < p0;
-      lo[1] += p1;
-      hi[1] += lo[1] < p1;
-      lo[2] += p2;
-      hi[2] += lo[2] < p2;
-      lo[3] += p3;
-      hi[3] += lo[3] < p3;
-      w3 = w2;
-      w2 = w1;
-      w1 = w0;
-      const int nxt = 2 + k0 - i;  // b[k0 - (i + 1)], needed while i + 1 <= k0 + 3
-      w0 = nxt >= 0 ? sb[nxt] : 0u;
-    }
+  const int ihi = k0 + C - 1 < na - 1 ? k0 + C - 1 : na - 1;
+  if (k0 < zL && ilo <= ihi && nb > 0) {
+    school_columns<C>(sa[grp], sbp[grp], k0, ilo, ihi, lo, hi);
   }
   // limb k receives w0_k + w1_{k-1} + w2_{k-2}; the previous thread's top
   // words come by shuffle (zero for the group's first thread)
-  uint32_t x0[4], x1[4], x2[4];
+  uint32_t x0[C], x1[C], x2[C];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < C; ++q) {
     x0[q] = (uint32_t)lo[q];
     x1[q] = (uint32_t)(lo[q] >> 32);
     x2[q] = hi[q];
   }
-  uint32_t p1 = __shfl_up(x1[3], 1, G), p22 = __shfl_up(x2[2], 1, G), p23 = __shfl_up(x2[3], 1, G);
+  uint32_t p1 = __shfl_up(x1[C - 1], 1, G), p22 = __shfl_up(x2[C - 2], 1, G),
+           p23 = __shfl_up(x2[C - 1], 1, G);
   if (lane == 0) p1 = p22 = p23 = 0;
-  uint64_t s[4];
+  uint64_t s[C];
   s[0] = (uint64_t)x0[0] + p1 + p22;
   s[1] = (uint64_t)x0[1] + x1[0] + p23;
-  s[2] = (uint64_t)x0[2] + x1[1] + x2[0];
-  s[3] = (uint64_t)x0[3] + x1[2] + x2[1];
+#pragma unroll
+  for (int q = 2; q < C; ++q) s[q] = (uint64_t)x0[q] + x1[q - 1] + x2[q - 2];
   CarryFn f;
   f.base = 0;
   f.T = 0;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < C; ++q) {
     CarryFn g;
     g.base = s[q] >> 32;
     g.T = (1ull << 32) - (s[q] & 0xffffffffull);
@@ -176,7 +179,7 @@ __global__ __launch_bounds__(256) void k_batch_school_g(const BatchDesc* descs, 
   if (lane == 0) excl.T = 0;
   uint64_t c = cf_eval0(excl);
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < C; ++q) {
     const uint64_t t = (s[q] & 0xffffffffull) + c;
     if (k0 + q < zL) d.out[k0 + q] = (uint32_t)t;
     c = (s[q] >> 32) + (t >> 32);

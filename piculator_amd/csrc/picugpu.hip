@@ -691,8 +691,9 @@ int ensure_pool(Ctx& c, size_t limbs, size_t ndesc) {
 
 // Size classes of the batched paths.  Small products (zL = na + nb in
 // (2^(b-1), 2^b], b <= 12) go to the class g_small_class[b]: 0 the
-// 256-thread schoolbook (zL <= 1024), 1 and 2 the grouped schoolbook with 16
-// or 64 threads per product (zL <= 64, <= 256), 10..12 the single-workgroup
+// 256-thread schoolbook (zL <= 1024), 1, 2 and 3 the grouped schoolbook with
+// 16 threads x 4 columns, 64 x 4 and 64 x 8 per product (zL <= 64, <= 256,
+// <= 512), 10..12 the single-workgroup
 // NTT of 2^k points, 110..112 the batched pass kernels at 2^(k - 100)
 // points (spectra through HBM); the autotuner (tune_small) picks among them
 // per bucket on the device at first use.  Classes 13..17 are the multi-pass
@@ -703,7 +704,7 @@ int g_small_class[kSmallNttMaxLog + 1];
 bool g_small_ready = false;
 
 // Largest zL of a schoolbook class.
-int school_max(int cls) { return cls == 1 ? 64 : cls == 2 ? 256 : kSchoolMaxLimbs; }
+int school_max(int cls) { return cls == 1 ? 64 : cls == 2 ? 256 : cls == 3 ? 512 : kSchoolMaxLimbs; }
 
 int small_class_default(int b) {
   if ((1 << b) <= 64) return 1;
@@ -715,21 +716,21 @@ int small_class_default(int b) {
 constexpr int kPassClass = 100;  // + log2 N: batched pass kernels
 
 bool small_class_valid(int b, int cls) {
-  if (cls >= 0 && cls <= 2) return (1 << b) <= school_max(cls);
+  if (cls >= 0 && cls <= 3) return (1 << b) <= school_max(cls);
   if (cls >= kPassClass + kSmallNttMinLog && cls <= kPassClass + kSmallNttMaxLog) return cls - kPassClass >= b;
   return cls >= kSmallNttMinLog && cls <= kSmallNttMaxLog && cls >= b;
 }
 
-// Slot of a small class in launch order (0..8).
+// Slot of a small class in launch order (0..9).
 int small_slot(int cls) {
-  if (cls <= 2) return cls;
-  if (cls < kPassClass) return cls - kSmallNttMinLog + 3;
-  return cls - kPassClass - kSmallNttMinLog + 6;
+  if (cls <= 3) return cls;
+  if (cls < kPassClass) return cls - kSmallNttMinLog + 4;
+  return cls - kPassClass - kSmallNttMinLog + 7;
 }
 int small_slot_class(int slot) {
-  if (slot <= 2) return slot;
-  if (slot <= 5) return slot - 3 + kSmallNttMinLog;
-  return slot - 6 + kSmallNttMinLog + kPassClass;
+  if (slot <= 3) return slot;
+  if (slot <= 6) return slot - 4 + kSmallNttMinLog;
+  return slot - 7 + kSmallNttMinLog + kPassClass;
 }
 
 int batch_class(uint64_t na, uint64_t nb) {
@@ -826,7 +827,8 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, b
   return PCG_OK;
 }
 
-// Launches one small class (0..2 schoolbook, 10..12 single-workgroup NTT)
+// Launches one small class (0..3 schoolbook, 10..12 single-workgroup NTT,
+// 110..112 batched passes)
 // over n consecutive descriptors (already on the device).
 int launch_small_class(Ctx& c, int cls, const BatchDesc* base, size_t n) {
   if (n == 0) return PCG_OK;
@@ -835,11 +837,15 @@ int launch_small_class(Ctx& c, int cls, const BatchDesc* base, size_t n) {
       hipLaunchKernelGGL(k_batch_school, dim3((unsigned)n), dim3(256), 0, c.stream, base);
       break;
     case 1:
-      hipLaunchKernelGGL(k_batch_school_g<16>, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, c.stream,
+      hipLaunchKernelGGL((k_batch_school_g<16, 4>), dim3((unsigned)((n + 15) / 16)), dim3(256), 0, c.stream,
                          base, (uint32_t)n);
       break;
     case 2:
-      hipLaunchKernelGGL(k_batch_school_g<64>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.stream,
+      hipLaunchKernelGGL((k_batch_school_g<64, 4>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.stream,
+                         base, (uint32_t)n);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_batch_school_g<64, 8>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.stream,
                          base, (uint32_t)n);
       break;
     case 10:
@@ -869,7 +875,8 @@ __global__ void k_fill_pattern(uint32_t* p, uint64_t n) {
 }
 
 // Autotuner of the small-product classes: for each bucket b = 4..12 it
-// times 2048 products of na = nb = 3 * 2^(b-3) limbs (zL = 3/4 of the
+// times a launch of 2048 to 16384 products (about the bucket's share of one
+// level of a large subtree) of na = nb = 3 * 2^(b-3) limbs (zL = 3/4 of the
 // bucket's top) through every class valid for the bucket (the schoolbook
 // kernels, the single-workgroup NTT and the batched pass kernels at
 // 2^max(b, 10) points; best of three launches, HIP events on the library
@@ -878,14 +885,17 @@ __global__ void k_fill_pattern(uint32_t* p, uint64_t n) {
 // crossovers.
 int tune_small(Ctx& c) {
   for (int b = 0; b <= kSmallNttMaxLog; ++b) g_small_class[b] = small_class_default(b);
-  constexpr int kCount = 2048;
-  constexpr int kMaxHalf = 3 << 9;  // na at b = 12
+  // products per timed launch: about what one level of a 2^19-leaf subtree
+  // holds in the bucket (4 d / 2^(b-3)), between 2048 and 16384
+  auto count_of = [](int b) { return std::min(16384, std::max(2048, 1 << (23 - b))); };
+  constexpr int kMaxCount = 16384;
+  constexpr size_t kMaxLimbs = (size_t)2048 * 2 * (3 << 9);  // count x zL, equal for b >= 9
   uint32_t* in = nullptr;
   uint32_t* out = nullptr;
   BatchDesc* dd = nullptr;
-  const size_t in_limbs = (size_t)kCount * 2 * kMaxHalf, out_limbs = (size_t)kCount * 2 * kMaxHalf;
+  const size_t in_limbs = kMaxLimbs, out_limbs = kMaxLimbs;
   if (hipMalloc(&in, in_limbs * 4) != hipSuccess || hipMalloc(&out, out_limbs * 4) != hipSuccess ||
-      hipMalloc(&dd, kCount * sizeof(BatchDesc)) != hipSuccess) {
+      hipMalloc(&dd, kMaxCount * sizeof(BatchDesc)) != hipSuccess) {
     (void)hipGetLastError();
     (void)hipFree(in);
     (void)hipFree(out);
@@ -893,10 +903,15 @@ int tune_small(Ctx& c) {
   }
   hipLaunchKernelGGL(k_fill_pattern, dim3((unsigned)((in_limbs + 255) / 256)), dim3(256), 0, c.stream, in,
                      (uint64_t)in_limbs);
-  std::vector<BatchDesc> h(kCount);
+  std::vector<BatchDesc> h(kMaxCount);
   int rc = PCG_OK;
   for (int b = 4; b <= kSmallNttMaxLog && rc == PCG_OK; ++b) {
     const uint32_t n = 3u << (b - 3);
+    const int kCount = count_of(b);
+    if ((size_t)kCount * 2 * n > kMaxLimbs) {
+      rc = fail(PCG_ERR_INTERNAL, "autotuner batch exceeds its buffers");
+      break;
+    }
     for (int i = 0; i < kCount; ++i) {
       h[i].a = in + (size_t)i * 2 * n;
       h[i].b = h[i].a + n;
@@ -910,10 +925,10 @@ int tune_small(Ctx& c) {
       break;
     }
     const int lg = b < kSmallNttMinLog ? kSmallNttMinLog : b;
-    const int cls[5] = {1, 2, 0, lg, kPassClass + lg};
+    const int cls[6] = {1, 2, 3, 0, lg, kPassClass + lg};
     float best_ms = 1e30f;
     int best_cls = small_class_default(b);
-    for (int m = 0; m < 5 && rc == PCG_OK; ++m) {
+    for (int m = 0; m < 6 && rc == PCG_OK; ++m) {
       if (!small_class_valid(b, cls[m])) continue;
       float best = 1e30f;
       for (int rep = 0; rep < 4 && rc == PCG_OK; ++rep) {
@@ -945,10 +960,10 @@ int tune_small(Ctx& c) {
 int ensure_tuned(Ctx& c) { return g_small_ready ? PCG_OK : tune_small(c); }
 
 int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, bool dev) {
-  // class slots: 0..2 schoolbook classes 0..2, 3..5 single-workgroup NTT
-  // (2^10 .. 2^12), 6..8 batched passes at 2^10 .. 2^12, 9..13 batched
+  // class slots: 0..3 schoolbook classes 0..3, 4..6 single-workgroup NTT
+  // (2^10 .. 2^12), 7..9 batched passes at 2^10 .. 2^12, 10..14 batched
   // multi-pass (2^13 .. 2^17)
-  constexpr int kSmallSlots = 9;
+  constexpr int kSmallSlots = 10;
   constexpr int kSlots = kSmallSlots + (kBatchMaxLog - kSmallNttMaxLog);
   int rc0 = ensure_tuned(c);
   if (rc0) return rc0;

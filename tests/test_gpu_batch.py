@@ -84,7 +84,7 @@ def test_batch_dev_pointers(gpu):
         assert np.array_equal(out, exact_product(a, b))
 
 
-SCHOOL_MAX = {0: 1024, 1: 64, 2: 256}
+SCHOOL_MAX = {0: 1024, 1: 64, 2: 256, 3: 512}
 
 
 def class_valid(b, c):
@@ -105,12 +105,13 @@ def test_autotune_table_is_valid(gpu):
     assert gpu.batch_table() == t
 
 
-@pytest.mark.parametrize("forced", [0, 1, 2, 10, 110])
+@pytest.mark.parametrize("forced", [0, 1, 2, 3, 10, 110])
 def test_forced_small_classes_exact(gpu, forced):
     """Every kernel of the tuned crossover (schoolbook with 256 threads, with
-    16 or 64 threads per product, 2^10-point single-workgroup NTT, batched
-    pass kernels at 2^10 points) gives the exact product in every bucket it
-    may serve, including all-ones carry chains and unbalanced shapes."""
+    16 or 64 threads per product and 4 or 8 columns per thread, 2^10-point
+    single-workgroup NTT, batched pass kernels at 2^10 points) gives the
+    exact product in every bucket it may serve, including all-ones carry
+    chains and unbalanced shapes."""
     saved = gpu.batch_table()
     table = [forced if class_valid(b, forced) else c for b, c in enumerate(saved)]
     gpu.batch_table(table)
