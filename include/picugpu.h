@@ -140,6 +140,13 @@ typedef struct pcg_bsr_value {
 int pcg_bsr_exact(uint32_t a, uint32_t b, uint64_t max_limbs, pcg_bsr_value* P, pcg_bsr_value* Q,
                   pcg_bsr_value* R);
 int pcg_bsr_bound(uint32_t a, uint32_t b, uint64_t* bound);
+/* Starts building, on host threads, the evaluation plans of subtrees of the
+ * given sizes (b - a) and returns at once; a later pcg_bsr_exact or
+ * pcg_df_bsr_exact of such a size waits for its plan instead of building it.
+ * The reference has no plans (Pi_BSR recurses, PI.h:64-112); a caller about
+ * to evaluate many subtrees of a few sizes calls this first so the host-side
+ * planning overlaps other work. */
+int pcg_bsr_prepare(const uint32_t* sizes, uint64_t n);
 
 /* ---- Device-resident BigFloat values ----
  * A BigFloat (BigFloat.h:33-36) whose limbs stay in device memory, so the

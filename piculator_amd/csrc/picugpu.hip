@@ -10,12 +10,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <future>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -702,6 +705,7 @@ constexpr int kBatchMaxLog = 17;
 constexpr int kSmallNttMinLog = 10;
 int g_small_class[kSmallNttMaxLog + 1];
 bool g_small_ready = false;
+uint64_t g_table_gen = 1;  // bumped whenever g_small_class changes
 
 // Largest zL of a schoolbook class.
 int school_max(int cls) { return cls == 1 ? 64 : cls == 2 ? 256 : cls == 3 ? 512 : kSchoolMaxLimbs; }
@@ -743,9 +747,9 @@ int batch_class(uint64_t na, uint64_t nb) {
 
 // Class of a product of at most zL limbs on the device-resident path:
 // the small classes above, else the transform length (log2).
-int bsr_class(uint64_t zL) {
+int bsr_class(const std::array<int, kSmallNttMaxLog + 1>& tab, uint64_t zL) {
   const int lg = ceil_log2(zL);
-  if (lg <= kSmallNttMaxLog) return g_small_class[lg];
+  if (lg <= kSmallNttMaxLog) return tab[lg];
   return lg < kMinLogN ? kMinLogN : lg;
 }
 
@@ -954,6 +958,7 @@ int tune_small(Ctx& c) {
   (void)hipFree(dd);
   if (rc) return rc;
   g_small_ready = true;
+  ++g_table_gen;
   return PCG_OK;
 }
 
@@ -1199,7 +1204,24 @@ void bsr_plan_free(BsrPlan& P) {
   P = BsrPlan();
 }
 
-int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
+// Host side of a subtree plan (CPU only, no GPU call): the tree of d leaves
+// (PI.h:90 midpoints), value bounds, arena offsets, and per height the
+// products (grouped by class with the given small-class table, paired
+// around shared operands) and the adds.
+struct BsrHostPlan {
+  BsrPlan P;  // sizes and heights; no device memory yet
+  std::vector<BsrVal> meta;
+  std::vector<uint32_t> leaf_rel, leaf_ids;
+  std::vector<BsrMul> muls;
+  std::vector<BsrAdd> adds;
+  size_t max_large = 0, max_large_chunks = 0;
+  uint64_t table_gen = 0;  // generation of the class table it was planned with
+};
+
+using SmallTable = std::array<int, kSmallNttMaxLog + 1>;
+
+void bsr_host_plan(uint32_t d, const SmallTable& tab, BsrHostPlan& HP) {
+  BsrPlan& P = HP.P;
   std::vector<BsrNode> nodes;
   nodes.reserve(2 * (size_t)d);
   bsr_build(nodes, 0, d);
@@ -1207,7 +1229,8 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
   P.d = d;
   P.maxb = bsr_bounds(nodes, bnd);
   const size_t nv = nodes.size() * 5;
-  std::vector<BsrVal> meta(nv);
+  std::vector<BsrVal>& meta = HP.meta;
+  meta.resize(nv);
   uint64_t off = 0;
   for (size_t v = 0; v < nv; ++v) {
     meta[v].off = off;
@@ -1221,7 +1244,8 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
     P.root_off[j] = meta[j].off;
   }
   P.H = nodes[0].height;
-  std::vector<uint32_t> leaf_rel, leaf_ids;
+  std::vector<uint32_t>& leaf_rel = HP.leaf_rel;
+  std::vector<uint32_t>& leaf_ids = HP.leaf_ids;
   std::vector<std::vector<int32_t>> by_h(P.H + 1);
   for (int32_t i = 0; i < (int32_t)nodes.size(); ++i) {
     by_h[nodes[i].height].push_back(i);
@@ -1232,9 +1256,10 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
   }
   P.nleaves = leaf_rel.size();
   P.hp.assign(P.H + 1, BsrHeight());
-  std::vector<BsrMul> muls;
-  std::vector<BsrAdd> adds;
-  size_t max_large = 0, max_large_chunks = 0;
+  std::vector<BsrMul>& muls = HP.muls;
+  std::vector<BsrAdd>& adds = HP.adds;
+  size_t& max_large = HP.max_large;
+  size_t& max_large_chunks = HP.max_large_chunks;
   for (int32_t h = 2; h <= P.H; ++h) {
     std::vector<std::pair<int, BsrMul>> ms;
     std::vector<std::pair<int, BsrMulPair>> mp;
@@ -1251,8 +1276,8 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
       const BsrMul m4[4] = {{Rt + 1, Lf + 0, I + 3}, {Rt + 1, Lf + 1, I + 1},
                             {Lf + 2, Rt + 0, I + 4}, {Lf + 2, Rt + 2, I + 2}};
       for (int q = 0; q < 4; q += 2) {
-        const int c0 = bsr_class(bnd[m4[q].a] + bnd[m4[q].b]);
-        const int c1 = bsr_class(bnd[m4[q + 1].a] + bnd[m4[q + 1].b]);
+        const int c0 = bsr_class(tab, bnd[m4[q].a] + bnd[m4[q].b]);
+        const int c1 = bsr_class(tab, bnd[m4[q + 1].a] + bnd[m4[q + 1].b]);
         if (c0 == c1 && class_pairs(c0)) {
           mp.push_back({c0, BsrMulPair{m4[q], m4[q + 1]}});
         } else {
@@ -1306,6 +1331,18 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
     adds.insert(adds.end(), small.begin(), small.end());
     adds.insert(adds.end(), large.begin(), large.end());
   }
+}
+
+// Device side: allocates the plan's buffers and arena and uploads it.
+int bsr_plan_upload(Ctx& c, const BsrHostPlan& HP, BsrPlan& P) {
+  P = HP.P;
+  const size_t nv = HP.meta.size();
+  const std::vector<BsrVal>& meta = HP.meta;
+  const std::vector<uint32_t>& leaf_rel = HP.leaf_rel;
+  const std::vector<uint32_t>& leaf_ids = HP.leaf_ids;
+  const std::vector<BsrMul>& muls = HP.muls;
+  const std::vector<BsrAdd>& adds = HP.adds;
+  const size_t max_large = HP.max_large, max_large_chunks = HP.max_large_chunks;
   const size_t need = nv * sizeof(BsrVal) + leaf_rel.size() * 4 + leaf_ids.size() * 4 +
                       muls.size() * sizeof(BsrMul) + adds.size() * sizeof(BsrAdd) +
                       muls.size() * sizeof(BatchDesc) + max_large * sizeof(BsrAddSlot) +
@@ -1314,7 +1351,7 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
     (void)hipGetLastError();
     bsr_plan_free(P);
     return fail(PCG_ERR_OOM, "cannot allocate the binary-splitting plan (%llu arena limbs)",
-                (unsigned long long)off);
+                (unsigned long long)HP.P.arena_limbs);
   }
   char* p = static_cast<char*>(P.buf);
   P.meta = carve<BsrVal>(p, nv);
@@ -1333,14 +1370,32 @@ int bsr_plan_build(Ctx& c, uint32_t d, BsrPlan& P) {
     HIPCHK(hipMemcpyAsync(P.muls, muls.data(), muls.size() * sizeof(BsrMul), hipMemcpyHostToDevice, c.stream));
   if (!adds.empty())
     HIPCHK(hipMemcpyAsync(P.adds, adds.data(), adds.size() * sizeof(BsrAdd), hipMemcpyHostToDevice, c.stream));
-  HIPCHK(hipStreamSynchronize(c.stream));  // host vectors go out of scope
+  HIPCHK(hipStreamSynchronize(c.stream));  // the host plan may be freed after this
   return PCG_OK;
 }
+
 
 // Two cached plans: the binary splitting produces at most two subtree sizes
 // at a given depth.
 BsrPlan g_bsr_plans[2];
 int g_bsr_next = 0;
+
+// Host plans being built on other threads (pcg_bsr_prepare), by subtree
+// size; touched only under the library lock.
+std::map<uint32_t, std::shared_future<std::shared_ptr<BsrHostPlan>>> g_prep;
+
+SmallTable small_table() {
+  SmallTable t;
+  for (int b = 0; b <= kSmallNttMaxLog; ++b) t[b] = g_small_class[b];
+  return t;
+}
+
+std::shared_ptr<BsrHostPlan> host_plan_job(uint32_t d, SmallTable tab, uint64_t gen) {
+  auto h = std::make_shared<BsrHostPlan>();
+  bsr_host_plan(d, tab, *h);
+  h->table_gen = gen;
+  return h;
+}
 
 // Evaluates the subtree [a, b) into the plan's arena; rv receives the
 // metadata of P, Q, R (their limbs stay in (*Pout)->arena until the next call).
@@ -1358,7 +1413,17 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
     g_bsr_next ^= 1;
     HIPCHK(hipStreamSynchronize(c.stream));
     bsr_plan_free(*P);
-    int rc = bsr_plan_build(c, d, *P);
+    // a plan prepared in the background for this size and class table, or
+    // one built here
+    std::shared_ptr<BsrHostPlan> hp;
+    auto it = g_prep.find(d);
+    if (it != g_prep.end()) {
+      const std::shared_future<std::shared_ptr<BsrHostPlan>> fut = it->second;
+      g_prep.erase(it);
+      hp = fut.get();
+    }
+    if (!hp || hp->table_gen != g_table_gen) hp = host_plan_job(d, small_table(), g_table_gen);
+    int rc = bsr_plan_upload(c, *hp, *P);
     if (rc) return rc;
   }
   if (P->maxb > max_limbs)
@@ -1893,6 +1958,7 @@ void pcg_shutdown(void) {
   (void)hipFree(c.bspec);
   (void)hipFree(c.bagg);
   (void)hipFree(c.bcin);
+  g_prep.clear();  // waits for background plan builds
   for (BsrPlan& q : g_bsr_plans) bsr_plan_free(q);
   (void)df_release_pool(c);
   (void)hipFree(c.df_agg);
@@ -2272,6 +2338,23 @@ int pcg_df_bsr_exact(uint32_t a, uint32_t b, uint64_t max_limbs, pcg_dfloat* P, 
   return PCG_OK;
 }
 
+int pcg_bsr_prepare(const uint32_t* sizes, uint64_t n) {
+  if (!sizes && n) return fail(PCG_ERR_INVALID, "null pointer");
+  DF_ENTER();
+  int rc = ensure_tuned(c);  // plans record the small-product classes
+  if (rc) return rc;
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint32_t d = sizes[i];
+    if (d == 0 || g_prep.count(d)) continue;
+    bool have = false;
+    for (const BsrPlan& q : g_bsr_plans)
+      if (q.d == d && q.buf) have = true;
+    if (have) continue;
+    g_prep[d] = std::async(std::launch::async, host_plan_job, d, small_table(), g_table_gen).share();
+  }
+  return PCG_OK;
+}
+
 int pcg_autotune(int* table, int n) {
   DF_ENTER();
   HIPCHK(hipStreamSynchronize(c.stream));
@@ -2293,6 +2376,7 @@ int pcg_batch_table(int* table, int n, int set) {
     for (BsrPlan& q : g_bsr_plans) bsr_plan_free(q);
     for (int b = 0; b < n && b <= kSmallNttMaxLog; ++b) g_small_class[b] = table[b];
     g_small_ready = true;
+    ++g_table_gen;
     return PCG_OK;
   }
   for (int b = 0; b < n && b <= kSmallNttMaxLog; ++b) table[b] = g_small_class[b];

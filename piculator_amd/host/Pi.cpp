@@ -174,6 +174,22 @@ bool bsr_device_resident(DPQR& out, uint32_t a, uint32_t b, size_t p) {
   return true;
 }
 
+// Distinct sizes of the subtrees bsr_resident hands to the GPU (the same
+// walk and the same bound test as bsr_device_resident).
+void resident_sizes(uint32_t a, uint32_t b, size_t p, std::vector<uint32_t>& sizes) {
+  if (b - a <= kDeviceLeaves) {
+    uint64_t bound = 0;
+    if (pcg_bsr_bound(a, b, &bound) == PCG_OK && bound <= p + YCL_BIGFLOAT_EXTRA_PRECISION) {
+      if (std::find(sizes.begin(), sizes.end(), b - a) == sizes.end()) sizes.push_back(b - a);
+      return;
+    }
+  }
+  if (b - a <= 64) return;
+  const uint32_t m = (a + b) / 2;
+  resident_sizes(a, m, p, sizes);
+  resident_sizes(m, b, p, sizes);
+}
+
 // The series with every value resident in device memory: GPU subtrees
 // below, the node combine of PI.h:95-110 on DeviceFloat above (the same
 // truncation as BigFloat, so the same values).
@@ -230,47 +246,60 @@ size_t pi_hex_into(size_t decimal_digits, PiMode mode, char* out, size_t cap, Pi
   if ((uint32_t)terms != terms) throw "Limit Exceeded";
   const uint64_t m0 = big_mul_count();
   const double t0 = now_s();
-  double t1, t2, t3, t4;
+  PiTimes pt;  // phase durations
   size_t n = 0;
   if (mode == PiMode::Plain) {
     PQR s;
     bsr_plain(s, 0, (uint32_t)terms, p);
     BigFloat P = s.Q.mul(13591409u).add(s.P, p);
     BigFloat Q = s.Q.mul(4270934400u);
-    t1 = now_s();
+    const double t1 = now_s();
+    pt.series = t1 - t0;
     P = Q.div(P, p);
-    t2 = now_s();
+    const double t2 = now_s();
+    pt.division = t2 - t1;
     Q = invsqrt(10005, p);
-    t3 = now_s();
+    const double t3 = now_s();
+    pt.invsqrt = t3 - t2;
     P = P.mul(Q, p);
-    t4 = now_s();
+    const double t4 = now_s();
+    pt.final_mul = t4 - t3;
     const std::string str = P.to_string_hex(hex);
     if (str.size() + 1 > cap) throw "output capacity too small";
     std::memcpy(out, str.c_str(), str.size() + 1);
     n = str.size();
+    pt.conversion = now_s() - t4;
   } else {
+    // The subtree plans are built on host threads while the GPU computes
+    // 1/sqrt(10005) (independent of the series, so computing it first
+    // changes no value).
+    std::vector<uint32_t> sizes;
+    resident_sizes(0, (uint32_t)terms, p, sizes);
+    if (!sizes.empty() && pcg_bsr_prepare(sizes.data(), sizes.size()) != PCG_OK)
+      throw pcg_last_error_string();
+    const double ti = now_s();
+    const DeviceFloat S = invsqrt_device(10005, p);
+    const double tb = now_s();
+    pt.invsqrt = tb - ti;
     DPQR s;
     bsr_resident(s, 0, (uint32_t)terms, p, verbose);
     DeviceFloat P = s.Q.mul(13591409u).add(s.P, p);
-    DeviceFloat Q = s.Q.mul(4270934400u);
+    const DeviceFloat Q = s.Q.mul(4270934400u);
     s = DPQR();
-    t1 = now_s();
+    const double t1 = now_s();
+    pt.series = t1 - tb;
     P = Q.div(P, p);
-    t2 = now_s();
-    Q = invsqrt_device(10005, p);
-    t3 = now_s();
-    P = P.mul(Q, p);
-    t4 = now_s();
+    const double t2 = now_s();
+    pt.division = t2 - t1;
+    P = P.mul(S, p);
+    const double t4 = now_s();
+    pt.final_mul = t4 - t2;
     n = P.to_hex_into(hex, out, cap);
+    pt.conversion = now_s() - t4;
   }
-  const double t5 = now_s();
   if (times) {
-    times->series = t1 - t0;
-    times->division = t2 - t1;
-    times->invsqrt = t3 - t2;
-    times->final_mul = t4 - t3;
-    times->conversion = t5 - t4;
-    times->total = t5 - t0;
+    *times = pt;
+    times->total = now_s() - t0;
     times->big_muls = big_mul_count() - m0;
   }
   return n;

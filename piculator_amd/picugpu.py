@@ -33,10 +33,10 @@ EXPORTS = [
     "pcg_mul_dev", "pcg_mul_batch", "pcg_mul_batch_dev", "pcg_sync", "pcg_dev_alloc",
     "pcg_dev_free", "pcg_memcpy_h2d", "pcg_memcpy_d2h", "pcg_fft_last_roundoff",
     "pcg_last_profile", "pcg_ntt_plan", "pcg_timer_start", "pcg_timer_stop",
-    "pcg_bsr_exact", "pcg_bsr_bound", "pcg_df_free", "pcg_df_set", "pcg_df_read", "pcg_df_copy",
-    "pcg_df_ucmp", "pcg_df_add", "pcg_df_sub", "pcg_df_mul", "pcg_df_mul_pair", "pcg_df_mul_u32",
-    "pcg_df_to_hex",
-    "pcg_df_bsr_exact", "pcg_df_release_pool", "pcg_autotune", "pcg_batch_table",
+    "pcg_bsr_exact", "pcg_bsr_bound", "pcg_bsr_prepare", "pcg_df_free", "pcg_df_set",
+    "pcg_df_read", "pcg_df_copy", "pcg_df_ucmp", "pcg_df_add", "pcg_df_sub", "pcg_df_mul",
+    "pcg_df_mul_pair", "pcg_df_mul_u32", "pcg_df_to_hex", "pcg_df_bsr_exact",
+    "pcg_df_release_pool", "pcg_autotune", "pcg_batch_table",
 ]
 
 
@@ -95,6 +95,7 @@ def load():
     L.pcg_timer_stop.argtypes = [C.POINTER(C.c_double)]
     L.pcg_bsr_exact.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64] + [C.POINTER(BsrValue)] * 3
     L.pcg_bsr_bound.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
+    L.pcg_bsr_prepare.argtypes = [C.POINTER(C.c_uint32), C.c_uint64]
     L.pcg_autotune.argtypes = [C.POINTER(C.c_int), C.c_int]
     L.pcg_batch_table.argtypes = [C.POINTER(C.c_int), C.c_int, C.c_int]
     dfp = C.POINTER(DFloat)
@@ -257,6 +258,13 @@ def bsr_bound(a: int, b: int) -> int:
     v = C.c_uint64()
     _check(load().pcg_bsr_bound(a, b, C.byref(v)))
     return v.value
+
+
+def bsr_prepare(sizes):
+    """Starts building the plans of subtrees of these sizes in the background."""
+    ensure_init()
+    arr = (C.c_uint32 * max(1, len(sizes)))(*sizes)
+    _check(load().pcg_bsr_prepare(arr, len(sizes)))
 
 
 def bsr_exact(a: int, b: int, max_limbs: int):

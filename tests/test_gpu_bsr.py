@@ -40,3 +40,14 @@ def test_device_resident_results(gpu, a, b):
     got = gpu.df_bsr_exact(a, b, BIG_P)
     P, Q, R = O.pi_bsr(a, b, BIG_P)
     assert got == [P.tuple(), Q.tuple(), R.tuple()]
+
+
+def test_prepared_plans(gpu):
+    """Plans built in the background (pcg_bsr_prepare) give the same values;
+    a size prepared twice, or never evaluated, is harmless."""
+    gpu.bsr_prepare([8000, 8001, 8000, 123])
+    for a, b in [(5000, 13000), (40, 8041)]:
+        got = gpu.bsr_exact(a, b, BIG_P)
+        P, Q, R = O.pi_bsr(a, b, BIG_P)
+        assert got == [P.tuple(), Q.tuple(), R.tuple()]
+    gpu.bsr_prepare([])
