@@ -106,6 +106,10 @@ int pcg_sync(void);
 /* Device memory helpers for callers without their own allocator. */
 int pcg_dev_alloc(void** ptr, uint64_t bytes);
 int pcg_dev_free(void* ptr);
+/* Page-locked host memory (hipHostMalloc) for large results such as the hex
+ * text of pcg_df_to_hex, which then arrives at full PCIe bandwidth. */
+int pcg_host_alloc(void** ptr, uint64_t bytes);
+int pcg_host_free(void* ptr);
 int pcg_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
 int pcg_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
 

@@ -2105,6 +2105,19 @@ int pcg_dev_free(void* ptr) {
   HIPCHK(hipFree(ptr));
   return PCG_OK;
 }
+int pcg_host_alloc(void** ptr, uint64_t bytes) {
+  if (!ptr) return fail(PCG_ERR_INVALID, "null pointer");
+  if (hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    *ptr = nullptr;
+    return fail(PCG_ERR_OOM, "hipHostMalloc(%llu) failed", (unsigned long long)bytes);
+  }
+  return PCG_OK;
+}
+int pcg_host_free(void* ptr) {
+  HIPCHK(hipHostFree(ptr));
+  return PCG_OK;
+}
 int pcg_memcpy_h2d(void* dst, const void* src, uint64_t bytes) {
   HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
   return PCG_OK;

@@ -29,24 +29,32 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "GPU init failed: %s\n", pcg_last_error_string());
     return 1;
   }
+  char* buf = nullptr;  // pi.txt in page-locked memory (the text comes off the GPU)
   try {
+    size_t hex = 0, p = 0, terms = 0;
+    piculator::pi_sizes(digits, &hex, &p, &terms);
+    const size_t cap = hex + 32;
+    if (pcg_host_alloc(reinterpret_cast<void**>(&buf), cap) != PCG_OK) throw pcg_last_error_string();
     piculator::PiTimes t;
-    const std::string s = piculator::pi_hex(
-        digits, mode == "plain" ? piculator::PiMode::Plain : piculator::PiMode::Batched, &t, true);
-    std::printf("Piculator (MI355X)\nDecimal Digits: %zu  Hex Digits: %zu  mode: %s\n", digits,
-                s.size() - 2, mode.c_str());
+    const size_t n = piculator::pi_hex_into(
+        digits, mode == "plain" ? piculator::PiMode::Plain : piculator::PiMode::Batched, buf, cap, &t,
+        true);
+    std::printf("Piculator (MI355X)\nDecimal Digits: %zu  Hex Digits: %zu  mode: %s\n", digits, n - 2,
+                mode.c_str());
     std::printf("series %.3f s, division %.3f s, invsqrt %.3f s, final multiply %.3f s, "
                 "conversion %.3f s, total %.3f s, big multiplies %llu\n",
                 t.series, t.division, t.invsqrt, t.final_mul, t.conversion, t.total,
                 (unsigned long long)t.big_muls);
     FILE* f = std::fopen(out.c_str(), "wb");
     if (!f) throw "Cannot Create File";
-    std::fwrite(s.data(), 1, s.size(), f);
+    std::fwrite(buf, 1, n, f);
     std::fclose(f);
   } catch (const char* e) {
     std::fprintf(stderr, "error: %s\n", e);
+    if (buf) pcg_host_free(buf);
     return 1;
   }
+  pcg_host_free(buf);
   pcg_shutdown();
   return 0;
 }

@@ -31,7 +31,8 @@ STATUS = {
 EXPORTS = [
     "pcg_init", "pcg_shutdown", "pcg_last_error_string", "pcg_reserve", "pcg_mul",
     "pcg_mul_dev", "pcg_mul_batch", "pcg_mul_batch_dev", "pcg_sync", "pcg_dev_alloc",
-    "pcg_dev_free", "pcg_memcpy_h2d", "pcg_memcpy_d2h", "pcg_fft_last_roundoff",
+    "pcg_dev_free", "pcg_host_alloc", "pcg_host_free", "pcg_memcpy_h2d", "pcg_memcpy_d2h",
+    "pcg_fft_last_roundoff",
     "pcg_last_profile", "pcg_ntt_plan", "pcg_timer_start", "pcg_timer_stop",
     "pcg_bsr_exact", "pcg_bsr_bound", "pcg_bsr_prepare", "pcg_df_free", "pcg_df_set",
     "pcg_df_read", "pcg_df_copy", "pcg_df_ucmp", "pcg_df_add", "pcg_df_sub", "pcg_df_mul",
@@ -87,6 +88,8 @@ def load():
     L.pcg_mul_batch_dev.argtypes = [C.POINTER(MulDesc), u64, C.c_uint32]
     L.pcg_dev_alloc.argtypes = [C.POINTER(vp), u64]
     L.pcg_dev_free.argtypes = [vp]
+    L.pcg_host_alloc.argtypes = [C.POINTER(vp), u64]
+    L.pcg_host_free.argtypes = [vp]
     L.pcg_memcpy_h2d.argtypes = [vp, vp, u64]
     L.pcg_memcpy_d2h.argtypes = [vp, vp, u64]
     L.pcg_fft_last_roundoff.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int)]
@@ -205,6 +208,19 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+def host_buffer(nbytes: int):
+    """A ctypes char array of nbytes in page-locked host memory (pcg_host_alloc),
+    freed when the array is garbage-collected."""
+    import weakref
+
+    ensure_init()
+    p = C.c_void_p()
+    _check(load().pcg_host_alloc(C.byref(p), nbytes))
+    arr = (C.c_char * nbytes).from_address(p.value)
+    weakref.finalize(arr, load().pcg_host_free, p.value)
+    return arr
 
 
 def mul_dev(d_a: int, na: int, d_b: int, nb: int, d_out: int, engine: int = ENGINE_AUTO,
