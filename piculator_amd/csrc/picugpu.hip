@@ -1105,40 +1105,66 @@ int32_t bsr_build(std::vector<BsrNode>& nodes, uint32_t a, uint32_t b) {
   return id;
 }
 
-constexpr uint64_t kLeafBound = 5;  // leaf terms are below 2^160
+// Bit bounds of a leaf's P, Q, R for any b < 2^32 (PI.h:67-88, with the
+// factors 2b-1, 6b-5, 6b-1 in uint32 as the reference computes them):
+// |P| < (13591409 + 545140134 b) 2^96 < 2^158, Q = b^3 26726400 409297880
+// < 2^150, R < 2^96.
+constexpr uint64_t kLeafBits[3] = {158, 150, 96};
 
-// Limb bounds of P, Q, R of every node (bottom-up), max over the subtree.
-uint64_t bsr_bounds(const std::vector<BsrNode>& nodes, std::vector<uint64_t>& bnd) {
-  bnd.assign(nodes.size() * 5, 0);
+uint64_t limbs_of_bits(uint64_t bits) { return (bits + 31) / 32; }
+
+// Bounds of every value (P Q R x y per node) of a subtree, bottom-up:
+// bits (x = P_L Q_R and y = P_R R_L add their operands' bits, P = x + y one
+// more, Q and R add), tlen = limbs of the finished value (ceil(bits / 32):
+// a product's one-limb trim and add/sub's carry limb and strip leave exactly
+// the value's length), slot = limbs written before that (a product's
+// na + nb, an addition's longer operand + 1).  Returns the largest tlen.
+uint64_t bsr_bounds(const std::vector<BsrNode>& nodes, std::vector<uint64_t>& tlen,
+                    std::vector<uint64_t>& slot) {
+  std::vector<uint64_t> bits(nodes.size() * 5, 0);
+  tlen.assign(nodes.size() * 5, 0);
+  slot.assign(nodes.size() * 5, 0);
   uint64_t mx = 0;
   for (size_t i = nodes.size(); i-- > 0;) {
     const BsrNode& n = nodes[i];
-    uint64_t* v = &bnd[i * 5];  // P Q R x y
+    const size_t v = i * 5;  // P Q R x y
     if (n.left < 0) {
-      v[0] = v[1] = v[2] = kLeafBound;
+      for (int j = 0; j < 3; ++j) bits[v + j] = kLeafBits[j];
     } else {
-      const uint64_t* l = &bnd[n.left * 5];
-      const uint64_t* r = &bnd[n.right * 5];
-      v[3] = l[0] + r[1];
-      v[4] = r[0] + l[2];
-      v[0] = std::max(v[3], v[4]) + 1;
-      v[1] = l[1] + r[1];
-      v[2] = l[2] + r[2];
+      const size_t l = (size_t)n.left * 5, r = (size_t)n.right * 5;
+      bits[v + 3] = bits[l + 0] + bits[r + 1];
+      bits[v + 4] = bits[r + 0] + bits[l + 2];
+      bits[v + 0] = std::max(bits[v + 3], bits[v + 4]) + 1;
+      bits[v + 1] = bits[l + 1] + bits[r + 1];
+      bits[v + 2] = bits[l + 2] + bits[r + 2];
     }
-    for (int j = 0; j < 5; ++j) mx = std::max(mx, v[j]);
+    for (int j = 0; j < 5; ++j) tlen[v + j] = limbs_of_bits(bits[v + j]);
+    if (n.left < 0) {
+      for (int j = 0; j < 3; ++j) slot[v + j] = tlen[v + j];
+    } else {
+      const size_t l = (size_t)n.left * 5, r = (size_t)n.right * 5;
+      slot[v + 3] = tlen[l + 0] + tlen[r + 1];
+      slot[v + 4] = tlen[r + 0] + tlen[l + 2];
+      slot[v + 0] = std::max(tlen[v + 3], tlen[v + 4]) + 1;
+      slot[v + 1] = tlen[l + 1] + tlen[r + 1];
+      slot[v + 2] = tlen[l + 2] + tlen[r + 2];
+    }
+    for (int j = 0; j < 5; ++j) mx = std::max(mx, tlen[v + j]);
   }
   return mx;
 }
 
-// The same bounds for the root of a subtree of d leaves, by recursion on the
-// sizes (left d / 2, right d - d / 2): O(log d) distinct sizes, memoised.
+// The largest value length (tlen) of a subtree of d leaves, by recursion on
+// the sizes (left d / 2, right d - d / 2): O(log d) distinct sizes, memoised.
 struct BsrRootBound {
-  uint64_t P, Q, R, mx;
+  uint64_t P, Q, R, mx;  // bits of the root's P, Q, R; largest tlen
 };
 BsrRootBound bsr_bound_of(uint32_t d) {
   static std::mutex mu;
   static std::map<uint32_t, BsrRootBound> memo;
-  if (d == 1) return BsrRootBound{kLeafBound, kLeafBound, kLeafBound, kLeafBound};
+  if (d == 1)
+    return BsrRootBound{kLeafBits[0], kLeafBits[1], kLeafBits[2],
+                        limbs_of_bits(std::max({kLeafBits[0], kLeafBits[1], kLeafBits[2]}))};
   {
     std::lock_guard<std::mutex> lk(mu);
     auto it = memo.find(d);
@@ -1150,7 +1176,7 @@ BsrRootBound bsr_bound_of(uint32_t d) {
   v.P = std::max(x, y) + 1;
   v.Q = l.Q + r.Q;
   v.R = l.R + r.R;
-  v.mx = std::max({l.mx, r.mx, x, y, v.P, v.Q, v.R});
+  v.mx = std::max({l.mx, r.mx, limbs_of_bits(v.P), limbs_of_bits(v.Q), limbs_of_bits(v.R)});
   std::lock_guard<std::mutex> lk(mu);
   memo[d] = v;
   return v;
@@ -1225,9 +1251,9 @@ void bsr_host_plan(uint32_t d, const SmallTable& tab, BsrHostPlan& HP) {
   std::vector<BsrNode> nodes;
   nodes.reserve(2 * (size_t)d);
   bsr_build(nodes, 0, d);
-  std::vector<uint64_t> bnd;
+  std::vector<uint64_t> tlen, slot;
   P.d = d;
-  P.maxb = bsr_bounds(nodes, bnd);
+  P.maxb = bsr_bounds(nodes, tlen, slot);
   const size_t nv = nodes.size() * 5;
   std::vector<BsrVal>& meta = HP.meta;
   meta.resize(nv);
@@ -1236,11 +1262,11 @@ void bsr_host_plan(uint32_t d, const SmallTable& tab, BsrHostPlan& HP) {
     meta[v].off = off;
     meta[v].L = 0;
     meta[v].sign = 1;
-    off += bnd[v] + 1;  // one spare limb for an addition's carry
+    off += slot[v] + 1;  // one spare limb
   }
   P.arena_limbs = off;
   for (int j = 0; j < 3; ++j) {
-    P.root_bound[j] = bnd[j];
+    P.root_bound[j] = tlen[j];
     P.root_off[j] = meta[j].off;
   }
   P.H = nodes[0].height;
@@ -1276,8 +1302,8 @@ void bsr_host_plan(uint32_t d, const SmallTable& tab, BsrHostPlan& HP) {
       const BsrMul m4[4] = {{Rt + 1, Lf + 0, I + 3}, {Rt + 1, Lf + 1, I + 1},
                             {Lf + 2, Rt + 0, I + 4}, {Lf + 2, Rt + 2, I + 2}};
       for (int q = 0; q < 4; q += 2) {
-        const int c0 = bsr_class(tab, bnd[m4[q].a] + bnd[m4[q].b]);
-        const int c1 = bsr_class(tab, bnd[m4[q + 1].a] + bnd[m4[q + 1].b]);
+        const int c0 = bsr_class(tab, tlen[m4[q].a] + tlen[m4[q].b]);
+        const int c1 = bsr_class(tab, tlen[m4[q + 1].a] + tlen[m4[q + 1].b]);
         if (c0 == c1 && class_pairs(c0)) {
           mp.push_back({c0, BsrMulPair{m4[q], m4[q + 1]}});
         } else {
@@ -1286,11 +1312,11 @@ void bsr_host_plan(uint32_t d, const SmallTable& tab, BsrHostPlan& HP) {
         }
       }
       const BsrAdd ad{I + 3, I + 4, I + 0};  // P = x.add(y)
-      if (bnd[I] <= 1024) {
+      if (slot[I] <= 1024) {
         small.push_back(ad);
       } else {
         large.push_back(ad);
-        large_max = std::max(large_max, bnd[I]);
+        large_max = std::max(large_max, slot[I]);
       }
     }
     // group the height's products by class (a height has only a few

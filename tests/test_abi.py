@@ -45,26 +45,43 @@ def test_plan_limits():
 
 
 def _tree_bound(d: int) -> int:
-    """Limb bound of every value of a subtree of d leaves, node by node over
-    the explicit tree (midpoints m = (a + b) / 2 of PI.h:90): leaves 5 limbs,
-    x = P_L + Q_R, y = P_R + R_L, P = max(x, y) + 1, Q and R sums."""
+    """Largest value length (limbs) of a subtree of d leaves, node by node over
+    the explicit tree (midpoints m = (a + b) / 2 of PI.h:90), from bit bounds:
+    leaves P < 2^158, Q < 2^150, R < 2^96 for any b < 2^32 (the reference's
+    uint32 factors), x = P_L Q_R and y = P_R R_L add bits, P = max(x, y) + 1,
+    Q and R add; a value's length is ceil(bits / 32)."""
     mx = 0
+
+    def limbs(bits):
+        return (bits + 31) // 32
 
     def rec(a, b):
         nonlocal mx
         if b - a == 1:
-            mx = max(mx, 5)
-            return 5, 5, 5
+            mx = max(mx, limbs(158))
+            return 158, 150, 96
         m = (a + b) // 2
         lp, lq, lr = rec(a, m)
         rp, rq, rr = rec(m, b)
         x, y = lp + rq, rp + lr
         v = (max(x, y) + 1, lq + rq, lr + rr)
-        mx = max(mx, x, y, *v)
+        mx = max(mx, limbs(x), limbs(y), *(limbs(t) for t in v))
         return v
 
     rec(1000, 1000 + d)
     return mx
+
+
+def test_leaf_bit_bounds():
+    """The leaf bounds hold for the largest leaf index (P, Q, R of PI.h:67-88
+    with the reference's uint32 factors 2b-1, 6b-5, 6b-1)."""
+    m32 = (1 << 32) - 1
+    for b in (1, 2, 3, 70513670, 715827882, 715827883, m32 - 1, m32):
+        f1, f2, f3 = (2 * b - 1) & m32, (6 * b - 5) & m32, (6 * b - 1) & m32
+        P = (13591409 + 545140134 * b) * f1 * f2 * f3
+        Q = b ** 3 * 26726400 * 409297880
+        R = f1 * f2 * f3
+        assert P.bit_length() <= 158 and Q.bit_length() <= 150 and R.bit_length() <= 96
 
 
 def test_bsr_bound_matches_tree_walk():
