@@ -186,61 +186,110 @@ __global__ __launch_bounds__(256) void k_batch_school_g(const BatchDesc* descs, 
   }
 }
 
-// Block size N / kEPT (N = 2^logN, 2^10 <= N <= 2^12); all products of one
-// launch share logN.  prm points at the constants for this N.
+// One product per workgroup of N / kEPT threads (N = 2^logN, 2^10 <= N <=
+// 2^12); all products of one launch share logN, prm points at the constants
+// for this N.  The operands are read once, in first-stage order, into
+// registers.  Per prime: the forward DFTs of a and b run from registers to
+// registers (one LDS line for the middle stages), the pointwise product with
+// N^-1 stays in registers, one pass through LDS puts it in first-stage order
+// for the inverse DFT; the prime-0 result waits in registers.  CRT words go
+// through LDS into limb order for the block carry.  LDS: one padded line (or
+// the 3 N CRT words) and one stage-twiddle table, reloaded for prime 1.
 template <int LOGN>
 __global__ __launch_bounds__((1 << LOGN) / kEPT) void k_batch_ntt(const BatchDesc* descs,
                                                                   const Prime* prm, double2 crt) {
   using GE = Geo<LOGN, 0, true>;
   constexpr int N = 1 << LOGN;
-  constexpr int LP = GE::LP;  // padded line length
-  // [0, LP): operand a's line; [LP, 2 LP): operand b's; [2 LP, 2 LP + N):
-  // prime-0 results (natural order)
-  __shared__ double lds[2 * LP + N];
+  constexpr int LOGR1 = LOGN % 3 == 1 ? 1 : (LOGN % 3 == 2 ? 2 : 3);
+  constexpr int R1 = 1 << LOGR1;
+  constexpr int NB1 = kEPT / R1;
+  constexpr int LDSD = GE::LP > (3 * N) / 2 ? GE::LP : (3 * N) / 2;  // doubles
+  __shared__ double lds[LDSD];
   __shared__ CarryFn wtot[(N / kEPT + 63) / 64];
   __shared__ double2 ltw[lds_tw_size<LOGN>()];
   const BatchDesc d = descs[blockIdx.x];
-  const int tid = threadIdx.x, nth = blockDim.x;
+  const int tid = threadIdx.x;
   const int na = (int)d.na, nb = (int)d.nb, zL = na + nb;
-  double* la = lds;
-  double* lb = lds + LP;
-  double* r0 = lds + 2 * LP;
+  const Prime P0 = prm[0], P1 = prm[1];
+  uint32_t ua[kEPT], ub[kEPT];
+#pragma unroll
+  for (int b = 0; b < NB1; ++b)
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+      int lam, n;
+      stage_pos<GE, LOGR1>(b, r, lam, n);
+      ua[b * R1 + r] = n < na ? d.a[n] : 0u;
+      ub[b * R1 + r] = n < nb ? d.b[n] : 0u;
+    }
+  double z0[8];
+  double y[1][8];
   for (int pi = 0; pi < 2; ++pi) {
-    const Prime P = prm[pi];
+    const Prime& P = pi ? P1 : P0;
+    if (pi) __syncthreads();  // prime 0's last stage has read LDS and the table
     const double2* tw = stage_twiddles<GE>(ltw, P);
-    for (int i = tid; i < N; i += nth) {
-      la[GE::idx(i, 0)] = i < na ? (double)d.a[i] : 0.0;
-      lb[GE::idx(i, 0)] = i < nb ? (double)d.b[i] : 0.0;
+    double x[kEPT], yb[1][8];
+#pragma unroll
+    for (int q = 0; q < kEPT; ++q) x[q] = (double)ua[q];
+    line_dft_regs<GE>(x, y, lds, tw, P, false);
+#pragma unroll
+    for (int q = 0; q < kEPT; ++q) x[q] = (double)ub[q];
+    __syncthreads();
+    line_dft_regs<GE>(x, yb, lds, tw, P, false);
+    // pointwise (natural order: point tid + r N / 8) with N^-1, then the
+    // inverse DFT's first-stage order through LDS
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const double t = mm_gen(y[0][r], yb[0][r], P.p, P.pinv);
+      y[0][r] = mm_shoup(t, P.ninv.x, P.ninv.y, P.p);
     }
     __syncthreads();
-    line_dft<GE>(la, tw, P, false);
-    line_dft<GE>(lb, tw, P, false);
-    for (int i = tid; i < N; i += nth) {
-      const int li = GE::idx(i, 0);
-      const double x = mm_gen(la[li], lb[li], P.p, P.pinv);
-      la[li] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      int lam, n;
+      stage_pos<GE, 3>(0, r, lam, n);
+      lds[GE::idx(n, 0)] = y[0][r];
     }
     __syncthreads();
-    line_dft<GE>(la, tw, P, true);
+#pragma unroll
+    for (int b = 0; b < NB1; ++b)
+#pragma unroll
+      for (int r = 0; r < R1; ++r) {
+        int lam, n;
+        stage_pos<GE, LOGR1>(b, r, lam, n);
+        x[b * R1 + r] = lds[GE::idx(n, 0)];
+      }
+    __syncthreads();
+    line_dft_regs<GE>(x, y, lds, tw, P, true);
     if (pi == 0) {
 #pragma unroll
-      for (int q = 0; q < kEPT; ++q) r0[tid * kEPT + q] = la[GE::idx(tid * kEPT + q, 0)];
-      __syncthreads();
+      for (int r = 0; r < 8; ++r) z0[r] = y[0][r];
     }
   }
-  const Prime P0 = prm[0], P1 = prm[1];
+  // CRT of coefficient n = tid + r N / 8; words to LDS, then limb order
+  __syncthreads();
+  uint32_t* sw = reinterpret_cast<uint32_t*>(lds);  // w0 [0, N), w1 [N, 2N), w2 [2N, 3N)
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    int lam, n;
+    stage_pos<GE, 3>(0, r, lam, n);
+    uint32_t c0 = 0, c1 = 0, c2 = 0;
+    if (n < zL) crt_words(z0[r], y[0][r], P0, P1, crt, c0, c1, c2);
+    sw[n] = c0;
+    sw[N + n] = c1;
+    sw[2 * N + n] = c2;
+  }
+  __syncthreads();
   uint32_t w0[kEPT], w1[kEPT], w2[kEPT], D[kEPT];
 #pragma unroll
   for (int q = 0; q < kEPT; ++q) {
     const int k = tid * kEPT + q;
-    w0[q] = w1[q] = w2[q] = 0;
-    if (k < zL) crt_words(r0[k], la[GE::idx(k, 0)], P0, P1, crt, w0[q], w1[q], w2[q]);
+    w0[q] = sw[k];
+    w1[q] = sw[N + k];
+    w2[q] = sw[2 * N + k];
   }
-  __syncthreads();  // sw1/sw2 below reuse the operand lines
-  uint32_t* sw1 = reinterpret_cast<uint32_t*>(lds);
-  uint32_t* sw2 = sw1 + N;
+  __syncthreads();  // block_carry's scratch reuses the LDS
   uint64_t c, spill;
-  block_carry<kEPT>(w0, w1, w2, sw1, sw2, wtot, D, c, spill);
+  block_carry<kEPT>(w0, w1, w2, sw, sw + N, wtot, D, c, spill);
 #pragma unroll
   for (int q = 0; q < kEPT; ++q) {
     const int k = tid * kEPT + q;
