@@ -101,10 +101,9 @@ void parallel_for(size_t n, F body) {
   for (std::thread& x : th) x.join();
 }
 
-// Largest subtree evaluated as one explicit forest at a time.
-constexpr uint32_t kChunkLeaves = 1u << 18;
-
-// Evaluates the subtree [a, b) bottom-up, batching every height.
+// Evaluates the subtree [a, b) bottom-up on host BigFloats, batching every
+// height's products (MulBatch); the fallback for the small subtrees whose
+// values exceed the precision window at very low digit counts.
 void bsr_subtree_batched(PQR& out, uint32_t a, uint32_t b, size_t p, bool verbose) {
   std::vector<Node> nodes;
   nodes.reserve(2 * (size_t)(b - a));
