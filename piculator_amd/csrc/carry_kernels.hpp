@@ -97,7 +97,32 @@ struct CarryArgs {
   // FFT engine: unweighted inverse-transform output and round-off maximum
   const double2* z;
   unsigned long long* maxdist;  // bits of a non-negative double
+  // Wrapped product (top != null): r holds the cyclic convolution of length
+  // N < zL - 1, so coefficient k < top_w also holds coefficient k + N; top
+  // holds the exact coefficients t = k - N + top_w - 1 of those (prime 1 at
+  // top + top_stride), which are subtracted below N and used from N up.
+  const double* top;
+  uint64_t top_stride;
+  uint64_t top_w;
 };
+
+// Residues of coefficient k (k < zL - 1) of an NTT product, both primes,
+// with the wrapped coefficients of a short cyclic convolution taken out.
+__device__ __forceinline__ void coeff_residues(const CarryArgs& a, uint64_t k, double& r0, double& r1) {
+  if (k < a.N) {
+    r0 = a.r[k];
+    r1 = a.r[a.N + k];
+    if (a.top && k < a.top_w) {
+      const uint64_t t = k + a.top_w - 1;
+      r0 -= a.top[t];
+      r1 -= a.top[a.top_stride + t];
+    }
+  } else {
+    const uint64_t t = k - a.N + a.top_w - 1;
+    r0 = a.top[t];
+    r1 = a.top[a.top_stride + t];
+  }
+}
 
 // Limb k of the FFT engine: digits 2k, 2k+1 are real parts of points 2k, 2k+1
 // (below N) or imaginary parts of points 2k-N, 2k+1-N.  The rounded digit
@@ -239,11 +264,18 @@ __global__ __launch_bounds__(kCarryThreads) void k_carry_local(CarryArgs a) {
     if ((tid & 63) == 0) atomicMax(a.maxdist, (unsigned long long)__double_as_longlong(dmax));
   } else {
     const Prime P0 = a.prm[0], P1 = a.prm[1];
+    // coefficients 0 .. zL - 2 (the top limb only receives carries); from
+    // N up they exist only in a wrapped product
+    const uint64_t kend = a.top ? a.zL - 1 : (a.zL - 1 < a.N ? a.zL - 1 : a.N);
 #pragma unroll
     for (int q = 0; q < kCarryLPT; ++q) {
       const uint64_t k = base + tid * kCarryLPT + q;
       w0[q] = w1[q] = w2[q] = 0;
-      if (k < a.zL && k < a.N) crt_words(a.r[k], a.r[a.N + k], P0, P1, a.crt, w0[q], w1[q], w2[q]);
+      if (k < kend) {
+        double r0, r1;
+        coeff_residues(a, k, r0, r1);
+        crt_words(r0, r1, P0, P1, a.crt, w0[q], w1[q], w2[q]);
+      }
     }
   }
   uint64_t c, spill;

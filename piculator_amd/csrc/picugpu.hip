@@ -203,6 +203,8 @@ struct Ctx {
   // workspace
   double* spec = nullptr;  // [2 operands][2 primes][N]
   size_t spec_points = 0;
+  double* top_spec = nullptr;  // the short product of a wrapped one, [2][2][M]
+  size_t top_points = 0;
   CarryFn* agg = nullptr;
   uint64_t* cin = nullptr;
   size_t nblk_cap = 0;
@@ -463,14 +465,14 @@ int launch_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double* 
   return PCG_OK;
 }
 
+// Transform length of a product of zL limbs: its zL - 1 coefficients.
 int ntt_log_points(uint64_t zL) {
-  const int logN = ceil_log2(zL);
+  const int logN = ceil_log2(zL > 1 ? zL - 1 : 1);
   return logN < kMinLogN ? kMinLogN : logN;
 }
 
-// Forward transform of operand a into the spectrum A = c.spec[0, 2N).
-int ntt_forward_a(Ctx& c, const Plan& pl, const uint32_t* d_a, uint64_t na) {
-  double* A = c.spec;
+// Forward transform of operand a into the spectrum A ([2][N]).
+int ntt_forward_into(Ctx& c, const Plan& pl, const uint32_t* d_a, uint64_t na, double* A) {
   for (size_t j = 0; j < pl.passes.size(); ++j) {
     const int rc = launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : A, d_a, na, A, nullptr);
     if (rc) return rc;
@@ -478,33 +480,10 @@ int ntt_forward_a(Ctx& c, const Plan& pl, const uint32_t* d_a, uint64_t na) {
   return PCG_OK;
 }
 
-// The rest of a product with the spectrum of a already in A: forward
-// transform of b (its last pass fused with the pointwise product and the
-// first inverse pass), inverse passes, CRT and carry into d_out[0, zL).
-int ntt_mul_tail(Ctx& c, const Plan& pl, const uint32_t* d_b, uint64_t nb, uint64_t zL, uint32_t* d_out,
-                 bool profile);
-
-// Enqueue the whole NTT product on the library stream (device pointers).
-int ntt_mul_enqueue(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_b, uint64_t nb,
-                    uint32_t* d_out, bool profile) {
-  const uint64_t zL = na + nb;
-  const int logN = ntt_log_points(zL);
-  int rc = ensure_workspace(c, logN, zL);
-  if (rc) return rc;
-  const Plan pl = make_plan(logN);
-  if (profile) HIPCHK(hipEventRecord(c.ev[0], c.stream));
-  rc = ntt_forward_a(c, pl, d_a, na);
-  if (rc) return rc;
-  if (profile) HIPCHK(hipEventRecord(c.ev[1], c.stream));
-  return ntt_mul_tail(c, pl, d_b, nb, zL, d_out, profile);
-}
-
-int ntt_mul_tail(Ctx& c, const Plan& pl, const uint32_t* d_b, uint64_t nb, uint64_t zL, uint32_t* d_out,
-                 bool profile) {
-  const int logN = pl.logN;
-  const uint64_t N = 1ull << logN;
-  double* A = c.spec;
-  double* B = c.spec + 2 * N;
+// With the spectrum of a in A: forward transform of b (its last pass fused
+// with the pointwise product and the first inverse pass) and the inverse
+// passes; B receives the convolution's residues ([2][N]).
+int ntt_tail_into(Ctx& c, const Plan& pl, const uint32_t* d_b, uint64_t nb, const double* A, double* B) {
   const size_t m = pl.passes.size();
   int rc;
   for (size_t j = 0; j + 1 < m; ++j) {
@@ -517,12 +496,76 @@ int ntt_mul_tail(Ctx& c, const Plan& pl, const uint32_t* d_b, uint64_t nb, uint6
     rc = launch_pass(c, kInv, pl, pl.passes[j], B, nullptr, 0, B, nullptr);
     if (rc) return rc;
   }
-  if (profile) HIPCHK(hipEventRecord(c.ev[2], c.stream));
+  return PCG_OK;
+}
+
+// A product whose coefficient count zL - 1 exceeds a power of two N1 by at
+// most N1 / 8 runs as the cyclic convolution of length N1 (the top
+// coefficients wrap onto the bottom ones) plus the short product of the top
+// w = zL - 1 - N1 limbs of each operand, which gives those top coefficients
+// exactly; the carry subtracts them below N1 and uses them from N1 up.
+// Cost: a transform of N1 and one of 2^ceil(log2(2w - 1)) points instead of
+// one of 2 N1 (the products of the largest tree levels and of the Newton
+// steps sit a few percent above a power of two).
+struct WrapPlan {
+  bool use = false;
+  int logN1 = 0, logM = 0;
+  uint64_t w = 0;
+};
+constexpr int kWrapMinLog = 20;
+
+WrapPlan wrap_plan(uint64_t na, uint64_t nb) {
+  WrapPlan wp;
+  const uint64_t zL = na + nb;
+  const int logN = ntt_log_points(zL);
+  if (logN - 1 < kWrapMinLog) return wp;
+  const uint64_t N1 = 1ull << (logN - 1);
+  const uint64_t w = zL - 1 - N1;  // >= 1: logN is the length the coefficients need
+  if (w > N1 / 8 || na > N1 || nb > N1 || na < w || nb < w) return wp;
+  wp.use = true;
+  wp.logN1 = logN - 1;
+  wp.logM = ntt_log_points(2 * w);  // 2w - 1 coefficients
+  wp.w = w;
+  return wp;
+}
+
+// Scratch of the short product ([2 operands][2 primes][M]).
+int ensure_top_scratch(Ctx& c, int logM) {
+  const size_t pts = (size_t)1 << logM;
+  if (pts <= c.top_points) return PCG_OK;
+  HIPCHK(hipStreamSynchronize(c.stream));
+  if (c.top_spec) HIPCHK(hipFree(c.top_spec));
+  c.top_spec = nullptr;
+  c.top_points = 0;
+  if (hipMalloc(&c.top_spec, pts * 4 * sizeof(double)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(PCG_ERR_OOM, "cannot allocate %zu bytes of transform scratch", pts * 32);
+  }
+  c.top_points = pts;
+  return PCG_OK;
+}
+
+// The short product of a wrapped one: the top w limbs of a and b convolved
+// into c.top_spec's second half (residues, no carry).
+int ntt_wrap_top(Ctx& c, const WrapPlan& wp, const uint32_t* d_a, uint64_t na, const uint32_t* d_b,
+                 uint64_t nb) {
+  const Plan pm = make_plan(wp.logM);
+  const uint64_t M = 1ull << wp.logM;
+  double* A2 = c.top_spec;
+  double* B2 = c.top_spec + 2 * M;
+  int rc = ntt_forward_into(c, pm, d_a + (na - wp.w), wp.w, A2);
+  if (rc) return rc;
+  return ntt_tail_into(c, pm, d_b + (nb - wp.w), wp.w, A2, B2);
+}
+
+// CRT and carries of the residues in B into d_out[0, zL); wp.use: B holds a
+// wrapped convolution and c.top_spec the top coefficients.
+int ntt_carry(Ctx& c, const Plan& pl, const double* B, uint64_t zL, uint32_t* d_out, const WrapPlan& wp) {
   CarryArgs ca;
   memset(&ca, 0, sizeof ca);
-  ca.prm = c.d_prime + logN * 2;
+  ca.prm = c.d_prime + pl.logN * 2;
   ca.r = B;
-  ca.N = N;
+  ca.N = 1ull << pl.logN;
   ca.zL = zL;
   ca.out = d_out;
   ca.agg = c.agg;
@@ -531,15 +574,69 @@ int ntt_mul_tail(Ctx& c, const Plan& pl, const uint32_t* d_b, uint64_t nb, uint6
   ca.tot = c.agg + c.nblk_cap;
   ca.crt = c.crt;
   ca.status = c.status;
-  ca.z = nullptr;
-  ca.maxdist = nullptr;
+  if (wp.use) {
+    const uint64_t M = 1ull << wp.logM;
+    ca.top = c.top_spec + 2 * M;
+    ca.top_stride = M;
+    ca.top_w = wp.w;
+  }
   hipLaunchKernelGGL(k_carry_local<false>, dim3((unsigned)ca.nblk), dim3(kCarryThreads), 0, c.stream, ca);
   launch_carry_scan(c, ca);
   hipLaunchKernelGGL(k_carry_fix, dim3((unsigned)((ca.nblk + 255) / 256)), dim3(256), 0, c.stream,
                      ca);
-  if (profile) HIPCHK(hipEventRecord(c.ev[3], c.stream));
   HIPCHK(hipGetLastError());
   return PCG_OK;
+}
+
+// Plan of a product: the full transform, or the wrapped one.
+struct MulPlan {
+  WrapPlan wp;
+  int logN;  // the main transform
+};
+MulPlan mul_plan(uint64_t na, uint64_t nb) {
+  MulPlan mp;
+  mp.wp = wrap_plan(na, nb);
+  mp.logN = mp.wp.use ? mp.wp.logN1 : ntt_log_points(na + nb);
+  return mp;
+}
+
+int ensure_mul_workspace(Ctx& c, const MulPlan& mp, uint64_t zL) {
+  int rc = ensure_workspace(c, mp.logN, zL);
+  if (!rc && mp.wp.use) rc = ensure_top_scratch(c, mp.wp.logM);
+  return rc;
+}
+
+// Forward a (into c.spec's first half) is done; the rest of the product
+// of a (na limbs) and b into d_out.
+int ntt_mul_tail(Ctx& c, const Plan& pl, const MulPlan& mp, const uint32_t* d_a, uint64_t na,
+                 const uint32_t* d_b, uint64_t nb, uint32_t* d_out, bool profile) {
+  const uint64_t N = 1ull << pl.logN;
+  if (mp.wp.use) {
+    const int rc = ntt_wrap_top(c, mp.wp, d_a, na, d_b, nb);
+    if (rc) return rc;
+  }
+  int rc = ntt_tail_into(c, pl, d_b, nb, c.spec, c.spec + 2 * N);
+  if (rc) return rc;
+  if (profile) HIPCHK(hipEventRecord(c.ev[2], c.stream));
+  rc = ntt_carry(c, pl, c.spec + 2 * N, na + nb, d_out, mp.wp);
+  if (rc) return rc;
+  if (profile) HIPCHK(hipEventRecord(c.ev[3], c.stream));
+  return PCG_OK;
+}
+
+// Enqueue the whole NTT product on the library stream (device pointers).
+int ntt_mul_enqueue(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_b, uint64_t nb,
+                    uint32_t* d_out, bool profile) {
+  const uint64_t zL = na + nb;
+  const MulPlan mp = mul_plan(na, nb);
+  int rc = ensure_mul_workspace(c, mp, zL);
+  if (rc) return rc;
+  const Plan pl = make_plan(mp.logN);
+  if (profile) HIPCHK(hipEventRecord(c.ev[0], c.stream));
+  rc = ntt_forward_into(c, pl, d_a, na, c.spec);
+  if (rc) return rc;
+  if (profile) HIPCHK(hipEventRecord(c.ev[1], c.stream));
+  return ntt_mul_tail(c, pl, mp, d_a, na, d_b, nb, d_out, profile);
 }
 
 
@@ -1855,19 +1952,24 @@ int df_mul_pair_locked(Ctx& c, pcg_dfloat* z1, const pcg_dfloat* px1, pcg_dfloat
     const int rc = check_sizes(x[i].L, sc[i].L);
     if (rc) return rc;
   }
+  MulPlan mp[2];
+  for (int i = 0; i < 2; ++i)
+    if (zL[i]) mp[i] = mul_plan(sc[i].L, x[i].L);
   const bool shared = zL[0] && zL[1] && eng != PCG_ENGINE_FFT && sc[0].L == sc[1].L &&
-                      ntt_log_points(zL[0]) == ntt_log_points(zL[1]);
+                      mp[0].logN == mp[1].logN;
   uint32_t* buf[2] = {nullptr, nullptr};
   uint64_t cap[2] = {0, 0};
   int rc = PCG_OK;
   for (int i = 0; i < 2 && !rc; ++i)
     if (zL[i]) rc = df_alloc(c, zL[i], &buf[i], &cap[i]);
   if (!rc && shared) {
-    const int logN = ntt_log_points(zL[0]);
-    rc = ensure_workspace(c, logN, std::max(zL[0], zL[1]));
-    const Plan pl = make_plan(logN);
-    if (!rc) rc = ntt_forward_a(c, pl, sc[0].t, sc[0].L);
-    for (int i = 0; i < 2 && !rc; ++i) rc = ntt_mul_tail(c, pl, x[i].t, x[i].L, zL[i], buf[i], false);
+    // one forward transform of s (the main length of both products), then
+    // each product's short top product if it wraps, its tail and carry
+    for (int i = 0; i < 2 && !rc; ++i) rc = ensure_mul_workspace(c, mp[i], zL[i]);
+    const Plan pl = make_plan(mp[0].logN);
+    if (!rc) rc = ntt_forward_into(c, pl, sc[0].t, sc[0].L, c.spec);
+    for (int i = 0; i < 2 && !rc; ++i)
+      rc = ntt_mul_tail(c, pl, mp[i], sc[i].t, sc[i].L, x[i].t, x[i].L, buf[i], false);
   } else {
     for (int i = 0; i < 2 && !rc; ++i)
       if (zL[i]) rc = mul_dev_locked(c, x[i].t, x[i].L, sc[i].t, sc[i].L, buf[i], eng);
@@ -1973,6 +2075,7 @@ void pcg_shutdown(void) {
   (void)hipFree(c.d_maxdist);
   (void)hipFree(c.d_prime);
   (void)hipFree(c.spec);
+  (void)hipFree(c.top_spec);
   (void)hipFree(c.agg);
   (void)hipFree(c.cin);
   (void)hipFree(c.status);
@@ -2000,6 +2103,8 @@ void pcg_shutdown(void) {
   c.stream = nullptr;
   c.spec = nullptr;
   c.spec_points = 0;
+  c.top_spec = nullptr;
+  c.top_points = 0;
   c.agg = nullptr;
   c.cin = nullptr;
   c.nblk_cap = 0;

@@ -222,3 +222,26 @@ def test_mul_pair_rejects_one_result_twice(gpu):
         assert gpu.df_to_tuple(x) == (True, 0, [1, 2, 3])
     finally:
         gpu.load().pcg_df_free(r(x))
+
+
+def test_mul_pair_wrapped_products(gpu):
+    """A pair whose products wrap (zL - 1 just above 2^20): one forward
+    transform of s of 2^20 points, each product's short top product; exact
+    (p = 0) results checked by residues and by their lowest limbs."""
+    from tests.helpers import residue, CHECK_PRIMES, from_int, to_int
+
+    rng = np.random.default_rng(77)
+    n = (1 << 19) + 5000
+    s = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    x1 = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    x2 = rng.integers(0, 2**32, n - 3000, dtype=np.uint64).astype(np.uint32)
+    for v in (s, x1, x2):
+        v[-1] |= 1
+    got1, got2 = gpu.df_mul_pair((True, 0, x1), (True, 0, x2), (True, 0, s), 0)
+    for x, got in ((x1, got1), (x2, got2)):
+        z = np.asarray(got[2], dtype=np.uint32)
+        assert z.size in (x.size + s.size, x.size + s.size - 1)
+        for q in CHECK_PRIMES:
+            assert residue(z, q) == residue(x, q) * residue(s, q) % q
+        k = 12000
+        assert np.array_equal(z[:k], from_int((to_int(x[:k]) * to_int(s[:k])) % (1 << (32 * k)), k))

@@ -83,3 +83,45 @@ def test_workload_large_residues(gpu, logn, unbalanced):
     assert got.size == a.size + b.size
     assert int(got[0]) == (int(a[0]) * int(b[0])) & 0xFFFFFFFF
     assert residues_match(a, b, got)
+
+
+def _low_limbs(a, b, k):
+    """The lowest k limbs of a*b (they depend only on the lowest k limbs)."""
+    from tests.helpers import from_int, to_int
+
+    m = (1 << (32 * k)) - 1
+    return from_int((to_int(a[:k]) * to_int(b[:k])) & m, k)
+
+
+# Products whose zL - 1 coefficients exceed 2^20 by at most 2^17 run as a
+# cyclic convolution of 2^20 points plus the short product of the operands'
+# top limbs (the wrapped coefficients); the boundary cases are one wrapped
+# coefficient, none (zL - 1 = 2^20 fits), and the largest wrap.
+@pytest.mark.parametrize("na,nb", [((1 << 19) + 4096, (1 << 19) + 4096),
+                                   ((1 << 20) - 5, 70000),
+                                   ((1 << 19) + 1, (1 << 19) + 1),
+                                   (1 << 19, (1 << 19) + 1),
+                                   ((1 << 19) + 65536, (1 << 19) + 65536)])
+def test_wrapped_products(gpu, na, nb):
+    rng = np.random.default_rng(na ^ (nb << 3))
+    a = rng.integers(0, 2**32, na, dtype=np.uint64).astype(np.uint32)
+    b = rng.integers(0, 2**32, nb, dtype=np.uint64).astype(np.uint32)
+    a[-1] |= 1
+    b[-1] |= 1
+    got = gpu.mul(a, b, gpu.ENGINE_NTT)
+    assert got.size == na + nb
+    assert residues_match(a, b, got)
+    k = min(na, nb, 140000)  # covers every wrapped (corrected) coefficient
+    assert np.array_equal(got[:k], _low_limbs(a, b, k))
+
+
+@pytest.mark.parametrize("n", [(1 << 19) + 3000])
+def test_wrapped_carry_chains(gpu, n):
+    a = np.full(n, M32, dtype=np.uint32)
+    got = gpu.mul(a, a, gpu.ENGINE_NTT)
+    # (2^(32n) - 1)^2 = 2^(64n) - 2^(32n+1) + 1: limbs 1, 0..0, 0xFFFFFFFE, F..F
+    want = np.zeros(2 * n, dtype=np.uint32)
+    want[0] = 1
+    want[n] = 0xFFFFFFFE
+    want[n + 1:] = M32
+    assert np.array_equal(got, want)
