@@ -500,13 +500,13 @@ int ntt_tail_into(Ctx& c, const Plan& pl, const uint32_t* d_b, uint64_t nb, cons
 }
 
 // A product whose coefficient count zL - 1 exceeds a power of two N1 by at
-// most N1 / 8 runs as the cyclic convolution of length N1 (the top
+// most N1 / 4 runs as the cyclic convolution of length N1 (the top
 // coefficients wrap onto the bottom ones) plus the short product of the top
 // w = zL - 1 - N1 limbs of each operand, which gives those top coefficients
 // exactly; the carry subtracts them below N1 and uses them from N1 up.
-// Cost: a transform of N1 and one of 2^ceil(log2(2w - 1)) points instead of
-// one of 2 N1 (the products of the largest tree levels and of the Newton
-// steps sit a few percent above a power of two).
+// Cost: a transform of N1 and one of 2^ceil(log2(2w - 1)) <= N1 / 2 points
+// instead of one of 2 N1 (the products of the largest tree levels and of
+// the Newton steps sit a few percent above a power of two).
 struct WrapPlan {
   bool use = false;
   int logN1 = 0, logM = 0;
@@ -521,7 +521,7 @@ WrapPlan wrap_plan(uint64_t na, uint64_t nb) {
   if (logN - 1 < kWrapMinLog) return wp;
   const uint64_t N1 = 1ull << (logN - 1);
   const uint64_t w = zL - 1 - N1;  // >= 1: logN is the length the coefficients need
-  if (w > N1 / 8 || na > N1 || nb > N1 || na < w || nb < w) return wp;
+  if (w > N1 / 4 || na > N1 || nb > N1 || na < w || nb < w) return wp;
   wp.use = true;
   wp.logN1 = logN - 1;
   wp.logM = ntt_log_points(2 * w);  // 2w - 1 coefficients

@@ -93,7 +93,7 @@ def _low_limbs(a, b, k):
     return from_int((to_int(a[:k]) * to_int(b[:k])) & m, k)
 
 
-# Products whose zL - 1 coefficients exceed 2^20 by at most 2^17 run as a
+# Products whose zL - 1 coefficients exceed 2^20 by at most 2^18 run as a
 # cyclic convolution of 2^20 points plus the short product of the operands'
 # top limbs (the wrapped coefficients); the boundary cases are one wrapped
 # coefficient, none (zL - 1 = 2^20 fits), and the largest wrap.
@@ -101,7 +101,8 @@ def _low_limbs(a, b, k):
                                    ((1 << 20) - 5, 70000),
                                    ((1 << 19) + 1, (1 << 19) + 1),
                                    (1 << 19, (1 << 19) + 1),
-                                   ((1 << 19) + 65536, (1 << 19) + 65536)])
+                                   ((1 << 19) + 65536, (1 << 19) + 65536),
+                                   ((1 << 19) + 131072, (1 << 19) + 131072)])
 def test_wrapped_products(gpu, na, nb):
     rng = np.random.default_rng(na ^ (nb << 3))
     a = rng.integers(0, 2**32, na, dtype=np.uint64).astype(np.uint32)
@@ -111,7 +112,7 @@ def test_wrapped_products(gpu, na, nb):
     got = gpu.mul(a, b, gpu.ENGINE_NTT)
     assert got.size == na + nb
     assert residues_match(a, b, got)
-    k = min(na, nb, 140000)  # covers every wrapped (corrected) coefficient
+    k = min(na, nb, 270000)  # covers every wrapped (corrected) coefficient
     assert np.array_equal(got[:k], _low_limbs(a, b, k))
 
 
