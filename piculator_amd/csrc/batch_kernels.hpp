@@ -305,10 +305,13 @@ __global__ __launch_bounds__((1 << LOGN) / kEPT) __attribute__((amdgpu_waves_per
 // k_carry_local); k_batch_carry_scan scans each product's chunk functions
 // with one wave; k_batch_carry_fix adds the carry-ins.  r: inverse-transform
 // outputs of product z at r + z * zstride, prime q at offset q * N.
-// agg / cin: nch entries per product.
+// agg / cin: nch entries per product.  Wrapped products (top != null): a
+// product with w = zL - 1 - N > 0 holds the cyclic convolution of length N,
+// and its top coefficients t = k - N + w - 1 at top + z * zstride (prime 1
+// at + M) are subtracted below w and used from N up (as coeff_residues).
 __global__ __launch_bounds__(kCarryThreads) void k_batch_carry_local(
     const BatchDesc* descs, const double* r, uint64_t zstride, uint64_t N, const Prime* prm,
-    double2 crt, CarryFn* agg, uint32_t nch) {
+    double2 crt, CarryFn* agg, uint32_t nch, const double* top, uint64_t M) {
   __shared__ uint32_t sw1[kCarryBlock], sw2[kCarryBlock];
   __shared__ CarryFn wtot[kCarryThreads / 64];
   __shared__ uint32_t d01[2];
@@ -318,6 +321,9 @@ __global__ __launch_bounds__(kCarryThreads) void k_batch_carry_local(
   const uint64_t base = (uint64_t)blockIdx.x * kCarryBlock;
   if (base >= zL) return;  // whole block: the chunk lies past this product
   const double* rz = r + (uint64_t)z * zstride;
+  const double* tz = top ? top + (uint64_t)z * zstride : nullptr;
+  const uint64_t w = (top && zL > N + 1) ? zL - 1 - N : 0;  // wrapped coefficients
+  const uint64_t kend = w ? zL - 1 : (zL < N ? zL : N);
   const Prime P0 = prm[0], P1 = prm[1];
   const int tid = threadIdx.x;
   uint32_t w0[kCarryLPT], w1[kCarryLPT], w2[kCarryLPT], D[kCarryLPT];
@@ -325,7 +331,21 @@ __global__ __launch_bounds__(kCarryThreads) void k_batch_carry_local(
   for (int q = 0; q < kCarryLPT; ++q) {
     const uint64_t k = base + tid * kCarryLPT + q;
     w0[q] = w1[q] = w2[q] = 0;
-    if (k < zL) crt_words(rz[k], rz[N + k], P0, P1, crt, w0[q], w1[q], w2[q]);
+    if (k < kend) {
+      double r0, r1;
+      if (k < N) {
+        r0 = rz[k];
+        r1 = rz[N + k];
+        if (k < w) {
+          r0 -= tz[k + w - 1];
+          r1 -= tz[M + k + w - 1];
+        }
+      } else {
+        r0 = tz[k - N + w - 1];
+        r1 = tz[M + k - N + w - 1];
+      }
+      crt_words(r0, r1, P0, P1, crt, w0[q], w1[q], w2[q]);
+    }
   }
   uint64_t c, spill;
   block_carry<kCarryLPT>(w0, w1, w2, sw1, sw2, wtot, D, c, spill);
@@ -349,6 +369,30 @@ __global__ __launch_bounds__(kCarryThreads) void k_batch_carry_local(
     }
     agg[(uint64_t)z * nch + blockIdx.x] = g;
   }
+}
+
+// Top descriptors of wrapped products: the w = zL - 1 - N highest limbs of
+// each operand (an empty product when the coefficients fit N).
+__global__ void k_wrap_descs(const BatchDesc* descs, uint64_t count, uint64_t N, BatchDesc* tops) {
+  const uint64_t z = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= count) return;
+  const BatchDesc d = descs[z];
+  const uint64_t zL = (uint64_t)d.na + d.nb;
+  BatchDesc t;
+  t.out = nullptr;
+  if (zL > N + 1) {
+    const uint32_t w = (uint32_t)(zL - 1 - N);
+    t.a = d.a + (d.na - w);
+    t.b = d.b + (d.nb - w);
+    t.na = w;
+    t.nb = w;
+  } else {
+    t.a = d.a;
+    t.b = d.b;
+    t.na = 0;
+    t.nb = 0;
+  }
+  tops[z] = t;
 }
 
 // One wave per product: exclusive scan of its chunk functions, 64 at a time.

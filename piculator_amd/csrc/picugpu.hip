@@ -222,6 +222,8 @@ struct Ctx {
   // scratch of batched multi-pass products
   double* bspec = nullptr;
   size_t bspec_doubles = 0;
+  BatchDesc* wrap_desc = nullptr;  // top descriptors of wrapped batched products
+  size_t wrap_desc_cap = 0;
   CarryFn* bagg = nullptr;  // chunk carry functions of batched products
   uint64_t* bcin = nullptr;
   size_t bcarry_cap = 0;
@@ -850,6 +852,25 @@ int bsr_class(const std::array<int, kSmallNttMaxLog + 1>& tab, uint64_t zL) {
   return lg < kMinLogN ? kMinLogN : lg;
 }
 
+// Launch key of a subtree product whose operands have at most ta and tb
+// limbs: its class (bsr_class) times 32 plus, for a multi-pass product whose
+// ta + tb - 1 coefficients exceed a power of two N1 >= 2^13 by at most N1 / 4,
+// log2 of the short top product's transform (the product then runs wrapped
+// at N1 points, as ntt_mul_enqueue's wrapped products).
+constexpr int kWrapKey = 32;
+constexpr int kBatchWrapMinLog = 13;
+int bsr_key(const std::array<int, kSmallNttMaxLog + 1>& tab, uint64_t ta, uint64_t tb) {
+  const int cls = bsr_class(tab, ta + tb);
+  if (cls <= kSmallNttMaxLog || cls >= kPassClass) return cls * kWrapKey;
+  const uint64_t C = ta + tb - 1;  // coefficients
+  const int lg = ceil_log2(C);
+  if (lg < cls && lg > kSmallNttMaxLog) return lg * kWrapKey;  // coefficients fit one size down
+  if (lg - 1 < kBatchWrapMinLog) return cls * kWrapKey;
+  const uint64_t N1 = 1ull << (lg - 1), wb = C - N1;
+  if (wb > N1 / 4 || ta > N1 || tb > N1) return cls * kWrapKey;
+  return (lg - 1) * kWrapKey + ntt_log_points(2 * wb);
+}
+
 // Classes that run on the batched pass kernels (a transform spectrum in HBM
 // that two products can share).
 bool class_pairs(int cls) { return cls >= kPassClass || cls > kSmallNttMaxLog; }
@@ -858,15 +879,19 @@ bool class_pairs(int cls) { return cls >= kPassClass || cls > kSmallNttMaxLog; }
 // kernels carry a product grid dimension; scratch is 4N doubles per product.
 // pairs: products 2k and 2k+1 have the same operand a, whose forward
 // transform runs once (count is even).
-int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, bool pairs = false) {
+int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, bool pairs = false,
+                      int logM = 0) {
   const uint64_t N = 1ull << logN;
-  const uint64_t per = 4 * N;
+  const uint64_t M = logM ? 1ull << logM : 0;
+  const uint64_t per = 4 * N + 4 * M;  // main spectra, then the top product's
   if (pairs && (count & 1)) return fail(PCG_ERR_INTERNAL, "odd count of paired products");
   size_t group_max = std::max<size_t>(1, ((size_t)1 << 28) / per);  // 2 GiB of scratch
   if (pairs) group_max = std::max<size_t>(2, group_max & ~(size_t)1);
   const size_t want = std::min(count, group_max) * per;
-  // chunk carry functions: (products per group) x (chunks per product)
-  const size_t carry_need = std::min(count, group_max) * (size_t)(N / kCarryBlock);
+  // chunk carry functions: (products per group) x (chunks per product); a
+  // wrapped product has up to N + M / 2 + 1 limbs
+  const uint32_t nch = (uint32_t)((N + M / 2 + 1 + kCarryBlock - 1) / kCarryBlock);
+  const size_t carry_need = std::min(count, group_max) * (size_t)nch;
   if (carry_need > c.bcarry_cap) {
     HIPCHK(hipStreamSynchronize(c.stream));
     if (c.bagg) HIPCHK(hipFree(c.bagg));
@@ -889,6 +914,15 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, b
       return fail(PCG_ERR_OOM, "cannot allocate batched transform scratch");
     }
     c.bspec_doubles = want;
+  }
+  if (logM && std::min(count, group_max) > c.wrap_desc_cap) {
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (c.wrap_desc) HIPCHK(hipFree(c.wrap_desc));
+    c.wrap_desc = nullptr;
+    c.wrap_desc_cap = 0;
+    const size_t cap = std::max<size_t>(std::min(count, group_max), 1024);
+    HIPCHK(hipMalloc(&c.wrap_desc, cap * sizeof(BatchDesc)));
+    c.wrap_desc_cap = cap;
   }
   const Plan pl = make_plan(logN);
   const size_t m = pl.passes.size();
@@ -916,10 +950,37 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, b
       rc = launch_pass(c, kInv, pl, pl.passes[j], B, nullptr, 0, B, nullptr, dd, 1, per, k);
       if (rc) return rc;
     }
-    const uint32_t nch = (uint32_t)(N / kCarryBlock);
+    const double* top = nullptr;
+    if (logM) {
+      // the short products of the wrapped ones: top limbs of each operand
+      hipLaunchKernelGGL(k_wrap_descs, dim3((k + 255) / 256), dim3(256), 0, c.stream, dd, (uint64_t)k, N,
+                         c.wrap_desc);
+      const Plan pm = make_plan(logM);
+      const size_t mm = pm.passes.size();
+      double* A2 = c.bspec + 4 * N;
+      double* B2 = A2 + 2 * M;
+      for (size_t j = 0; j < mm; ++j) {
+        rc = launch_pass(c, kFwd, pm, pm.passes[j], j == 0 ? nullptr : A2, nullptr, 0, A2, nullptr,
+                         c.wrap_desc, 0, per, k);
+        if (rc) return rc;
+      }
+      for (size_t j = 0; j + 1 < mm; ++j) {
+        rc = launch_pass(c, kFwd, pm, pm.passes[j], j == 0 ? nullptr : B2, nullptr, 0, B2, nullptr,
+                         c.wrap_desc, 1, per, k);
+        if (rc) return rc;
+      }
+      rc = launch_pass(c, kFused, pm, pm.passes[mm - 1], mm == 1 ? nullptr : B2, nullptr, 0, B2, A2,
+                       c.wrap_desc, 1, per, k);
+      if (rc) return rc;
+      for (size_t j = mm - 1; j-- > 0;) {
+        rc = launch_pass(c, kInv, pm, pm.passes[j], B2, nullptr, 0, B2, nullptr, c.wrap_desc, 1, per, k);
+        if (rc) return rc;
+      }
+      top = B2;
+    }
     if ((size_t)k * nch > c.bcarry_cap) return fail(PCG_ERR_INTERNAL, "batched carry blocks exceed the scratch");
     hipLaunchKernelGGL(k_batch_carry_local, dim3(nch, k), dim3(kCarryThreads), 0, c.stream, dd, B, per,
-                       N, c.d_prime + logN * 2, c.crt, c.bagg, nch);
+                       N, c.d_prime + logN * 2, c.crt, c.bagg, nch, top, M);
     hipLaunchKernelGGL(k_batch_carry_scan, dim3(k), dim3(64), 0, c.stream, dd, c.bagg, c.bcin, nch);
     hipLaunchKernelGGL(k_batch_carry_fix, dim3((unsigned)(((uint64_t)k * nch + 255) / 256)), dim3(256), 0,
                        c.stream, dd, c.bcin, nch, (uint64_t)k);
@@ -1296,6 +1357,7 @@ struct BsrHeight {
     int cls;
     size_t n;    // products
     bool pairs;  // products 2k, 2k+1 share operand a (one forward transform)
+    int wrap;    // > 0: wrapped at 2^cls points, top products of 2^wrap
   };
   std::vector<Launch> classes;  // in launch order
 };
@@ -1399,9 +1461,9 @@ void bsr_host_plan(uint32_t d, const SmallTable& tab, BsrHostPlan& HP) {
       const BsrMul m4[4] = {{Rt + 1, Lf + 0, I + 3}, {Rt + 1, Lf + 1, I + 1},
                             {Lf + 2, Rt + 0, I + 4}, {Lf + 2, Rt + 2, I + 2}};
       for (int q = 0; q < 4; q += 2) {
-        const int c0 = bsr_class(tab, tlen[m4[q].a] + tlen[m4[q].b]);
-        const int c1 = bsr_class(tab, tlen[m4[q + 1].a] + tlen[m4[q + 1].b]);
-        if (c0 == c1 && class_pairs(c0)) {
+        const int c0 = bsr_key(tab, tlen[m4[q].a], tlen[m4[q].b]);
+        const int c1 = bsr_key(tab, tlen[m4[q + 1].a], tlen[m4[q + 1].b]);
+        if (c0 == c1 && class_pairs(c0 / kWrapKey)) {
           mp.push_back({c0, BsrMulPair{m4[q], m4[q + 1]}});
         } else {
           ms.push_back({c0, m4[q]});
@@ -1436,14 +1498,14 @@ void bsr_host_plan(uint32_t d, const SmallTable& tab, BsrHostPlan& HP) {
           muls.push_back(e.second.m[1]);
           cnt += 2;
         }
-      if (cnt) hh.classes.push_back({kind, cnt, true});
+      if (cnt) hh.classes.push_back({kind / kWrapKey, cnt, true, kind % kWrapKey});
       cnt = 0;
       for (const auto& e : ms)
         if (e.first == kind) {
           muls.push_back(e.second);
           ++cnt;
         }
-      if (cnt) hh.classes.push_back({kind, cnt, false});
+      if (cnt) hh.classes.push_back({kind / kWrapKey, cnt, false, kind % kWrapKey});
     }
     hh.add0 = adds.size();
     hh.nsmall = small.size();
@@ -1567,11 +1629,12 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
     for (const auto& cl : hh.classes) {
       int rc;
       if (cl.pairs)
-        rc = batched_multipass(c, hd + o, cl.n, cl.cls >= kPassClass ? cl.cls - kPassClass : cl.cls, true);
+        rc = batched_multipass(c, hd + o, cl.n, cl.cls >= kPassClass ? cl.cls - kPassClass : cl.cls, true,
+                               cl.wrap);
       else if (cl.cls <= kSmallNttMaxLog || cl.cls >= kPassClass)
         rc = launch_small_class(c, cl.cls, hd + o, cl.n);
       else
-        rc = batched_multipass(c, hd + o, cl.n, cl.cls);
+        rc = batched_multipass(c, hd + o, cl.n, cl.cls, false, cl.wrap);
       if (rc) return rc;
       o += cl.n;
     }
@@ -2085,6 +2148,7 @@ void pcg_shutdown(void) {
   (void)hipHostFree(c.h_desc);
   (void)hipFree(c.d_desc);
   (void)hipFree(c.bspec);
+  (void)hipFree(c.wrap_desc);
   (void)hipFree(c.bagg);
   (void)hipFree(c.bcin);
   g_prep.clear();  // waits for background plan builds
@@ -2119,6 +2183,8 @@ void pcg_shutdown(void) {
   c.desc_cap = 0;
   c.bspec = nullptr;
   c.bspec_doubles = 0;
+  c.wrap_desc = nullptr;
+  c.wrap_desc_cap = 0;
   c.bagg = nullptr;
   c.bcin = nullptr;
   c.bcarry_cap = 0;
