@@ -854,11 +854,11 @@ int bsr_class(const std::array<int, kSmallNttMaxLog + 1>& tab, uint64_t zL) {
 
 // Launch key of a subtree product whose operands have at most ta and tb
 // limbs: its class (bsr_class) times 32 plus, for a multi-pass product whose
-// ta + tb - 1 coefficients exceed a power of two N1 >= 2^13 by at most N1 / 4,
+// ta + tb - 1 coefficients exceed a power of two N1 >= 2^12 by at most N1 / 4,
 // log2 of the short top product's transform (the product then runs wrapped
 // at N1 points, as ntt_mul_enqueue's wrapped products).
 constexpr int kWrapKey = 32;
-constexpr int kBatchWrapMinLog = 13;
+constexpr int kBatchWrapMinLog = 12;
 int bsr_key(const std::array<int, kSmallNttMaxLog + 1>& tab, uint64_t ta, uint64_t tb) {
   const int cls = bsr_class(tab, ta + tb);
   if (cls <= kSmallNttMaxLog || cls >= kPassClass) return cls * kWrapKey;
@@ -1628,9 +1628,9 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
     size_t o = 0;
     for (const auto& cl : hh.classes) {
       int rc;
-      if (cl.pairs)
-        rc = batched_multipass(c, hd + o, cl.n, cl.cls >= kPassClass ? cl.cls - kPassClass : cl.cls, true,
-                               cl.wrap);
+      if (cl.pairs || cl.wrap)
+        rc = batched_multipass(c, hd + o, cl.n, cl.cls >= kPassClass ? cl.cls - kPassClass : cl.cls,
+                               cl.pairs, cl.wrap);
       else if (cl.cls <= kSmallNttMaxLog || cl.cls >= kPassClass)
         rc = launch_small_class(c, cl.cls, hd + o, cl.n);
       else
