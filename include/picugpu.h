@@ -144,13 +144,21 @@ typedef struct pcg_bsr_value {
 int pcg_bsr_exact(uint32_t a, uint32_t b, uint64_t max_limbs, pcg_bsr_value* P, pcg_bsr_value* Q,
                   pcg_bsr_value* R);
 int pcg_bsr_bound(uint32_t a, uint32_t b, uint64_t* bound);
-/* Starts building, on host threads, the evaluation plans of subtrees of the
- * given sizes (b - a) and returns at once; a later pcg_bsr_exact or
- * pcg_df_bsr_exact of such a size waits for its plan instead of building it.
- * The reference has no plans (Pi_BSR recurses, PI.h:64-112); a caller about
- * to evaluate many subtrees of a few sizes calls this first so the host-side
- * planning overlaps other work. */
+/* Builds the evaluation plans of subtrees of the given sizes (b - a; the
+ * first two distinct ones, as many as the plan cache holds): arena offsets,
+ * product and add lists, laid out on the GPU from a table of the subtree
+ * sizes.  pcg_bsr_exact and pcg_df_bsr_exact build a missing plan
+ * themselves; calling this first moves the plan allocation ahead of the
+ * first evaluation.  The reference has no plans (Pi_BSR recurses,
+ * PI.h:64-112). */
 int pcg_bsr_prepare(const uint32_t* sizes, uint64_t n);
+/* Self-check, host only (no GPU call, no pcg_init needed): lays out the plan
+ * of a subtree of `leaves` leaves as the GPU does (the same per-node code,
+ * run on the CPU) with the small-product class table `table` (n entries, as
+ * pcg_batch_table) and compares every entry with a direct node-by-node
+ * construction.  PCG_OK when they agree, PCG_ERR_INTERNAL naming the first
+ * difference otherwise.  No counterpart in the reference. */
+int pcg_bsr_plan_check(uint32_t leaves, const int* table, int n);
 
 /* ---- Device-resident BigFloat values ----
  * A BigFloat (BigFloat.h:33-36) whose limbs stay in device memory, so the

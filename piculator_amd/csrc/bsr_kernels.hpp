@@ -500,4 +500,137 @@ __global__ void k_bsr_add_final(const BsrAdd* adds, BsrVal* meta, uint32_t* aren
   mo.sign = L ? pl.sign : 1;
 }
 
+// ------------------------------------------------------------------ layout
+// A subtree of d leaves splits at m = (a + b) / 2 (PI.h:90), so a node of s
+// leaves has children of floor(s / 2) and s - floor(s / 2) leaves, and
+// everything the plan records about a node (value bounds, arena slots,
+// height, the size classes of its products) depends on s alone.  The sizes
+// that occur are few (at most two per depth), so the plan is a small table
+// of them, and each node's entries are computed from its preorder id t by
+// walking down from the root:
+//   - node ids are preorder: the left child of t is t + 1, the right one
+//     t + 2 floor(s / 2); the arena holds the nodes in id order, each node's
+//     five values (P Q R x y) in a row, so a node's offset is the arena of
+//     everything before it in preorder;
+//   - the products and adds of a height are grouped by class, each group in
+//     id order; the nodes of one height are disjoint subtrees, so the nodes
+//     before t in a group are those of the left siblings met on the walk,
+//     counted per size with the table's subtree counts.
+constexpr int kBsrPeers = 4;  // most distinct sizes of one height
+
+struct BsrSizeInfo {
+  uint64_t s;         // leaves
+  uint64_t own;       // arena limbs of the node's five values (slot + 1 each)
+  uint64_t arena;     // arena limbs of the whole subtree
+  uint64_t pre[5];    // offset of value j from the node's first limb
+  int32_t l, r;       // size index of the children, -1 for a leaf
+  int32_t npeer;      // sizes of the same height (this one included)
+  int32_t peer[kBsrPeers];
+  uint64_t mbase[4];  // first index (in the plan's product list) of product j's group
+  uint32_t mloc[4];   // products of the same node ahead of product j in its group
+  uint32_t mmul[4][kBsrPeers];  // products a node of size peer[p] puts in product j's group
+  uint64_t abase;     // first index of the node's add group (small or large)
+  uint32_t amul[kBsrPeers];     // 1 if size peer[p] is in the same add group
+};
+
+struct BsrLayout {
+  const BsrSizeInfo* info;
+  const uint32_t* cnt;  // cnt[i * ns + q]: nodes of size q in a subtree of size i
+  int32_t ns, root;
+  uint64_t nodes, nmeta, nleaf, nmul, nadd;
+  BsrVal* meta;
+  uint32_t* leaf_rel;
+  uint32_t* leaf_ids;
+  BsrMul* muls;
+  BsrAdd* adds;
+  uint32_t* err;  // k_bsr_layout sets it when bsr_layout_node fails
+};
+
+// Writes every plan entry of node t (0 <= t < nodes); false if an index
+// fell outside its list (nothing is written there).  Host and device.
+__host__ __device__ inline bool bsr_layout_node(const BsrLayout& L, uint64_t t) {
+  const BsrSizeInfo* I = L.info;
+  // walk 1: the node's size, range start and arena offset
+  int32_t si = L.root;
+  uint64_t id = 0, a = 0, base = 0;
+  while (id != t) {
+    const BsrSizeInfo& n = I[si];
+    const uint64_t sl = I[n.l].s;
+    if (t < id + 2 * sl) {
+      id += 1;
+      base += n.own;
+      si = n.l;
+    } else {
+      id += 2 * sl;
+      base += n.own + I[n.l].arena;
+      a += sl;
+      si = n.r;
+    }
+  }
+  const BsrSizeInfo& me = I[si];
+  bool bad = false;
+  for (int j = 0; j < 5; ++j) {
+    const uint64_t v = 5 * t + j;
+    if (v < L.nmeta) {
+      L.meta[v].off = base + me.pre[j];
+      L.meta[v].L = 0;
+      L.meta[v].sign = 1;
+    } else {
+      bad = true;
+    }
+  }
+  if (me.l < 0) {  // leaf [a, a + 1)
+    if (a < L.nleaf) {
+      L.leaf_rel[a] = (uint32_t)(a + 1);
+      for (int j = 0; j < 3; ++j) L.leaf_ids[3 * a + j] = (uint32_t)(5 * t + j);
+    } else {
+      bad = true;
+    }
+  } else {
+    // walk 2: nodes of each size of this height ahead of t in preorder
+    uint64_t before[kBsrPeers] = {0, 0, 0, 0};
+    int32_t sj = L.root;
+    uint64_t jd = 0;
+    while (jd != t) {
+      const BsrSizeInfo& n = I[sj];
+      const uint64_t sl = I[n.l].s;
+      if (t < jd + 2 * sl) {
+        jd += 1;
+        sj = n.l;
+      } else {
+        for (int p = 0; p < me.npeer; ++p) before[p] += L.cnt[(uint64_t)n.l * L.ns + me.peer[p]];
+        jd += 2 * sl;
+        sj = n.r;
+      }
+    }
+    const uint32_t v = (uint32_t)(5 * t), lf = (uint32_t)(5 * (t + 1)),
+                   rt = (uint32_t)(5 * (t + 2 * I[me.l].s));
+    // x = P0 Q1, Q = Q0 Q1, y = P1 R0, R = R0 R1 (PI.h:110-112), as two
+    // pairs around the operands used twice
+    const BsrMul m4[4] = {{rt + 1, lf + 0, v + 3}, {rt + 1, lf + 1, v + 1},
+                          {lf + 2, rt + 0, v + 4}, {lf + 2, rt + 2, v + 2}};
+    for (int j = 0; j < 4; ++j) {
+      uint64_t k = me.mbase[j] + me.mloc[j];
+      for (int p = 0; p < me.npeer; ++p) k += before[p] * me.mmul[j][p];
+      if (k < L.nmul)
+        L.muls[k] = m4[j];
+      else
+        bad = true;
+    }
+    uint64_t k = me.abase;
+    for (int p = 0; p < me.npeer; ++p) k += before[p] * me.amul[p];
+    if (k < L.nadd)
+      L.adds[k] = BsrAdd{v + 3, v + 4, v + 0};  // P = x.add(y)
+    else
+      bad = true;
+  }
+  return !bad;
+}
+
+__global__ void k_bsr_layout(BsrLayout L) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < L.nodes; t += stride)
+    if (!bsr_layout_node(L, t)) atomicOr(L.err, 1u);
+}
+
 }  // namespace pcg

@@ -16,7 +16,6 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
-#include <future>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -1389,18 +1388,19 @@ void bsr_plan_free(BsrPlan& P) {
   P = BsrPlan();
 }
 
-// Host side of a subtree plan (CPU only, no GPU call): the tree of d leaves
-// (PI.h:90 midpoints), value bounds, arena offsets, and per height the
-// products (grouped by class with the given small-class table, paired
-// around shared operands) and the adds.
+// A subtree plan built node by node on the host (CPU only): the tree of d
+// leaves (PI.h:90 midpoints), value bounds, arena offsets, and per height
+// the products (grouped by class with the given small-class table, paired
+// around shared operands) and the adds.  Plans are laid out on the device
+// (bsr_plan_build); this direct construction is what pcg_bsr_plan_check
+// compares that layout with.
 struct BsrHostPlan {
-  BsrPlan P;  // sizes and heights; no device memory yet
+  BsrPlan P;  // sizes and heights; no device memory
   std::vector<BsrVal> meta;
   std::vector<uint32_t> leaf_rel, leaf_ids;
   std::vector<BsrMul> muls;
   std::vector<BsrAdd> adds;
   size_t max_large = 0, max_large_chunks = 0;
-  uint64_t table_gen = 0;  // generation of the class table it was planned with
 };
 
 using SmallTable = std::array<int, kSmallNttMaxLog + 1>;
@@ -1518,56 +1518,308 @@ void bsr_host_plan(uint32_t d, const SmallTable& tab, BsrHostPlan& HP) {
   }
 }
 
-// Device side: allocates the plan's buffers and arena and uploads it.
-int bsr_plan_upload(Ctx& c, const BsrHostPlan& HP, BsrPlan& P) {
-  P = HP.P;
-  const size_t nv = HP.meta.size();
-  const std::vector<BsrVal>& meta = HP.meta;
-  const std::vector<uint32_t>& leaf_rel = HP.leaf_rel;
-  const std::vector<uint32_t>& leaf_ids = HP.leaf_ids;
-  const std::vector<BsrMul>& muls = HP.muls;
-  const std::vector<BsrAdd>& adds = HP.adds;
-  const size_t max_large = HP.max_large, max_large_chunks = HP.max_large_chunks;
-  const size_t need = nv * sizeof(BsrVal) + leaf_rel.size() * 4 + leaf_ids.size() * 4 +
-                      muls.size() * sizeof(BsrMul) + adds.size() * sizeof(BsrAdd) +
-                      muls.size() * sizeof(BatchDesc) + max_large * sizeof(BsrAddSlot) +
-                      max_large_chunks * (sizeof(CarryFn) + sizeof(uint64_t)) + 11 * 256;
-  if (hipMalloc(&P.buf, need) != hipSuccess || hipMalloc(&P.arena, P.arena_limbs * 4) != hipSuccess) {
-    (void)hipGetLastError();
-    bsr_plan_free(P);
-    return fail(PCG_ERR_OOM, "cannot allocate the binary-splitting plan (%llu arena limbs)",
-                (unsigned long long)HP.P.arena_limbs);
+// Size table of a subtree plan (bsr_layout_node): one entry per distinct
+// subtree size, children before parents, with the value bounds of
+// bsr_bounds per size, and per height the product groups and adds.
+struct BsrSizes {
+  std::vector<BsrSizeInfo> info;
+  std::vector<std::array<uint64_t, 5>> bits, tlen, slot;
+  std::vector<int32_t> height;
+  std::vector<uint32_t> cnt;  // ns x ns subtree counts
+  std::map<uint64_t, int32_t> index;
+  int32_t root = -1;
+  size_t max_large = 0, max_large_chunks = 0;
+};
+
+int32_t bsr_size_index(BsrSizes& S, uint64_t s) {
+  auto it = S.index.find(s);
+  if (it != S.index.end()) return it->second;
+  int32_t l = -1, r = -1;
+  if (s > 1) {
+    l = bsr_size_index(S, s / 2);
+    r = bsr_size_index(S, s - s / 2);
   }
-  char* p = static_cast<char*>(P.buf);
-  P.meta = carve<BsrVal>(p, nv);
-  P.leaf_rel = carve<uint32_t>(p, leaf_rel.size());
-  P.leaf_ids = carve<uint32_t>(p, leaf_ids.size());
-  P.muls = carve<BsrMul>(p, muls.size());
-  P.adds = carve<BsrAdd>(p, adds.size());
-  P.descs = carve<BatchDesc>(p, muls.size());
-  P.add_slots = carve<BsrAddSlot>(p, max_large);
-  P.add_agg = carve<CarryFn>(p, max_large_chunks);
-  P.add_cin = carve<uint64_t>(p, max_large_chunks);
-  HIPCHK(hipMemcpyAsync(P.meta, meta.data(), nv * sizeof(BsrVal), hipMemcpyHostToDevice, c.stream));
-  HIPCHK(hipMemcpyAsync(P.leaf_rel, leaf_rel.data(), leaf_rel.size() * 4, hipMemcpyHostToDevice, c.stream));
-  HIPCHK(hipMemcpyAsync(P.leaf_ids, leaf_ids.data(), leaf_ids.size() * 4, hipMemcpyHostToDevice, c.stream));
-  if (!muls.empty())
-    HIPCHK(hipMemcpyAsync(P.muls, muls.data(), muls.size() * sizeof(BsrMul), hipMemcpyHostToDevice, c.stream));
-  if (!adds.empty())
-    HIPCHK(hipMemcpyAsync(P.adds, adds.data(), adds.size() * sizeof(BsrAdd), hipMemcpyHostToDevice, c.stream));
-  HIPCHK(hipStreamSynchronize(c.stream));  // the host plan may be freed after this
+  std::array<uint64_t, 5> bits{}, tl{}, sl{};
+  if (l < 0) {
+    for (int j = 0; j < 3; ++j) bits[j] = kLeafBits[j];
+  } else {
+    const auto &bl = S.bits[l], &br = S.bits[r];
+    bits[3] = bl[0] + br[1];
+    bits[4] = br[0] + bl[2];
+    bits[0] = std::max(bits[3], bits[4]) + 1;
+    bits[1] = bl[1] + br[1];
+    bits[2] = bl[2] + br[2];
+  }
+  for (int j = 0; j < 5; ++j) tl[j] = limbs_of_bits(bits[j]);
+  if (l < 0) {
+    for (int j = 0; j < 3; ++j) sl[j] = tl[j];
+  } else {
+    const auto &tll = S.tlen[l], &tlr = S.tlen[r];
+    sl[3] = tll[0] + tlr[1];
+    sl[4] = tlr[0] + tll[2];
+    sl[0] = std::max(tl[3], tl[4]) + 1;
+    sl[1] = tll[1] + tlr[1];
+    sl[2] = tll[2] + tlr[2];
+  }
+  BsrSizeInfo e;
+  memset(&e, 0, sizeof e);
+  e.s = s;
+  e.l = l;
+  e.r = r;
+  for (int j = 0; j < 5; ++j) {
+    e.pre[j] = e.own;
+    e.own += sl[j] + 1;  // one spare limb
+  }
+  e.arena = e.own + (l < 0 ? 0 : S.info[l].arena + S.info[r].arena);
+  const int32_t i = (int32_t)S.info.size();
+  S.info.push_back(e);
+  S.bits.push_back(bits);
+  S.tlen.push_back(tl);
+  S.slot.push_back(sl);
+  S.height.push_back(l < 0 ? 1 : std::max(S.height[l], S.height[r]) + 1);
+  S.index[s] = i;
+  return i;
+}
+
+// The plan of a subtree of d leaves from its size table: P's sizes and
+// heights (launch groups, adds) and S, whose info and cnt are what
+// bsr_layout_node reads.  Product groups follow bsr_key and class_pairs
+// with the class table tab, in the order of the direct construction.
+int bsr_plan_sizes(uint32_t d, const SmallTable& tab, BsrPlan& P, BsrSizes& S) {
+  P = BsrPlan();
+  S = BsrSizes();
+  if (d == 0) return fail(PCG_ERR_INVALID, "empty range");
+  S.root = bsr_size_index(S, d);
+  const size_t ns = S.info.size();
+  S.cnt.assign(ns * ns, 0);
+  for (size_t i = 0; i < ns; ++i) {  // children have smaller indices
+    uint32_t* ci = &S.cnt[i * ns];
+    ci[i] += 1;
+    if (S.info[i].l >= 0)
+      for (size_t q = 0; q < ns; ++q)
+        ci[q] += S.cnt[S.info[i].l * ns + q] + S.cnt[S.info[i].r * ns + q];
+  }
+  const uint32_t* croot = &S.cnt[(size_t)S.root * ns];
+  P.d = d;
+  P.maxb = 0;
+  for (size_t i = 0; i < ns; ++i)
+    for (int j = 0; j < 5; ++j) P.maxb = std::max(P.maxb, S.tlen[i][j]);
+  P.arena_limbs = S.info[S.root].arena;
+  for (int j = 0; j < 3; ++j) {
+    P.root_bound[j] = S.tlen[S.root][j];
+    P.root_off[j] = S.info[S.root].pre[j];
+  }
+  P.H = S.height[S.root];
+  P.nleaves = d;
+  P.hp.assign(P.H + 1, BsrHeight());
+  uint64_t mcur = 0, acur = 0;
+  for (int32_t h = 2; h <= P.H; ++h) {
+    std::vector<int32_t> peers;
+    for (size_t i = 0; i < ns; ++i)
+      if (S.height[i] == h) peers.push_back((int32_t)i);
+    if (peers.size() > (size_t)kBsrPeers)
+      return fail(PCG_ERR_INTERNAL, "%zu subtree sizes at height %d", peers.size(), (int)h);
+    // group code of each product: kind * 2, + 1 for a single (unpaired) one
+    std::vector<std::array<int, 4>> code(peers.size());
+    for (size_t p = 0; p < peers.size(); ++p) {
+      const BsrSizeInfo& e = S.info[peers[p]];
+      const auto &tl = S.tlen[e.l], &tr = S.tlen[e.r];
+      // operands as bsr_layout_node's m4: (Q1, P0), (Q1, Q0), (R0, P1), (R0, R1)
+      const int k[4] = {bsr_key(tab, tr[1], tl[0]), bsr_key(tab, tr[1], tl[1]),
+                        bsr_key(tab, tl[2], tr[0]), bsr_key(tab, tl[2], tr[2])};
+      for (int q = 0; q < 4; q += 2) {
+        const bool paired = k[q] == k[q + 1] && class_pairs(k[q] / kWrapKey);
+        code[p][q] = 2 * k[q] + (paired ? 0 : 1);
+        code[p][q + 1] = 2 * k[q + 1] + (paired ? 0 : 1);
+      }
+    }
+    std::vector<int> groups;
+    for (const auto& cp : code)
+      for (int c : cp)
+        if (std::find(groups.begin(), groups.end(), c) == groups.end()) groups.push_back(c);
+    std::sort(groups.begin(), groups.end());
+    BsrHeight& hh = P.hp[h];
+    hh.mul0 = mcur;
+    std::map<int, uint64_t> gbase;
+    for (int g : groups) {
+      uint64_t n = 0;
+      for (size_t p = 0; p < peers.size(); ++p)
+        for (int j = 0; j < 4; ++j)
+          if (code[p][j] == g) n += croot[peers[p]];
+      gbase[g] = mcur;
+      hh.classes.push_back({(g / 2) / kWrapKey, (size_t)n, (g & 1) == 0, (g / 2) % kWrapKey});
+      mcur += n;
+    }
+    hh.nmul = mcur - hh.mul0;
+    // adds: those with a result slot of at most 1024 limbs first
+    hh.add0 = acur;
+    hh.nsmall = hh.nlarge = 0;
+    uint64_t large_max = 0;
+    for (int32_t q : peers) {
+      if (S.slot[q][0] <= 1024) {
+        hh.nsmall += croot[q];
+      } else {
+        hh.nlarge += croot[q];
+        large_max = std::max(large_max, S.slot[q][0]);
+      }
+    }
+    hh.add_maxch = (uint32_t)((large_max + kCarryBlock - 1) / kCarryBlock);
+    S.max_large = std::max(S.max_large, hh.nlarge);
+    S.max_large_chunks = std::max(S.max_large_chunks, hh.nlarge * hh.add_maxch);
+    acur += hh.nsmall + hh.nlarge;
+    for (size_t p = 0; p < peers.size(); ++p) {
+      BsrSizeInfo& e = S.info[peers[p]];
+      e.npeer = (int32_t)peers.size();
+      for (size_t o = 0; o < peers.size(); ++o) e.peer[o] = peers[o];
+      for (int j = 0; j < 4; ++j) {
+        e.mbase[j] = gbase[code[p][j]];
+        e.mloc[j] = 0;
+        for (int i = 0; i < j; ++i) e.mloc[j] += code[p][i] == code[p][j];
+        for (size_t o = 0; o < peers.size(); ++o) {
+          e.mmul[j][o] = 0;
+          for (int i = 0; i < 4; ++i) e.mmul[j][o] += code[o][i] == code[p][j];
+        }
+      }
+      const bool small = S.slot[peers[p]][0] <= 1024;
+      e.abase = hh.add0 + (small ? 0 : hh.nsmall);
+      for (size_t o = 0; o < peers.size(); ++o) e.amul[o] = (S.slot[peers[o]][0] <= 1024) == small;
+    }
+  }
+  if (mcur != 4 * (uint64_t)(d - 1) || acur != (uint64_t)(d - 1))
+    return fail(PCG_ERR_INTERNAL, "subtree plan of %u leaves: %llu products, %llu adds", d,
+                (unsigned long long)mcur, (unsigned long long)acur);
   return PCG_OK;
 }
 
+BsrLayout bsr_layout_of(const BsrSizes& S, uint32_t d) {
+  BsrLayout L;
+  memset(&L, 0, sizeof L);
+  L.ns = (int32_t)S.info.size();
+  L.root = S.root;
+  L.nodes = 2 * (uint64_t)d - 1;
+  L.nmeta = 5 * L.nodes;
+  L.nleaf = d;
+  L.nmul = 4 * (uint64_t)(d - 1);
+  L.nadd = d - 1;
+  return L;
+}
+
+// Device side: allocates the plan's buffers and arena and lays the plan out
+// with k_bsr_layout (one thread per node); synchronises to check it.
+int bsr_plan_build(Ctx& c, uint32_t d, const SmallTable& tab, BsrPlan& P) {
+  BsrSizes S;
+  int rc = bsr_plan_sizes(d, tab, P, S);
+  if (rc) return rc;
+  BsrLayout L = bsr_layout_of(S, d);
+  const size_t ns = S.info.size();
+  const size_t need = L.nmeta * sizeof(BsrVal) + L.nleaf * 4 * 4 + L.nmul * sizeof(BsrMul) +
+                      L.nadd * sizeof(BsrAdd) + L.nmul * sizeof(BatchDesc) +
+                      S.max_large * sizeof(BsrAddSlot) +
+                      S.max_large_chunks * (sizeof(CarryFn) + sizeof(uint64_t)) +
+                      ns * sizeof(BsrSizeInfo) + ns * ns * 4 + 13 * 256;
+  if (hipMalloc(&P.buf, need) != hipSuccess || hipMalloc(&P.arena, P.arena_limbs * 4) != hipSuccess) {
+    (void)hipGetLastError();
+    const uint64_t al = P.arena_limbs;
+    bsr_plan_free(P);
+    return fail(PCG_ERR_OOM, "cannot allocate the binary-splitting plan (%llu arena limbs)",
+                (unsigned long long)al);
+  }
+  char* p = static_cast<char*>(P.buf);
+  P.meta = carve<BsrVal>(p, L.nmeta);
+  P.leaf_rel = carve<uint32_t>(p, L.nleaf);
+  P.leaf_ids = carve<uint32_t>(p, 3 * L.nleaf);
+  P.muls = carve<BsrMul>(p, L.nmul);
+  P.adds = carve<BsrAdd>(p, L.nadd);
+  P.descs = carve<BatchDesc>(p, L.nmul);
+  P.add_slots = carve<BsrAddSlot>(p, S.max_large);
+  P.add_agg = carve<CarryFn>(p, S.max_large_chunks);
+  P.add_cin = carve<uint64_t>(p, S.max_large_chunks);
+  BsrSizeInfo* d_info = carve<BsrSizeInfo>(p, ns);
+  uint32_t* d_cnt = carve<uint32_t>(p, ns * ns);
+  uint32_t* d_err = carve<uint32_t>(p, 1);
+  L.info = d_info;
+  L.cnt = d_cnt;
+  L.meta = P.meta;
+  L.leaf_rel = P.leaf_rel;
+  L.leaf_ids = P.leaf_ids;
+  L.muls = P.muls;
+  L.adds = P.adds;
+  L.err = d_err;
+  HIPCHK(hipMemcpyAsync(d_info, S.info.data(), ns * sizeof(BsrSizeInfo), hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipMemcpyAsync(d_cnt, S.cnt.data(), ns * ns * 4, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipMemsetAsync(d_err, 0, 4, c.stream));
+  const uint64_t blocks = std::min<uint64_t>((L.nodes + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_bsr_layout, dim3((unsigned)blocks), dim3(256), 0, c.stream, L);
+  HIPCHK(hipGetLastError());
+  uint32_t err = 0;
+  HIPCHK(hipMemcpyAsync(&err, d_err, 4, hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));  // S's tables are freed on return
+  if (err) {
+    bsr_plan_free(P);
+    return fail(PCG_ERR_INTERNAL, "subtree plan of %u leaves: an entry fell outside its list", d);
+  }
+  return PCG_OK;
+}
+
+// The layout of bsr_plan_build run on the host, compared entry by entry with
+// the direct construction (bsr_host_plan).  Host only.
+int bsr_plan_check(uint32_t d, const SmallTable& tab) {
+  BsrPlan P;
+  BsrSizes S;
+  int rc = bsr_plan_sizes(d, tab, P, S);
+  if (rc) return rc;
+  BsrLayout L = bsr_layout_of(S, d);
+  std::vector<BsrVal> meta(L.nmeta);
+  std::vector<uint32_t> leaf_rel(L.nleaf), leaf_ids(3 * L.nleaf);
+  std::vector<BsrMul> muls(L.nmul);
+  std::vector<BsrAdd> adds(L.nadd);
+  L.info = S.info.data();
+  L.cnt = S.cnt.data();
+  L.meta = meta.data();
+  L.leaf_rel = leaf_rel.data();
+  L.leaf_ids = leaf_ids.data();
+  L.muls = muls.data();
+  L.adds = adds.data();
+  for (uint64_t t = 0; t < L.nodes; ++t)
+    if (!bsr_layout_node(L, t)) return fail(PCG_ERR_INTERNAL, "d %u: node %llu outside its lists", d, (unsigned long long)t);
+  BsrHostPlan H;
+  bsr_host_plan(d, tab, H);
+  const BsrPlan& Q = H.P;
+  auto differ = [&](const char* what) { return fail(PCG_ERR_INTERNAL, "d %u: %s differs", d, what); };
+  if (P.maxb != Q.maxb || P.arena_limbs != Q.arena_limbs || P.H != Q.H || P.nleaves != Q.nleaves)
+    return differ("plan size");
+  for (int j = 0; j < 3; ++j)
+    if (P.root_bound[j] != Q.root_bound[j] || P.root_off[j] != Q.root_off[j]) return differ("root");
+  if (S.max_large != H.max_large || S.max_large_chunks != H.max_large_chunks) return differ("add scratch");
+  for (int32_t h = 2; h <= P.H; ++h) {
+    const BsrHeight &x = P.hp[h], &y = Q.hp[h];
+    if (x.mul0 != y.mul0 || x.nmul != y.nmul || x.add0 != y.add0 || x.nsmall != y.nsmall ||
+        x.nlarge != y.nlarge || x.add_maxch != y.add_maxch || x.classes.size() != y.classes.size())
+      return differ("height");
+    for (size_t k = 0; k < x.classes.size(); ++k) {
+      const auto &u = x.classes[k], &v = y.classes[k];
+      if (u.cls != v.cls || u.n != v.n || u.pairs != v.pairs || u.wrap != v.wrap) return differ("launch group");
+    }
+  }
+  if (meta.size() != H.meta.size() || muls.size() != H.muls.size() || adds.size() != H.adds.size() ||
+      leaf_rel != H.leaf_rel || leaf_ids != H.leaf_ids)
+    return differ("leaves or list sizes");
+  for (size_t v = 0; v < meta.size(); ++v)
+    if (meta[v].off != H.meta[v].off || meta[v].L != H.meta[v].L || meta[v].sign != H.meta[v].sign)
+      return differ("arena offset");
+  for (size_t k = 0; k < muls.size(); ++k)
+    if (muls[k].a != H.muls[k].a || muls[k].b != H.muls[k].b || muls[k].out != H.muls[k].out)
+      return differ("product list");
+  for (size_t k = 0; k < adds.size(); ++k)
+    if (adds[k].x != H.adds[k].x || adds[k].y != H.adds[k].y || adds[k].out != H.adds[k].out)
+      return differ("add list");
+  return PCG_OK;
+}
 
 // Two cached plans: the binary splitting produces at most two subtree sizes
 // at a given depth.
 BsrPlan g_bsr_plans[2];
 int g_bsr_next = 0;
-
-// Host plans being built on other threads (pcg_bsr_prepare), by subtree
-// size; touched only under the library lock.
-std::map<uint32_t, std::shared_future<std::shared_ptr<BsrHostPlan>>> g_prep;
 
 SmallTable small_table() {
   SmallTable t;
@@ -1575,11 +1827,22 @@ SmallTable small_table() {
   return t;
 }
 
-std::shared_ptr<BsrHostPlan> host_plan_job(uint32_t d, SmallTable tab, uint64_t gen) {
-  auto h = std::make_shared<BsrHostPlan>();
-  bsr_host_plan(d, tab, *h);
-  h->table_gen = gen;
-  return h;
+// The cached plan of a subtree of d leaves, built if needed (evicting the
+// older of the two).
+int bsr_plan_get(Ctx& c, uint32_t d, BsrPlan** out) {
+  for (BsrPlan& q : g_bsr_plans)
+    if (q.d == d && q.buf) {
+      *out = &q;
+      return PCG_OK;
+    }
+  BsrPlan* P = &g_bsr_plans[g_bsr_next];
+  g_bsr_next ^= 1;
+  HIPCHK(hipStreamSynchronize(c.stream));
+  bsr_plan_free(*P);
+  const int rc = bsr_plan_build(c, d, small_table(), *P);
+  if (rc) return rc;
+  *out = P;
+  return PCG_OK;
 }
 
 // Evaluates the subtree [a, b) into the plan's arena; rv receives the
@@ -1591,26 +1854,8 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
   if (rct) return rct;
   const uint32_t d = b - a;
   BsrPlan* P = nullptr;
-  for (BsrPlan& q : g_bsr_plans)
-    if (q.d == d && q.buf) P = &q;
-  if (!P) {
-    P = &g_bsr_plans[g_bsr_next];
-    g_bsr_next ^= 1;
-    HIPCHK(hipStreamSynchronize(c.stream));
-    bsr_plan_free(*P);
-    // a plan prepared in the background for this size and class table, or
-    // one built here
-    std::shared_ptr<BsrHostPlan> hp;
-    auto it = g_prep.find(d);
-    if (it != g_prep.end()) {
-      const std::shared_future<std::shared_ptr<BsrHostPlan>> fut = it->second;
-      g_prep.erase(it);
-      hp = fut.get();
-    }
-    if (!hp || hp->table_gen != g_table_gen) hp = host_plan_job(d, small_table(), g_table_gen);
-    int rc = bsr_plan_upload(c, *hp, *P);
-    if (rc) return rc;
-  }
+  int rc = bsr_plan_get(c, d, &P);
+  if (rc) return rc;
   if (P->maxb > max_limbs)
     return fail(PCG_ERR_TOO_LARGE, "subtree values may reach %llu limbs > %llu",
                 (unsigned long long)P->maxb, (unsigned long long)max_limbs);
@@ -2151,7 +2396,6 @@ void pcg_shutdown(void) {
   (void)hipFree(c.wrap_desc);
   (void)hipFree(c.bagg);
   (void)hipFree(c.bcin);
-  g_prep.clear();  // waits for background plan builds
   for (BsrPlan& q : g_bsr_plans) bsr_plan_free(q);
   (void)df_release_pool(c);
   (void)hipFree(c.df_agg);
@@ -2553,16 +2797,26 @@ int pcg_bsr_prepare(const uint32_t* sizes, uint64_t n) {
   DF_ENTER();
   int rc = ensure_tuned(c);  // plans record the small-product classes
   if (rc) return rc;
-  for (uint64_t i = 0; i < n; ++i) {
-    const uint32_t d = sizes[i];
-    if (d == 0 || g_prep.count(d)) continue;
-    bool have = false;
-    for (const BsrPlan& q : g_bsr_plans)
-      if (q.d == d && q.buf) have = true;
-    if (have) continue;
-    g_prep[d] = std::async(std::launch::async, host_plan_job, d, small_table(), g_table_gen).share();
+  // the plan cache holds two sizes: the first two distinct ones
+  std::vector<uint32_t> want;
+  for (uint64_t i = 0; i < n && want.size() < 2; ++i)
+    if (sizes[i] && std::find(want.begin(), want.end(), sizes[i]) == want.end()) want.push_back(sizes[i]);
+  for (uint32_t d : want) {
+    BsrPlan* P = nullptr;
+    rc = bsr_plan_get(c, d, &P);
+    if (rc) return rc;
   }
   return PCG_OK;
+}
+
+int pcg_bsr_plan_check(uint32_t leaves, const int* table, int n) {
+  if (!table || n != kSmallNttMaxLog + 1) return fail(PCG_ERR_INVALID, "the class table has %d entries", kSmallNttMaxLog + 1);
+  SmallTable tab;
+  for (int b = 0; b < n; ++b) {
+    if (!small_class_valid(b, table[b])) return fail(PCG_ERR_INVALID, "class %d is not valid for bucket %d", table[b], b);
+    tab[b] = table[b];
+  }
+  return bsr_plan_check(leaves, tab);
 }
 
 int pcg_autotune(int* table, int n) {

@@ -151,8 +151,12 @@ void bsr_subtree_batched(PQR& out, uint32_t a, uint32_t b, size_t p, bool verbos
   out = std::move(val[root]);
 }
 
-// Largest subtree handed to the GPU-resident evaluation at once.
-constexpr uint32_t kDeviceLeaves = 1u << 20;
+// Largest subtree handed to the GPU-resident evaluation at once.  Each
+// level of such a subtree runs as one batched launch set, where the combines
+// above it run node by node, so larger is faster while the plans fit: at
+// 10^9 digits this gives 32 subtrees of 2.2 M leaves (4.1 GB of arena per
+// plan, two plans).
+constexpr uint32_t kDeviceLeaves = 1u << 22;
 
 // Values of a node kept in device memory.
 struct DPQR {
@@ -269,9 +273,8 @@ size_t pi_hex_into(size_t decimal_digits, PiMode mode, char* out, size_t cap, Pi
     n = str.size();
     pt.conversion = now_s() - t4;
   } else {
-    // The subtree plans are built on host threads while the GPU computes
-    // 1/sqrt(10005) (independent of the series, so computing it first
-    // changes no value).
+    // The subtree plans (laid out on the GPU), then 1/sqrt(10005): it is
+    // independent of the series, so computing it first changes no value.
     std::vector<uint32_t> sizes;
     resident_sizes(0, (uint32_t)terms, p, sizes);
     if (!sizes.empty() && pcg_bsr_prepare(sizes.data(), sizes.size()) != PCG_OK)

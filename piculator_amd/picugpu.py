@@ -37,7 +37,7 @@ EXPORTS = [
     "pcg_bsr_exact", "pcg_bsr_bound", "pcg_bsr_prepare", "pcg_df_free", "pcg_df_set",
     "pcg_df_read", "pcg_df_copy", "pcg_df_ucmp", "pcg_df_add", "pcg_df_sub", "pcg_df_mul",
     "pcg_df_mul_pair", "pcg_df_mul_u32", "pcg_df_to_hex", "pcg_df_bsr_exact",
-    "pcg_df_release_pool", "pcg_autotune", "pcg_batch_table",
+    "pcg_df_release_pool", "pcg_autotune", "pcg_batch_table", "pcg_bsr_plan_check",
 ]
 
 
@@ -99,6 +99,7 @@ def load():
     L.pcg_bsr_exact.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64] + [C.POINTER(BsrValue)] * 3
     L.pcg_bsr_bound.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
     L.pcg_bsr_prepare.argtypes = [C.POINTER(C.c_uint32), C.c_uint64]
+    L.pcg_bsr_plan_check.argtypes = [C.c_uint32, C.POINTER(C.c_int), C.c_int]
     L.pcg_autotune.argtypes = [C.POINTER(C.c_int), C.c_int]
     L.pcg_batch_table.argtypes = [C.POINTER(C.c_int), C.c_int, C.c_int]
     dfp = C.POINTER(DFloat)
@@ -277,10 +278,18 @@ def bsr_bound(a: int, b: int) -> int:
 
 
 def bsr_prepare(sizes):
-    """Starts building the plans of subtrees of these sizes in the background."""
+    """Builds the plans of subtrees of these sizes (the first two distinct)."""
     ensure_init()
     arr = (C.c_uint32 * max(1, len(sizes)))(*sizes)
     _check(load().pcg_bsr_prepare(arr, len(sizes)))
+
+
+def bsr_plan_check(leaves: int, table) -> None:
+    """Host-only self-check of the subtree plan layout (no GPU needed):
+    raises PcgError naming the first entry that differs from the direct
+    node-by-node construction."""
+    arr = (C.c_int * len(table))(*table)
+    _check(load().pcg_bsr_plan_check(leaves, arr, len(table)))
 
 
 def bsr_exact(a: int, b: int, max_limbs: int):

@@ -89,3 +89,47 @@ def test_bsr_bound_matches_tree_walk():
 
     for d in (1, 2, 3, 5, 64, 100, 1000, 4097, 70515):
         assert picugpu.bsr_bound(1000, 1000 + d) == _tree_bound(d), d
+
+
+# Small-product class tables (pcg_batch_table layout, buckets 0..12): the
+# defaults before tuning, and tables that move products into the
+# single-workgroup NTT and batched-pass classes, so the plan's launch groups
+# (paired, single, wrapped) change with them.
+_PLAN_TABLES = [
+    [1, 1, 1, 1, 1, 1, 1, 2, 2, 0, 0, 11, 12],
+    [1, 1, 1, 1, 1, 1, 1, 2, 2, 0, 10, 11, 12],
+    [1, 1, 1, 1, 1, 1, 1, 3, 3, 3, 110, 111, 112],
+    [1, 1, 1, 1, 1, 1, 2, 2, 3, 110, 10, 11, 112],
+]
+
+
+def test_bsr_plan_layout_matches_direct_construction():
+    """The subtree plan the GPU lays out from the size table (the same
+    per-node code run on the CPU) equals the direct node-by-node
+    construction: arena offsets, leaf lists, product and add lists in launch
+    order, per-height launch groups.  Every size up to 300 leaves (all the
+    small shapes, odd and even splits) and larger ones around powers of two."""
+    from piculator_amd import picugpu
+
+    sizes = list(range(1, 301)) + [1023, 1024, 1025, 4097, 12345, 65537, 100003]
+    for t in _PLAN_TABLES:
+        for d in sizes:
+            picugpu.bsr_plan_check(d, t)
+
+
+def test_bsr_plan_layout_at_pi_subtree_sizes():
+    """The two subtree sizes of pi to 10^9 digits at depth 5 (70,513,670
+    terms / 32) with the default table."""
+    from piculator_amd import picugpu
+
+    for d in (2203552, 2203553):
+        picugpu.bsr_plan_check(d, _PLAN_TABLES[0])
+
+
+def test_bsr_plan_check_rejects_bad_tables():
+    from piculator_amd import picugpu
+
+    with pytest.raises(picugpu.PcgError):
+        picugpu.bsr_plan_check(100, [1] * 12)
+    with pytest.raises(picugpu.PcgError):
+        picugpu.bsr_plan_check(100, [0] * 12 + [5])  # class 5 is not a class

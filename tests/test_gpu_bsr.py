@@ -43,8 +43,9 @@ def test_device_resident_results(gpu, a, b):
 
 
 def test_prepared_plans(gpu):
-    """Plans built in the background (pcg_bsr_prepare) give the same values;
-    a size prepared twice, or never evaluated, is harmless."""
+    """Plans built ahead (pcg_bsr_prepare, laid out on the GPU) give the same
+    values; a size named twice, a third size (beyond the two cached plans)
+    or one never evaluated is harmless."""
     gpu.bsr_prepare([8000, 8001, 8000, 123])
     for a, b in [(5000, 13000), (40, 8041)]:
         got = gpu.bsr_exact(a, b, BIG_P)
