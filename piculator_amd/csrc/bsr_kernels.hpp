@@ -22,77 +22,173 @@ struct BsrVal {
 };
 
 // ------------------------------------------------------------------ leaves
-// Fixed-capacity unsigned integer for the leaf terms (< 2^160).
-struct Small {
-  uint32_t t[6];
-  int L;
+// Leaves and the nodes of two leaves, one thread each, in fixed-size limb
+// arrays (registers): a leaf's P < 2^158, Q < 2^150, R < 2^96 take 5, 5, 3
+// limbs, so a two-leaf node's products take at most 10 limbs and its
+// P = x + y at most 11.  A two-leaf node is computed from its leaves
+// directly, so neither its leaves nor its products pass through memory.
+// Values are normalised (no leading zero limb; zero has L = 0 and sign 1),
+// as BigFloat leaves them: the product's one-limb trim and usub's strip
+// (BigFloat.cpp:411-412, 277-284) give exactly the significant limbs.
+
+// z[0..N] = z[0..N) * x (z[N] receives the carry)
+template <int N>
+__host__ __device__ __forceinline__ void fx_mul_u32(uint32_t* z, uint32_t x) {
+  uint64_t c = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    c += (uint64_t)z[i] * x;
+    z[i] = (uint32_t)c;
+    c >>= 32;
+  }
+  z[N] = (uint32_t)c;
+}
+
+// z[0..NA+NB) = a[0..NA) * b[0..NB)
+template <int NA, int NB>
+__host__ __device__ __forceinline__ void fx_mul(const uint32_t* a, const uint32_t* b, uint32_t* z) {
+#pragma unroll
+  for (int k = 0; k < NA + NB; ++k) z[k] = 0;
+#pragma unroll
+  for (int i = 0; i < NA; ++i) {
+    uint64_t c = 0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      c += (uint64_t)a[i] * b[j] + z[i + j];
+      z[i + j] = (uint32_t)c;
+      c >>= 32;
+    }
+    z[i + NB] = (uint32_t)c;
+  }
+}
+
+// Number of significant limbs of z[0..N).
+template <int N>
+__host__ __device__ __forceinline__ uint32_t fx_len(const uint32_t* z) {
+  uint32_t L = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (z[i]) L = i + 1;
+  return L;
+}
+
+// P, Q, R of the leaf b (PI.h:67-88) with the reference's uint32
+// arithmetic for 2b-1, 6b-5, 6b-1; P is negative for odd b.
+struct LeafVals {
+  uint32_t P[5], Q[5], R[3];
+  int32_t sP;  // sign of P: 1 positive, 0 negative
 };
 
-__device__ __forceinline__ void small_set(Small& z, uint32_t x) {
-  z.L = x ? 1 : 0;
-  z.t[0] = x;
-}
-// z = z * x as BigFloat::mul(uint32_t) (BigFloat.cpp:156-183)
-__device__ __forceinline__ void small_mul(Small& z, uint32_t x) {
-  if (z.L == 0 || x == 0) {
-    z.L = 0;
-    return;
+__host__ __device__ __forceinline__ LeafVals bsr_leaf(uint32_t b) {
+  LeafVals v;
+  const uint32_t f1 = 2 * b - 1, f2 = 6 * b - 5, f3 = 6 * b - 1;
+  // P = (13591409 + 545140134 b) f1 f2 f3
+  uint32_t p[6] = {b, 0, 0, 0, 0, 0};
+  fx_mul_u32<1>(p, 545140134u);
+  const uint64_t lo = (uint64_t)p[0] + 13591409u;
+  p[0] = (uint32_t)lo;
+  p[1] += (uint32_t)(lo >> 32);  // b 545140134 + 13591409 < 2^63: no carry out of p[1]
+  fx_mul_u32<2>(p, f1);
+  fx_mul_u32<3>(p, f2);
+  fx_mul_u32<4>(p, f3);
+  // Q = b^3 26726400 409297880
+  uint32_t q[6] = {b, 0, 0, 0, 0, 0};
+  fx_mul_u32<1>(q, b);
+  fx_mul_u32<2>(q, b);
+  fx_mul_u32<3>(q, 26726400u);
+  fx_mul_u32<4>(q, 409297880u);
+  // R = f1 f2 f3
+  uint32_t r[4] = {f1, 0, 0, 0};
+  fx_mul_u32<1>(r, f2);
+  fx_mul_u32<2>(r, f3);
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    v.P[i] = p[i];
+    v.Q[i] = q[i];
   }
-  uint64_t carry = 0;
-  for (int c = 0; c < z.L; ++c) {
-    carry += (uint64_t)z.t[c] * x;
-    z.t[c] = (uint32_t)carry;
-    carry >>= 32;
-  }
-  if (carry) z.t[z.L++] = (uint32_t)carry;
-}
-// z = z + x (both non-negative) as BigFloat::uadd with p = 0
-__device__ __forceinline__ void small_add(Small& z, uint32_t x) {
-  uint64_t carry = x;
-  for (int c = 0; c < z.L && carry; ++c) {
-    carry += z.t[c];
-    z.t[c] = (uint32_t)carry;
-    carry >>= 32;
-  }
-  if (carry) z.t[z.L++] = (uint32_t)carry;
-  if (z.L == 0 && x) {
-    z.t[0] = x;
-    z.L = 1;
-  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v.R[i] = r[i];
+  v.sP = (b & 1) ? 0 : 1;
+  return v;
 }
 
-// Leaf b = a + rel[i] (PI.h:67-88), with the reference's uint32 arithmetic
-// for 2b-1, 6b-5, 6b-1.  ids: value ids of (P, Q, R) for each leaf.
-__global__ void k_bsr_leaves(const uint32_t* rel, uint32_t a, const uint32_t* ids, uint64_t n,
-                             BsrVal* meta, uint32_t* arena) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t b = a + rel[i];
-  Small P, Q, R;
-  // P = (13591409 + 545140134 b)(2b-1)(6b-5)(6b-1), negative for odd b
-  small_set(P, b);
-  small_mul(P, 545140134u);
-  small_add(P, 13591409u);
-  small_mul(P, 2 * b - 1);
-  small_mul(P, 6 * b - 5);
-  small_mul(P, 6 * b - 1);
-  // Q = b^3 * 26726400 * 409297880
-  small_set(Q, b);
-  small_mul(Q, b);
-  small_mul(Q, b);
-  small_mul(Q, 26726400u);
-  small_mul(Q, 409297880u);
-  // R = (2b-1)(6b-5)(6b-1)
-  small_set(R, 2 * b - 1);
-  small_mul(R, 6 * b - 5);
-  small_mul(R, 6 * b - 1);
-  const Small* v[3] = {&P, &Q, &R};
-  for (int j = 0; j < 3; ++j) {
-    BsrVal& m = meta[ids[3 * i + j]];
-    for (int c = 0; c < v[j]->L; ++c) arena[m.off + c] = v[j]->t[c];
-    m.L = (uint32_t)v[j]->L;
-    m.sign = (j == 0 && (b & 1) && P.L) ? 0 : 1;
+template <int N>
+__host__ __device__ __forceinline__ void bsr_put(BsrVal* meta, uint32_t id, uint32_t* arena, const uint32_t* z, int32_t sign) {
+  BsrVal& m = meta[id];
+  const uint32_t L = fx_len<N>(z);
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if ((uint32_t)i < L) arena[m.off + i] = z[i];
+  m.L = L;
+  m.sign = L ? sign : 1;
+}
+
+// One entry per leaf of the subtree (rel, ids as the plan lays them out):
+// rel = b - a of the leaf, with bit 31 set when the leaf is the left child
+// of a two-leaf node, whose P, Q, R (value ids in ids) this thread
+// computes; ids[3i] = 0xFFFFFFFF for the right child of such a node
+// (nothing to do); otherwise the leaf's own P, Q, R.
+constexpr uint32_t kBsrPairBit = 0x80000000u;
+constexpr uint32_t kBsrSkip = 0xFFFFFFFFu;
+
+__host__ __device__ inline void bsr_base_entry(const uint32_t* rel, uint32_t a, const uint32_t* ids,
+                                               uint64_t i, BsrVal* meta, uint32_t* arena) {
+  const uint32_t id0 = ids[3 * i];
+  if (id0 == kBsrSkip) return;
+  const uint32_t rv = rel[i];
+  const uint32_t b = a + (rv & ~kBsrPairBit);
+  if (!(rv & kBsrPairBit)) {
+    const LeafVals v = bsr_leaf(b);
+    bsr_put<5>(meta, id0, arena, v.P, v.sP);
+    bsr_put<5>(meta, ids[3 * i + 1], arena, v.Q, 1);
+    bsr_put<3>(meta, ids[3 * i + 2], arena, v.R, 1);
+    return;
   }
+  // node [b - 1, b + 1): x = P0 Q1, Q = Q0 Q1, y = P1 R0, R = R0 R1,
+  // P = x + y (PI.h:95-112)
+  const LeafVals l = bsr_leaf(b), r = bsr_leaf(b + 1);
+  uint32_t x[11], y[10], Q[10], R[6];
+  fx_mul<5, 5>(l.P, r.Q, x);
+  fx_mul<5, 5>(l.Q, r.Q, Q);
+  fx_mul<5, 3>(r.P, l.R, y);
+  fx_mul<3, 3>(l.R, r.R, R);
+  x[10] = 0;
+  y[8] = y[9] = 0;
+  uint32_t Pz[11];
+  int32_t sign = l.sP;  // x has P0's sign, y has P1's (Q and R are positive)
+  if (l.sP == r.sP) {
+    uint64_t c = 0;
+#pragma unroll
+    for (int k = 0; k < 11; ++k) {
+      c += (uint64_t)x[k] + (k < 10 ? y[k] : 0u);
+      Pz[k] = (uint32_t)c;
+      c >>= 32;
+    }
+  } else {
+    // |x| - |y| or |y| - |x|, the larger magnitude's sign
+    int cmp = 0;
+#pragma unroll
+    for (int k = 9; k >= 0; --k)
+      if (cmp == 0 && x[k] != y[k]) cmp = x[k] > y[k] ? 1 : -1;
+    int64_t br = 0;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+      const int64_t d = cmp >= 0 ? (int64_t)x[k] - y[k] + br : (int64_t)y[k] - x[k] + br;
+      Pz[k] = (uint32_t)d;
+      br = d < 0 ? -1 : 0;
+    }
+    Pz[10] = 0;
+    if (cmp < 0) sign = r.sP;
+  }
+  bsr_put<11>(meta, id0, arena, Pz, sign);
+  bsr_put<10>(meta, ids[3 * i + 1], arena, Q, 1);
+  bsr_put<6>(meta, ids[3 * i + 2], arena, R, 1);
+}
+
+__global__ void k_bsr_base(const uint32_t* rel, uint32_t a, const uint32_t* ids, uint64_t n,
+                           BsrVal* meta, uint32_t* arena) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) bsr_base_entry(rel, a, ids, i, meta, arena);
 }
 
 // ---------------------------------------------------------------- products
@@ -553,9 +649,12 @@ __host__ __device__ inline bool bsr_layout_node(const BsrLayout& L, uint64_t t) 
   // walk 1: the node's size, range start and arena offset
   int32_t si = L.root;
   uint64_t id = 0, a = 0, base = 0;
+  uint64_t pid = 0, ps = 0;  // parent's id and size
   while (id != t) {
     const BsrSizeInfo& n = I[si];
     const uint64_t sl = I[n.l].s;
+    pid = id;
+    ps = n.s;
     if (t < id + 2 * sl) {
       id += 1;
       base += n.own;
@@ -579,10 +678,17 @@ __host__ __device__ inline bool bsr_layout_node(const BsrLayout& L, uint64_t t) 
       bad = true;
     }
   }
-  if (me.l < 0) {  // leaf [a, a + 1)
+  if (me.l < 0) {  // leaf [a, a + 1): an entry of k_bsr_base
     if (a < L.nleaf) {
-      L.leaf_rel[a] = (uint32_t)(a + 1);
-      for (int j = 0; j < 3; ++j) L.leaf_ids[3 * a + j] = (uint32_t)(5 * t + j);
+      uint32_t rel = (uint32_t)(a + 1), id0 = (uint32_t)(5 * t);
+      if (ps == 2 && t == pid + 1) {  // left leaf of a two-leaf node: the node
+        rel |= kBsrPairBit;
+        id0 = (uint32_t)(5 * pid);
+      } else if (ps == 2) {  // right leaf of one: nothing
+        id0 = kBsrSkip;
+      }
+      L.leaf_rel[a] = rel;
+      for (int j = 0; j < 3; ++j) L.leaf_ids[3 * a + j] = id0 == kBsrSkip ? kBsrSkip : id0 + j;
     } else {
       bad = true;
     }

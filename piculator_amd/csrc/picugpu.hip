@@ -1432,11 +1432,22 @@ void bsr_host_plan(uint32_t d, const SmallTable& tab, BsrHostPlan& HP) {
   std::vector<uint32_t>& leaf_rel = HP.leaf_rel;
   std::vector<uint32_t>& leaf_ids = HP.leaf_ids;
   std::vector<std::vector<int32_t>> by_h(P.H + 1);
+  std::vector<int32_t> parent(nodes.size(), -1);
+  for (int32_t i = 0; i < (int32_t)nodes.size(); ++i)
+    if (nodes[i].left >= 0) parent[nodes[i].left] = parent[nodes[i].right] = i;
   for (int32_t i = 0; i < (int32_t)nodes.size(); ++i) {
     by_h[nodes[i].height].push_back(i);
     if (nodes[i].left < 0) {
-      leaf_rel.push_back(nodes[i].b);
-      for (int j = 0; j < 3; ++j) leaf_ids.push_back((uint32_t)(5 * i + j));
+      // k_bsr_base entries: a two-leaf node through its left leaf
+      const int32_t q = parent[i];
+      const bool pair = q >= 0 && nodes[q].b - nodes[q].a == 2;
+      if (pair && nodes[q].left == i) {
+        leaf_rel.push_back(nodes[i].b | kBsrPairBit);
+        for (int j = 0; j < 3; ++j) leaf_ids.push_back((uint32_t)(5 * q + j));
+      } else {
+        leaf_rel.push_back(nodes[i].b);
+        for (int j = 0; j < 3; ++j) leaf_ids.push_back(pair ? kBsrSkip : (uint32_t)(5 * i + j));
+      }
     }
   }
   P.nleaves = leaf_rel.size();
@@ -1589,6 +1600,7 @@ int bsr_plan_sizes(uint32_t d, const SmallTable& tab, BsrPlan& P, BsrSizes& S) {
   P = BsrPlan();
   S = BsrSizes();
   if (d == 0) return fail(PCG_ERR_INVALID, "empty range");
+  if (d >= kBsrPairBit) return fail(PCG_ERR_TOO_LARGE, "subtree of %u leaves", d);
   S.root = bsr_size_index(S, d);
   const size_t ns = S.info.size();
   S.cnt.assign(ns * ns, 0);
@@ -1862,9 +1874,10 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
   if (caps)
     for (int j = 0; j < 3; ++j)
       if (caps[j] < P->root_bound[j]) return fail(PCG_ERR_TOO_LARGE, "result buffer too small");
-  hipLaunchKernelGGL(k_bsr_leaves, dim3((unsigned)((P->nleaves + 255) / 256)), dim3(256), 0, c.stream,
+  // leaves and two-leaf nodes (every node of height 2) in one launch
+  hipLaunchKernelGGL(k_bsr_base, dim3((unsigned)((P->nleaves + 255) / 256)), dim3(256), 0, c.stream,
                      P->leaf_rel, a, P->leaf_ids, P->nleaves, P->meta, P->arena);
-  for (int32_t h = 2; h <= P->H; ++h) {
+  for (int32_t h = 3; h <= P->H; ++h) {
     const BsrHeight& hh = P->hp[h];
     const BsrMul* hm = P->muls + hh.mul0;
     BatchDesc* hd = P->descs + hh.mul0;

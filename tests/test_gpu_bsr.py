@@ -28,6 +28,17 @@ def test_large_leaf_indices(gpu):
     assert got == [P.tuple(), Q.tuple(), R.tuple()]
 
 
+@pytest.mark.parametrize("a,b", [(715827870, 715827901), (1431655700, 1431655800)])
+def test_wrapping_leaf_factors(gpu, a, b):
+    # 6b-5 and 6b-1 wrap mod 2^32 from b = 715827883 (twice from
+    # 1431655766): leaves and two-leaf nodes keep the reference's uint32
+    # factors.  (Ranges stay below a + b = 2^32, where PI.h:90's uint32
+    # midpoint itself wraps.)
+    got = gpu.bsr_exact(a, b, BIG_P)
+    P, Q, R = O.pi_bsr(a, b, BIG_P)
+    assert got == [P.tuple(), Q.tuple(), R.tuple()]
+
+
 def test_window_too_small_is_refused(gpu):
     with pytest.raises(gpu.PcgError) as e:
         gpu.bsr_exact(0, 5000, 100)
