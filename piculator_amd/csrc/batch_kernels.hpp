@@ -303,7 +303,8 @@ __global__ __launch_bounds__((1 << LOGN) / kEPT) __attribute__((amdgpu_waves_per
 // (chunk, z) resolves limbs [1024 chunk, 1024 chunk + 1024) of product z
 // with carry-in 0 and records the chunk's carry function (the form of
 // k_carry_local); k_batch_carry_scan scans each product's chunk functions
-// with one wave; k_batch_carry_fix adds the carry-ins.  r: inverse-transform
+// with one wave (k_batch_carry_scan_wide: one workgroup, above 64 chunks);
+// k_batch_carry_fix adds the carry-ins.  r: inverse-transform
 // outputs of product z at r + z * zstride, prime q at offset q * N.
 // agg / cin: nch entries per product.  Wrapped products (top != null): a
 // product with w = zL - 1 - N > 0 holds the cyclic convolution of length N,
@@ -426,6 +427,18 @@ __global__ __launch_bounds__(64) void k_batch_carry_scan(const BatchDesc* descs,
     tot.T = __shfl(incl.T, 63, 64);
     run = cf_compose(run, tot);
   }
+}
+
+// The same scan with one workgroup per product, for products of more than
+// 64 chunks (cf_scan_runs).
+__global__ __launch_bounds__(256) void k_batch_carry_scan_wide(const BatchDesc* descs, const CarryFn* agg,
+                                                               uint64_t* cin, uint32_t nch) {
+  __shared__ CarryFn wtot[4], tot;
+  const uint32_t z = blockIdx.x;
+  const BatchDesc d = descs[z];
+  const uint64_t zL = (uint64_t)d.na + d.nb;
+  const uint32_t nc = (uint32_t)((zL + kCarryBlock - 1) / kCarryBlock);
+  (void)cf_scan_runs(agg + (uint64_t)z * nch, nc, cin + (uint64_t)z * nch, wtot, &tot);
 }
 
 // One thread per (product, chunk): ripple the chunk's carry-in.

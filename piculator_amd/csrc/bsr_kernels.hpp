@@ -358,7 +358,7 @@ __global__ __launch_bounds__(NT) void k_bsr_add(const BsrAdd* adds, BsrVal* meta
 // of every node in its own workgroup: the comparison of equal-length
 // magnitudes (k_bsr_add_cmp), block-local sums with carry-in 0 and the
 // chunk carry functions (k_bsr_add_local), one wave per node scanning them
-// (k_bsr_add_scan), the carry-in ripple (k_bsr_add_fix), the top nonzero
+// (k_bsr_add_scan; k_bsr_add_scan_wide above 64 chunks), the carry-in ripple (k_bsr_add_fix), the top nonzero
 // limb of a difference (k_bsr_add_topnz) and the result length and sign
 // (k_bsr_add_final).  Decisions are recomputed from the metadata in each
 // kernel (bsr_add_plan), so they agree everywhere.
@@ -538,6 +538,20 @@ __global__ __launch_bounds__(64) void k_bsr_add_scan(const BsrAdd* adds, const B
     run = cf_compose(run, tot);
   }
   if (lane == 0) slots[node].carry = cf_eval0(run);
+}
+
+// The same scan with one workgroup per node, for adds of more than 64
+// chunks (cf_scan_runs).
+__global__ __launch_bounds__(256) void k_bsr_add_scan_wide(const BsrAdd* adds, const BsrVal* meta,
+                                                          uint32_t* arena, BsrAddSlot* slots,
+                                                          const CarryFn* agg, uint64_t* cin, uint32_t maxch) {
+  __shared__ CarryFn wtot[4], tot;
+  const uint32_t node = blockIdx.x;
+  const BsrAddPlan pl = bsr_add_plan(adds[node], meta, arena, slots[node].cmp);
+  if (pl.zero) return;  // the same for the whole workgroup
+  const uint32_t nc = (uint32_t)((pl.LA + kCarryBlock - 1) / kCarryBlock);
+  const CarryFn t = cf_scan_runs(agg + (uint64_t)node * maxch, nc, cin + (uint64_t)node * maxch, wtot, &tot);
+  if (threadIdx.x == 0) slots[node].carry = cf_eval0(t);
 }
 
 __global__ __launch_bounds__(256) void k_bsr_add_fix(const BsrAdd* adds, const BsrVal* meta, uint32_t* arena,

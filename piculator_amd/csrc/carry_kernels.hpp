@@ -81,6 +81,48 @@ __device__ __forceinline__ CarryFn cf_block_exclusive(CarryFn f, CarryFn* wave_t
   return cf_compose(pre, excl);
 }
 
+// Exclusive scan of the n chunk functions a[0..n) by one whole workgroup
+// (any n): thread t composes the run [t K, t K + K), K = ceil(n / threads),
+// the runs are scanned across the workgroup, and each thread walks its run
+// again from its prefix, writing out[i] = the carry into chunk i.  Loads go
+// four at a time so a long run is not one memory latency per chunk.
+// Returns the composition of all n (every thread).  wave_tot: LDS of
+// threads / 64 entries; tot: one LDS entry.
+__device__ __forceinline__ CarryFn cf_scan_runs(const CarryFn* a, uint32_t n, uint64_t* out,
+                                                CarryFn* wave_tot, CarryFn* tot) {
+  const uint32_t K = (n + blockDim.x - 1) / blockDim.x;
+  const uint32_t c0 = min(n, (uint32_t)threadIdx.x * K), c1 = min(n, c0 + K);
+  CarryFn f;
+  f.base = 0;
+  f.T = 0;
+  uint32_t c = c0;
+  for (; c + 4 <= c1; c += 4) {
+    const CarryFn g0 = a[c], g1 = a[c + 1], g2 = a[c + 2], g3 = a[c + 3];
+    f = cf_compose(f, cf_compose(cf_compose(g0, g1), cf_compose(g2, g3)));
+  }
+  for (; c < c1; ++c) f = cf_compose(f, a[c]);
+  CarryFn run = cf_block_exclusive(f, wave_tot);
+  for (c = c0; c + 4 <= c1; c += 4) {
+    const CarryFn g0 = a[c], g1 = a[c + 1], g2 = a[c + 2], g3 = a[c + 3];
+    out[c] = cf_eval0(run);
+    run = cf_compose(run, g0);
+    out[c + 1] = cf_eval0(run);
+    run = cf_compose(run, g1);
+    out[c + 2] = cf_eval0(run);
+    run = cf_compose(run, g2);
+    out[c + 3] = cf_eval0(run);
+    run = cf_compose(run, g3);
+  }
+  for (; c < c1; ++c) {
+    out[c] = cf_eval0(run);
+    run = cf_compose(run, a[c]);
+  }
+  // the last thread's run ends at n (an empty run starts there)
+  if (threadIdx.x == blockDim.x - 1) *tot = run;
+  __syncthreads();
+  return *tot;
+}
+
 struct CarryArgs {
   const Prime* prm;  // the two primes (p, pinv)
   const double* r;   // [2][N] inverse-transform outputs
