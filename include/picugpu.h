@@ -140,7 +140,8 @@ typedef struct pcg_bsr_value {
  * integer (exponent 0), and the results equal those of the BigFloat
  * recursion bit for bit.  Returns PCG_ERR_TOO_LARGE when a length bound
  * exceeds max_limbs or a buffer is too small (nothing is written then).
- * pcg_bsr_bound gives the limb bound of P, Q and R for sizing buffers. */
+ * pcg_bsr_bound gives the limb bound of P, Q and R for sizing buffers (from
+ * the bit lengths of the leaf terms at b, which bound every leaf of [a, b)). */
 int pcg_bsr_exact(uint32_t a, uint32_t b, uint64_t max_limbs, pcg_bsr_value* P, pcg_bsr_value* Q,
                   pcg_bsr_value* R);
 int pcg_bsr_bound(uint32_t a, uint32_t b, uint64_t* bound);
@@ -153,12 +154,14 @@ int pcg_bsr_bound(uint32_t a, uint32_t b, uint64_t* bound);
  * PI.h:64-112). */
 int pcg_bsr_prepare(const uint32_t* sizes, uint64_t n);
 /* Self-check, host only (no GPU call, no pcg_init needed): lays out the plan
- * of a subtree of `leaves` leaves as the GPU does (the same per-node code,
- * run on the CPU) with the small-product class table `table` (n entries, as
- * pcg_batch_table) and compares every entry with a direct node-by-node
- * construction.  PCG_OK when they agree, PCG_ERR_INTERNAL naming the first
- * difference otherwise.  No counterpart in the reference. */
-int pcg_bsr_plan_check(uint32_t leaves, const int* table, int n);
+ * of a subtree of `leaves` leaves whose largest leaf index is bmax (the
+ * plans size every value from the leaf bounds at bmax; 0 = any leaf) as the
+ * GPU does (the same per-node code, run on the CPU) with the small-product
+ * class table `table` (n entries, as pcg_batch_table) and compares every
+ * entry with a direct node-by-node construction.  PCG_OK when they agree,
+ * PCG_ERR_INTERNAL naming the first difference otherwise.  No counterpart in
+ * the reference. */
+int pcg_bsr_plan_check(uint32_t leaves, uint32_t bmax, const int* table, int n);
 
 /* ---- Device-resident BigFloat values ----
  * A BigFloat (BigFloat.h:33-36) whose limbs stay in device memory, so the

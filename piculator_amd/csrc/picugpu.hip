@@ -1270,6 +1270,36 @@ int32_t bsr_build(std::vector<BsrNode>& nodes, uint32_t a, uint32_t b) {
 // |P| < (13591409 + 545140134 b) 2^96 < 2^158, Q = b^3 26726400 409297880
 // < 2^150, R < 2^96.
 constexpr uint64_t kLeafBits[3] = {158, 150, 96};
+using LeafBits = std::array<uint64_t, 3>;
+
+// Bit lengths of |P|, Q, R of the leaf b = bmax, which bound those of every
+// leaf b <= bmax while no factor wraps (6 bmax - 1 < 2^32: the values then
+// grow with b); kLeafBits beyond.
+LeafBits leaf_bits_for(uint32_t bmax) {
+  if (bmax == 0 || bmax > 715827882u) return LeafBits{kLeafBits[0], kLeafBits[1], kLeafBits[2]};
+  auto bits_of = [](uint64_t lead, std::initializer_list<uint32_t> fs) {
+    uint32_t z[8] = {(uint32_t)lead, (uint32_t)(lead >> 32), 0, 0, 0, 0, 0, 0};
+    int n = 2;
+    for (uint32_t f : fs) {
+      uint64_t c = 0;
+      for (int i = 0; i < n; ++i) {
+        c += (uint64_t)z[i] * f;
+        z[i] = (uint32_t)c;
+        c >>= 32;
+      }
+      z[n++] = (uint32_t)c;
+    }
+    int top = n - 1;
+    while (top > 0 && z[top] == 0) --top;
+    uint64_t b = 32 * (uint64_t)top;
+    for (uint32_t v = z[top]; v; v >>= 1) ++b;
+    return b;
+  };
+  const uint64_t b = bmax;
+  const uint32_t f1 = 2 * bmax - 1, f2 = 6 * bmax - 5, f3 = 6 * bmax - 1;
+  return LeafBits{bits_of(13591409ull + 545140134ull * b, {f1, f2, f3}),
+                  bits_of(b * b, {bmax, 26726400u, 409297880u}), bits_of(f1, {f2, f3})};
+}
 
 uint64_t limbs_of_bits(uint64_t bits) { return (bits + 31) / 32; }
 
@@ -1279,7 +1309,7 @@ uint64_t limbs_of_bits(uint64_t bits) { return (bits + 31) / 32; }
 // a product's one-limb trim and add/sub's carry limb and strip leave exactly
 // the value's length), slot = limbs written before that (a product's
 // na + nb, an addition's longer operand + 1).  Returns the largest tlen.
-uint64_t bsr_bounds(const std::vector<BsrNode>& nodes, std::vector<uint64_t>& tlen,
+uint64_t bsr_bounds(const std::vector<BsrNode>& nodes, const LeafBits& leaf, std::vector<uint64_t>& tlen,
                     std::vector<uint64_t>& slot) {
   std::vector<uint64_t> bits(nodes.size() * 5, 0);
   tlen.assign(nodes.size() * 5, 0);
@@ -1289,7 +1319,7 @@ uint64_t bsr_bounds(const std::vector<BsrNode>& nodes, std::vector<uint64_t>& tl
     const BsrNode& n = nodes[i];
     const size_t v = i * 5;  // P Q R x y
     if (n.left < 0) {
-      for (int j = 0; j < 3; ++j) bits[v + j] = kLeafBits[j];
+      for (int j = 0; j < 3; ++j) bits[v + j] = leaf[j];
     } else {
       const size_t l = (size_t)n.left * 5, r = (size_t)n.right * 5;
       bits[v + 3] = bits[l + 0] + bits[r + 1];
@@ -1312,34 +1342,6 @@ uint64_t bsr_bounds(const std::vector<BsrNode>& nodes, std::vector<uint64_t>& tl
     for (int j = 0; j < 5; ++j) mx = std::max(mx, tlen[v + j]);
   }
   return mx;
-}
-
-// The largest value length (tlen) of a subtree of d leaves, by recursion on
-// the sizes (left d / 2, right d - d / 2): O(log d) distinct sizes, memoised.
-struct BsrRootBound {
-  uint64_t P, Q, R, mx;  // bits of the root's P, Q, R; largest tlen
-};
-BsrRootBound bsr_bound_of(uint32_t d) {
-  static std::mutex mu;
-  static std::map<uint32_t, BsrRootBound> memo;
-  if (d == 1)
-    return BsrRootBound{kLeafBits[0], kLeafBits[1], kLeafBits[2],
-                        limbs_of_bits(std::max({kLeafBits[0], kLeafBits[1], kLeafBits[2]}))};
-  {
-    std::lock_guard<std::mutex> lk(mu);
-    auto it = memo.find(d);
-    if (it != memo.end()) return it->second;
-  }
-  const BsrRootBound l = bsr_bound_of(d / 2), r = bsr_bound_of(d - d / 2);
-  BsrRootBound v;
-  const uint64_t x = l.P + r.Q, y = r.P + l.R;
-  v.P = std::max(x, y) + 1;
-  v.Q = l.Q + r.Q;
-  v.R = l.R + r.R;
-  v.mx = std::max({l.mx, r.mx, limbs_of_bits(v.P), limbs_of_bits(v.Q), limbs_of_bits(v.R)});
-  std::lock_guard<std::mutex> lk(mu);
-  memo[d] = v;
-  return v;
 }
 
 template <class T>
@@ -1365,15 +1367,18 @@ struct BsrHeight {
 };
 struct BsrPlan {
   uint32_t d = 0;
+  LeafBits leaf{};  // leaf bounds the plan was sized with
   uint64_t maxb = 0, arena_limbs = 0;
   uint64_t root_bound[3] = {0, 0, 0};
   uint64_t root_off[3] = {0, 0, 0};
   int32_t H = 0;
   std::vector<BsrHeight> hp;
   uint64_t nleaves = 0;
-  // device
+  // device (buffers kept for the next plan laid out in this slot)
   void* buf = nullptr;
   uint32_t* arena = nullptr;
+  size_t buf_cap = 0;       // bytes
+  uint64_t arena_cap = 0;   // limbs
   BsrVal* meta = nullptr;
   uint32_t* leaf_rel = nullptr;  // b - a of each leaf
   uint32_t* leaf_ids = nullptr;
@@ -1408,14 +1413,14 @@ struct BsrHostPlan {
 
 using SmallTable = std::array<int, kSmallNttMaxLog + 1>;
 
-void bsr_host_plan(uint32_t d, const SmallTable& tab, BsrHostPlan& HP) {
+void bsr_host_plan(uint32_t d, const LeafBits& leaf, const SmallTable& tab, BsrHostPlan& HP) {
   BsrPlan& P = HP.P;
   std::vector<BsrNode> nodes;
   nodes.reserve(2 * (size_t)d);
   bsr_build(nodes, 0, d);
   std::vector<uint64_t> tlen, slot;
   P.d = d;
-  P.maxb = bsr_bounds(nodes, tlen, slot);
+  P.maxb = bsr_bounds(nodes, leaf, tlen, slot);
   const size_t nv = nodes.size() * 5;
   std::vector<BsrVal>& meta = HP.meta;
   meta.resize(nv);
@@ -1536,6 +1541,7 @@ void bsr_host_plan(uint32_t d, const SmallTable& tab, BsrHostPlan& HP) {
 // subtree size, children before parents, with the value bounds of
 // bsr_bounds per size, and per height the product groups and adds.
 struct BsrSizes {
+  LeafBits leaf{};
   std::vector<BsrSizeInfo> info;
   std::vector<std::array<uint64_t, 5>> bits, tlen, slot;
   std::vector<int32_t> height;
@@ -1555,7 +1561,7 @@ int32_t bsr_size_index(BsrSizes& S, uint64_t s) {
   }
   std::array<uint64_t, 5> bits{}, tl{}, sl{};
   if (l < 0) {
-    for (int j = 0; j < 3; ++j) bits[j] = kLeafBits[j];
+    for (int j = 0; j < 3; ++j) bits[j] = S.leaf[j];
   } else {
     const auto &bl = S.bits[l], &br = S.bits[r];
     bits[3] = bl[0] + br[1];
@@ -1599,11 +1605,13 @@ int32_t bsr_size_index(BsrSizes& S, uint64_t s) {
 // heights (launch groups, adds) and S, whose info and cnt are what
 // bsr_layout_node reads.  Product groups follow bsr_key and class_pairs
 // with the class table tab, in the order of the direct construction.
-int bsr_plan_sizes(uint32_t d, const SmallTable& tab, BsrPlan& P, BsrSizes& S) {
+int bsr_plan_sizes(uint32_t d, const LeafBits& leaf, const SmallTable& tab, BsrPlan& P, BsrSizes& S) {
   P = BsrPlan();
   S = BsrSizes();
   if (d == 0) return fail(PCG_ERR_INVALID, "empty range");
   if (d >= kBsrPairBit) return fail(PCG_ERR_TOO_LARGE, "subtree of %u leaves", d);
+  S.leaf = leaf;
+  P.leaf = leaf;
   S.root = bsr_size_index(S, d);
   const size_t ns = S.info.size();
   S.cnt.assign(ns * ns, 0);
@@ -1706,6 +1714,18 @@ int bsr_plan_sizes(uint32_t d, const SmallTable& tab, BsrPlan& P, BsrSizes& S) {
   return PCG_OK;
 }
 
+// The largest value length (tlen) of a subtree of d leaves whose leaves are
+// bounded by leaf: over the size table, O(log d) sizes.
+uint64_t bsr_max_limbs(uint32_t d, const LeafBits& leaf) {
+  BsrSizes S;
+  S.leaf = leaf;
+  bsr_size_index(S, d);
+  uint64_t mx = 0;
+  for (const auto& t : S.tlen)
+    for (uint64_t v : t) mx = std::max(mx, v);
+  return mx;
+}
+
 BsrLayout bsr_layout_of(const BsrSizes& S, uint32_t d) {
   BsrLayout L;
   memset(&L, 0, sizeof L);
@@ -1719,11 +1739,12 @@ BsrLayout bsr_layout_of(const BsrSizes& S, uint32_t d) {
   return L;
 }
 
-// Device side: allocates the plan's buffers and arena and lays the plan out
-// with k_bsr_layout (one thread per node); synchronises to check it.
-int bsr_plan_build(Ctx& c, uint32_t d, const SmallTable& tab, BsrPlan& P) {
+// Device side: lays the plan out with k_bsr_layout (one thread per node)
+// into the slot's buffers, grown when too small; synchronises to check it.
+int bsr_plan_build(Ctx& c, uint32_t d, const LeafBits& leaf, const SmallTable& tab, BsrPlan& P) {
   BsrSizes S;
-  int rc = bsr_plan_sizes(d, tab, P, S);
+  BsrPlan NP;
+  int rc = bsr_plan_sizes(d, leaf, tab, NP, S);
   if (rc) return rc;
   BsrLayout L = bsr_layout_of(S, d);
   const size_t ns = S.info.size();
@@ -1732,13 +1753,35 @@ int bsr_plan_build(Ctx& c, uint32_t d, const SmallTable& tab, BsrPlan& P) {
                       S.max_large * sizeof(BsrAddSlot) +
                       S.max_large_chunks * (sizeof(CarryFn) + sizeof(uint64_t)) +
                       ns * sizeof(BsrSizeInfo) + ns * ns * 4 + 13 * 256;
-  if (hipMalloc(&P.buf, need) != hipSuccess || hipMalloc(&P.arena, P.arena_limbs * 4) != hipSuccess) {
+  HIPCHK(hipStreamSynchronize(c.stream));  // the slot's old plan: last use
+  NP.buf = P.buf;
+  NP.arena = P.arena;
+  NP.buf_cap = P.buf_cap;
+  NP.arena_cap = P.arena_cap;
+  P = BsrPlan();  // the slot holds no plan until this one is laid out
+  bool ok = true;
+  if (NP.buf_cap < need) {
+    (void)hipFree(NP.buf);
+    NP.buf = nullptr;
+    NP.buf_cap = 0;
+    ok = hipMalloc(&NP.buf, need) == hipSuccess;
+    if (ok) NP.buf_cap = need;
+  }
+  if (ok && NP.arena_cap < NP.arena_limbs) {
+    (void)hipFree(NP.arena);
+    NP.arena = nullptr;
+    NP.arena_cap = 0;
+    ok = hipMalloc(&NP.arena, NP.arena_limbs * 4) == hipSuccess;
+    if (ok) NP.arena_cap = NP.arena_limbs;
+  }
+  if (!ok) {
     (void)hipGetLastError();
-    const uint64_t al = P.arena_limbs;
-    bsr_plan_free(P);
+    const uint64_t al = NP.arena_limbs;
+    bsr_plan_free(NP);
     return fail(PCG_ERR_OOM, "cannot allocate the binary-splitting plan (%llu arena limbs)",
                 (unsigned long long)al);
   }
+  P = std::move(NP);
   char* p = static_cast<char*>(P.buf);
   P.meta = carve<BsrVal>(p, L.nmeta);
   P.leaf_rel = carve<uint32_t>(p, L.nleaf);
@@ -1778,10 +1821,10 @@ int bsr_plan_build(Ctx& c, uint32_t d, const SmallTable& tab, BsrPlan& P) {
 
 // The layout of bsr_plan_build run on the host, compared entry by entry with
 // the direct construction (bsr_host_plan).  Host only.
-int bsr_plan_check(uint32_t d, const SmallTable& tab) {
+int bsr_plan_check(uint32_t d, const LeafBits& leaf, const SmallTable& tab) {
   BsrPlan P;
   BsrSizes S;
-  int rc = bsr_plan_sizes(d, tab, P, S);
+  int rc = bsr_plan_sizes(d, leaf, tab, P, S);
   if (rc) return rc;
   BsrLayout L = bsr_layout_of(S, d);
   std::vector<BsrVal> meta(L.nmeta);
@@ -1798,7 +1841,7 @@ int bsr_plan_check(uint32_t d, const SmallTable& tab) {
   for (uint64_t t = 0; t < L.nodes; ++t)
     if (!bsr_layout_node(L, t)) return fail(PCG_ERR_INTERNAL, "d %u: node %llu outside its lists", d, (unsigned long long)t);
   BsrHostPlan H;
-  bsr_host_plan(d, tab, H);
+  bsr_host_plan(d, leaf, tab, H);
   const BsrPlan& Q = H.P;
   auto differ = [&](const char* what) { return fail(PCG_ERR_INTERNAL, "d %u: %s differs", d, what); };
   if (P.maxb != Q.maxb || P.arena_limbs != Q.arena_limbs || P.H != Q.H || P.nleaves != Q.nleaves)
@@ -1842,19 +1885,17 @@ SmallTable small_table() {
   return t;
 }
 
-// The cached plan of a subtree of d leaves, built if needed (evicting the
-// older of the two).
-int bsr_plan_get(Ctx& c, uint32_t d, BsrPlan** out) {
+// The cached plan of a subtree of d leaves with leaf bounds leaf, laid out
+// if needed in the older of the two slots (reusing its buffers).
+int bsr_plan_get(Ctx& c, uint32_t d, const LeafBits& leaf, BsrPlan** out) {
   for (BsrPlan& q : g_bsr_plans)
-    if (q.d == d && q.buf) {
+    if (q.d == d && q.leaf == leaf && q.meta) {
       *out = &q;
       return PCG_OK;
     }
   BsrPlan* P = &g_bsr_plans[g_bsr_next];
   g_bsr_next ^= 1;
-  HIPCHK(hipStreamSynchronize(c.stream));
-  bsr_plan_free(*P);
-  const int rc = bsr_plan_build(c, d, small_table(), *P);
+  const int rc = bsr_plan_build(c, d, leaf, small_table(), *P);
   if (rc) return rc;
   *out = P;
   return PCG_OK;
@@ -1869,7 +1910,7 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
   if (rct) return rct;
   const uint32_t d = b - a;
   BsrPlan* P = nullptr;
-  int rc = bsr_plan_get(c, d, &P);
+  int rc = bsr_plan_get(c, d, leaf_bits_for(b), &P);  // the leaves are a + 1 .. b
   if (rc) return rc;
   if (P->maxb > max_limbs)
     return fail(PCG_ERR_TOO_LARGE, "subtree values may reach %llu limbs > %llu",
@@ -2616,7 +2657,7 @@ int pcg_bsr_exact(uint32_t a, uint32_t b, uint64_t max_limbs, pcg_bsr_value* P, 
 
 int pcg_bsr_bound(uint32_t a, uint32_t b, uint64_t* bound) {
   if (!bound || b <= a) return fail(PCG_ERR_INVALID, "bad arguments");
-  *bound = bsr_bound_of(b - a).mx;
+  *bound = bsr_max_limbs(b - a, leaf_bits_for(b));
   return PCG_OK;
 }
 
@@ -2821,22 +2862,24 @@ int pcg_bsr_prepare(const uint32_t* sizes, uint64_t n) {
   std::vector<uint32_t> want;
   for (uint64_t i = 0; i < n && want.size() < 2; ++i)
     if (sizes[i] && std::find(want.begin(), want.end(), sizes[i]) == want.end()) want.push_back(sizes[i]);
+  // laid out with the bounds of any leaf, so their buffers hold every later
+  // plan of these sizes
   for (uint32_t d : want) {
     BsrPlan* P = nullptr;
-    rc = bsr_plan_get(c, d, &P);
+    rc = bsr_plan_get(c, d, leaf_bits_for(0), &P);
     if (rc) return rc;
   }
   return PCG_OK;
 }
 
-int pcg_bsr_plan_check(uint32_t leaves, const int* table, int n) {
+int pcg_bsr_plan_check(uint32_t leaves, uint32_t bmax, const int* table, int n) {
   if (!table || n != kSmallNttMaxLog + 1) return fail(PCG_ERR_INVALID, "the class table has %d entries", kSmallNttMaxLog + 1);
   SmallTable tab;
   for (int b = 0; b < n; ++b) {
     if (!small_class_valid(b, table[b])) return fail(PCG_ERR_INVALID, "class %d is not valid for bucket %d", table[b], b);
     tab[b] = table[b];
   }
-  return bsr_plan_check(leaves, tab);
+  return bsr_plan_check(leaves, leaf_bits_for(bmax), tab);
 }
 
 int pcg_autotune(int* table, int n) {

@@ -99,7 +99,7 @@ def load():
     L.pcg_bsr_exact.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64] + [C.POINTER(BsrValue)] * 3
     L.pcg_bsr_bound.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
     L.pcg_bsr_prepare.argtypes = [C.POINTER(C.c_uint32), C.c_uint64]
-    L.pcg_bsr_plan_check.argtypes = [C.c_uint32, C.POINTER(C.c_int), C.c_int]
+    L.pcg_bsr_plan_check.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_int), C.c_int]
     L.pcg_autotune.argtypes = [C.POINTER(C.c_int), C.c_int]
     L.pcg_batch_table.argtypes = [C.POINTER(C.c_int), C.c_int, C.c_int]
     dfp = C.POINTER(DFloat)
@@ -284,12 +284,12 @@ def bsr_prepare(sizes):
     _check(load().pcg_bsr_prepare(arr, len(sizes)))
 
 
-def bsr_plan_check(leaves: int, table) -> None:
-    """Host-only self-check of the subtree plan layout (no GPU needed):
-    raises PcgError naming the first entry that differs from the direct
-    node-by-node construction."""
+def bsr_plan_check(leaves: int, table, bmax: int = 0) -> None:
+    """Host-only self-check of the subtree plan layout (no GPU needed) for
+    leaves up to index bmax (0: any): raises PcgError naming the first entry
+    that differs from the direct node-by-node construction."""
     arr = (C.c_int * len(table))(*table)
-    _check(load().pcg_bsr_plan_check(leaves, arr, len(table)))
+    _check(load().pcg_bsr_plan_check(leaves, bmax, arr, len(table)))
 
 
 def bsr_exact(a: int, b: int, max_limbs: int):

@@ -44,12 +44,21 @@ def test_plan_limits():
         picugpu.ntt_plan(2 ** 31)
 
 
-def _tree_bound(d: int) -> int:
-    """Largest value length (limbs) of a subtree of d leaves, node by node over
+def _leaf_bits(b: int):
+    """Bit lengths of |P|, Q, R of the leaf b (PI.h:67-88, uint32 factors)."""
+    m32 = (1 << 32) - 1
+    f1, f2, f3 = (2 * b - 1) & m32, (6 * b - 5) & m32, (6 * b - 1) & m32
+    P = (13591409 + 545140134 * b) * f1 * f2 * f3
+    Q = b ** 3 * 26726400 * 409297880
+    return P.bit_length(), Q.bit_length(), (f1 * f2 * f3).bit_length()
+
+
+def _tree_bound(a: int, b: int, leaf) -> int:
+    """Largest value length (limbs) of the subtree [a, b), node by node over
     the explicit tree (midpoints m = (a + b) / 2 of PI.h:90), from bit bounds:
-    leaves P < 2^158, Q < 2^150, R < 2^96 for any b < 2^32 (the reference's
-    uint32 factors), x = P_L Q_R and y = P_R R_L add bits, P = max(x, y) + 1,
-    Q and R add; a value's length is ceil(bits / 32)."""
+    every leaf's P, Q, R below 2^leaf[0], 2^leaf[1], 2^leaf[2], x = P_L Q_R
+    and y = P_R R_L add bits, P = max(x, y) + 1, Q and R add; a value's
+    length is ceil(bits / 32)."""
     mx = 0
 
     def limbs(bits):
@@ -58,8 +67,8 @@ def _tree_bound(d: int) -> int:
     def rec(a, b):
         nonlocal mx
         if b - a == 1:
-            mx = max(mx, limbs(158))
-            return 158, 150, 96
+            mx = max(mx, limbs(max(leaf)))
+            return leaf
         m = (a + b) // 2
         lp, lq, lr = rec(a, m)
         rp, rq, rr = rec(m, b)
@@ -68,7 +77,7 @@ def _tree_bound(d: int) -> int:
         mx = max(mx, limbs(x), limbs(y), *(limbs(t) for t in v))
         return v
 
-    rec(1000, 1000 + d)
+    rec(a, b)
     return mx
 
 
@@ -84,11 +93,27 @@ def test_leaf_bit_bounds():
         assert P.bit_length() <= 158 and Q.bit_length() <= 150 and R.bit_length() <= 96
 
 
+def test_leaf_bits_grow_with_b():
+    """Below the wrap of 6b - 1 (b <= 715827882) every leaf term grows with
+    b, so the bits of the leaf b bound every leaf of [a, b); checked leaf by
+    leaf on ranges up to the wrap."""
+    for a, b in ((0, 3000), (70510000, 70513670), (715825000, 715827882)):
+        top = _leaf_bits(b)
+        for k in range(a + 1, b + 1):
+            assert all(x <= y for x, y in zip(_leaf_bits(k), top)), k
+
+
 def test_bsr_bound_matches_tree_walk():
+    """pcg_bsr_bound: the tree walk from the bits of the leaf b, and from the
+    bounds of any leaf (158, 150, 96) past the wrap."""
     from piculator_amd import picugpu
 
     for d in (1, 2, 3, 5, 64, 100, 1000, 4097, 70515):
-        assert picugpu.bsr_bound(1000, 1000 + d) == _tree_bound(d), d
+        assert picugpu.bsr_bound(1000, 1000 + d) == _tree_bound(1000, 1000 + d, _leaf_bits(1000 + d)), d
+    for a, d in ((70000000, 4097), (715827000, 882), (715827000, 883), (2000000000, 1000)):
+        b = a + d
+        leaf = _leaf_bits(b) if b <= 715827882 else (158, 150, 96)
+        assert picugpu.bsr_bound(a, b) == _tree_bound(a, b, leaf), (a, b)
 
 
 # Small-product class tables (pcg_batch_table layout, buckets 0..12): the
@@ -115,15 +140,20 @@ def test_bsr_plan_layout_matches_direct_construction():
     for t in _PLAN_TABLES:
         for d in sizes:
             picugpu.bsr_plan_check(d, t)
+    # plans sized from the leaf bounds at the largest leaf index
+    for bmax in (300, 2203553, 70513670, 715827882):
+        for d in (1, 2, 3, 7, 100, 300, 4097):
+            picugpu.bsr_plan_check(d, _PLAN_TABLES[0], bmax)
 
 
 def test_bsr_plan_layout_at_pi_subtree_sizes():
     """The two subtree sizes of pi to 10^9 digits at depth 5 (70,513,670
-    terms / 32) with the default table."""
+    terms / 32) with the default table, for any leaf and for the last one."""
     from piculator_amd import picugpu
 
     for d in (2203552, 2203553):
         picugpu.bsr_plan_check(d, _PLAN_TABLES[0])
+        picugpu.bsr_plan_check(d, _PLAN_TABLES[0], 70513670)
 
 
 def test_bsr_plan_check_rejects_bad_tables():
