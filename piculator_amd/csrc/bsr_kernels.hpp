@@ -22,11 +22,12 @@ struct BsrVal {
 };
 
 // ------------------------------------------------------------------ leaves
-// Leaves and the nodes of two leaves, one thread each, in fixed-size limb
-// arrays (registers): a leaf's P < 2^158, Q < 2^150, R < 2^96 take 5, 5, 3
-// limbs, so a two-leaf node's products take at most 10 limbs and its
-// P = x + y at most 11.  A two-leaf node is computed from its leaves
-// directly, so neither its leaves nor its products pass through memory.
+// Leaves and the nodes of up to four leaves, one thread each, in fixed-size
+// limb arrays (registers): a leaf's P < 2^158, Q < 2^150, R < 2^96 take 5,
+// 5, 3 limbs, a two-leaf node's P, Q, R at most 11, 10, 6 and a node of
+// three or four leaves' 22, 20, 12.  Such a node is computed from its leaves
+// directly, so neither its leaves nor the products below it pass through
+// memory.
 // Values are normalised (no leading zero limb; zero has L = 0 and sign 1),
 // as BigFloat leaves them: the product's one-limb trim and usub's strip
 // (BigFloat.cpp:411-412, 277-284) give exactly the significant limbs.
@@ -123,66 +124,138 @@ __host__ __device__ __forceinline__ void bsr_put(BsrVal* meta, uint32_t id, uint
   m.sign = L ? sign : 1;
 }
 
-// One entry per leaf of the subtree (rel, ids as the plan lays them out):
-// rel = b - a of the leaf, with bit 31 set when the leaf is the left child
-// of a two-leaf node, whose P, Q, R (value ids in ids) this thread
-// computes; ids[3i] = 0xFFFFFFFF for the right child of such a node
-// (nothing to do); otherwise the leaf's own P, Q, R.
-constexpr uint32_t kBsrPairBit = 0x80000000u;
-constexpr uint32_t kBsrSkip = 0xFFFFFFFFu;
+// P, Q, R of a node in fixed-size arrays (zero above the value's limbs).
+template <int NP, int NQ, int NR>
+struct NodeVals {
+  uint32_t P[NP], Q[NQ], R[NR];
+  int32_t sP;  // sign of P: 1 positive (or zero), 0 negative
+};
 
-__host__ __device__ inline void bsr_base_entry(const uint32_t* rel, uint32_t a, const uint32_t* ids,
-                                               uint64_t i, BsrVal* meta, uint32_t* arena) {
-  const uint32_t id0 = ids[3 * i];
-  if (id0 == kBsrSkip) return;
-  const uint32_t rv = rel[i];
-  const uint32_t b = a + (rv & ~kBsrPairBit);
-  if (!(rv & kBsrPairBit)) {
-    const LeafVals v = bsr_leaf(b);
-    bsr_put<5>(meta, id0, arena, v.P, v.sP);
-    bsr_put<5>(meta, ids[3 * i + 1], arena, v.Q, 1);
-    bsr_put<3>(meta, ids[3 * i + 2], arena, v.R, 1);
-    return;
-  }
-  // node [b - 1, b + 1): x = P0 Q1, Q = Q0 Q1, y = P1 R0, R = R0 R1,
-  // P = x + y (PI.h:95-112)
-  const LeafVals l = bsr_leaf(b), r = bsr_leaf(b + 1);
-  uint32_t x[11], y[10], Q[10], R[6];
-  fx_mul<5, 5>(l.P, r.Q, x);
-  fx_mul<5, 5>(l.Q, r.Q, Q);
-  fx_mul<5, 3>(r.P, l.R, y);
-  fx_mul<3, 3>(l.R, r.R, R);
-  x[10] = 0;
-  y[8] = y[9] = 0;
-  uint32_t Pz[11];
-  int32_t sign = l.sP;  // x has P0's sign, y has P1's (Q and R are positive)
+// The node combine of PI.h:95-112 on fixed-size values: x = P0 Q1,
+// Q = Q0 Q1, y = P1 R0, R = R0 R1, P = x + y (x has P0's sign and y has
+// P1's: Q and R are positive).  z.P needs NPL + NQR + 1 >= NPR + NRL + 1
+// limbs.
+template <int NPL, int NQL, int NRL, int NPR, int NQR, int NRR>
+__host__ __device__ __forceinline__ void bsr_combine(const NodeVals<NPL, NQL, NRL>& l,
+                                                     const NodeVals<NPR, NQR, NRR>& r,
+                                                     NodeVals<NPL + NQR + 1, NQL + NQR, NRL + NRR>& z) {
+  constexpr int NX = NPL + NQR, NY = NPR + NRL;
+  static_assert(NY <= NX, "y is the shorter product");
+  uint32_t x[NX], y[NX];
+  fx_mul<NPL, NQR>(l.P, r.Q, x);
+  fx_mul<NQL, NQR>(l.Q, r.Q, z.Q);
+  fx_mul<NPR, NRL>(r.P, l.R, y);
+#pragma unroll
+  for (int k = NY; k < NX; ++k) y[k] = 0;
+  fx_mul<NRL, NRR>(l.R, r.R, z.R);
+  int32_t sign = l.sP;
   if (l.sP == r.sP) {
     uint64_t c = 0;
 #pragma unroll
-    for (int k = 0; k < 11; ++k) {
-      c += (uint64_t)x[k] + (k < 10 ? y[k] : 0u);
-      Pz[k] = (uint32_t)c;
+    for (int k = 0; k < NX; ++k) {
+      c += (uint64_t)x[k] + y[k];
+      z.P[k] = (uint32_t)c;
       c >>= 32;
     }
+    z.P[NX] = (uint32_t)c;
   } else {
     // |x| - |y| or |y| - |x|, the larger magnitude's sign
     int cmp = 0;
 #pragma unroll
-    for (int k = 9; k >= 0; --k)
+    for (int k = NX - 1; k >= 0; --k)
       if (cmp == 0 && x[k] != y[k]) cmp = x[k] > y[k] ? 1 : -1;
     int64_t br = 0;
 #pragma unroll
-    for (int k = 0; k < 10; ++k) {
+    for (int k = 0; k < NX; ++k) {
       const int64_t d = cmp >= 0 ? (int64_t)x[k] - y[k] + br : (int64_t)y[k] - x[k] + br;
-      Pz[k] = (uint32_t)d;
+      z.P[k] = (uint32_t)d;
       br = d < 0 ? -1 : 0;
     }
-    Pz[10] = 0;
+    z.P[NX] = 0;
     if (cmp < 0) sign = r.sP;
   }
-  bsr_put<11>(meta, id0, arena, Pz, sign);
-  bsr_put<10>(meta, ids[3 * i + 1], arena, Q, 1);
-  bsr_put<6>(meta, ids[3 * i + 2], arena, R, 1);
+  z.sP = sign;
+}
+
+using LeafNode = NodeVals<5, 5, 3>;
+using PairNode = NodeVals<11, 10, 6>;
+using QuadNode = NodeVals<22, 20, 12>;
+
+__host__ __device__ __forceinline__ LeafNode bsr_leaf_node(uint32_t b) {
+  const LeafVals v = bsr_leaf(b);
+  LeafNode n;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    n.P[i] = v.P[i];
+    n.Q[i] = v.Q[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) n.R[i] = v.R[i];
+  n.sP = v.sP;
+  return n;
+}
+
+// The node of the leaves b and b + 1.
+__host__ __device__ __forceinline__ PairNode bsr_pair_node(uint32_t b) {
+  PairNode z;
+  bsr_combine(bsr_leaf_node(b), bsr_leaf_node(b + 1), z);
+  return z;
+}
+
+template <int NP, int NQ, int NR>
+__host__ __device__ __forceinline__ void bsr_put_node(BsrVal* meta, const uint32_t* id, uint32_t* arena,
+                                                      const NodeVals<NP, NQ, NR>& v) {
+  bsr_put<NP>(meta, id[0], arena, v.P, v.sP);
+  bsr_put<NQ>(meta, id[1], arena, v.Q, 1);
+  bsr_put<NR>(meta, id[2], arena, v.R, 1);
+}
+
+// One entry per leaf of the subtree (rel, ids as the plan lays them out):
+// rel = b - a of the leaf, plus kBsrPairBit when this leaf is the left leaf
+// of a two-leaf node computed here (one whose parent has 5 leaves, or the
+// root), or kBsrQuadBit (and kBsrFourBit for 4 leaves) when it is the first
+// leaf of a node of 3 or 4 leaves (height 3: a leaf and a two-leaf node, or
+// two two-leaf nodes) computed here; ids = the value ids of P, Q, R of the
+// leaf or node, or kBsrSkip for a leaf that another entry covers.
+constexpr uint32_t kBsrPairBit = 0x80000000u;
+constexpr uint32_t kBsrQuadBit = 0x40000000u;
+constexpr uint32_t kBsrFourBit = 0x20000000u;
+constexpr uint32_t kBsrRelMask = 0x1FFFFFFFu;
+constexpr uint32_t kBsrSkip = 0xFFFFFFFFu;
+
+__host__ __device__ inline void bsr_base_entry(const uint32_t* rel, uint32_t a, const uint32_t* ids,
+                                               uint64_t i, BsrVal* meta, uint32_t* arena) {
+  const uint32_t* id = ids + 3 * i;
+  if (id[0] == kBsrSkip) return;
+  const uint32_t rv = rel[i];
+  const uint32_t b = a + (rv & kBsrRelMask);  // the entry's first leaf
+  if (rv & kBsrQuadBit) {
+    // [b - 1, b + 3) as (b, b+1) + (b+2, b+3), or [b - 1, b + 2) as
+    // b + (b+1, b+2); a lone left leaf takes the pair-size arrays (zeros
+    // above), so both shapes run the same code
+    PairNode l, r;
+    if (rv & kBsrFourBit) {
+      l = bsr_pair_node(b);
+      r = bsr_pair_node(b + 2);
+    } else {
+      const LeafNode v = bsr_leaf_node(b);
+#pragma unroll
+      for (int k = 0; k < 11; ++k) l.P[k] = k < 5 ? v.P[k] : 0u;
+#pragma unroll
+      for (int k = 0; k < 10; ++k) l.Q[k] = k < 5 ? v.Q[k] : 0u;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) l.R[k] = k < 3 ? v.R[k] : 0u;
+      l.sP = v.sP;
+      r = bsr_pair_node(b + 1);
+    }
+    QuadNode z;
+    bsr_combine(l, r, z);
+    bsr_put_node(meta, id, arena, z);
+  } else if (rv & kBsrPairBit) {
+    bsr_put_node(meta, id, arena, bsr_pair_node(b));
+  } else {
+    bsr_put_node(meta, id, arena, bsr_leaf_node(b));
+  }
 }
 
 __global__ void k_bsr_base(const uint32_t* rel, uint32_t a, const uint32_t* ids, uint64_t n,
@@ -663,12 +736,17 @@ __host__ __device__ inline bool bsr_layout_node(const BsrLayout& L, uint64_t t) 
   // walk 1: the node's size, range start and arena offset
   int32_t si = L.root;
   uint64_t id = 0, a = 0, base = 0;
-  uint64_t pid = 0, ps = 0;  // parent's id and size
+  uint64_t pid = 0, ps = 0, pa = 0;  // parent's id, size and first leaf
+  uint64_t gid = 0, gs = 0, ga = 0;  // grandparent's (size 0: none)
   while (id != t) {
     const BsrSizeInfo& n = I[si];
     const uint64_t sl = I[n.l].s;
+    gid = pid;
+    gs = ps;
+    ga = pa;
     pid = id;
     ps = n.s;
+    pa = a;
     if (t < id + 2 * sl) {
       id += 1;
       base += n.own;
@@ -694,12 +772,25 @@ __host__ __device__ inline bool bsr_layout_node(const BsrLayout& L, uint64_t t) 
   }
   if (me.l < 0) {  // leaf [a, a + 1): an entry of k_bsr_base
     if (a < L.nleaf) {
+      // a leaf's parent has 2 or 3 leaves, a two-leaf node's 3, 4 or 5
       uint32_t rel = (uint32_t)(a + 1), id0 = (uint32_t)(5 * t);
-      if (ps == 2 && t == pid + 1) {  // left leaf of a two-leaf node: the node
-        rel |= kBsrPairBit;
+      if (ps == 3) {  // the lone left leaf of a three-leaf node: the node
+        rel |= kBsrQuadBit;
         id0 = (uint32_t)(5 * pid);
-      } else if (ps == 2) {  // right leaf of one: nothing
-        id0 = kBsrSkip;
+      } else if (ps == 2 && (gs == 3 || gs == 4)) {  // under a node of height 3
+        if (a == ga) {
+          rel |= kBsrQuadBit | (gs == 4 ? kBsrFourBit : 0u);
+          id0 = (uint32_t)(5 * gid);
+        } else {
+          id0 = kBsrSkip;
+        }
+      } else if (ps == 2) {  // a two-leaf node under a five-leaf one, or the root
+        if (t == pid + 1) {
+          rel |= kBsrPairBit;
+          id0 = (uint32_t)(5 * pid);
+        } else {
+          id0 = kBsrSkip;
+        }
       }
       L.leaf_rel[a] = rel;
       for (int j = 0; j < 3; ++j) L.leaf_ids[3 * a + j] = id0 == kBsrSkip ? kBsrSkip : id0 + j;

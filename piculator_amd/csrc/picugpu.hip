@@ -1446,16 +1446,25 @@ void bsr_host_plan(uint32_t d, const LeafBits& leaf, const SmallTable& tab, BsrH
   for (int32_t i = 0; i < (int32_t)nodes.size(); ++i) {
     by_h[nodes[i].height].push_back(i);
     if (nodes[i].left < 0) {
-      // k_bsr_base entries: a two-leaf node through its left leaf
+      // k_bsr_base entries: a node of height 3 through its first leaf, a
+      // two-leaf node outside one through its left leaf, a lone root leaf
       const int32_t q = parent[i];
-      const bool pair = q >= 0 && nodes[q].b - nodes[q].a == 2;
-      if (pair && nodes[q].left == i) {
-        leaf_rel.push_back(nodes[i].b | kBsrPairBit);
-        for (int j = 0; j < 3; ++j) leaf_ids.push_back((uint32_t)(5 * q + j));
-      } else {
-        leaf_rel.push_back(nodes[i].b);
-        for (int j = 0; j < 3; ++j) leaf_ids.push_back(pair ? kBsrSkip : (uint32_t)(5 * i + j));
+      const int32_t g = q >= 0 ? parent[q] : -1;
+      const uint32_t qs = q >= 0 ? nodes[q].b - nodes[q].a : 0, gs = g >= 0 ? nodes[g].b - nodes[g].a : 0;
+      int32_t owner = i;  // node whose values the entry computes, -1: none
+      uint32_t rel = nodes[i].b;
+      if (qs == 3) {
+        owner = q;
+        rel |= kBsrQuadBit;
+      } else if (qs == 2 && (gs == 3 || gs == 4)) {
+        owner = nodes[i].a == nodes[g].a ? g : -1;
+        rel |= kBsrQuadBit | (gs == 4 ? kBsrFourBit : 0u);
+      } else if (qs == 2) {
+        owner = nodes[q].left == i ? q : -1;
+        rel |= kBsrPairBit;
       }
+      leaf_rel.push_back(owner < 0 ? nodes[i].b : rel);
+      for (int j = 0; j < 3; ++j) leaf_ids.push_back(owner < 0 ? kBsrSkip : (uint32_t)(5 * owner + j));
     }
   }
   P.nleaves = leaf_rel.size();
@@ -1609,7 +1618,7 @@ int bsr_plan_sizes(uint32_t d, const LeafBits& leaf, const SmallTable& tab, BsrP
   P = BsrPlan();
   S = BsrSizes();
   if (d == 0) return fail(PCG_ERR_INVALID, "empty range");
-  if (d >= kBsrPairBit) return fail(PCG_ERR_TOO_LARGE, "subtree of %u leaves", d);
+  if (d > kBsrRelMask) return fail(PCG_ERR_TOO_LARGE, "subtree of %u leaves", d);
   S.leaf = leaf;
   P.leaf = leaf;
   S.root = bsr_size_index(S, d);
@@ -1918,10 +1927,10 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
   if (caps)
     for (int j = 0; j < 3; ++j)
       if (caps[j] < P->root_bound[j]) return fail(PCG_ERR_TOO_LARGE, "result buffer too small");
-  // leaves and two-leaf nodes (every node of height 2) in one launch
+  // leaves and every node of height 2 and 3 in one launch
   hipLaunchKernelGGL(k_bsr_base, dim3((unsigned)((P->nleaves + 255) / 256)), dim3(256), 0, c.stream,
                      P->leaf_rel, a, P->leaf_ids, P->nleaves, P->meta, P->arena);
-  for (int32_t h = 3; h <= P->H; ++h) {
+  for (int32_t h = 4; h <= P->H; ++h) {
     const BsrHeight& hh = P->hp[h];
     const BsrMul* hm = P->muls + hh.mul0;
     BatchDesc* hd = P->descs + hh.mul0;
