@@ -647,20 +647,36 @@ __global__ __launch_bounds__(256) void k_bsr_add_fix(const BsrAdd* adds, const B
   }
 }
 
-__global__ __launch_bounds__(256) void k_bsr_add_topnz(const BsrAdd* adds, const BsrVal* meta,
-                                                       uint32_t* arena, BsrAddSlot* slots) {
-  const uint32_t node = blockIdx.y;
+// One workgroup (kCarryThreads) per node: the top nonzero limb of a
+// difference, scanning 1024-limb chunks from the top down (usually the top
+// chunk holds it).
+__global__ __launch_bounds__(kCarryThreads) void k_bsr_add_topnz(const BsrAdd* adds, const BsrVal* meta,
+                                                                 uint32_t* arena, BsrAddSlot* slots) {
+  __shared__ unsigned long long sh_hit;
+  const uint32_t node = blockIdx.x;
   const BsrAddPlan pl = bsr_add_plan(adds[node], meta, arena, slots[node].cmp);
-  const uint64_t base = (uint64_t)blockIdx.x * kCarryBlock;
-  if (!pl.sub || pl.zero || base >= pl.LA) return;
-  unsigned long long hit = 0;
+  if (!pl.sub || pl.zero) return;  // the same for the whole workgroup
+  for (uint64_t end = pl.LA; end > 0;) {
+    const uint64_t base = end > kCarryBlock ? end - kCarryBlock : 0;
+    if (threadIdx.x == 0) sh_hit = 0;
+    __syncthreads();
+    unsigned long long hit = 0;
 #pragma unroll
-  for (int q = 0; q < kCarryLPT; ++q) {
-    const uint64_t k = base + (uint64_t)threadIdx.x * kCarryLPT + q;
-    if (k < pl.LA && pl.O[k] != 0) hit = k + 1;
+    for (int q = 0; q < kCarryLPT; ++q) {
+      const uint64_t k = base + (uint64_t)threadIdx.x * kCarryLPT + q;
+      if (k < end && pl.O[k] != 0) hit = k + 1;
+    }
+    hit = wave_max_u64(hit);
+    if ((threadIdx.x & 63) == 0 && hit) atomicMax(&sh_hit, hit);
+    __syncthreads();
+    const unsigned long long h = sh_hit;
+    if (h) {
+      if (threadIdx.x == 0) slots[node].top = h;
+      return;
+    }
+    __syncthreads();  // every thread has read sh_hit before the next reset
+    end = base;
   }
-  hit = wave_max_u64(hit);
-  if ((threadIdx.x & 63) == 0 && hit) atomicMax(&slots[node].top, hit);
 }
 
 __global__ void k_bsr_add_final(const BsrAdd* adds, BsrVal* meta, uint32_t* arena, const BsrAddSlot* slots,
@@ -838,7 +854,21 @@ __host__ __device__ inline bool bsr_layout_node(const BsrLayout& L, uint64_t t) 
   return !bad;
 }
 
-__global__ void k_bsr_layout(BsrLayout L) {
+// The size table is read on every step of both walks: a copy in LDS when it
+// has at most kBsrLdsSizes sizes (subtrees below 2^31 leaves have at most
+// two per depth).
+constexpr int kBsrLdsSizes = 64;
+
+__global__ __launch_bounds__(256) void k_bsr_layout(BsrLayout L) {
+  __shared__ BsrSizeInfo s_info[kBsrLdsSizes];
+  __shared__ uint32_t s_cnt[kBsrLdsSizes * kBsrLdsSizes];
+  if (L.ns <= kBsrLdsSizes) {
+    for (int i = threadIdx.x; i < L.ns; i += blockDim.x) s_info[i] = L.info[i];
+    for (int i = threadIdx.x; i < L.ns * L.ns; i += blockDim.x) s_cnt[i] = L.cnt[i];
+    __syncthreads();
+    L.info = s_info;
+    L.cnt = s_cnt;
+  }
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < L.nodes; t += stride)
     if (!bsr_layout_node(L, t)) atomicOr(L.err, 1u);

@@ -1972,8 +1972,8 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
                            P->add_slots, P->add_agg, P->add_cin, mc);
       hipLaunchKernelGGL(k_bsr_add_fix, dim3((unsigned)((n * mc + 255) / 256)), dim3(256), 0, c.stream, la,
                          P->meta, P->arena, P->add_slots, P->add_cin, mc, n);
-      hipLaunchKernelGGL(k_bsr_add_topnz, grid, dim3(256), 0, c.stream, la, P->meta, P->arena,
-                         P->add_slots);
+      hipLaunchKernelGGL(k_bsr_add_topnz, dim3((unsigned)n), dim3(kCarryThreads), 0, c.stream, la, P->meta,
+                         P->arena, P->add_slots);
       hipLaunchKernelGGL(k_bsr_add_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, la,
                          P->meta, P->arena, P->add_slots, n);
     }
