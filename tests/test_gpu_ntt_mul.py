@@ -126,3 +126,32 @@ def test_wrapped_carry_chains(gpu, n):
     want[n] = 0xFFFFFFFE
     want[n + 1:] = M32
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("n", [3000, 5000, (1 << 16) + 7, (1 << 19) + 65536, (1 << 20) - 3,
+                               (1 << 21) + 100000])
+def test_squares_on_one_device_buffer(gpu, n):
+    """a * a with both operands at one device address takes the square path
+    (a's forward transform reused as the fused pass's input); plain, wrapped
+    (2n - 1 coefficients a little above a power of two) and all-ones."""
+    import ctypes as C
+
+    L = gpu.load()
+    rng = random.Random(n)
+    for a in (rand_limbs(rng, n), np.full(n, M32, dtype=np.uint32)):
+        d_a, d_o = C.c_void_p(), C.c_void_p()
+        gpu._check(L.pcg_dev_alloc(C.byref(d_a), 4 * n))
+        gpu._check(L.pcg_dev_alloc(C.byref(d_o), 8 * n))
+        try:
+            gpu._check(L.pcg_memcpy_h2d(d_a, a.ctypes.data, 4 * n))
+            gpu.mul_dev(d_a.value, n, d_a.value, n, d_o.value, gpu.ENGINE_NTT, sync=True)
+            got = np.empty(2 * n, dtype=np.uint32)
+            gpu._check(L.pcg_memcpy_d2h(got.ctypes.data, d_o, 8 * n))
+        finally:
+            L.pcg_dev_free(d_a)
+            L.pcg_dev_free(d_o)
+        if n <= 1 << 17:
+            assert np.array_equal(got, exact_product(a, a))
+        else:  # the general path (operands in two buffers) and residues
+            assert np.array_equal(got, gpu.mul(a, a, gpu.ENGINE_NTT))
+            assert residues_match(a, a, got)
