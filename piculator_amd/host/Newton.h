@@ -41,7 +41,11 @@ F newton_invsqrt(uint32_t x, size_t p) {
   if (p == 0) return F::seed(1. / std::sqrt((double)x), 0, true);
   const F r0 = newton_invsqrt<F>(x, half_precision(p));
   F t = r0.mul(r0, p);
-  t = t.mul(F(x), 0);
+  // BigFloat.cpp:562 multiplies by BigFloat(x) with p = 0: no chop, the
+  // exact product of |t| + 1 limbs less one leading zero limb, the sign and
+  // exponent of t.  mul(uint32_t) (BigFloat.cpp:156-183) gives that same
+  // value in one pass instead of a transform of |t| limbs.
+  t = t.mul(x);
   t = t.sub(F(1), p);
   t = t.mul(2147483648u);
   t.shift_exp(-1);
