@@ -197,7 +197,14 @@ int pcg_df_add(pcg_dfloat* z, const pcg_dfloat* a, const pcg_dfloat* b, uint64_t
 int pcg_df_sub(pcg_dfloat* z, const pcg_dfloat* a, const pcg_dfloat* b, uint64_t p);
 /* z = a.mul(b, p) (BigFloat.cpp:326-444): operands cut to their top p + 2
  * limbs (p = 0: exact), exact product on the GPU engine chosen by `flags`
- * (PCG_ENGINE_*), one zero top limb trimmed. */
+ * (PCG_ENGINE_*), one zero top limb trimmed.  PCG_KEEP_B in flags keeps b's
+ * forward transform for the next product; PCG_REUSE_B says that b is the
+ * operand kept that way (the same pcg_dfloat, unchanged, and the same p):
+ * its transform is reused when nothing has overwritten it and the transform
+ * length agrees, else recomputed.  Results are identical either way.  The
+ * Newton steps of BigFloat.cpp:445-567 multiply r0 twice in a row. */
+#define PCG_KEEP_B 0x1000u
+#define PCG_REUSE_B 0x2000u
 int pcg_df_mul(pcg_dfloat* z, const pcg_dfloat* a, const pcg_dfloat* b, uint64_t p, uint32_t flags);
 /* z1 = x1.mul(s, p) and z2 = x2.mul(s, p), each exactly as pcg_df_mul, with
  * the forward transform of the shared operand computed once when both

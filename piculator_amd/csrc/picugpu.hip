@@ -204,6 +204,9 @@ struct Ctx {
   size_t spec_points = 0;
   double* top_spec = nullptr;  // the short product of a wrapped one, [2][2][M]
   size_t top_points = 0;
+  // Generation of the transforms in spec / top_spec: bumped whenever a
+  // product writes operand a's half of them or they are reallocated.
+  uint64_t spec_gen = 1;
   CarryFn* agg = nullptr;
   uint64_t* cin = nullptr;
   size_t nblk_cap = 0;
@@ -330,6 +333,7 @@ int ensure_workspace(Ctx& c, int logN, uint64_t zL) {
     if (c.spec) HIPCHK(hipFree(c.spec));
     c.spec = nullptr;
     c.spec_points = 0;
+    ++c.spec_gen;
     if (hipMalloc(&c.spec, pts * 4 * sizeof(double)) != hipSuccess) {
       (void)hipGetLastError();
       return fail(PCG_ERR_OOM, "cannot allocate %zu bytes of transform scratch", pts * 32);
@@ -538,6 +542,7 @@ int ensure_top_scratch(Ctx& c, int logM) {
   if (c.top_spec) HIPCHK(hipFree(c.top_spec));
   c.top_spec = nullptr;
   c.top_points = 0;
+  ++c.spec_gen;
   if (hipMalloc(&c.top_spec, pts * 4 * sizeof(double)) != hipSuccess) {
     (void)hipGetLastError();
     return fail(PCG_ERR_OOM, "cannot allocate %zu bytes of transform scratch", pts * 32);
@@ -549,13 +554,15 @@ int ensure_top_scratch(Ctx& c, int logM) {
 // The short product of a wrapped one: the top w limbs of a and b convolved
 // into c.top_spec's second half (residues, no carry).
 int ntt_wrap_top(Ctx& c, const WrapPlan& wp, const uint32_t* d_a, uint64_t na, const uint32_t* d_b,
-                 uint64_t nb) {
+                 uint64_t nb, bool a_kept = false) {
   const Plan pm = make_plan(wp.logM);
   const uint64_t M = 1ull << wp.logM;
   double* A2 = c.top_spec;
   double* B2 = c.top_spec + 2 * M;
-  int rc = ntt_forward_into(c, pm, d_a + (na - wp.w), wp.w, A2);
-  if (rc) return rc;
+  if (!a_kept) {
+    const int rc = ntt_forward_into(c, pm, d_a + (na - wp.w), wp.w, A2);
+    if (rc) return rc;
+  }
   return ntt_tail_into(c, pm, d_b + (nb - wp.w), wp.w, A2, B2);
 }
 
@@ -589,6 +596,17 @@ int ntt_carry(Ctx& c, const Plan& pl, const double* B, uint64_t zL, uint32_t* d_
   return PCG_OK;
 }
 
+// The operand whose transform spec's first half (and top_spec's, for a
+// wrapped product) holds, kept for the next product (PCG_KEEP_B).
+struct KeptSpectrum {
+  const uint32_t* t = nullptr;
+  uint64_t L = 0;
+  int logN = 0;
+  WrapPlan wp;
+  uint64_t gen = 0;
+};
+KeptSpectrum g_kept;
+
 // Plan of a product: the full transform, or the wrapped one.
 struct MulPlan {
   WrapPlan wp;
@@ -610,10 +628,10 @@ int ensure_mul_workspace(Ctx& c, const MulPlan& mp, uint64_t zL) {
 // Forward a (into c.spec's first half) is done; the rest of the product
 // of a (na limbs) and b into d_out.
 int ntt_mul_tail(Ctx& c, const Plan& pl, const MulPlan& mp, const uint32_t* d_a, uint64_t na,
-                 const uint32_t* d_b, uint64_t nb, uint32_t* d_out, bool profile) {
+                 const uint32_t* d_b, uint64_t nb, uint32_t* d_out, bool profile, bool a_kept = false) {
   const uint64_t N = 1ull << pl.logN;
   if (mp.wp.use) {
-    const int rc = ntt_wrap_top(c, mp.wp, d_a, na, d_b, nb);
+    const int rc = ntt_wrap_top(c, mp.wp, d_a, na, d_b, nb, a_kept);
     if (rc) return rc;
   }
   int rc = ntt_tail_into(c, pl, d_b, nb, c.spec, c.spec + 2 * N);
@@ -626,8 +644,12 @@ int ntt_mul_tail(Ctx& c, const Plan& pl, const MulPlan& mp, const uint32_t* d_a,
 }
 
 // Enqueue the whole NTT product on the library stream (device pointers).
+// keep: kKeepA (record operand a's transform for the next product) and/or
+// kReuseA (operand a's transform is the one recorded: skip it when the plan
+// matches and nothing has overwritten it since).
+constexpr uint32_t kKeepA = 1, kReuseA = 2;
 int ntt_mul_enqueue(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_b, uint64_t nb,
-                    uint32_t* d_out, bool profile) {
+                    uint32_t* d_out, bool profile, uint32_t keep = 0) {
   const uint64_t zL = na + nb;
   const MulPlan mp = mul_plan(na, nb);
   int rc = ensure_mul_workspace(c, mp, zL);
@@ -635,6 +657,18 @@ int ntt_mul_enqueue(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_
   const Plan pl = make_plan(mp.logN);
   if (profile) HIPCHK(hipEventRecord(c.ev[0], c.stream));
   const size_t m = pl.passes.size();
+  const KeptSpectrum& K = g_kept;
+  // the main transform of a depends on its limbs and the length only; the
+  // short one of a wrapped product on the wrap too
+  const bool reuse = (keep & kReuseA) && K.t == d_a && K.L == na && K.gen == c.spec_gen &&
+                     K.logN == mp.logN;
+  const bool top_kept = reuse && mp.wp.use && K.wp.use && K.wp.logM == mp.wp.logM && K.wp.w == mp.wp.w;
+  if (!reuse) ++c.spec_gen;
+  if (keep & kKeepA)
+    g_kept = KeptSpectrum{d_a, na, mp.logN, mp.wp, c.spec_gen};
+  else if (reuse && mp.wp.use)
+    g_kept.wp = mp.wp;  // the short transform is redone for this product's wrap
+  if (reuse) return ntt_mul_tail(c, pl, mp, d_a, na, d_b, nb, d_out, profile, top_kept);
   if (d_a == d_b && na == nb && m >= 2) {
     // a square: the fused pass takes a's forward transform but its last
     // pass as its input, so the last pass also writes into A (out of place)
@@ -705,6 +739,7 @@ int fft_mul_enqueue(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_
   const uint64_t zL = na + nb;
   int logN = ceil_log2(zL);
   if (logN < kMinLogN) logN = kMinLogN;
+  ++c.spec_gen;  // the FFT's data overwrite the kept transform
   int rc = ensure_workspace(c, logN, zL);
   if (rc) return rc;
   const Plan pl = make_fft_plan(logN);
@@ -741,7 +776,7 @@ int fft_mul_enqueue(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_
 }
 
 int mul_dev_locked(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_b, uint64_t nb,
-                   uint32_t* d_out, uint32_t flags) {
+                   uint32_t* d_out, uint32_t flags, uint32_t keep = 0) {
   int rc = check_sizes(na, nb);
   if (rc) return rc;
   const uint64_t zL = na + nb;
@@ -767,7 +802,7 @@ int mul_dev_locked(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_b
     if (!c.fft_recomputed) return PCG_OK;
   }
   const bool prof = (flags & PCG_PROFILE) != 0;
-  rc = ntt_mul_enqueue(c, d_a, na, d_b, nb, d_out, prof);
+  rc = ntt_mul_enqueue(c, d_a, na, d_b, nb, d_out, prof, keep);
   if (rc) return rc;
   if (prof) {
     HIPCHK(hipEventSynchronize(c.ev[3]));
@@ -2323,7 +2358,13 @@ int df_mul_locked(Ctx& c, pcg_dfloat* z, const pcg_dfloat* pa, const pcg_dfloat*
   uint64_t cap = 0;
   int rc = df_alloc(c, a.L + b.L, &buf, &cap);
   if (rc) return rc;
-  rc = mul_dev_locked(c, a.t, a.L, b.t, b.L, buf, flags & PCG_ENGINE_MASK);
+  if (flags & (PCG_KEEP_B | PCG_REUSE_B)) {
+    // b first: its transform is the one kept or reused (exact products commute)
+    const uint32_t keep = ((flags & PCG_KEEP_B) ? kKeepA : 0u) | ((flags & PCG_REUSE_B) ? kReuseA : 0u);
+    rc = mul_dev_locked(c, b.t, b.L, a.t, a.L, buf, flags & PCG_ENGINE_MASK, keep);
+  } else {
+    rc = mul_dev_locked(c, a.t, a.L, b.t, b.L, buf, flags & PCG_ENGINE_MASK);
+  }
   uint32_t topl = 0;
   if (!rc) rc = df_read_limb(c, buf + a.L + b.L - 1, &topl);
   if (!rc) rc = sync_locked(c);
@@ -2371,6 +2412,7 @@ int df_mul_pair_locked(Ctx& c, pcg_dfloat* z1, const pcg_dfloat* px1, pcg_dfloat
   if (!rc && shared) {
     // one forward transform of s (the main length of both products), then
     // each product's short top product if it wraps, its tail and carry
+    ++c.spec_gen;
     for (int i = 0; i < 2 && !rc; ++i) rc = ensure_mul_workspace(c, mp[i], zL[i]);
     const Plan pl = make_plan(mp[0].logN);
     if (!rc) rc = ntt_forward_into(c, pl, sc[0].t, sc[0].L, c.spec);

@@ -47,6 +47,10 @@ class BigFloat {
   BigFloat add(const BigFloat& x, size_t p = 0) const;
   BigFloat sub(const BigFloat& x, size_t p = 0) const;
   BigFloat mul(const BigFloat& x, size_t p = 0, int threads = 1) const;
+  // The products of the Newton steps that multiply r0 twice (the device
+  // type keeps r0's transform between them); plain products here.
+  BigFloat mul_keep(const BigFloat& x, size_t p) const { return mul(x, p); }
+  BigFloat mul_reuse(const BigFloat& x, size_t p) const { return mul(x, p); }
   BigFloat rcp(size_t p, int threads = 1) const;
   BigFloat div(const BigFloat& x, size_t p, int threads = 1) const;
 

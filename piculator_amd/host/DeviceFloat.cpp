@@ -115,6 +115,20 @@ DeviceFloat DeviceFloat::mul(const DeviceFloat& x, size_t p) const {
   return z;
 }
 
+DeviceFloat DeviceFloat::mul_keep(const DeviceFloat& x, size_t p) const {
+  DeviceFloat z;
+  call([&] { return pcg_df_mul(&z.v_, &v_, &x.v_, p, mul_engine_flags() | PCG_KEEP_B); });
+  if (v_.L && x.v_.L) count_big_muls(1);
+  return z;
+}
+
+DeviceFloat DeviceFloat::mul_reuse(const DeviceFloat& x, size_t p) const {
+  DeviceFloat z;
+  call([&] { return pcg_df_mul(&z.v_, &v_, &x.v_, p, mul_engine_flags() | PCG_REUSE_B); });
+  if (v_.L && x.v_.L) count_big_muls(1);
+  return z;
+}
+
 std::pair<DeviceFloat, DeviceFloat> DeviceFloat::mul_pair(const DeviceFloat& x1, const DeviceFloat& x2,
                                                           size_t p) const {
   std::pair<DeviceFloat, DeviceFloat> z;

@@ -38,6 +38,10 @@ class DeviceFloat {
   DeviceFloat add(const DeviceFloat& x, size_t p = 0) const;
   DeviceFloat sub(const DeviceFloat& x, size_t p = 0) const;
   DeviceFloat mul(const DeviceFloat& x, size_t p = 0) const;
+  // mul(x, p), keeping x's forward transform for a next product by x
+  // (PCG_KEEP_B); mul_reuse: x is that operand, unchanged (PCG_REUSE_B)
+  DeviceFloat mul_keep(const DeviceFloat& x, size_t p) const;
+  DeviceFloat mul_reuse(const DeviceFloat& x, size_t p) const;
   // {x1.mul(*this, p), x2.mul(*this, p)}, this value's forward transform
   // shared by the two products (pcg_df_mul_pair)
   std::pair<DeviceFloat, DeviceFloat> mul_pair(const DeviceFloat& x1, const DeviceFloat& x2,

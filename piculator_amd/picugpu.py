@@ -339,6 +339,30 @@ def df_mul_pair(x1, x2, s, p: int = 0, flags: int = 0, alias: str = ""):
             load().pcg_df_free(C.byref(v))
 
 
+KEEP_B = 0x1000
+REUSE_B = 0x2000
+
+
+def df_mul_seq(s, xs, p: int = 0, flags=None, between=None):
+    """[x.mul(s, p) for x in xs] through pcg_df_mul on device values made from
+    (sign, exp, limbs) tuples, with flags[i] for product i (PCG_KEEP_B /
+    PCG_REUSE_B: s's transform kept and reused); between(i), if given, runs
+    after product i (e.g. another product that overwrites the kept one)."""
+    flags = flags or [0] * len(xs)
+    sv = df_from_tuple(s)
+    vals = [df_from_tuple(x) for x in xs]
+    zs = [DFloat() for _ in xs]
+    try:
+        for i, (x, z) in enumerate(zip(vals, zs)):
+            _check(load().pcg_df_mul(C.byref(z), C.byref(x), C.byref(sv), p, flags[i]))
+            if between:
+                between(i)
+        return [df_to_tuple(z) for z in zs]
+    finally:
+        for v in [sv] + vals + zs:
+            load().pcg_df_free(C.byref(v))
+
+
 def df_bsr_exact(a: int, b: int, max_limbs: int):
     """(sign, exp, limbs) of P, Q, R of the subtree [a, b) through
     pcg_df_bsr_exact (results left in device memory, then read back)."""

@@ -245,3 +245,38 @@ def test_mul_pair_wrapped_products(gpu):
             assert residue(z, q) == residue(x, q) * residue(s, q) % q
         k = 12000
         assert np.array_equal(z[:k], from_int((to_int(x[:k]) * to_int(s[:k])) % (1 << (32 * k)), k))
+
+
+@pytest.mark.parametrize("ns,nx,p", [(3000, 6000, 0), (30000, 60000, 0), (40000, 90000, 50000),
+                                     ((1 << 19) + 9000, (1 << 19) + 20000, 0),
+                                     ((1 << 19) + 9000, (1 << 19) - 8993, 0)])
+def test_kept_transform_reused(gpu, ns, nx, p):
+    """x.mul(s) with PCG_KEEP_B, then more products by s with PCG_REUSE_B
+    (the Newton pattern): every result exactly as without the flags.  The
+    second operand is 7 limbs shorter: the same transform length, for the
+    wrapped sizes another wrap (a new short product) or, at the last size,
+    no wrap (2^20 - 1 coefficients after 2^20 + 6); the last operand has a
+    different transform length (recomputed)."""
+    rng = random.Random(ns + nx + p)
+    s = rand_val(rng, ns)
+    xs = [rand_val(rng, nx), rand_val(rng, nx - 7), rand_val(rng, nx // 3)]
+    got = gpu.df_mul_seq(s, xs, p, [gpu.KEEP_B, gpu.REUSE_B, gpu.REUSE_B])
+    for g, x in zip(got, xs):
+        check(g, O.BF(*x).mul(O.BF(*s), p).tuple())
+
+
+def test_kept_transform_overwritten(gpu):
+    """Another product between the keep and the reuse overwrites the kept
+    transform: the reuse recomputes it, results exact."""
+    rng = random.Random(5)
+    s, x0, x1 = rand_val(rng, 20000), rand_val(rng, 20000), rand_val(rng, 20001)
+    other = (rand_val(rng, 20000), rand_val(rng, 19000))
+
+    def between(i):
+        if i == 0:
+            gpu.df_mul_pair(other[0], other[1], other[1], 0)
+            gpu.mul(np.array(other[0][2], dtype=np.uint32), np.array(other[1][2], dtype=np.uint32))
+
+    got = gpu.df_mul_seq(s, [x0, x1], 0, [gpu.KEEP_B, gpu.REUSE_B], between)
+    check(got[0], O.BF(*x0).mul(O.BF(*s), 0).tuple())
+    check(got[1], O.BF(*x1).mul(O.BF(*s), 0).tuple())
