@@ -336,8 +336,8 @@ def main():
         summarize_traffic(*args.summarize_traffic)
         return
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path "
+              "(the binary-splitting tree would shard by subtrees, one GPU each)", file=sys.stderr)
         sys.exit(2)
     if args.calib_copy:
         # a plain copy of known size (512 MiB read + 512 MiB written), so that
