@@ -26,6 +26,16 @@ constexpr int kSmallNttMaxLog = 12;    // N bound of k_batch_ntt (rows padded by
 // before this step's products, so their LDS latency overlaps the
 // multiply-adds; sa[ihi + 1] is read but not used (ihi + 1 <= na <= the
 // array's length - 1, as nb >= 1).
+// (hi:lo) += a * b as 96 bits: the 64-bit multiply-add with its carry out
+// (in VCC) added into hi.  The wait between the VCC write and its read as
+// a carry is explicit (the hazard recognizer does not look inside asm).
+__device__ __forceinline__ void mac96(uint64_t& lo, uint32_t& hi, uint32_t a, uint32_t b) {
+  asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\ts_nop 1\n\tv_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+      : "+v"(lo), "+v"(hi)
+      : "v"(a), "v"(b)
+      : "vcc");
+}
+
 template <int C>
 __device__ __forceinline__ void school_columns(const uint32_t* sa, const uint32_t* sb, int k0, int ilo,
                                                int ihi, uint64_t (&lo)[C], uint32_t (&hi)[C]) {
@@ -35,15 +45,11 @@ __device__ __forceinline__ void school_columns(const uint32_t* sa, const uint32_
   uint32_t an = sa[ilo];
 #pragma unroll 2
   for (int i = ilo; i <= ihi; ++i) {
-    const uint64_t ai = an;
+    const uint32_t ai = an;
     an = sa[i + 1];
     const uint32_t bn = sb[3 + k0 - i];  // b[k0 - (i + 1)]
 #pragma unroll
-    for (int q = 0; q < C; ++q) {
-      const uint64_t pq = ai * w[q];
-      lo[q] += pq;
-      hi[q] += lo[q] < pq;
-    }
+    for (int q = 0; q < C; ++q) mac96(lo[q], hi[q], ai, w[q]);
 #pragma unroll
     for (int q = C - 1; q > 0; --q) w[q] = w[q - 1];
     w[0] = bn;
