@@ -26,14 +26,21 @@ constexpr int kSmallNttMaxLog = 12;    // N bound of k_batch_ntt (rows padded by
 // before this step's products, so their LDS latency overlaps the
 // multiply-adds; sa[ihi + 1] is read but not used (ihi + 1 <= na <= the
 // array's length - 1, as nb >= 1).
-// (hi:lo) += a * b as 96 bits: the 64-bit multiply-add with its carry out
-// (in VCC) added into hi.  The wait between the VCC write and its read as
-// a carry is explicit (the hazard recognizer does not look inside asm).
-__device__ __forceinline__ void mac96(uint64_t& lo, uint32_t& hi, uint32_t a, uint32_t b) {
-  asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\ts_nop 1\n\tv_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
-      : "+v"(lo), "+v"(hi)
-      : "v"(a), "v"(b)
-      : "vcc");
+// (hi0:lo0) += a * b0 and (hi1:lo1) += a * b1 as 96-bit sums: each a
+// 64-bit multiply-add with its carry out (to an SGPR pair) added into the
+// high word.  The waits between each carry's write and its read are
+// explicit (the hazard recognizer does not look inside asm): the second
+// multiply-add and one wait state.
+__device__ __forceinline__ void mac96x2(uint64_t& lo0, uint32_t& hi0, uint64_t& lo1, uint32_t& hi1, uint32_t a,
+                                        uint32_t b0, uint32_t b1) {
+  uint64_t c0, c1;
+  asm("v_mad_u64_u32 %0, %4, %6, %7, %0\n\t"
+      "v_mad_u64_u32 %1, %5, %6, %8, %1\n\t"
+      "s_nop 0\n\t"
+      "v_addc_co_u32_e64 %2, %4, 0, %2, %4\n\t"
+      "v_addc_co_u32_e64 %3, %5, 0, %3, %5"
+      : "+v"(lo0), "+v"(lo1), "+v"(hi0), "+v"(hi1), "=&s"(c0), "=&s"(c1)
+      : "v"(a), "v"(b0), "v"(b1));
 }
 
 template <int C>
@@ -49,7 +56,8 @@ __device__ __forceinline__ void school_columns(const uint32_t* sa, const uint32_
     an = sa[i + 1];
     const uint32_t bn = sb[3 + k0 - i];  // b[k0 - (i + 1)]
 #pragma unroll
-    for (int q = 0; q < C; ++q) mac96(lo[q], hi[q], ai, w[q]);
+    for (int q = 0; q < C; q += 2) mac96x2(lo[q], hi[q], lo[q + 1], hi[q + 1], ai, w[q], w[q + 1]);
+    static_assert(C % 2 == 0, "columns go in pairs");
 #pragma unroll
     for (int q = C - 1; q > 0; --q) w[q] = w[q - 1];
     w[0] = bn;
