@@ -432,8 +432,8 @@ __global__ __launch_bounds__(NT) void k_bsr_add(const BsrAdd* adds, BsrVal* meta
 // magnitudes (k_bsr_add_cmp), block-local sums with carry-in 0 and the
 // chunk carry functions (k_bsr_add_local), one wave per node scanning them
 // (k_bsr_add_scan; k_bsr_add_scan_wide above 64 chunks), the carry-in ripple (k_bsr_add_fix), the top nonzero
-// limb of a difference (k_bsr_add_topnz) and the result length and sign
-// (k_bsr_add_final).  Decisions are recomputed from the metadata in each
+// limb of a difference with the result length and sign (k_bsr_add_finish,
+// which also clears the node's slot).  Decisions are recomputed from the metadata in each
 // kernel (bsr_add_plan), so they agree everywhere.
 struct BsrAddSlot {
   unsigned long long cmp;  // 1 + highest index where equal-length magnitudes differ
@@ -502,14 +502,6 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     v = o > v ? o : v;
   }
   return v;
-}
-
-__global__ void k_bsr_add_prep(uint64_t n, BsrAddSlot* slots) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  slots[i].cmp = 0;
-  slots[i].top = 0;
-  slots[i].carry = 0;
 }
 
 __global__ __launch_bounds__(256) void k_bsr_add_cmp(const BsrAdd* adds, const BsrVal* meta,
@@ -647,56 +639,59 @@ __global__ __launch_bounds__(256) void k_bsr_add_fix(const BsrAdd* adds, const B
   }
 }
 
-// One workgroup (kCarryThreads) per node: the top nonzero limb of a
-// difference, scanning 1024-limb chunks from the top down (usually the top
-// chunk holds it).
-__global__ __launch_bounds__(kCarryThreads) void k_bsr_add_topnz(const BsrAdd* adds, const BsrVal* meta,
-                                                                 uint32_t* arena, BsrAddSlot* slots) {
+// One workgroup (kCarryThreads) per node, last: the length and sign of the
+// result — a difference's top nonzero limb, scanning 1024-limb chunks from
+// the top down (usually the top chunk holds it), or a sum's carry limb —
+// then the node's slot is cleared for the next height (the slots start
+// zeroed when the plan is laid out).
+__global__ __launch_bounds__(kCarryThreads) void k_bsr_add_finish(const BsrAdd* adds, BsrVal* meta,
+                                                                  uint32_t* arena, BsrAddSlot* slots) {
   __shared__ unsigned long long sh_hit;
   const uint32_t node = blockIdx.x;
-  const BsrAddPlan pl = bsr_add_plan(adds[node], meta, arena, slots[node].cmp);
-  if (!pl.sub || pl.zero) return;  // the same for the whole workgroup
-  for (uint64_t end = pl.LA; end > 0;) {
-    const uint64_t base = end > kCarryBlock ? end - kCarryBlock : 0;
-    if (threadIdx.x == 0) sh_hit = 0;
-    __syncthreads();
-    unsigned long long hit = 0;
-#pragma unroll
-    for (int q = 0; q < kCarryLPT; ++q) {
-      const uint64_t k = base + (uint64_t)threadIdx.x * kCarryLPT + q;
-      if (k < end && pl.O[k] != 0) hit = k + 1;
-    }
-    hit = wave_max_u64(hit);
-    if ((threadIdx.x & 63) == 0 && hit) atomicMax(&sh_hit, hit);
-    __syncthreads();
-    const unsigned long long h = sh_hit;
-    if (h) {
-      if (threadIdx.x == 0) slots[node].top = h;
-      return;
-    }
-    __syncthreads();  // every thread has read sh_hit before the next reset
-    end = base;
-  }
-}
-
-__global__ void k_bsr_add_final(const BsrAdd* adds, BsrVal* meta, uint32_t* arena, const BsrAddSlot* slots,
-                                uint64_t n) {
-  const uint64_t node = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (node >= n) return;
   const BsrAdd op = adds[node];
   const BsrAddPlan pl = bsr_add_plan(op, meta, arena, slots[node].cmp);
-  uint32_t L;
+  const uint64_t carry = slots[node].carry;
+  uint64_t L = 0;
   if (pl.zero) {
     L = 0;
-  } else if (pl.sub) {
-    L = (uint32_t)slots[node].top;  // every leading zero stripped (usub)
+  } else if (pl.sub) {  // the same for the whole workgroup
+    for (uint64_t end = pl.LA; end > 0;) {
+      const uint64_t base = end > kCarryBlock ? end - kCarryBlock : 0;
+      if (threadIdx.x == 0) sh_hit = 0;
+      __syncthreads();
+      unsigned long long hit = 0;
+#pragma unroll
+      for (int q = 0; q < kCarryLPT; ++q) {
+        const uint64_t k = base + (uint64_t)threadIdx.x * kCarryLPT + q;
+        if (k < end && pl.O[k] != 0) hit = k + 1;
+      }
+      hit = wave_max_u64(hit);
+      if ((threadIdx.x & 63) == 0 && hit) atomicMax(&sh_hit, hit);
+      __syncthreads();
+      const unsigned long long h = sh_hit;
+      __syncthreads();  // every thread has read sh_hit before the next reset
+      if (h) {
+        L = h;  // every leading zero stripped (usub)
+        break;
+      }
+      end = base;
+    }
   } else {
     L = pl.LA;
-    if (slots[node].carry) pl.O[L++] = 1;  // carry limb (uadd)
+    if (carry) {
+      if (threadIdx.x == 0) pl.O[L] = 1;  // carry limb (uadd)
+      ++L;
+    }
   }
-  BsrVal& mo = meta[op.out];
-  mo.L = L;
-  mo.sign = L ? pl.sign : 1;
+  __syncthreads();  // every thread has read the slot
+  if (threadIdx.x == 0) {
+    BsrVal& mo = meta[op.out];
+    mo.L = (uint32_t)L;
+    mo.sign = L ? pl.sign : 1;
+    slots[node].cmp = 0;
+    slots[node].top = 0;
+    slots[node].carry = 0;
+  }
 }
 
 // ------------------------------------------------------------------ layout

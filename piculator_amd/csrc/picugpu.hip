@@ -1449,6 +1449,7 @@ struct BsrPlan {
   BsrAdd* adds = nullptr;
   BatchDesc* descs = nullptr;
   BsrAddSlot* add_slots = nullptr;  // per large add of a height
+  size_t nslots = 0;
   CarryFn* add_agg = nullptr;       // per (large add, chunk)
   uint64_t* add_cin = nullptr;
 };
@@ -1878,6 +1879,7 @@ int bsr_plan_build(Ctx& c, uint32_t d, const LeafBits& leaf, const SmallTable& t
   HIPCHK(hipMemcpyAsync(d_info, S.info.data(), ns * sizeof(BsrSizeInfo), hipMemcpyHostToDevice, c.stream));
   HIPCHK(hipMemcpyAsync(d_cnt, S.cnt.data(), ns * ns * 4, hipMemcpyHostToDevice, c.stream));
   HIPCHK(hipMemsetAsync(d_err, 0, 4, c.stream));
+  P.nslots = S.max_large;
   const uint64_t blocks = std::min<uint64_t>((L.nodes + 255) / 256, 8192);
   hipLaunchKernelGGL(k_bsr_layout, dim3((unsigned)blocks), dim3(256), 0, c.stream, L);
   HIPCHK(hipGetLastError());
@@ -1990,6 +1992,8 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
   if (caps)
     for (int j = 0; j < 3; ++j)
       if (caps[j] < P->root_bound[j]) return fail(PCG_ERR_TOO_LARGE, "result buffer too small");
+  // the large adds' slots start zeroed (k_bsr_add_finish clears each after use)
+  if (P->nslots) HIPCHK(hipMemsetAsync(P->add_slots, 0, P->nslots * sizeof(BsrAddSlot), c.stream));
   // leaves and every node of height 2 and 3 in one launch
   hipLaunchKernelGGL(k_bsr_base, dim3((unsigned)((P->nleaves + 255) / 256)), dim3(256), 0, c.stream,
                      P->leaf_rel, a, P->leaf_ids, P->nleaves, P->meta, P->arena);
@@ -2022,8 +2026,7 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
       const uint64_t n = hh.nlarge;
       const uint32_t mc = hh.add_maxch;
       const dim3 grid(mc, (unsigned)n);
-      hipLaunchKernelGGL(k_bsr_add_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, n,
-                         P->add_slots);
+      // the slots are zero here (zeroed before the leaves, cleared after use)
       hipLaunchKernelGGL(k_bsr_add_cmp, grid, dim3(256), 0, c.stream, la, P->meta, P->arena, P->add_slots);
       hipLaunchKernelGGL(k_bsr_add_local, grid, dim3(kCarryThreads), 0, c.stream, la, P->meta, P->arena,
                          P->add_slots, P->add_agg, mc);
@@ -2035,10 +2038,8 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
                            P->add_slots, P->add_agg, P->add_cin, mc);
       hipLaunchKernelGGL(k_bsr_add_fix, dim3((unsigned)((n * mc + 255) / 256)), dim3(256), 0, c.stream, la,
                          P->meta, P->arena, P->add_slots, P->add_cin, mc, n);
-      hipLaunchKernelGGL(k_bsr_add_topnz, dim3((unsigned)n), dim3(kCarryThreads), 0, c.stream, la, P->meta,
+      hipLaunchKernelGGL(k_bsr_add_finish, dim3((unsigned)n), dim3(kCarryThreads), 0, c.stream, la, P->meta,
                          P->arena, P->add_slots);
-      hipLaunchKernelGGL(k_bsr_add_final, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, la,
-                         P->meta, P->arena, P->add_slots, n);
     }
   }
   HIPCHK(hipGetLastError());
