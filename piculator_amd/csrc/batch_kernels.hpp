@@ -317,8 +317,8 @@ __global__ __launch_bounds__((1 << LOGN) / kEPT) __attribute__((amdgpu_waves_per
 // (chunk, z) resolves limbs [1024 chunk, 1024 chunk + 1024) of product z
 // with carry-in 0 and records the chunk's carry function (the form of
 // k_carry_local); k_batch_carry_scan scans each product's chunk functions
-// with one wave (k_batch_carry_scan_wide: one workgroup, above 64 chunks);
-// k_batch_carry_fix adds the carry-ins.  r: inverse-transform
+// with one wave and adds the carry-ins; above 64 chunks
+// k_batch_carry_scan_wide (one workgroup) scans and k_batch_carry_fix adds.  r: inverse-transform
 // outputs of product z at r + z * zstride, prime q at offset q * N.
 // agg / cin: nch entries per product.  Wrapped products (top != null): a
 // product with w = zL - 1 - N > 0 holds the cyclic convolution of length N,
@@ -410,7 +410,8 @@ __global__ void k_wrap_descs(const BatchDesc* descs, uint64_t count, uint64_t N,
   tops[z] = t;
 }
 
-// One wave per product: exclusive scan of its chunk functions, 64 at a time.
+// One wave per product: exclusive scan of its chunk functions, 64 at a time,
+// each chunk's carry-in rippled into its limbs (no k_batch_carry_fix).
 __global__ __launch_bounds__(64) void k_batch_carry_scan(const BatchDesc* descs, const CarryFn* agg,
                                                          uint64_t* cin, uint32_t nch) {
   const uint32_t z = blockIdx.x;
@@ -435,7 +436,18 @@ __global__ __launch_bounds__(64) void k_batch_carry_scan(const BatchDesc* descs,
     }
     CarryFn excl = cf_shfl_up(incl, 1);
     if (lane == 0) excl.T = 0;
-    if (idx < nc) cin[(uint64_t)z * nch + idx] = cf_eval0(cf_compose(run, excl));
+    if (idx < nc) {
+      // the chunk's carry-in, rippled into its limbs here (as k_batch_carry_fix)
+      uint64_t c = cf_eval0(cf_compose(run, excl));
+      uint64_t k = (uint64_t)idx * kCarryBlock;
+      const uint64_t end = k + kCarryBlock < zL ? k + kCarryBlock : zL;
+      while (c != 0 && k < end) {
+        const uint64_t v = (uint64_t)d.out[k] + (c & 0xffffffffull);
+        d.out[k] = (uint32_t)v;
+        c = (c >> 32) + (v >> 32);
+        ++k;
+      }
+    }
     CarryFn tot;
     tot.base = __shfl(incl.base, 63, 64);
     tot.T = __shfl(incl.T, 63, 64);

@@ -431,7 +431,8 @@ __global__ __launch_bounds__(NT) void k_bsr_add(const BsrAdd* adds, BsrVal* meta
 // of every node in its own workgroup: the comparison of equal-length
 // magnitudes (k_bsr_add_cmp), block-local sums with carry-in 0 and the
 // chunk carry functions (k_bsr_add_local), one wave per node scanning them
-// (k_bsr_add_scan; k_bsr_add_scan_wide above 64 chunks), the carry-in ripple (k_bsr_add_fix), the top nonzero
+// and adding the carry-ins (k_bsr_add_scan; above 64 chunks k_bsr_add_scan_wide
+// and the carry-in ripple k_bsr_add_fix), the top nonzero
 // limb of a difference with the result length and sign (k_bsr_add_finish,
 // which also clears the node's slot).  Decisions are recomputed from the metadata in each
 // kernel (bsr_add_plan), so they agree everywhere.
@@ -596,7 +597,18 @@ __global__ __launch_bounds__(64) void k_bsr_add_scan(const BsrAdd* adds, const B
     }
     CarryFn excl = cf_shfl_up(incl, 1);
     if (lane == 0) excl.T = 0;
-    if (idx < nc) cin[(uint64_t)node * maxch + idx] = cf_eval0(cf_compose(run, excl));
+    if (idx < nc) {
+      // the chunk's carry-in, rippled into its limbs here (as k_bsr_add_fix)
+      uint64_t c = cf_eval0(cf_compose(run, excl));
+      uint64_t k = (uint64_t)idx * kCarryBlock;
+      const uint64_t end = k + kCarryBlock < pl.LA ? k + kCarryBlock : pl.LA;
+      while (c != 0 && k < end) {
+        const uint64_t v = (uint64_t)pl.O[k] + (c & 0xffffffffull);
+        pl.O[k] = (uint32_t)v;
+        c = (c >> 32) + (v >> 32);
+        ++k;
+      }
+    }
     CarryFn tot;
     tot.base = __shfl(incl.base, 63, 64);
     tot.T = __shfl(incl.T, 63, 64);

@@ -1043,12 +1043,13 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, b
     if ((size_t)k * nch > c.bcarry_cap) return fail(PCG_ERR_INTERNAL, "batched carry blocks exceed the scratch");
     hipLaunchKernelGGL(k_batch_carry_local, dim3(nch, k), dim3(kCarryThreads), 0, c.stream, dd, B, per,
                        N, c.d_prime + logN * 2, c.crt, c.bagg, nch, top, M);
-    if (nch > 64)
+    if (nch > 64) {
       hipLaunchKernelGGL(k_batch_carry_scan_wide, dim3(k), dim3(256), 0, c.stream, dd, c.bagg, c.bcin, nch);
-    else
+      hipLaunchKernelGGL(k_batch_carry_fix, dim3((unsigned)(((uint64_t)k * nch + 255) / 256)), dim3(256), 0,
+                         c.stream, dd, c.bcin, nch, (uint64_t)k);
+    } else {  // the scan adds the carry-ins itself
       hipLaunchKernelGGL(k_batch_carry_scan, dim3(k), dim3(64), 0, c.stream, dd, c.bagg, c.bcin, nch);
-    hipLaunchKernelGGL(k_batch_carry_fix, dim3((unsigned)(((uint64_t)k * nch + 255) / 256)), dim3(256), 0,
-                       c.stream, dd, c.bcin, nch, (uint64_t)k);
+    }
   }
   HIPCHK(hipGetLastError());
   return PCG_OK;
@@ -2030,14 +2031,15 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
       hipLaunchKernelGGL(k_bsr_add_cmp, grid, dim3(256), 0, c.stream, la, P->meta, P->arena, P->add_slots);
       hipLaunchKernelGGL(k_bsr_add_local, grid, dim3(kCarryThreads), 0, c.stream, la, P->meta, P->arena,
                          P->add_slots, P->add_agg, mc);
-      if (mc > 64)
+      if (mc > 64) {
         hipLaunchKernelGGL(k_bsr_add_scan_wide, dim3((unsigned)n), dim3(256), 0, c.stream, la, P->meta,
                            P->arena, P->add_slots, P->add_agg, P->add_cin, mc);
-      else
+        hipLaunchKernelGGL(k_bsr_add_fix, dim3((unsigned)((n * mc + 255) / 256)), dim3(256), 0, c.stream, la,
+                           P->meta, P->arena, P->add_slots, P->add_cin, mc, n);
+      } else {  // the scan adds the carry-ins itself
         hipLaunchKernelGGL(k_bsr_add_scan, dim3((unsigned)n), dim3(64), 0, c.stream, la, P->meta, P->arena,
                            P->add_slots, P->add_agg, P->add_cin, mc);
-      hipLaunchKernelGGL(k_bsr_add_fix, dim3((unsigned)((n * mc + 255) / 256)), dim3(256), 0, c.stream, la,
-                         P->meta, P->arena, P->add_slots, P->add_cin, mc, n);
+      }
       hipLaunchKernelGGL(k_bsr_add_finish, dim3((unsigned)n), dim3(kCarryThreads), 0, c.stream, la, P->meta,
                          P->arena, P->add_slots);
     }
