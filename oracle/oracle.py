@@ -83,6 +83,8 @@ def lib():
         L.orc_pi_hex.restype = C.c_long
         L.orc_mul_count.restype = C.c_uint64
         L.orc_pi_bsr.argtypes = [bfp, bfp, bfp, C.c_uint32, C.c_uint32, sz, C.c_int, C.c_int]
+        L.orc_set_plant.argtypes = [C.c_int]
+        L.orc_set_plant.restype = None
         L.orc_bbp_hex.argtypes = [C.c_uint64, C.c_int]
         L.orc_bbp_hex.restype = C.c_uint32
         _lib = L
@@ -265,6 +267,12 @@ class BF:
         if n < 0:
             raise ValueError("hex conversion out of range")
         return buf.value.decode()
+
+
+def set_plant(k: int) -> None:
+    """Plant a mistake in the Newton steps (tests' self-checks only):
+    1 = the last sub keeps p + 1 limbs, 2 = the schedule s = p / 2; 0 = none."""
+    lib().orc_set_plant(k)
 
 
 def invsqrt(x: int, p: int, engine: int = ENGINE_NTT, threads: int = 1) -> BF:

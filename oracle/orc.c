@@ -695,6 +695,20 @@ void orc_bf_mul(orc_bf* out, const orc_bf* a, const orc_bf* x, size_t p, int eng
 }
 
 /* BigFloat.cpp:445-511: Newton reciprocal, r1 = r0 - (r0*x - 1)*r0. */
+/* Planted mistakes for the tests' self-checks (0 = none, the default):
+ * 1 = the last sub of each Newton step keeps p + 1 limbs instead of p + 2,
+ * 2 = the half-precision schedule s = p / 2 instead of p / 2 + 1.
+ * Tests set it only to show that their checks fail on such a slip. */
+static int g_plant = 0;
+void orc_set_plant(int k) { g_plant = k; }
+
+static size_t newton_half(size_t p) {
+  size_t s = g_plant == 2 ? p / 2 : p / 2 + 1; /* BigFloat.cpp:502 */
+  if (p == 1) s = 0;
+  if (p == 2) s = 1;
+  return s;
+}
+
 int orc_bf_rcp(orc_bf* out, const orc_bf* a, size_t p, int engine, int threads) {
   if (a->L == 0) return -1; /* "Divide by Zero" */
   int64_t Aexp = a->exp;
@@ -729,9 +743,7 @@ int orc_bf_rcp(orc_bf* out, const orc_bf* a, size_t p, int engine, int threads) 
     bf_move(out, &o);
     return 0;
   }
-  size_t s = p / 2 + 1;
-  if (p == 1) s = 0;
-  if (p == 2) s = 1;
+  size_t s = newton_half(p);
   orc_bf T, t1, one, t2, t3;
   bf_init(&T); bf_init(&t1); bf_init(&one); bf_init(&t2); bf_init(&t3);
   int rc = orc_bf_rcp(&T, a, s, engine, threads);
@@ -740,7 +752,7 @@ int orc_bf_rcp(orc_bf* out, const orc_bf* a, size_t p, int engine, int threads) 
   orc_bf_from_u32(&one, 1, 1);
   orc_bf_sub(&t2, &t1, &one, p);
   orc_bf_mul(&t3, &t2, &T, p, engine, threads);
-  orc_bf_sub(out, &T, &t3, p);
+  orc_bf_sub(out, &T, &t3, g_plant == 1 ? p - 1 : p);
   orc_bf_free(&T); orc_bf_free(&t1); orc_bf_free(&one); orc_bf_free(&t2); orc_bf_free(&t3);
   return 0;
 }
@@ -777,9 +789,7 @@ int orc_bf_invsqrt(orc_bf* out, uint32_t x, size_t p, int engine, int threads) {
     bf_move(out, &o);
     return 0;
   }
-  size_t s = p / 2 + 1;
-  if (p == 1) s = 0;
-  if (p == 2) s = 1;
+  size_t s = newton_half(p);
   orc_bf T, temp, xb, one, t;
   bf_init(&T); bf_init(&temp); bf_init(&xb); bf_init(&one); bf_init(&t);
   int rc = orc_bf_invsqrt(&T, x, s, engine, threads);
@@ -796,7 +806,7 @@ int orc_bf_invsqrt(orc_bf* out, uint32_t x, size_t p, int engine, int threads) {
   temp.exp--;
   orc_bf_mul(&t, &temp, &T, p, engine, threads);        /* ... * r0       */
   bf_move(&temp, &t);
-  orc_bf_sub(out, &T, &temp, p);
+  orc_bf_sub(out, &T, &temp, g_plant == 1 ? p - 1 : p);
   orc_bf_free(&T); orc_bf_free(&temp); orc_bf_free(&xb); orc_bf_free(&one); orc_bf_free(&t);
   return 0;
 }

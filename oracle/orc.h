@@ -74,6 +74,8 @@ void orc_bf_mul(orc_bf* z, const orc_bf* a, const orc_bf* b, size_t p, int engin
 int orc_bf_rcp(orc_bf* z, const orc_bf* a, size_t p, int engine, int threads);                   /* BigFloat.cpp:445-511 */
 int orc_bf_div(orc_bf* z, const orc_bf* a, const orc_bf* b, size_t p, int engine, int threads);  /* BigFloat.cpp:512-515 */
 int orc_bf_invsqrt(orc_bf* z, uint32_t x, size_t p, int engine, int threads);                    /* BigFloat.cpp:516-567 */
+/* Test self-checks only: 1 = p + 1 chop in the last sub of a Newton step, 2 = s = p / 2. */
+void orc_set_plant(int k);
 /* to_string_hex (BigFloat.cpp:95-106); returns bytes written (2 + digits) or -1. */
 long orc_bf_to_hex(const orc_bf* a, size_t digits, char* out, size_t cap);
 

@@ -7,8 +7,6 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
-#include <iomanip>
-#include <sstream>
 
 #include "../../include/picugpu.h"
 #include "Newton.h"
@@ -91,18 +89,7 @@ BigFloat BigFloat::from_parts(bool s, int64_t e, const uint32_t* limbs, size_t n
   return z;
 }
 
-// ------------------------------------------------ strings (BigFloat.cpp:83-106)
-std::string BigFloat::to_string_dec(size_t digits) const {
-  BigFloat temp = *this;
-  std::stringstream s;
-  s << temp.T[temp.L - 1] << '.';
-  for (size_t i = 0; i < (digits + 8) / 9; i++) {
-    temp.T[temp.L - 1] = 0;
-    temp = temp.mul(1000000000u);
-    s << std::setw(9) << std::setfill('0') << temp.T[temp.L - 1];
-  }
-  return s.str().substr(0, 2 + digits);
-}
+// ------------------------------------------------ strings (BigFloat.cpp:95-106)
 std::string BigFloat::to_string_hex(size_t digits) const {
   // top limb in decimal, '.', then 8 lowercase hex digits per lower limb
   const size_t nl = (digits + 7) / 8;

@@ -36,7 +36,6 @@ class BigFloat {
   BigFloat();
   BigFloat(uint32_t x, bool sign = true);
 
-  std::string to_string_dec(size_t digits = 0) const;
   std::string to_string_hex(size_t digits = 0) const;
   size_t get_precision() const { return L; }
   int64_t get_exponent() const { return exp; }

@@ -1,8 +1,11 @@
 // C entry points of the host library libpiculator.so (for Python tests and
 // bench.py): pi computation and single BigFloat operations on the GPU
-// BigFloat.  C++ exceptions (the reference's string literals) are turned into
-// a status code and kept for pch_last_error().
+// BigFloat.  C++ exceptions (the reference's string literals, and any
+// std::exception such as std::bad_alloc from a host allocation) are turned
+// into a status code and kept for pch_last_error(); none crosses extern "C".
 #include <cstring>
+#include <exception>
+#include <new>
 #include <string>
 
 #include "../../include/picugpu.h"
@@ -67,6 +70,12 @@ long pch_pi_hex(uint64_t digits, int mode, char* out, uint64_t cap, double* time
     return (long)n;
   } catch (const char* e) {
     return catch_all(e);
+  } catch (const std::bad_alloc&) {
+    return catch_all("out of host memory");
+  } catch (const std::exception& e) {
+    return catch_all(e.what());
+  } catch (...) {
+    return catch_all("unknown C++ exception");
   }
 }
 
@@ -92,6 +101,12 @@ int pch_bf_op(int op, int sa, int64_t ea, const uint32_t* la, uint64_t na, int s
     return export_bf(z, sz, ez, lz, cap, nz);
   } catch (const char* e) {
     return catch_all(e);
+  } catch (const std::bad_alloc&) {
+    return catch_all("out of host memory");
+  } catch (const std::exception& e) {
+    return catch_all(e.what());
+  } catch (...) {
+    return catch_all("unknown C++ exception");
   }
 }
 
@@ -126,6 +141,12 @@ int pch_df_op(int op, int sa, int64_t ea, const uint32_t* la, uint64_t na, int s
     return export_bf(z.to_host(), sz, ez, lz, cap, nz);
   } catch (const char* e) {
     return catch_all(e);
+  } catch (const std::bad_alloc&) {
+    return catch_all("out of host memory");
+  } catch (const std::exception& e) {
+    return catch_all(e.what());
+  } catch (...) {
+    return catch_all("unknown C++ exception");
   }
 }
 

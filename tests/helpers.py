@@ -85,3 +85,46 @@ def pi_spot_checks(s: bytes | str) -> dict:
             res["prefix_ok"] &= hashlib.sha256(pre).hexdigest() == fx["prefix_sha256"]
     res["ok"] = bool(res["bbp_ok"] and res["prefix_ok"] and res["bbp_positions"])
     return res
+
+
+def limb_digest(limbs) -> str:
+    """SHA-256 of the little-endian uint32 limbs (the mul_digests.json form)."""
+    import hashlib
+
+    return hashlib.sha256(np.ascontiguousarray(limbs, dtype="<u4").tobytes()).hexdigest()
+
+
+_MUL_DIGESTS = None
+
+
+def mul_digests() -> dict:
+    """tests/golden/mul_digests.json: oracle product digests of the workloads."""
+    global _MUL_DIGESTS
+    if _MUL_DIGESTS is None:
+        import json
+        import os
+
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mul_digests.json")
+        with open(path) as f:
+            _MUL_DIGESTS = json.load(f)
+    return _MUL_DIGESTS
+
+
+def single_product_digest(na: int, nb: int) -> str:
+    return mul_digests()["single"]["%dx%d" % (na, nb)]["sha256"]
+
+
+def oracle_threads() -> int:
+    """Threads for oracle products inside tests: the CPU share of a GPU box
+    is 16 cores whatever os.cpu_count() reports there."""
+    import os
+
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def oracle_product(a, b) -> np.ndarray:
+    """The oracle's exact product (its restatement of the reference NTT
+    multiply), multi-threaded; for operands too large for Python integers."""
+    from oracle import oracle as O
+
+    return O.mul_limbs(a, b, O.ENGINE_NTT, oracle_threads())

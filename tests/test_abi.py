@@ -163,3 +163,24 @@ def test_bsr_plan_check_rejects_bad_tables():
         picugpu.bsr_plan_check(100, [1] * 12)
     with pytest.raises(picugpu.PcgError):
         picugpu.bsr_plan_check(100, [0] * 12 + [5])  # class 5 is not a class
+
+
+def test_host_library_maps_allocation_failure_to_status():
+    """A std::bad_alloc inside the host library (here: an operand of 2^60
+    limbs, whose limb array cannot be allocated) comes back as status -1
+    with a message instead of crossing extern "C" and terminating the
+    process.  No GPU call is made: the operand is built before any product."""
+    import ctypes as C
+
+    import numpy as np
+
+    from piculator_amd import pihost
+
+    L = pihost.load()
+    la = np.ones(4, dtype=np.uint32)
+    sz, ez, nz = C.c_int(), C.c_int64(), C.c_uint64()
+    out = np.zeros(8, dtype=np.uint32)
+    rc = L.pch_bf_op(1, 1, 0, la.ctypes.data, 1 << 60, 1, 0, la.ctypes.data, 1, 0,
+                     C.byref(sz), C.byref(ez), out.ctypes.data, out.size, C.byref(nz))
+    assert rc == -1
+    assert b"memory" in L.pch_last_error()

@@ -5,7 +5,12 @@ Closed forms used (values are sum T[i] 2^(32 (exp + i)), sign-magnitude):
     chopped to their top p+2 limbs (p = 0: no chop), one zero top limb trimmed;
   * add/sub (BigFloat.cpp:184-325): exact sum/difference of the operands
     with every word below top - (p+2) dropped;
-  * rcp/div/invsqrt (BigFloat.cpp:445-567): accuracy bounds;
+  * rcp/div/invsqrt (BigFloat.cpp:445-567): the IEEE-double seeds at p = 0,
+    the exponent and length the last sub of a Newton step fixes, an
+    accuracy bound at the p + 2-limb working precision, and one Newton step
+    rebuilt from the mul/sub closed forms above on the oracle's own
+    half-precision result (the schedule s = p/2 + 1 read from
+    BigFloat.cpp:502-504,553-555 -- pinned by this restatement only);
   * Pi (PI.h:138-210): hex digits of pi (mpmath) on the correct prefix,
     Bailey-Borwein-Plouffe digit extraction at spot positions.
 """
@@ -119,6 +124,184 @@ def test_invsqrt_accuracy(p):
     r = O.invsqrt(10005, p)
     err = abs(val(r) ** 2 * 10005 - 1)
     assert err < Fraction(2) ** (-32 * (p - 1)) * 8
+
+
+# ---- Newton steps rebuilt from the closed forms (exact rational arithmetic)
+class X:
+    """(sign, exp, magnitude m): value (-1)^(!sign) m 2^(32 exp); its limb
+    count is that of m (every BigFloat result has a nonzero top limb)."""
+
+    def __init__(self, sign, exp, m):
+        self.sign, self.exp, self.m = (True, 0, 0) if m == 0 else (bool(sign), exp, m)
+
+    @staticmethod
+    def of(bf):
+        s, e, limbs = bf.tuple()
+        return X(s, e, to_int(limbs))
+
+    @property
+    def L(self):
+        return (self.m.bit_length() + 31) // 32
+
+    def tuple(self):
+        return (self.sign, self.exp, [(self.m >> (32 * i)) & 0xFFFFFFFF for i in range(self.L)])
+
+
+def x_chop(x, pp):
+    """x cut to its top pp limbs (BigFloat.cpp:355-366)."""
+    c = x.L - pp
+    return X(x.sign, x.exp + c, x.m >> (32 * c)) if c > 0 else x
+
+
+def x_mul(a, b, p):
+    """BigFloat::mul closed form: exact product of the chopped operands."""
+    if a.m == 0 or b.m == 0:
+        return X(True, 0, 0)
+    pp = a.L + b.L if p == 0 else p + 2
+    a, b = x_chop(a, pp), x_chop(b, pp)
+    return X(a.sign == b.sign, a.exp + b.exp, a.m * b.m)
+
+
+def x_add(a, b, p):
+    """BigFloat::add closed form: words below top - (p + 2) dropped."""
+    if a.m == 0:
+        return b
+    if b.m == 0:
+        return a
+    top = max(a.exp + a.L, b.exp + b.L)
+    bot = min(a.exp, b.exp)
+    if p and top - bot > p + 2:
+        bot = top - (p + 2)
+
+    def aligned(x):
+        v = (x.m >> (32 * (bot - x.exp))) if bot > x.exp else x.m << (32 * (x.exp - bot))
+        return v if x.sign else -v
+
+    v = aligned(a) + aligned(b)
+    return X(v >= 0, bot, abs(v))
+
+
+def x_sub(a, b, p):
+    return x_add(a, X(not b.sign, b.exp, b.m), p)
+
+
+def newton_half(p):
+    """BigFloat.cpp:502-504 (rcp) and 553-555 (invsqrt)."""
+    return 0 if p == 1 else (1 if p == 2 else p // 2 + 1)
+
+
+def rcp_step(x, T, p):
+    """r1 = r0 - (r0 x - 1) r0 at precision p (BigFloat.cpp:510)."""
+    one = X(True, 0, 1)
+    return x_sub(T, x_mul(x_sub(x_mul(x, T, p), one, p), T, p), p)
+
+
+def invsqrt_step(xv, T, p):
+    """r1 = r0 - (r0^2 x - 1) / 2 r0 at precision p (BigFloat.cpp:561-567)."""
+    one = X(True, 0, 1)
+    t = x_mul(T, T, p)
+    t = x_mul(t, X(True, 0, xv), 0)
+    t = x_sub(t, one, p)
+    t = X(t.sign, t.exp - 1, t.m * 2147483648)
+    t = x_mul(t, T, p)
+    return x_sub(T, t, p)
+
+
+NEWTON_P = list(range(1, 24)) + [33, 64, 100, 257]
+
+
+def newton_operand(p):
+    rng = random.Random(p)
+    return rand_bf(rng, 50, exp=-10, sign=bool(p % 2))
+
+
+def top_position(v: Fraction) -> int:
+    """t with 2^(32 (t - 1)) <= |v| < 2^(32 t)."""
+    v, t = abs(v), 0
+    while v >= Fraction(2) ** (32 * t):
+        t += 1
+    while v < Fraction(2) ** (32 * (t - 1)):
+        t -= 1
+    return t
+
+
+def test_newton_seeds_are_ieee_doubles():
+    """p = 0 seeds (BigFloat.cpp:459-499, 527-551): 1/x of the top three
+    limbs and 1/sqrt(x) in IEEE double, scaled by 2^32 until >= 2^32, kept
+    as two limbs -- recomputed here with Python floats (the same IEEE
+    operations, correctly rounded)."""
+    import math
+
+    for p in (1, 5, 33):
+        x = newton_operand(p)
+        r = x.rcp(0)
+        limbs, e = x.limbs[-3:].tolist(), x.exp + max(0, len(x.limbs) - 3)
+        v = float(limbs[0]) + limbs[1] * 4294967296. + limbs[2] * 4294967296. * 4294967296.
+        v, ex = 1. / v, -e
+        while v < 4294967296.:
+            v *= 4294967296.
+            ex -= 1
+        w = int(v)
+        assert r.tuple() == (x.sign, ex, [w & 0xFFFFFFFF, w >> 32])
+    for xv in (1, 2, 10005, 0xFFFFFFFF):
+        v, ex = 1. / math.sqrt(float(xv)), 0
+        while v < 4294967296.:
+            v *= 4294967296.
+            ex -= 1
+        w = int(v)
+        assert O.invsqrt(xv, 0).tuple() == (True, ex, [w & 0xFFFFFFFF, w >> 32])
+
+
+def _newton_invariants_hold(p) -> bool:
+    """What the last sub of a Newton step fixes: the result's bottom is cut
+    p + 2 limbs below the top of 1/x (so exp = top - (p + 2)), its length is
+    p + 2 (+1 carry limb, -1 stripped zero), and |r x - 1| stays within a
+    few units of 2^(-32 (p + 1)), the p + 2-limb working precision."""
+    x = newton_operand(p)
+    r = x.rcp(p)
+    t = top_position(1 / val(x))
+    ok = r.exp == t - (p + 2) and p + 1 <= len(r.limbs) <= p + 3
+    ok &= abs(val(r) * val(x) - 1) < 4 * Fraction(2) ** (-32 * (p + 1))
+    q = O.invsqrt(10005, p)
+    ok &= q.exp == -(p + 2) and p + 1 <= len(q.limbs) <= p + 3  # 1/sqrt(10005) < 2^0
+    ok &= abs(val(q) ** 2 * 10005 - 1) < 4 * Fraction(2) ** (-32 * (p + 1))
+    return bool(ok)
+
+
+@pytest.mark.parametrize("p", NEWTON_P)
+def test_newton_result_invariants(p):
+    assert _newton_invariants_hold(p)
+
+
+@pytest.mark.parametrize("p", NEWTON_P)
+def test_newton_step_from_closed_forms(p):
+    """rcp(p) and invsqrt(p) equal one Newton step, computed with the
+    mul/sub closed forms in exact arithmetic, on the oracle's own result at
+    the half precision s = p/2 + 1: limb for limb, exponent and sign."""
+    s = newton_half(p)
+    x = newton_operand(p)
+    assert x.rcp(p).tuple() == rcp_step(X.of(x), X.of(x.rcp(s)), p).tuple()
+    assert O.invsqrt(10005, p).tuple() == invsqrt_step(10005, X.of(O.invsqrt(10005, s)), p).tuple()
+    y = rand_bf(random.Random(p + 7), 40, exp=3, sign=False)
+    assert y.div(x, p).tuple() == x_mul(X.of(y), X.of(x.rcp(p)), p).tuple()
+
+
+def test_planted_newton_mistakes_are_caught():
+    """A p + 1 chop in the last sub of each Newton step fails the
+    invariants; the schedule s = p/2 passes every accuracy bound (its error
+    is as small as the right schedule's) but fails the rebuilt step."""
+    try:
+        O.set_plant(1)
+        assert not all(_newton_invariants_hold(p) for p in (3, 5, 33))
+        O.set_plant(2)
+        assert all(_newton_invariants_hold(p) for p in (5, 33))
+        caught = False
+        for p in (5, 33, 64):
+            x = newton_operand(p)
+            caught |= x.rcp(p).tuple() != rcp_step(X.of(x), X.of(x.rcp(newton_half(p))), p).tuple()
+        assert caught
+    finally:
+        O.set_plant(0)
 
 
 def test_divide_by_zero():

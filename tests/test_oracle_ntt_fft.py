@@ -159,3 +159,28 @@ def test_planted_mistake_is_caught():
     x = np.arange(1, n + 1, dtype=np.uint64)
     wrong = brute_dft(x, pow(W, 3, p), p)  # W^3 instead of W
     assert [int(v) for v in O.ntt_forward(x, k)] != wrong
+
+
+def test_mul_digest_fixture_reproduces():
+    """tests/golden/mul_digests.json (written by make_mul_digests.py): the
+    small single products and a sample of the batch reproduce from the
+    oracle and from Python integers; every entry has the workload's shape."""
+    from piculator_amd import workload
+    from tests.helpers import limb_digest, mul_digests
+
+    fx = mul_digests()
+    for key in ("65536x65536", "65536x32768"):
+        e = fx["single"][key]
+        a, b = workload.operand_pair(e["na"], e["nb"])
+        z = O.mul_limbs(a, b, O.ENGINE_REFERENCE, 2)
+        assert limb_digest(z) == e["sha256"]
+        assert np.array_equal(z, exact_product(a, b))
+    for logn in (16, 20, 24, 26):
+        n = 1 << logn
+        for nb in (n, n // 2):
+            assert ("%dx%d" % (n, nb)) in fx["single"]
+    pairs = workload.batch_operands(4096)
+    assert len(fx["batch"]["sha256"]) == 4096
+    for i in (0, 1, 777, 2048, 4095):
+        a, b = pairs[i]
+        assert limb_digest(exact_product(a, b)) == fx["batch"]["sha256"][i]
