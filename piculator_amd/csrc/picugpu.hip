@@ -74,10 +74,21 @@ struct Pass {
   int logL, logS, logLines, logG, logB;
 };
 
+// Passes of one transform in a fixed array (at most 4 for 2^30 points), so
+// planning a product never touches the heap.
+struct PassList {
+  static constexpr size_t kCap = 6;
+  Pass v[kCap];
+  size_t n = 0;
+  void push_back(const Pass& p) { v[n++] = p; }
+  size_t size() const { return n; }
+  const Pass& operator[](size_t i) const { return v[i]; }
+};
+
 struct Plan {
   int logN;
   int logE;
-  std::vector<Pass> passes;
+  PassList passes;
 };
 
 // NTT pass plan: strided passes take at most 2^8-point lines, so a
@@ -413,13 +424,17 @@ void launch_carry_scan(Ctx& c, const CarryArgs& a) {
   hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntot), dim3(kScanChunk), 0, c.stream, a);
 }
 
-// Resident workgroups per compute unit of a pass kernel instance (cached).
+// Resident workgroups per compute unit of a pass kernel instance, cached in
+// a fixed table (no heap use: the allocation-free entry point plans through
+// it); an instance beyond the table's capacity is queried every time.
 int pass_occupancy(PassFn fn, int threads) {
   static std::mutex mu;
-  static std::map<PassFn, int> memo;
+  static PassFn keys[128];
+  static int vals[128];
+  static int used = 0;
   std::lock_guard<std::mutex> lk(mu);
-  auto it = memo.find(fn);
-  if (it != memo.end()) return it->second;
+  for (int i = 0; i < used; ++i)
+    if (keys[i] == fn) return vals[i];
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(fn), threads, 0) !=
           hipSuccess ||
@@ -427,7 +442,11 @@ int pass_occupancy(PassFn fn, int threads) {
     (void)hipGetLastError();
     nb = 2;
   }
-  memo[fn] = nb;
+  if (used < 128) {
+    keys[used] = fn;
+    vals[used] = nb;
+    ++used;
+  }
   return nb;
 }
 
@@ -566,9 +585,17 @@ int ntt_wrap_top(Ctx& c, const WrapPlan& wp, const uint32_t* d_a, uint64_t na, c
   return ntt_tail_into(c, pm, d_b + (nb - wp.w), wp.w, A2, B2);
 }
 
+// Scratch of the carry scan: block functions, scan totals, carry-ins.
+struct CarryScratch {
+  CarryFn* agg;
+  CarryFn* tot;
+  uint64_t* cin;
+};
+
 // CRT and carries of the residues in B into d_out[0, zL); wp.use: B holds a
 // wrapped convolution and c.top_spec the top coefficients.
-int ntt_carry(Ctx& c, const Plan& pl, const double* B, uint64_t zL, uint32_t* d_out, const WrapPlan& wp) {
+int ntt_carry(Ctx& c, const Plan& pl, const double* B, uint64_t zL, uint32_t* d_out, const WrapPlan& wp,
+              const CarryScratch& cs) {
   CarryArgs ca;
   memset(&ca, 0, sizeof ca);
   ca.prm = c.d_prime + pl.logN * 2;
@@ -576,10 +603,10 @@ int ntt_carry(Ctx& c, const Plan& pl, const double* B, uint64_t zL, uint32_t* d_
   ca.N = 1ull << pl.logN;
   ca.zL = zL;
   ca.out = d_out;
-  ca.agg = c.agg;
-  ca.cin = c.cin;
+  ca.agg = cs.agg;
+  ca.cin = cs.cin;
   ca.nblk = (zL + kCarryBlock - 1) / kCarryBlock;
-  ca.tot = c.agg + c.nblk_cap;
+  ca.tot = cs.tot;
   ca.crt = c.crt;
   ca.status = c.status;
   if (wp.use) {
@@ -594,6 +621,10 @@ int ntt_carry(Ctx& c, const Plan& pl, const double* B, uint64_t zL, uint32_t* d_
                      ca);
   HIPCHK(hipGetLastError());
   return PCG_OK;
+}
+
+int ntt_carry(Ctx& c, const Plan& pl, const double* B, uint64_t zL, uint32_t* d_out, const WrapPlan& wp) {
+  return ntt_carry(c, pl, B, zL, d_out, wp, CarryScratch{c.agg, c.agg + c.nblk_cap, c.cin});
 }
 
 // The operand whose transform spec's first half (and top_spec's, for a
@@ -2476,17 +2507,8 @@ int df_mul_u32_locked(Ctx& c, pcg_dfloat* z, const pcg_dfloat* pa, uint32_t x) {
   return PCG_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int pcg_init(int device) {
-  Ctx& c = g_ctx;
-  std::lock_guard<std::mutex> lk(c.mu);
-  if (c.ready) {
-    if (device != c.device) return fail(PCG_ERR_INVALID, "already initialised on device %d", c.device);
-    return PCG_OK;
-  }
+// Set-up of pcg_init; on failure the caller releases whatever was built.
+int init_locked(Ctx& c, int device) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
     (void)hipGetLastError();
@@ -2515,43 +2537,42 @@ int pcg_init(int device) {
   return PCG_OK;
 }
 
-void pcg_shutdown(void) {
-  Ctx& c = g_ctx;
-  std::lock_guard<std::mutex> lk(c.mu);
-  if (!c.ready) return;
-  (void)hipStreamSynchronize(c.stream);
+// Frees every buffer, event and the stream (each only if it exists) and
+// returns the context to its initial state.
+void release_locked(Ctx& c) {
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
   for (int pi = 0; pi < 2; ++pi)
-    for (int t = 0; t < 3; ++t) (void)hipFree(c.tw[pi][t]);
-  for (int t = 0; t < 3; ++t) (void)hipFree(c.ftab[t]);
-  (void)hipFree(c.d_maxdist);
-  (void)hipFree(c.d_prime);
-  (void)hipFree(c.spec);
-  (void)hipFree(c.top_spec);
-  (void)hipFree(c.agg);
-  (void)hipFree(c.cin);
-  (void)hipFree(c.status);
-  (void)hipFree(c.d_stage);
-  (void)hipHostFree(c.h_pool);
-  (void)hipFree(c.d_pool);
-  (void)hipHostFree(c.h_desc);
-  (void)hipFree(c.d_desc);
-  (void)hipFree(c.bspec);
-  (void)hipFree(c.wrap_desc);
-  (void)hipFree(c.bagg);
-  (void)hipFree(c.bcin);
+    for (int t = 0; t < 3; ++t) {
+      (void)hipFree(c.tw[pi][t]);
+      c.tw[pi][t] = nullptr;
+    }
+  for (int t = 0; t < 3; ++t) {
+    (void)hipFree(c.ftab[t]);
+    c.ftab[t] = nullptr;
+  }
   for (BsrPlan& q : g_bsr_plans) bsr_plan_free(q);
-  (void)df_release_pool(c);
-  (void)hipFree(c.df_agg);
-  (void)hipFree(c.df_cin);
-  (void)hipFree(c.df_slot);
-  (void)hipHostFree(c.h_slot);
-  for (int i = 0; i < 4; ++i) (void)hipEventDestroy(c.ev[i]);
-  for (int i = 0; i < 2; ++i) (void)hipEventDestroy(c.tev[i]);
-  (void)hipStreamDestroy(c.stream);
-  Ctx fresh_defaults;
+  if (c.stream) (void)df_release_pool(c);
+  void* dev[] = {c.d_maxdist, c.d_prime, c.spec, c.top_spec, c.agg, c.cin, c.status, c.d_stage,
+                 c.d_pool, c.d_desc, c.bspec, c.wrap_desc, c.bagg, c.bcin, c.df_agg, c.df_cin,
+                 c.df_slot};
+  for (void* q : dev) (void)hipFree(q);
+  void* host[] = {c.h_pool, c.h_desc, c.h_slot};
+  for (void* q : host) (void)hipHostFree(q);
+  for (int i = 0; i < 4; ++i) {
+    if (c.ev[i]) (void)hipEventDestroy(c.ev[i]);
+    c.ev[i] = nullptr;
+  }
+  for (int i = 0; i < 2; ++i) {
+    if (c.tev[i]) (void)hipEventDestroy(c.tev[i]);
+    c.tev[i] = nullptr;
+  }
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  (void)hipGetLastError();
   c.ready = false;
   c.device = -1;
   c.stream = nullptr;
+  c.d_maxdist = nullptr;
+  c.d_prime = nullptr;
   c.spec = nullptr;
   c.spec_points = 0;
   c.top_spec = nullptr;
@@ -2575,14 +2596,35 @@ void pcg_shutdown(void) {
   c.bagg = nullptr;
   c.bcin = nullptr;
   c.bcarry_cap = 0;
-  c.d_prime = nullptr;
   c.prof_valid = false;
   c.df_agg = nullptr;
   c.df_cin = nullptr;
   c.df_nblk_cap = 0;
   c.df_slot = nullptr;
   c.h_slot = nullptr;
-  (void)fresh_defaults;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pcg_init(int device) {
+  Ctx& c = g_ctx;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (c.ready) {
+    if (device != c.device) return fail(PCG_ERR_INVALID, "already initialised on device %d", c.device);
+    return PCG_OK;
+  }
+  const int rc = init_locked(c, device);
+  if (rc) release_locked(c);  // nothing half-built survives a failed init
+  return rc;
+}
+
+void pcg_shutdown(void) {
+  Ctx& c = g_ctx;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (!c.ready) return;
+  release_locked(c);
 }
 
 const char* pcg_last_error_string(void) { return g_err.c_str(); }
