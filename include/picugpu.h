@@ -79,6 +79,20 @@ int pcg_mul(const uint32_t* a, uint64_t na, const uint32_t* b, uint64_t nb, uint
 int pcg_mul_dev(const uint32_t* d_a, uint64_t na, const uint32_t* d_b, uint64_t nb,
                 uint32_t* d_out, uint32_t flags);
 
+/* pcg_mul_dev for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The same exact product on the NTT engine
+ * (PCG_ENGINE_AUTO or PCG_ENGINE_NTT; PCG_SYNC as for pcg_mul_dev), with
+ * every scratch buffer carved from one static device array of 32.1 MiB that
+ * each GPU holds from library load to process end: the call makes no
+ * hipMalloc, hipFree or other allocation.  Limit: na + nb <= 2^20 limbs,
+ * else PCG_ERR_TOO_LARGE (nothing is launched).  All calls on a GPU share
+ * its array and leave their transforms in it; calls that overlap on one GPU
+ * (e.g. graph replays on different streams) race. */
+#define PCG_NOALLOC_MAX_LIMBS (1ull << 20)
+int pcg_mul_dev_noalloc(const uint32_t* d_a, uint64_t na, const uint32_t* d_b, uint64_t nb,
+                        uint32_t* d_out, uint32_t flags);
+
 /* One product of a batch. */
 typedef struct pcg_mul_desc {
   const uint32_t* a;

@@ -258,9 +258,32 @@ struct Ctx {
   size_t df_nblk_cap = 0;
   unsigned long long* df_slot = nullptr;  // [4] device
   unsigned long long* h_slot = nullptr;   // [4] pinned host
+  // device address of the static scratch array of pcg_mul_dev_noalloc on
+  // this context's device (hipGetSymbolAddress at pcg_init)
+  unsigned char* noalloc = nullptr;
 };
 
 Ctx g_ctx;
+
+// ---- scratch of the allocation-free product (pcg_mul_dev_noalloc) ----
+// Products of na + nb <= 2^20 limbs run the plain (unwrapped) transform of
+// at most 2^20 points: spectra [2 operands][2 primes][2^20] doubles, then the
+// carry scan's block functions, scan totals and carry-ins.
+constexpr uint64_t kNoAllocMaxLimbs = 1ull << 20;
+constexpr int kNoAllocMaxLogN = 20;
+constexpr size_t kNoAllocSpecBytes = (size_t)32 << kNoAllocMaxLogN;
+constexpr size_t kNoAllocBlocks = kNoAllocMaxLimbs / kCarryBlock + 1;
+constexpr size_t kNoAllocAggEntries = kNoAllocBlocks + kNoAllocBlocks / kScanChunk + 2;
+constexpr size_t kNoAllocAggBytes = (kNoAllocAggEntries * sizeof(CarryFn) + 255) & ~(size_t)255;
+constexpr size_t kNoAllocCinBytes = (kNoAllocBlocks * sizeof(uint64_t) + 255) & ~(size_t)255;
+constexpr size_t kNoAllocScratchBytes = kNoAllocSpecBytes + kNoAllocAggBytes + kNoAllocCinBytes;
+}  // namespace
+
+// In host code `scratch` names a host-side stand-in, not device memory: only
+// the address from hipGetSymbolAddress (Ctx::noalloc) reaches kernels.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
 
 int ceil_log2(uint64_t x) {
   int k = 0;
@@ -2507,6 +2530,40 @@ int df_mul_u32_locked(Ctx& c, pcg_dfloat* z, const pcg_dfloat* pa, uint32_t x) {
   return PCG_OK;
 }
 
+// The product of pcg_mul_dev on the NTT engine with every scratch buffer
+// carved from the static array (c.noalloc): no allocation of any kind.
+int mul_noalloc_locked(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_b, uint64_t nb,
+                       uint32_t* d_out, uint32_t flags) {
+  const uint32_t eng = flags & PCG_ENGINE_MASK;
+  if (eng != PCG_ENGINE_AUTO && eng != PCG_ENGINE_NTT)
+    return fail(PCG_ERR_INVALID, "pcg_mul_dev_noalloc runs the NTT engine only");
+  const uint64_t zL = na + nb;
+  if (na > kNoAllocMaxLimbs || nb > kNoAllocMaxLimbs || zL > kNoAllocMaxLimbs)
+    return fail(PCG_ERR_TOO_LARGE,
+                "pcg_mul_dev_noalloc: na + nb = %llu limbs exceeds its static scratch limit of %llu limbs",
+                (unsigned long long)zL, (unsigned long long)kNoAllocMaxLimbs);
+  if (zL == 0) return PCG_OK;
+  if (na == 0 || nb == 0) {
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((zL + 255) / 256)), dim3(256), 0, c.stream, d_out, zL);
+    HIPCHK(hipGetLastError());
+    return PCG_OK;
+  }
+  const int logN = ntt_log_points(zL);
+  if (logN > kNoAllocMaxLogN) return fail(PCG_ERR_INTERNAL, "noalloc transform of 2^%d points", logN);
+  const Plan pl = make_plan(logN);
+  const uint64_t N = 1ull << logN;
+  double* A = reinterpret_cast<double*>(c.noalloc);
+  double* B = A + 2 * N;
+  CarryFn* agg = reinterpret_cast<CarryFn*>(c.noalloc + kNoAllocSpecBytes);
+  uint64_t* cin = reinterpret_cast<uint64_t*>(c.noalloc + kNoAllocSpecBytes + kNoAllocAggBytes);
+  // every point of A and B and every carry entry read below is written
+  // first by this product, so nothing needs clearing between calls
+  int rc = ntt_forward_into(c, pl, d_a, na, A);
+  if (!rc) rc = ntt_tail_into(c, pl, d_b, nb, A, B);
+  if (!rc) rc = ntt_carry(c, pl, B, zL, d_out, WrapPlan(), CarryScratch{agg, agg + kNoAllocBlocks, cin});
+  return rc;
+}
+
 // Set-up of pcg_init; on failure the caller releases whatever was built.
 int init_locked(Ctx& c, int device) {
   int n = 0;
@@ -2532,6 +2589,9 @@ int init_locked(Ctx& c, int device) {
   for (int i = 0; i < 2; ++i) HIPCHK(hipEventCreate(&c.tev[i]));
   for (int b = 0; b <= kSmallNttMaxLog; ++b) g_small_class[b] = small_class_default(b);
   g_small_ready = false;
+  // this device's copy of the allocation-free product's scratch array
+  HIPCHK(hipGetSymbolAddress(reinterpret_cast<void**>(&c.noalloc), HIP_SYMBOL(scratch)));
+  if (!c.noalloc) return fail(PCG_ERR_HIP, "no device address for the static scratch array");
   c.device = device;
   c.ready = true;
   return PCG_OK;
@@ -2602,6 +2662,7 @@ void release_locked(Ctx& c) {
   c.df_nblk_cap = 0;
   c.df_slot = nullptr;
   c.h_slot = nullptr;
+  c.noalloc = nullptr;
 }
 
 }  // namespace
@@ -2646,6 +2707,18 @@ int pcg_mul_dev(const uint32_t* d_a, uint64_t na, const uint32_t* d_b, uint64_t 
   if (!c.ready) return fail(PCG_ERR_NOT_INIT, "pcg_init has not been called");
   if ((na && !d_a) || (nb && !d_b) || (na + nb && !d_out)) return fail(PCG_ERR_INVALID, "null pointer");
   int rc = mul_dev_locked(c, d_a, na, d_b, nb, d_out, flags);
+  if (rc) return rc;
+  if (flags & PCG_SYNC) return sync_locked(c);
+  return PCG_OK;
+}
+
+int pcg_mul_dev_noalloc(const uint32_t* d_a, uint64_t na, const uint32_t* d_b, uint64_t nb,
+                        uint32_t* d_out, uint32_t flags) {
+  Ctx& c = g_ctx;
+  std::lock_guard<std::mutex> lk(c.mu);
+  if (!c.ready) return fail(PCG_ERR_NOT_INIT, "pcg_init has not been called");
+  if ((na && !d_a) || (nb && !d_b) || (na + nb && !d_out)) return fail(PCG_ERR_INVALID, "null pointer");
+  int rc = mul_noalloc_locked(c, d_a, na, d_b, nb, d_out, flags);
   if (rc) return rc;
   if (flags & PCG_SYNC) return sync_locked(c);
   return PCG_OK;

@@ -30,7 +30,7 @@ STATUS = {
 # Every symbol include/picugpu.h declares.
 EXPORTS = [
     "pcg_init", "pcg_shutdown", "pcg_last_error_string", "pcg_reserve", "pcg_mul",
-    "pcg_mul_dev", "pcg_mul_batch", "pcg_mul_batch_dev", "pcg_sync", "pcg_dev_alloc",
+    "pcg_mul_dev", "pcg_mul_dev_noalloc", "pcg_mul_batch", "pcg_mul_batch_dev", "pcg_sync", "pcg_dev_alloc",
     "pcg_dev_free", "pcg_host_alloc", "pcg_host_free", "pcg_memcpy_h2d", "pcg_memcpy_d2h",
     "pcg_fft_last_roundoff",
     "pcg_last_profile", "pcg_ntt_plan", "pcg_timer_start", "pcg_timer_stop",
@@ -84,6 +84,7 @@ def load():
     L.pcg_reserve.argtypes = [u64]
     L.pcg_mul.argtypes = [vp, u64, vp, u64, vp, C.c_uint32]
     L.pcg_mul_dev.argtypes = [vp, u64, vp, u64, vp, C.c_uint32]
+    L.pcg_mul_dev_noalloc.argtypes = [vp, u64, vp, u64, vp, C.c_uint32]
     L.pcg_mul_batch.argtypes = [C.POINTER(MulDesc), u64, C.c_uint32]
     L.pcg_mul_batch_dev.argtypes = [C.POINTER(MulDesc), u64, C.c_uint32]
     L.pcg_dev_alloc.argtypes = [C.POINTER(vp), u64]
@@ -229,6 +230,18 @@ def mul_dev(d_a: int, na: int, d_b: int, nb: int, d_out: int, engine: int = ENGI
     ensure_init()
     flags = engine | (SYNC if sync else 0) | (PROFILE if profile else 0)
     _check(load().pcg_mul_dev(d_a, na, d_b, nb, d_out, flags))
+
+
+NOALLOC_MAX_LIMBS = 1 << 20
+
+
+def mul_dev_noalloc(d_a: int, na: int, d_b: int, nb: int, d_out: int, engine: int = ENGINE_AUTO,
+                    sync: bool = False):
+    """pcg_mul_dev_noalloc: the product with the library's static scratch
+    array (na + nb <= NOALLOC_MAX_LIMBS); allocates nothing."""
+    ensure_init()
+    flags = engine | (SYNC if sync else 0)
+    _check(load().pcg_mul_dev_noalloc(d_a, na, d_b, nb, d_out, flags))
 
 
 def sync():
