@@ -1,28 +1,34 @@
 """Benchmark of pi to 10^9 decimal digits on one MI355X, with its big-number
-multiply hot path measured against the HBM roofline.
+multiply hot path measured against the HBM and FP64-mulmod rooflines.
 
 Step = one computation of pi.txt for --digits decimal digits (default 10^9)
 through the batched schedule: Chudnovsky binary splitting with the lower tree
 evaluated on the GPU (pcg_bsr_exact), the top combines, division, 1/sqrt(10005)
 and the final product on the GPU BigFloat, and the hex conversion.  W untimed
-steps, then K timed steps bracketed by device synchronisation; `value` is the
-mean wall time of a step.  The result of the last step is checked: BBP digits
-at fixed positions and the prefix digest of the oracle's 10^8-digit pi.txt
-(and the full digest where the oracle fixture has this digit count).
+steps, then K timed steps, each bracketed by device synchronisation; `value`
+is the mean wall time of a step.  The output of every timed step is checked
+outside its timed window (SHA-256 against the oracle's pi.txt digest where a
+fixture exists, BBP digits, the prefix digest of the oracle's 10^8-digit
+pi.txt); if any check fails, `value` is null and the exit status is 1.
 
 Also reported on the same JSON line:
   roofline      the 2^24 x 2^24-limb NTT multiply (operands resident in HBM):
                 algorithmic bytes / time vs 8 TB/s, time from HIP events on
                 the library stream; traffic from the committed counter profile
-  ntt_mul       that multiply's stage times and product check
+  mulmod        the same multiply's algorithmic mulmods per second against
+                the FP64 mulmod rate measured by a register-resident probe
+  ntt_mul       that multiply's stage times and full-product digest check
+  fft_mul       the same operands on the FP64 FFT engine: time, HBM
+                fraction, maximum round-off, NTT recompute, digest check
   batch         4096 uneven products: one batched call vs one launch sequence
                 per product
   pi_small      pi to 10^6 digits, batched vs plain schedule, both checked
                 against the oracle digest
   cpu_baseline  the oracle (CPU restatement of the reference, reference engine
-                choice) timed on pi to 10^7 digits and scaled to --digits by
-                the ratio of full oracle runs at both sizes (10^9: 87 min on
-                8 threads, committed with its pi.txt digest)
+                choice) timed in this run on the box's CPU share: pi to 10^8
+                digits beside the GPU's own 10^8 run, scaled to --digits by the
+                ratio of full oracle runs, and the oracle's 2^24 x 2^24
+                multiply (NTT at k = 27, BigFloat.cpp:417-441) beside the GPU's
 Prints exactly one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -51,42 +57,97 @@ def pi_cost(digits: int) -> float:
     return n * math.log2(n) ** 2
 
 
-def cpu_baseline(threads: int, digits: int, sample_digits: int = 10000000):
+def cpu_share():
+    """(threads, how): the reference's default thread count is
+    std::thread::hardware_concurrency() (main.cpp:6), the CPUs this process
+    may run on: its affinity mask, capped by a cgroup CPU quota and by
+    OMP_NUM_THREADS (the GPU box pins both to its CPU share)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    how = ["affinity %d" % n]
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            q = max(1, int(int(quota) / int(period)))
+            how.append("cgroup quota %d" % q)
+            n = min(n, q)
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        how.append("OMP_NUM_THREADS %s" % omp)
+        n = min(n, int(omp))
+    how.append("os.cpu_count %s" % os.cpu_count())
+    return max(1, n), ", ".join(how)
+
+
+def cpu_baseline(threads: int, how: str, digits: int, sample_digits: int, gpu_sample_s, logn: int,
+                 gpu_mul_ms: float):
     """The oracle's Pi() (reference engine crossover: FFT below k = 27, NTT
-    above, BigFloat.cpp:386) on a bounded sample, scaled to `digits`."""
+    above, BigFloat.cpp:386) on `sample_digits`, scaled to `digits`, and the
+    oracle's 2^logn x 2^logn multiply on the reference engine."""
     from oracle import oracle as O
+    from piculator_amd import workload
+    from tests.helpers import limb_digest, single_product_digest
 
     t0 = time.perf_counter()
     s, _ = O.pi_hex(sample_digits, O.ENGINE_REFERENCE, threads=threads)
     dt = time.perf_counter() - t0
     ok = None
-    gold = os.path.join(ROOT, "tests", "golden", "pi_digests.json")
-    with open(gold) as f:
+    with open(os.path.join(ROOT, "tests", "golden", "pi_digests.json")) as f:
         fx = json.load(f)
     if str(sample_digits) in fx:
         ok = hashlib.sha256(s.encode()).hexdigest() == fx[str(sample_digits)]["sha256"]
-    # scale: the ratio of full oracle runs at both sizes measured in the
-    # build container where they exist, else the N log^2 N model
+    del s
     with open(os.path.join(ROOT, "tests", "golden", "oracle_pi_times.json")) as f:
         secs = json.load(f)["seconds"]
     if str(digits) in secs and str(sample_digits) in secs:
         scale = secs[str(digits)] / secs[str(sample_digits)]
-        how = "the measured ratio of full oracle runs (%.0f s / %.2f s, 8 threads)" % (
+        scale_how = "the ratio of full oracle runs at both sizes (%.0f s / %.1f s, 8 threads)" % (
             secs[str(digits)], secs[str(sample_digits)])
     else:
         scale = pi_cost(digits) / pi_cost(sample_digits)
-        how = "N log^2 N in hex digits"
-    return {
+        scale_how = "N log^2 N in hex digits"
+    n = 1 << logn
+    a, b = workload.operand_pair(n)
+    t0 = time.perf_counter()
+    z = O.mul_limbs(a, b, O.ENGINE_REFERENCE, threads)
+    mul_s = time.perf_counter() - t0
+    mul_ok = limb_digest(z) == single_product_digest(n, n)
+    del z, a, b
+    res = {
         "value": dt * scale,
-        "unit": "s (pi to %d digits, extrapolated)" % digits,
+        "unit": "s (pi to %d digits; measured at %d, scaled)" % (digits, sample_digits),
         "cores": threads,
         "kind": "port",
-        "sample": "oracle Pi(%d) (reference engines, %d threads): %.2f s measured, digest %s; "
-                  "scaled x%.1f to %d digits by %s"
-                  % (sample_digits, threads, dt, "ok" if ok else ("MISMATCH" if ok is False else "n/a"),
-                     scale, digits, how),
+        "sample": "oracle Pi(%d), reference engines, %d threads (%s): %.2f s measured in this run, "
+                  "digest %s; x%.1f to %d digits by %s"
+                  % (sample_digits, threads, how, dt, "ok" if ok else ("MISMATCH" if ok is False else "n/a"),
+                     scale, digits, scale_how),
         "sample_s": dt,
+        "sample_digits": sample_digits,
+        "sample_digest_ok": ok,
+        "mul": {"na": n, "nb": n, "engine": "reference (NTT at k = 27, BigFloat.cpp:417-441)",
+                "threads": threads, "s": mul_s, "digest_ok": bool(mul_ok),
+                "gpu_ms": gpu_mul_ms, "gpu_speedup": mul_s * 1e3 / gpu_mul_ms},
     }
+    if gpu_sample_s:
+        res["gpu_sample_s"] = gpu_sample_s
+        res["same_size_speedup"] = dt / gpu_sample_s
+    return res
+
+
+def probe_rates():
+    """FP64 mulmod and FMA rates of the whole GPU (libpcg_probe.so)."""
+    import ctypes as C
+
+    lib = C.CDLL(os.path.join(ROOT, "piculator_amd", "libpcg_probe.so"))
+    mm, fma, cu = C.c_double(), C.c_double(), C.c_int()
+    if lib.pcg_probe_fp64_rates(C.byref(mm), C.byref(fma), C.byref(cu)) != 0:
+        raise RuntimeError("mulmod probe failed")
+    return {"mulmods_per_s": mm.value, "fma_per_s": fma.value, "compute_units": cu.value,
+            "probe": "register-resident chains of the engine's FP64 Shoup mulmod (6 FP64 ops) and of "
+                     "FP64 FMAs, 16 workgroups of 256 threads per CU (piculator_amd/csrc/probe_mulmod.hip)"}
 
 
 def ntt_mul(logn: int, steps: int, warmup: int):
@@ -95,7 +156,7 @@ def ntt_mul(logn: int, steps: int, warmup: int):
     import numpy as np
 
     from piculator_amd import picugpu, workload
-    from tests.helpers import residues_match
+    from tests.helpers import limb_digest, single_product_digest
 
     n = 1 << logn
     a, b = workload.operand_pair(n)
@@ -118,7 +179,8 @@ def ntt_mul(logn: int, steps: int, warmup: int):
     stages, total = picugpu.last_profile()
     out = np.empty(2 * n, dtype=np.uint32)
     dout.download(out)
-    ok = bool(residues_match(a, b, out)) and int(out[0]) == (int(a[0]) * int(b[0])) & 0xFFFFFFFF
+    ok = limb_digest(out) == single_product_digest(n, n)
+    del out
     for buf in (da, db, dout):
         buf.free()
     balg = workload.bytes_alg(n, n)
@@ -138,6 +200,41 @@ def ntt_mul(logn: int, steps: int, warmup: int):
     return roofline, info
 
 
+def fft_mul(logn: int, steps: int):
+    """The same operands on the FP64 FFT engine (PCG_ENGINE_FFT): mean time
+    from HIP events, round-off report, full-product digest."""
+    import numpy as np
+
+    from piculator_amd import picugpu, workload
+    from tests.helpers import limb_digest, single_product_digest
+
+    n = 1 << logn
+    a, b = workload.operand_pair(n)
+    da, db = picugpu.DeviceBuffer(a.nbytes), picugpu.DeviceBuffer(b.nbytes)
+    dout = picugpu.DeviceBuffer(a.nbytes + b.nbytes)
+    da.upload(a)
+    db.upload(b)
+    picugpu.mul_dev(da.ptr, n, db.ptr, n, dout.ptr, picugpu.ENGINE_FFT, sync=True)
+    picugpu.timer_start()
+    for _ in range(steps):
+        # each FFT product reads its round-off back (the recompute decision)
+        picugpu.mul_dev(da.ptr, n, db.ptr, n, dout.ptr, picugpu.ENGINE_FFT, sync=True)
+    ms = picugpu.timer_stop() / steps
+    dist, recomputed = picugpu.fft_last_roundoff()
+    out = np.empty(2 * n, dtype=np.uint32)
+    dout.download(out)
+    ok = limb_digest(out) == single_product_digest(n, n)
+    del out
+    for buf in (da, db, dout):
+        buf.free()
+    achieved = workload.bytes_alg(n, n) / (ms / 1e3) / 1e9
+    res = {"na": n, "nb": n, "ms": ms, "gbs_alg": achieved, "hbm_frac": achieved / HBM_PEAK_GBS,
+           "max_roundoff": dist, "ntt_recompute": bool(recomputed), "digest_ok": bool(ok),
+           "digits": "16-bit balanced, right-angle convolution in complex FP64"}
+    log("fft multiply: %.3f ms, round-off %.4f, check %s" % (ms, dist, ok))
+    return res
+
+
 def batch_compare(count: int = 4096, reps: int = 3):
     """Uneven batch (lengths log-uniform in 2^4..2^14 limbs, seed 0xC0DE),
     operands resident in HBM: one pcg_mul_batch_dev call vs the plain form
@@ -145,7 +242,7 @@ def batch_compare(count: int = 4096, reps: int = 3):
     import numpy as np
 
     from piculator_amd import picugpu, workload
-    from tests.helpers import residues_match
+    from tests.helpers import limb_digest, mul_digests
 
     pairs = workload.batch_operands(count)
     total = sum(a.size + b.size for a, b in pairs)
@@ -177,12 +274,12 @@ def batch_compare(count: int = 4096, reps: int = 3):
         res[name + "_ms"] = best
     res["speedup"] = res["plain_ms"] / res["batched_ms"]
     pool.download(host)
-    ok = True
-    for i in range(0, count, 97):
-        a, b = pairs[i]
+    want = mul_digests()["batch"]["sha256"] if count == 4096 else None
+    ok = want is not None
+    for i in range(count if want else 0):
         po, n = outs[i]
-        ok &= bool(residues_match(a, b, host[po:po + n]))
-    res["checked_products_ok"] = ok
+        ok &= limb_digest(host[po:po + n]) == want[i]
+    res["all_products_digest_ok"] = bool(ok)
     return res
 
 
@@ -216,7 +313,7 @@ def sweep(steps: int):
     import numpy as np
 
     from piculator_amd import picugpu, workload
-    from tests.helpers import residues_match
+    from tests.helpers import limb_digest, single_product_digest
 
     picugpu.init(0)
     for logn in (16, 20, 24, 26):
@@ -246,7 +343,7 @@ def sweep(steps: int):
                 rec["mulmods_per_s"] = workload.mulmods_alg(n, nb) / (ms / 1e3)
                 out = np.empty(n + nb, dtype=np.uint32)
                 dout.download(out)
-                rec["exact"] = bool(residues_match(a, b, out))
+                rec["exact"] = limb_digest(out) == single_product_digest(n, nb)
                 print(json.dumps(rec), flush=True)
             for buf in (da, db, dout):
                 buf.free()
@@ -317,7 +414,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-sample-digits", type=int, default=10000000)
+    ap.add_argument("--cpu-sample-digits", type=int, default=100000000)
+    ap.add_argument("--no-fft", action="store_true", help="skip the FFT-engine multiply line")
+    ap.add_argument("--no-probe", action="store_true", help="skip the FP64 mulmod peak probe")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run one device copy of 512 MiB (counter calibration)")
     ap.add_argument("--mul-only", action="store_true",
@@ -359,35 +458,55 @@ def main():
 
     picugpu.init(0)
     roofline, mul_info = ntt_mul(args.logn, args.mul_steps, 3)
+    rates = None
+    if not args.no_probe:
+        rates = probe_rates()
+        log("fp64 probe: %.3g mulmods/s, %.3g FMA/s" % (rates["mulmods_per_s"], rates["fma_per_s"]))
+    mulmod = None
+    if rates:
+        from piculator_amd import workload
+
+        mm = workload.mulmods_alg(1 << args.logn, 1 << args.logn) / (mul_info["ms"] / 1e3)
+        mulmod = {"bound": "fp64 mulmod", "achieved": mm, "peak": rates["mulmods_per_s"], "unit": "mulmod/s",
+                  "frac": mm / rates["mulmods_per_s"],
+                  "algorithmic_mulmods": workload.mulmods_alg(1 << args.logn, 1 << args.logn),
+                  "fp64_fma_per_s": rates["fma_per_s"], "probe": rates["probe"]}
     if args.mul_only:
-        print(json.dumps({"roofline": roofline, "ntt_mul": mul_info}), flush=True)
+        print(json.dumps({"roofline": roofline, "mulmod": mulmod, "ntt_mul": mul_info}), flush=True)
         return
 
     D = args.digits
+    with open(os.path.join(ROOT, "tests", "golden", "pi_digests.json")) as f:
+        fx = json.load(f)
     buf = pihost.pi_buffer(D)
+    cold = None
     for i in range(args.warmup):
         log("warm-up pi %d digits (%d/%d)..." % (D, i + 1, args.warmup))
+        t0 = time.perf_counter()
         n, t = pihost.pi_hex_into(D, pihost.MODE_BATCHED, buf)
+        if cold is None:
+            cold = time.perf_counter() - t0
         log("  %.3f s" % t["total"])
     log("timed: %d x pi %d digits..." % (args.steps, D))
-    phases = []
-    picugpu.sync()
-    t0 = time.perf_counter()
+    phases, walls, digests = [], [], []
     for _ in range(args.steps):
+        picugpu.sync()
+        t0 = time.perf_counter()
         n, t = pihost.pi_hex_into(D, pihost.MODE_BATCHED, buf)
+        picugpu.sync()
+        walls.append(time.perf_counter() - t0)
         phases.append(t)
-    picugpu.sync()
-    wall = (time.perf_counter() - t0) / args.steps
+        digests.append(hashlib.sha256(buf.raw[:n]).hexdigest())  # outside the timed window
+    wall = sum(walls) / len(walls)
     log("  %.3f s per pi computation" % wall)
     s = buf.raw[:n]
     del buf
     check = pi_spot_checks(s)
-    gold = os.path.join(ROOT, "tests", "golden", "pi_digests.json")
-    with open(gold) as f:
-        fx = json.load(f)
-    check["sha256"] = hashlib.sha256(s).hexdigest()
+    check["sha256"] = digests[-1]
+    check["every_step_same_digest"] = len(set(digests)) == 1
+    check["ok"] = bool(check["ok"] and check["every_step_same_digest"])
     if str(D) in fx:
-        check["full_digest_ok"] = check["sha256"] == fx[str(D)]["sha256"]
+        check["full_digest_ok"] = all(d == fx[str(D)]["sha256"] for d in digests)
         check["ok"] = bool(check["ok"] and check["full_digest_ok"])
     check["head"] = s[:34].decode()
     del s
@@ -413,12 +532,19 @@ def main():
                    "roofline_kernel": "2^%d x 2^%d-limb two-prime FP64 NTT multiply"
                                       % (args.logn, args.logn)},
         "roofline": roofline,
+        "mulmod": mulmod,
+        "step_s": walls,
+        "cold_first_run_s": cold,
+        "cold_note": "the first (warm-up) run in this process: includes the small-product autotuner "
+                     "and the first plan layouts; the 2^%d multiply ran before it" % args.logn,
         "phases_s": {k: last[k] for k in ("series", "division", "invsqrt", "final_mul",
                                           "conversion", "total")},
         "big_multiplies": int(last["big_muls"]),
         "pi_check": check,
         "ntt_mul": mul_info,
     }
+    if not args.no_fft:
+        res["fft_mul"] = fft_mul(args.logn, max(2, args.mul_steps // 4))
     if not args.no_batch:
         log("uneven batch of 4096 products...")
         res["batch"] = batch_compare()
@@ -428,11 +554,26 @@ def main():
         res["pi_small"] = pi_compare(args.small_digits, not args.no_plain)
         log("pi_small:", json.dumps(res["pi_small"]))
     if not args.no_cpu:
-        th = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log("cpu baseline (%d threads)..." % th)
-        res["cpu_baseline"] = cpu_baseline(th, D, args.cpu_sample_digits)
+        th, how = cpu_share()
+        if args.cpu_threads:
+            th, how = args.cpu_threads, "--cpu-threads"
+        sd = args.cpu_sample_digits
+        gpu_sample = None
+        if sd <= D:
+            s2, t2 = pihost.pi_hex(sd, pihost.MODE_BATCHED)
+            gpu_sample = t2["total"]
+            del s2
+        log("cpu baseline (%d threads: %s)..." % (th, how))
+        res["cpu_baseline"] = cpu_baseline(th, how, D, sd, gpu_sample, args.logn, mul_info["ms"])
         log("cpu baseline:", res["cpu_baseline"]["sample"])
+    ok = check["ok"] and mul_info["product_check"]
+    if not ok:
+        res["value"] = None
+        res["ms_per_step"] = None
+        res["invalid"] = "a result check failed (pi_check / ntt_mul.product_check): no timing is reported"
     print(json.dumps(res), flush=True)
+    if not ok:
+        sys.exit(1)
 
 
 if __name__ == "__main__":
