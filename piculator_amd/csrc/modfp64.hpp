@@ -28,6 +28,7 @@ constexpr int kLogRootMax = 30;               // tables are built on a 2^30-th r
 struct Prime {
   double p, pinv;           // p and fl(1/p)
   double2 w8f[3], w8i[3];   // omega_8^{1,2,3} and inverses, with (w, w/p) pairs
+  double2 w16f[3], w16i[3]; // omega_16^{1,3,9} and inverses
   double2 ninv;             // N^{-1} for the current transform length
   const double2* twL;       // omega_4096^e, e < 4096
   const double2* twHi;      // omega_{2^30}^{e * 2^15}, e < 2^15

@@ -1,0 +1,482 @@
+// Large-tile NTT passes for long transforms (gfx950: up to 160 KB of LDS per
+// workgroup).
+//
+// The same pass as k_ntt_pass (ntt_kernels.hpp) -- a line DFT of length
+// L = 2^LOGL over lines of stride s, with the same inter-pass twiddles --
+// but each workgroup stages 2^14 points of one prime (128 KB) and 1024
+// threads hold 16 points each through radix-16 Stockham stages.  Rows of
+// 2^14 points and columns of up to 2^11 points (8 adjacent columns, 64-byte
+// runs) make a 2^25-point transform two HBM sweeps instead of three.
+//
+// Stage twiddles omega_{R Ns}^{jm r} come from an LDS table of
+// omega_{L/16}^e (e < L/16) for every stage but the last; the last stage
+// (Ns = L/16) splits jm = 16 a + b and forms
+//   omega_L^{jm r} = omega_{L/16}^{a r} * omega_L^{b r}
+// from that table and a 256-entry one (b r < 256), one modular product per
+// twiddle.  Radix 16 is 4 x 4 with the internal omega_16 powers.
+#pragma once
+#include "ntt_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace pcg {
+
+constexpr int kBigLogE = 14;
+constexpr int kBigE = 1 << kBigLogE;
+constexpr int kBigTB = 256;  // entries of the last-stage table
+// 16 points per thread (1024 threads, at most 128 registers each); 32 points
+// per thread (512 threads) measured 19% slower at 2^25 points
+constexpr int kBigEPT = 16;
+constexpr int kBigNTH = kBigE / kBigEPT;
+
+static_assert(kBigEPT == 16 || kBigEPT == 32, "16 or 32 points per thread");
+
+// Stage-twiddle table entries hold w; its Shoup quotient w/p is formed at
+// use (one multiply; (w, w/p) pairs measured no faster and take twice the
+// LDS).
+using BigTw = double;
+__device__ __forceinline__ double2 tw_of(const BigTw* ta, int e, const Prime& P) {
+  const double w = ta[e];
+  return make_double2(w, w * P.pinv);
+}
+
+// Natural-order DFT of 16 inputs: r = 4 r1 + r2, k = k1 + 4 k2,
+// X_k = sum_r2 w4^{r2 k2} w16^{r2 k1} sum_r1 y_{4 r1 + r2} w4^{r1 k1}.
+__device__ __forceinline__ void dft16(double* v, const Prime& P, bool inv) {
+  const double2* w8 = inv ? P.w8i : P.w8f;    // w16^2, w16^4, w16^6
+  const double2* w16 = inv ? P.w16i : P.w16f;  // w16^1, w16^3, w16^9
+  double z[4][4];
+#pragma unroll
+  for (int r2 = 0; r2 < 4; ++r2) {
+    double t[4] = {v[r2], v[4 + r2], v[8 + r2], v[12 + r2]};
+    dft_core<4>(t, P, inv);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) z[r2][k1] = t[k1];
+  }
+  // w16^{r2 k1}: (1,1) w16, (1,2) w8, (1,3) w16^3, (2,1) w8, (2,2) w4,
+  // (2,3) w8^3, (3,1) w16^3, (3,2) w8^3, (3,3) w16^9
+  z[1][1] = mm_shoup(z[1][1], w16[0].x, w16[0].y, P.p);
+  z[1][2] = mm_shoup(z[1][2], w8[0].x, w8[0].y, P.p);
+  z[1][3] = mm_shoup(z[1][3], w16[1].x, w16[1].y, P.p);
+  z[2][1] = mm_shoup(z[2][1], w8[0].x, w8[0].y, P.p);
+  z[2][2] = mm_shoup(z[2][2], w8[1].x, w8[1].y, P.p);
+  z[2][3] = mm_shoup(z[2][3], w8[2].x, w8[2].y, P.p);
+  z[3][1] = mm_shoup(z[3][1], w16[1].x, w16[1].y, P.p);
+  z[3][2] = mm_shoup(z[3][2], w8[2].x, w8[2].y, P.p);
+  z[3][3] = mm_shoup(z[3][3], w16[2].x, w16[2].y, P.p);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    double t[4] = {z[0][k1], z[1][k1], z[2][k1], z[3][k1]};
+    dft_core<4>(t, P, inv);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void big_dft(double* v, const Prime& P, bool inv) {
+  if constexpr (R == 16)
+    dft16(v, P, inv);
+  else
+    dft_core<R>(v, P, inv);
+}
+
+// Tile geometry: rows (LINES = 1, one line of 2^14 points, one pad double
+// after every 32: a half-wave's 32 consecutive doubles then span all 64
+// banks once) or columns (LINES = 2^(14 - LOGL) adjacent lines, point
+// n of line lam at n * LINES + lam, plus 8 pad doubles per 8 points).
+template <int LOGL, bool ROW>
+struct BigGeo {
+  static constexpr int L = 1 << LOGL;
+  static constexpr int LOGLINES = kBigLogE - LOGL;
+  static constexpr int LINES = 1 << LOGLINES;
+  static constexpr int LDS = ROW ? kBigE + kBigE / 32 : kBigE + kBigE / 8;
+  static constexpr int LOGR1 = LOGL % 4 == 0 ? 4 : LOGL % 4;
+  static_assert(!ROW || LOGL == kBigLogE, "row tiles are one line");
+  // columns: 8 doubles of pad after every 8 points, so the first stage's
+  // writes (8 points of a butterfly, adjacent lines) spread over the banks
+  static constexpr int kColPad = 3;
+  __device__ static __forceinline__ int idx(int n, int lam) {
+    return ROW ? n + (n >> 5) : (n << LOGLINES) + lam + ((n >> kColPad) << 3);
+  }
+  // idx(n + o, lam) == idx(n, lam) + ioff(o) for every access of the stages
+  // (n a butterfly's base point, o = r L/R or r Ns a compile-time offset:
+  // checked exhaustively for every instance), so the per-point LDS
+  // addresses are one base plus immediates
+  __device__ static constexpr int ioff(int o) {
+    return ROW ? o + (o >> 5) : (o << LOGLINES) + ((o >> kColPad) << 3);
+  }
+  __device__ static __forceinline__ void split(int bf, int logLR, int& lam, int& j) {
+    if (ROW) {
+      lam = 0;
+      j = bf;
+    } else {
+      lam = bf & (LINES - 1);
+      j = bf >> LOGLINES;
+    }
+  }
+};
+
+// The thread index, opaque to the optimiser: address arithmetic derived
+// from it is recomputed where it is used (a few integer operations) instead
+// of being hoisted out of the tile loop and kept live -- with 1024 threads
+// and 128 registers the hoisted addresses would spill, and every spill
+// reload waits for the next tile's loads in flight.
+__device__ __forceinline__ int big_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
+// Twiddle omega_{2^(LOGR+LOGNS)}^{e}, e = jm r, for a stage of the line DFT.
+template <int LOGL, int LOGR, int LOGNS>
+__device__ __forceinline__ double2 big_stage_tw(const BigTw* ta, const double* tb, int jm, int r,
+                                                const Prime& P, bool inv) {
+  constexpr int LOGTA = LOGL - 4;  // table omega_{L/16}
+  if constexpr (LOGR + LOGNS <= LOGTA) {
+    constexpr int SH = LOGTA - LOGR - LOGNS;
+    int e = (jm * r) << SH;
+    if (inv) e = ((1 << LOGTA) - e) & ((1 << LOGTA) - 1);
+    return tw_of(ta, e, P);
+  } else {
+    // last stage (LOGR + LOGNS == LOGL): jm = 16 a + b
+    static_assert(LOGR + LOGNS == LOGL, "stage beyond the line");
+    int ea = (jm >> 4) * r;        // < L / 16
+    const int eb = (jm & 15) * r;  // < 256
+    if (inv) ea = ((1 << LOGTA) - ea) & ((1 << LOGTA) - 1);
+    const double2 wa = tw_of(ta, ea, P);
+    const double wb = tb[(inv ? kBigTB : 0) + eb];  // omega_L^{+-eb}
+    const double w = mm_shoup(wb, wa.x, wa.y, P.p);
+    return make_double2(w, w * P.pinv);
+  }
+}
+
+// Inputs of butterfly b of this thread in a stage of radix 2^LOGR: line lam,
+// point j + r L / R.
+template <class GE, int LOGR>
+__device__ __forceinline__ void big_pos(int t, int b, int r, int& lam, int& n) {
+  constexpr int LOGLR = __builtin_ctz(GE::L) - LOGR;
+  int j;
+  GE::split(t + b * kBigNTH, LOGLR, lam, j);
+  n = j + (r << LOGLR);
+}
+
+// Twiddles and DFTs of a stage on the registers v (butterfly b, input r).
+template <class GE, int LOGR, int LOGNS>
+__device__ __forceinline__ void big_compute(double (&v)[kBigEPT >> LOGR][1 << LOGR], const BigTw* ta,
+                                            const double* tb, const Prime& P, bool inv) {
+  constexpr int R = 1 << LOGR;
+  constexpr int LOGLR = __builtin_ctz(GE::L) - LOGR;
+  constexpr int NS = 1 << LOGNS;
+  const int t = big_tid();
+#pragma unroll
+  for (int b = 0; b < kBigEPT / R; ++b) {
+    if constexpr (LOGNS > 0) {
+      int lam, j;
+      GE::split(t + b * kBigNTH, LOGLR, lam, j);
+      const int jm = j & (NS - 1);
+      v[b][0] = reduce(v[b][0], P.p, P.pinv);
+      constexpr int LOGL = __builtin_ctz(GE::L);
+      constexpr int LOGTA = LOGL - 4;
+      if constexpr (LOGR + LOGNS <= LOGTA) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const double2 t = big_stage_tw<LOGL, LOGR, LOGNS>(ta, tb, jm, r, P, inv);
+          v[b][r] = mm_shoup(v[b][r], t.x, t.y, P.p);
+        }
+      } else {
+        // last stage: omega_L^{jm r} = omega_{L/16}^{a r} omega_L^{b r}, jm = 16 a + b,
+        // applied as two factors (the same two modular products as forming
+        // the twiddle first, with fewer values live)
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          int ea = (jm >> 4) * r;
+          if (inv) ea = ((1 << LOGTA) - ea) & ((1 << LOGTA) - 1);
+          const double wb = tb[(inv ? kBigTB : 0) + (jm & 15) * r];
+          const double2 wa = tw_of(ta, ea, P);
+          const double x = mm_shoup(v[b][r], wb, wb * P.pinv, P.p);
+          v[b][r] = mm_shoup(x, wa.x, wa.y, P.p);
+        }
+      }
+    }
+    big_dft<R>(v[b], P, inv);
+  }
+}
+
+template <class GE, int LOGR, int LOGNS>
+__device__ __forceinline__ void big_write(const double (&v)[kBigEPT >> LOGR][1 << LOGR], double* lds) {
+  constexpr int R = 1 << LOGR;
+  constexpr int LOGLR = __builtin_ctz(GE::L) - LOGR;
+  constexpr int NS = 1 << LOGNS;
+  const int t = big_tid();
+#pragma unroll
+  for (int b = 0; b < kBigEPT / R; ++b) {
+    int lam, j;
+    GE::split(t + b * kBigNTH, LOGLR, lam, j);
+    const int jm = j & (NS - 1);
+    const int idxD = ((j - jm) << LOGR) + jm;
+    double* base = lds + GE::idx(idxD, lam);
+#pragma unroll
+    for (int r = 0; r < R; ++r) base[GE::ioff(r << LOGNS)] = v[b][r];
+  }
+}
+
+template <class GE, int LOGR>
+__device__ __forceinline__ void big_read(double (&v)[kBigEPT >> LOGR][1 << LOGR], const double* lds) {
+  constexpr int LOGLR = __builtin_ctz(GE::L) - LOGR;
+  const int t = big_tid();
+#pragma unroll
+  for (int b = 0; b < (kBigEPT >> LOGR); ++b) {
+    int lam, n0;
+    big_pos<GE, LOGR>(t, b, 0, lam, n0);
+    const double* base = lds + GE::idx(n0, lam);
+#pragma unroll
+    for (int r = 0; r < (1 << LOGR); ++r) v[b][r] = base[GE::ioff(r << LOGLR)];
+  }
+}
+
+// A whole stage in LDS (read, compute, barrier, write, barrier).
+template <class GE, int LOGR, int LOGNS>
+__device__ __forceinline__ void big_stage(double* lds, const BigTw* ta, const double* tb,
+                                          const Prime& P, bool inv) {
+  double v[kBigEPT >> LOGR][1 << LOGR];
+  big_read<GE, LOGR>(v, lds);
+  big_compute<GE, LOGR, LOGNS>(v, ta, tb, P, inv);
+  __syncthreads();
+  big_write<GE, LOGR, LOGNS>(v, lds);
+  __syncthreads();
+}
+
+// Radix-16 stages from Ns = 2^LOGNS up to (not including) the last one.
+template <class GE, int LOGNS>
+__device__ __forceinline__ void big_middle(double* lds, const BigTw* ta, const double* tb,
+                                           const Prime& P, bool inv) {
+  if constexpr (LOGNS + 4 < __builtin_ctz(GE::L)) {
+    big_stage<GE, 4, LOGNS>(lds, ta, tb, P, inv);
+    big_middle<GE, LOGNS + 4>(lds, ta, tb, P, inv);
+  }
+}
+
+constexpr int kBigNBL = kBigEPT / 16;  // last-stage butterflies per thread
+
+// The last (radix-16) stage: LDS -> registers; y[b][r] is output point
+// j_b + r L / 16 of butterfly b's line (the place its input r came from).
+template <class GE>
+__device__ __forceinline__ void big_last(const double* lds, double (&y)[kBigNBL][16], const BigTw* ta,
+                                         const double* tb, const Prime& P, bool inv) {
+  constexpr int LOGL = __builtin_ctz(GE::L);
+  big_read<GE, 4>(y, lds);
+  big_compute<GE, 4, LOGL - 4>(y, ta, tb, P, inv);
+}
+
+// Fills the stage tables: ta[e] = omega_{L/16}^e (e < L/16; the inverse
+// direction negates the index), tb[e] = omega_L^e and tb[256 + e] =
+// omega_L^{-e} (e < 256).  Call before the first barrier.
+template <class GE>
+__device__ __forceinline__ void big_tables(BigTw* ta, double* tb, const Prime& P) {
+  constexpr int LOGL = __builtin_ctz(GE::L);
+  constexpr int LOGTA = LOGL - 4;
+  static_assert(LOGTA >= 1 && LOGTA <= kLogE, "line length out of range");
+  for (int e = threadIdx.x; e < (1 << LOGTA); e += blockDim.x)
+    ta[e] = P.twL[e << (kLogE - LOGTA)].x;  // twL = (omega_4096^e, its quotient)
+  for (int e = threadIdx.x; e < 2 * kBigTB; e += blockDim.x) {
+    const double2 w = root_pow((uint64_t)(e & (kBigTB - 1)), LOGL, e >= kBigTB, P);
+    tb[e] = w.x;
+  }
+}
+
+template <int MODE, int LOGL, bool ROW>
+__global__ __launch_bounds__(kBigNTH) void k_ntt_big(PassArgs a) {
+  using GE = BigGeo<LOGL, ROW>;
+  constexpr int LOGR1 = GE::LOGR1;
+  constexpr int R1 = 1 << LOGR1;
+  constexpr int NB1 = kBigEPT / R1;
+  constexpr int LOGTA = LOGL - 4;
+  __shared__ double lds[GE::LDS];
+  __shared__ BigTw ta[1 << LOGTA];
+  __shared__ double tb[2 * kBigTB];
+  const int pi = blockIdx.y;
+  const Prime P = a.prm[pi];
+  big_tables<GE>(ta, tb, P);
+  const int logS = a.logS;
+  uint64_t zoff = 0, soff = 0;
+  const uint32_t* limbs = a.limbs;
+  uint64_t nlimbs = a.nlimbs;
+  if (a.descs) {
+    const uint64_t zi = (uint64_t)blockIdx.z << a.zshift;
+    const BatchDesc d = a.descs[zi];
+    zoff = zi * a.zstride;
+    soff = ((zi >> a.sshift) << a.sshift) * a.zstride;
+    limbs = a.operand ? d.b : d.a;
+    nlimbs = a.operand ? d.nb : d.na;
+  }
+  const double* src = a.src ? a.src + zoff + pi * a.N : nullptr;
+  double* dst = a.dst + zoff + pi * a.N;
+  const int logRoot = LOGL + logS;
+  const uint64_t ntiles = a.ntiles;
+
+  auto tile_base = [&](uint64_t t) -> uint64_t {
+    if (ROW) return t << kBigLogE;
+    const int lgc = logS - GE::LOGLINES;
+    return ((t >> lgc) << (LOGL + logS)) + ((t & ((1ull << lgc) - 1)) << GE::LOGLINES);
+  };
+  auto gidx = [&](uint64_t base, int n, int lam) -> uint64_t {
+    return ROW ? base + (uint64_t)n : base + ((uint64_t)n << logS) + lam;
+  };
+  // inputs of this thread's first-stage butterflies: x[b R1 + r]; limbs
+  // past the operand are zeros (the load stays in bounds and is discarded,
+  // so every load issues without a branch)
+  auto load_tile = [&](uint64_t t, double (&x)[kBigEPT]) {
+    const uint64_t b0 = tile_base(t);
+    const int bt = big_tid();
+#pragma unroll
+    for (int b = 0; b < NB1; ++b)
+#pragma unroll
+      for (int r = 0; r < R1; ++r) {
+        int lam, n;
+        big_pos<GE, LOGR1>(bt, b, r, lam, n);
+        const uint64_t g = gidx(b0, n, lam);
+        if (src) {
+          x[b * R1 + r] = src[g];
+        } else {
+          const uint32_t v = limbs[g < nlimbs ? g : nlimbs - 1];
+          x[b * R1 + r] = g < nlimbs ? (double)v : 0.0;
+        }
+      }
+  };
+  // inter-pass twiddle omega^{c n} of a column pass along the thread's points
+  // n = n0 + k step: base omega^{c n0} times powers of omega^{c step}
+  auto col_twiddle = [&](uint64_t base, int lam, uint64_t n0, uint64_t step, double2& cur,
+                         double2& stp) {
+    const uint64_t c = (base & ((1ull << logS) - 1)) + (uint64_t)lam;
+    cur = root_pow(c * n0, logRoot, MODE == kInv, P);
+    stp = root_pow(c * step, logRoot, MODE == kInv, P);
+  };
+
+  // bases of the inverse pre-twiddles of tile t's first-stage butterflies
+  // (fetched with the tile's loads, so their table reads are in flight
+  // together with them)
+  auto inv_twiddles = [&](uint64_t t, double2 (&cur)[NB1], double2 (&stp)[NB1]) {
+    if constexpr (MODE == kInv && !ROW) {
+      const uint64_t b0 = tile_base(t);
+      const int bt = big_tid();
+#pragma unroll
+      for (int b = 0; b < NB1; ++b) {
+        int lam, n0;
+        big_pos<GE, LOGR1>(bt, b, 0, lam, n0);
+        col_twiddle(b0, lam, (uint64_t)n0, (uint64_t)(GE::L >> LOGR1), cur[b], stp[b]);
+      }
+    }
+  };
+
+  auto tile_step = [&](const uint64_t w, double (&xr)[kBigEPT], double2 (&icur)[NB1],
+                       double2 (&istp)[NB1], const uint64_t wload) {
+    const uint64_t base = tile_base(w);
+    const int bt = big_tid();
+    // bases of the forward post-twiddles, fetched now, used after the stages
+    double2 fcur[kBigNBL], fstp[kBigNBL];
+    if constexpr (MODE == kFwd && !ROW) {
+#pragma unroll
+      for (int b = 0; b < kBigNBL; ++b) {
+        int lam_o, n_o0;
+        big_pos<GE, 4>(bt, b, 0, lam_o, n_o0);
+        col_twiddle(base, lam_o, (uint64_t)n_o0, (uint64_t)(GE::L >> 4), fcur[b], fstp[b]);
+      }
+    }
+    if (MODE == kInv && !ROW) {
+#pragma unroll
+      for (int b = 0; b < NB1; ++b) {
+        double2 cur = icur[b], stp = istp[b];
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+          double& x = xr[b * R1 + r];
+          x = mm_shoup(x, cur.x, cur.y, P.p);
+          const double wv = mm_shoup(cur.x, stp.x, stp.y, P.p);
+          cur = make_double2(wv, wv * P.pinv);
+        }
+      }
+    }
+    {
+      double v1[NB1][R1];
+#pragma unroll
+      for (int b = 0; b < NB1; ++b)
+#pragma unroll
+        for (int r = 0; r < R1; ++r) v1[b][r] = xr[b * R1 + r];
+      big_compute<GE, LOGR1, 0>(v1, ta, tb, P, MODE == kInv);
+      big_write<GE, LOGR1, 0>(v1, lds);
+    }
+    __syncthreads();
+    if (wload < ntiles) {
+      load_tile(wload, xr);
+      inv_twiddles(wload, icur, istp);
+    }
+    big_middle<GE, LOGR1>(lds, ta, tb, P, MODE == kInv);
+    double y[kBigNBL][16];
+    big_last<GE>(lds, y, ta, tb, P, MODE == kInv);
+    if (MODE == kFused) {
+      const double* spec = a.spec + soff + pi * a.N;
+#pragma unroll
+      for (int b = 0; b < kBigNBL; ++b) {
+        int lam_o, n_o0;
+        big_pos<GE, 4>(bt, b, 0, lam_o, n_o0);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const uint64_t g = gidx(base, n_o0 + r * (GE::L >> 4), lam_o);
+          const double x = mm_gen(y[b][r], spec[g], P.p, P.pinv);
+          y[b][r] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
+        }
+      }
+      __syncthreads();  // every last-stage read of LDS is done
+#pragma unroll
+      for (int b = 0; b < kBigNBL; ++b) {
+        int lam_o, n_o0;
+        big_pos<GE, 4>(bt, b, 0, lam_o, n_o0);
+        double* ob = lds + GE::idx(n_o0, lam_o);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) ob[GE::ioff(r * (GE::L >> 4))] = y[b][r];
+      }
+      __syncthreads();
+      big_stage<GE, LOGR1, 0>(lds, ta, tb, P, true);
+      big_middle<GE, LOGR1>(lds, ta, tb, P, true);
+      big_last<GE>(lds, y, ta, tb, P, true);
+    }
+#pragma unroll
+    for (int b = 0; b < kBigNBL; ++b) {
+      int lam_o, n_o0;
+      big_pos<GE, 4>(bt, b, 0, lam_o, n_o0);
+      double2 cur = make_double2(1.0, 0.0), stp = make_double2(1.0, 0.0);
+      if constexpr (MODE == kFwd && !ROW) {
+        cur = fcur[b];
+        stp = fstp[b];
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        double x = y[b][r];
+        if (MODE == kFwd && !ROW) {
+          x = mm_shoup(x, cur.x, cur.y, P.p);
+          const double wv = mm_shoup(cur.x, stp.x, stp.y, P.p);
+          cur = make_double2(wv, wv * P.pinv);
+        } else {
+          x = reduce(x, P.p, P.pinv);
+        }
+        dst[gidx(base, n_o0 + r * (GE::L >> 4), lam_o)] = x;
+      }
+    }
+  };
+
+  uint64_t w = blockIdx.x;
+  if (w >= ntiles) return;
+  const uint64_t g = gridDim.x;
+  double xa[kBigEPT];
+  double2 icur[NB1], istp[NB1];
+  load_tile(w, xa);
+  inv_twiddles(w, icur, istp);
+  for (;;) {
+    tile_step(w, xa, icur, istp, w + g);
+    w += g;
+    if (w >= ntiles) break;
+    __syncthreads();  // the next tile's first stage overwrites LDS
+  }
+}
+
+}  // namespace pcg

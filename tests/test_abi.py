@@ -34,8 +34,11 @@ def test_calls_before_init_fail_cleanly():
 
 
 def test_plan_limits():
+    # 2^21 .. 2^25 points: a column pass and a 2^14-point row pass
     lg, passes = picugpu.ntt_plan(2 ** 25)
-    assert lg == 25 and passes == 3
+    assert lg == 25 and passes == 2
+    assert picugpu.ntt_plan(2 ** 21) == (21, 2)
+    assert picugpu.ntt_plan(2 ** 26)[1] == 3 and picugpu.ntt_plan(2 ** 30)[1] == 3
     # every transform length has a kernel instance for each of its passes
     for lg in range(8, 31):
         assert picugpu.ntt_plan(2 ** lg)[0] == lg
