@@ -400,6 +400,37 @@ def summarize_traffic(fetch_dir: str, write_dir: str):
     print(json.dumps(out, indent=1, sort_keys=True))
 
 
+def summarize_counters(dirname: str):
+    """Per-kernel averages per dispatch of every counter in a rocprofv3
+    --pmc run, with the ratios used for the compute-side evidence: VALU
+    instructions per wave-cycle (x4 waves per SIMD for the issue share),
+    wait share, LDS bank-conflict share."""
+    con = _rocpd(dirname)
+    rows = con.execute("select kernel_name, counter_name, dispatch_id, sum(value) from counters_collection "
+                       "group by dispatch_id, kernel_name, counter_name").fetchall()
+    acc, disp = {}, {}
+    for kn, cn, d, v in rows:
+        k = kn.split("(")[0].replace("void ", "")
+        acc.setdefault(k, {}).setdefault(cn, 0.0)
+        acc[k][cn] += v
+        disp.setdefault(k, set()).add(d)
+    out = {}
+    for k, m in acc.items():
+        n = len(disp[k])
+        e = {cn: v / n for cn, v in m.items()}
+        wc = e.get("SQ_WAVE_CYCLES")
+        if wc:
+            if "SQ_ACTIVE_INST_VALU" in e:
+                e["valu_per_wave_cycle"] = e["SQ_ACTIVE_INST_VALU"] / wc
+            if "SQ_WAIT_INST_ANY" in e:
+                e["wait_inst_share"] = e["SQ_WAIT_INST_ANY"] / wc
+        if e.get("SQ_LDS_IDX_ACTIVE") and "SQ_LDS_BANK_CONFLICT" in e:
+            e["lds_conflict_share"] = e["SQ_LDS_BANK_CONFLICT"] / e["SQ_LDS_IDX_ACTIVE"]
+        e["dispatches"] = n
+        out[k] = e
+    print(json.dumps(out, indent=1, sort_keys=True))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -427,7 +458,12 @@ def main():
                     help="print the kernel table of a rocprofv3 --kernel-trace run (no GPU)")
     ap.add_argument("--summarize-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="print HBM bytes per multiply from two rocprofv3 --pmc runs (no GPU)")
+    ap.add_argument("--summarize-counters", metavar="DIR",
+                    help="print per-kernel counter averages of a rocprofv3 --pmc run (no GPU)")
     args = ap.parse_args()
+    if args.summarize_counters:
+        summarize_counters(args.summarize_counters)
+        return
     if args.summarize_trace:
         summarize_trace(args.summarize_trace)
         return

@@ -464,9 +464,16 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_big(PassArgs a) {
     }
   };
 
-  uint64_t w = blockIdx.x;
-  if (w >= ntiles) return;
+  // XCD-aware tile order: workgroups are dispatched to the 8 XCDs round
+  // robin (linear id mod 8, and gridDim.x is a multiple of 8, so both
+  // primes' workgroups x sit on the same XCD), so rank them XCD by XCD:
+  // the workgroups of one XCD then take consecutive tiles, and a column
+  // tile's 64-byte runs and its neighbour's (the two halves of each
+  // 128-byte line) are read through the same L2 at the same time.
   const uint64_t g = gridDim.x;
+  uint64_t w = blockIdx.x;
+  if ((g & 7) == 0) w = (uint64_t)(blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
+  if (w >= ntiles) return;
   double xa[kBigEPT];
   double2 icur[NB1], istp[NB1];
   load_tile(w, xa);
