@@ -1,12 +1,12 @@
-// Large-tile NTT passes for long transforms (gfx950: up to 160 KB of LDS per
-// workgroup).
+// Large-tile NTT column passes for long transforms (gfx950: up to 160 KB of
+// LDS per workgroup), and the pieces the row pass (ntt_row.hpp) shares.
 //
 // The same pass as k_ntt_pass (ntt_kernels.hpp) -- a line DFT of length
 // L = 2^LOGL over lines of stride s, with the same inter-pass twiddles --
 // but each workgroup stages 2^14 points of one prime (128 KB) and 1024
-// threads hold 16 points each through radix-16 Stockham stages.  Rows of
-// 2^14 points and columns of up to 2^11 points (8 adjacent columns, 64-byte
-// runs) make a 2^25-point transform two HBM sweeps instead of three.
+// threads hold 16 points each through radix-16 Stockham stages.  Columns of
+// up to 2^11 points (8 adjacent columns, 64-byte runs) and 2^14-point rows
+// make a 2^25-point transform two HBM sweeps instead of three.
 //
 // Stage twiddles omega_{R Ns}^{jm r} come from an LDS table of
 // omega_{L/16}^e (e < L/16) for every stage but the last; the last stage
@@ -81,39 +81,30 @@ __device__ __forceinline__ void big_dft(double* v, const Prime& P, bool inv) {
     dft_core<R>(v, P, inv);
 }
 
-// Tile geometry: rows (LINES = 1, one line of 2^14 points, one pad double
-// after every 32: a half-wave's 32 consecutive doubles then span all 64
-// banks once) or columns (LINES = 2^(14 - LOGL) adjacent lines, point
-// n of line lam at n * LINES + lam, plus 8 pad doubles per 8 points).
-template <int LOGL, bool ROW>
+// Tile geometry: LINES = 2^(14 - LOGL) adjacent columns, point n of line
+// lam at n * LINES + lam, plus 8 pad doubles per 8 points.
+template <int LOGL>
 struct BigGeo {
   static constexpr int L = 1 << LOGL;
   static constexpr int LOGLINES = kBigLogE - LOGL;
   static constexpr int LINES = 1 << LOGLINES;
-  static constexpr int LDS = ROW ? kBigE + kBigE / 32 : kBigE + kBigE / 8;
+  static constexpr int LDS = kBigE + kBigE / 8;
   static constexpr int LOGR1 = LOGL % 4 == 0 ? 4 : LOGL % 4;
-  static_assert(!ROW || LOGL == kBigLogE, "row tiles are one line");
+  static_assert(LOGL >= 5 && LOGL <= 11, "column tiles of 8 or more lines");
   // columns: 8 doubles of pad after every 8 points, so the first stage's
   // writes (8 points of a butterfly, adjacent lines) spread over the banks
   static constexpr int kColPad = 3;
   __device__ static __forceinline__ int idx(int n, int lam) {
-    return ROW ? n + (n >> 5) : (n << LOGLINES) + lam + ((n >> kColPad) << 3);
+    return (n << LOGLINES) + lam + ((n >> kColPad) << 3);
   }
   // idx(n + o, lam) == idx(n, lam) + ioff(o) for every access of the stages
   // (n a butterfly's base point, o = r L/R or r Ns a compile-time offset:
   // checked exhaustively for every instance), so the per-point LDS
   // addresses are one base plus immediates
-  __device__ static constexpr int ioff(int o) {
-    return ROW ? o + (o >> 5) : (o << LOGLINES) + ((o >> kColPad) << 3);
-  }
+  __device__ static constexpr int ioff(int o) { return (o << LOGLINES) + ((o >> kColPad) << 3); }
   __device__ static __forceinline__ void split(int bf, int logLR, int& lam, int& j) {
-    if (ROW) {
-      lam = 0;
-      j = bf;
-    } else {
-      lam = bf & (LINES - 1);
-      j = bf >> LOGLINES;
-    }
+    lam = bf & (LINES - 1);
+    j = bf >> LOGLINES;
   }
 };
 
@@ -285,9 +276,10 @@ __device__ __forceinline__ void big_tables(BigTw* ta, double* tb, const Prime& P
   }
 }
 
-template <int MODE, int LOGL, bool ROW>
-__global__ __launch_bounds__(kBigNTH) void k_ntt_big(PassArgs a) {
-  using GE = BigGeo<LOGL, ROW>;
+template <int MODE, int LOGL>
+__global__ __launch_bounds__(kBigNTH) void k_ntt_col(PassArgs a) {
+  static_assert(MODE == kFwd || MODE == kInv, "column passes run forward or inverse");
+  using GE = BigGeo<LOGL>;
   constexpr int LOGR1 = GE::LOGR1;
   constexpr int R1 = 1 << LOGR1;
   constexpr int NB1 = kBigEPT / R1;
@@ -299,14 +291,13 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_big(PassArgs a) {
   const Prime P = a.prm[pi];
   big_tables<GE>(ta, tb, P);
   const int logS = a.logS;
-  uint64_t zoff = 0, soff = 0;
+  uint64_t zoff = 0;
   const uint32_t* limbs = a.limbs;
   uint64_t nlimbs = a.nlimbs;
   if (a.descs) {
     const uint64_t zi = (uint64_t)blockIdx.z << a.zshift;
     const BatchDesc d = a.descs[zi];
     zoff = zi * a.zstride;
-    soff = ((zi >> a.sshift) << a.sshift) * a.zstride;
     limbs = a.operand ? d.b : d.a;
     nlimbs = a.operand ? d.nb : d.na;
   }
@@ -316,12 +307,11 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_big(PassArgs a) {
   const uint64_t ntiles = a.ntiles;
 
   auto tile_base = [&](uint64_t t) -> uint64_t {
-    if (ROW) return t << kBigLogE;
     const int lgc = logS - GE::LOGLINES;
     return ((t >> lgc) << (LOGL + logS)) + ((t & ((1ull << lgc) - 1)) << GE::LOGLINES);
   };
   auto gidx = [&](uint64_t base, int n, int lam) -> uint64_t {
-    return ROW ? base + (uint64_t)n : base + ((uint64_t)n << logS) + lam;
+    return base + ((uint64_t)n << logS) + lam;
   };
   // inputs of this thread's first-stage butterflies: x[b R1 + r]; limbs
   // past the operand are zeros (the load stays in bounds and is discarded,
@@ -357,7 +347,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_big(PassArgs a) {
   // (fetched with the tile's loads, so their table reads are in flight
   // together with them)
   auto inv_twiddles = [&](uint64_t t, double2 (&cur)[NB1], double2 (&stp)[NB1]) {
-    if constexpr (MODE == kInv && !ROW) {
+    if constexpr (MODE == kInv) {
       const uint64_t b0 = tile_base(t);
       const int bt = big_tid();
 #pragma unroll
@@ -375,7 +365,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_big(PassArgs a) {
     const int bt = big_tid();
     // bases of the forward post-twiddles, fetched now, used after the stages
     double2 fcur[kBigNBL], fstp[kBigNBL];
-    if constexpr (MODE == kFwd && !ROW) {
+    if constexpr (MODE == kFwd) {
 #pragma unroll
       for (int b = 0; b < kBigNBL; ++b) {
         int lam_o, n_o0;
@@ -383,7 +373,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_big(PassArgs a) {
         col_twiddle(base, lam_o, (uint64_t)n_o0, (uint64_t)(GE::L >> 4), fcur[b], fstp[b]);
       }
     }
-    if (MODE == kInv && !ROW) {
+    if constexpr (MODE == kInv) {
 #pragma unroll
       for (int b = 0; b < NB1; ++b) {
         double2 cur = icur[b], stp = istp[b];
@@ -413,46 +403,19 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_big(PassArgs a) {
     big_middle<GE, LOGR1>(lds, ta, tb, P, MODE == kInv);
     double y[kBigNBL][16];
     big_last<GE>(lds, y, ta, tb, P, MODE == kInv);
-    if (MODE == kFused) {
-      const double* spec = a.spec + soff + pi * a.N;
-#pragma unroll
-      for (int b = 0; b < kBigNBL; ++b) {
-        int lam_o, n_o0;
-        big_pos<GE, 4>(bt, b, 0, lam_o, n_o0);
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const uint64_t g = gidx(base, n_o0 + r * (GE::L >> 4), lam_o);
-          const double x = mm_gen(y[b][r], spec[g], P.p, P.pinv);
-          y[b][r] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
-        }
-      }
-      __syncthreads();  // every last-stage read of LDS is done
-#pragma unroll
-      for (int b = 0; b < kBigNBL; ++b) {
-        int lam_o, n_o0;
-        big_pos<GE, 4>(bt, b, 0, lam_o, n_o0);
-        double* ob = lds + GE::idx(n_o0, lam_o);
-#pragma unroll
-        for (int r = 0; r < 16; ++r) ob[GE::ioff(r * (GE::L >> 4))] = y[b][r];
-      }
-      __syncthreads();
-      big_stage<GE, LOGR1, 0>(lds, ta, tb, P, true);
-      big_middle<GE, LOGR1>(lds, ta, tb, P, true);
-      big_last<GE>(lds, y, ta, tb, P, true);
-    }
 #pragma unroll
     for (int b = 0; b < kBigNBL; ++b) {
       int lam_o, n_o0;
       big_pos<GE, 4>(bt, b, 0, lam_o, n_o0);
       double2 cur = make_double2(1.0, 0.0), stp = make_double2(1.0, 0.0);
-      if constexpr (MODE == kFwd && !ROW) {
+      if constexpr (MODE == kFwd) {
         cur = fcur[b];
         stp = fstp[b];
       }
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         double x = y[b][r];
-        if (MODE == kFwd && !ROW) {
+        if constexpr (MODE == kFwd) {
           x = mm_shoup(x, cur.x, cur.y, P.p);
           const double wv = mm_shoup(cur.x, stp.x, stp.y, P.p);
           cur = make_double2(wv, wv * P.pinv);

@@ -1,0 +1,310 @@
+// Row pass of the large-tile plan (2^14-point lines, stride 1) with one
+// cross-wave exchange per transform.
+//
+// A 2^14-point DFT is four digit stages, decimation in frequency with the
+// digits of n = n1 2^10 + n2 2^6 + n3 2^2 + n4 taken from the top:
+//   S1: DFT16 over n1 for each m = n mod 2^10, twiddle omega_{2^14}^{k1 m}
+//   S2: DFT16 over n2 for each (k1, m2 = n mod 64), twiddle omega_1024^{k2 m2}
+//   S3: DFT16 over n3 for each (k1, k2, n4), twiddle omega_64^{k3 n4}
+//   S4: DFT4 over n4,
+// giving X[k1 + 16 k2 + 256 k3 + 4096 k4].  Thread t = 64 w + lane of the
+// 1024 holds 16 points.  S1 runs on thread t = m (its loads are 16
+// coalesced rows of the line); its outputs go to wave k1 through LDS (one
+// workgroup barrier on each side), after which wave k1 holds all of
+// subproblem k1, so the exchanges before S3 and S4 stay inside the wave:
+// each wave rewrites only the 1024 LDS doubles it has just read, ordered by
+// the wave's own in-order LDS queue, with no workgroup barrier.  The
+// inverse (fused mode) runs the same stages mirrored: twiddle^-1 then DFT^-1
+// in each stage's layout, exchanges reversed.
+//
+// The spectrum is stored in the thread layout of S4: register s 4 + k4 of
+// thread (k1 = w, lane = 4 k2 + q) at position (s 4 + k4) 2^10 + t, which is
+// frequency k1 + 16 k2 + 256 (4 q + s) + 4096 k4 of the line.  Only this
+// pass and its fused inverse read or write spectra in that order; the
+// pointwise product does not depend on it.
+//
+// LDS layouts (doubles; conflict-free for every 16- and 32-lane group of a
+// 64-bit access):
+//   cross-wave:  [k1][n2][lane]                 = xs[(k1 16 + n2) 64 + lane]
+//   before S3:   wave w, row n3, column 4 k2 + n4 swizzled by n3 << 2
+//   before S4:   wave w, row 4 s + n4, column 4 k2 + q swizzled by n4
+#pragma once
+#include "ntt_big.hpp"
+
+#pragma clang fp contract(off)
+
+namespace pcg {
+
+constexpr int kRowNTH = kBigNTH;  // 1024 threads x 16 points
+static_assert(kBigEPT == 16, "the row pass holds 16 points per thread");
+
+// Orders this wave's LDS accesses before and after it (the wave's LDS
+// operations execute in order, so no workgroup barrier is needed for an
+// exchange between its own lanes).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Tables: ta[e] = omega_1024^e (e < 1024); tb[e] = omega_{2^14}^e and
+// tb[256 + e] = omega_{2^14}^-e (e < 256).
+__device__ __forceinline__ double2 row_ta(const BigTw* ta, int e, bool inv, const Prime& P) {
+  return tw_of(ta, inv ? (1024 - e) & 1023 : e, P);
+}
+
+// S1 twiddles: v[k1] *= omega_{2^14}^{+-k1 m}, m = 16 a + b, as
+// omega_{2^14}^{b k1} (b k1 < 256) times omega_1024^{a k1} (a k1 < 1024).
+__device__ __forceinline__ void row_s1_twiddle(double (&v)[16], int m, const BigTw* ta,
+                                               const double* tb, const Prime& P, bool inv) {
+  const int a = m >> 4, b = m & 15;
+  v[0] = reduce(v[0], P.p, P.pinv);
+#pragma unroll
+  for (int k = 1; k < 16; ++k) {
+    const double wb = tb[(inv ? kBigTB : 0) + b * k];
+    const double2 wa = row_ta(ta, a * k, inv, P);
+    const double x = mm_shoup(v[k], wb, wb * P.pinv, P.p);
+    v[k] = mm_shoup(x, wa.x, wa.y, P.p);
+  }
+}
+
+// S2 twiddles: v[k2] *= omega_1024^{+-k2 m2}.
+__device__ __forceinline__ void row_s2_twiddle(double (&v)[16], int m2, const BigTw* ta,
+                                               const Prime& P, bool inv) {
+  v[0] = reduce(v[0], P.p, P.pinv);
+#pragma unroll
+  for (int k = 1; k < 16; ++k) {
+    const double2 w = row_ta(ta, k * m2, inv, P);
+    v[k] = mm_shoup(v[k], w.x, w.y, P.p);
+  }
+}
+
+// S3 twiddles: v[k3] *= omega_64^{+-k3 n4} = omega_1024^{+-16 k3 n4}.
+__device__ __forceinline__ void row_s3_twiddle(double (&v)[16], int n4, const BigTw* ta,
+                                               const Prime& P, bool inv) {
+  v[0] = reduce(v[0], P.p, P.pinv);
+#pragma unroll
+  for (int k = 1; k < 16; ++k) {
+    const double2 w = row_ta(ta, 16 * k * n4, inv, P);
+    v[k] = mm_shoup(v[k], w.x, w.y, P.p);
+  }
+}
+
+// Exchange before S3 (forward direction): this lane (n3 = lane >> 2,
+// n4 = lane & 3) holds k2 = 0..15 and receives n3 = 0..15 as lane 4 k2 + n4.
+// R is the wave's 1024 doubles.
+__device__ __forceinline__ void row_w1_put(const double (&v)[16], double* R, int lane) {
+  const int n3 = lane >> 2, n4 = lane & 3;
+  double* rb = R + n3 * 64 + n4;
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) rb[(k2 ^ n3) << 2] = v[k2];
+}
+__device__ __forceinline__ void row_w1_get(double (&v)[16], const double* R, int lane) {
+#pragma unroll
+  for (int n3 = 0; n3 < 16; ++n3) v[n3] = R[n3 * 64 + (lane ^ (n3 << 2))];
+}
+// Exchange before S4: this lane (k2 = lane >> 2, n4 = lane & 3) holds
+// k3 = 4 q + s and receives register 4 s + n4 as lane 4 k2 + q.
+__device__ __forceinline__ void row_w2_put(const double (&v)[16], double* R, int lane) {
+  const int k2 = lane >> 2, n4 = lane & 3;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double* rb = R + n4 * 64 + k2 * 4 + (q ^ n4);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) rb[s * 256] = v[4 * q + s];
+  }
+}
+__device__ __forceinline__ void row_w2_get(double (&v)[16], const double* R, int lane) {
+#pragma unroll
+  for (int row = 0; row < 16; ++row) v[row] = R[row * 64 + (lane ^ (row & 3))];
+}
+// The same two exchanges in the inverse direction (put where the forward
+// got, get where it put).
+__device__ __forceinline__ void row_w2_put_inv(const double (&v)[16], double* R, int lane) {
+#pragma unroll
+  for (int row = 0; row < 16; ++row) R[row * 64 + (lane ^ (row & 3))] = v[row];
+}
+__device__ __forceinline__ void row_w2_get_inv(double (&v)[16], const double* R, int lane) {
+  const int k2 = lane >> 2, n4 = lane & 3;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double* rb = R + n4 * 64 + k2 * 4 + (q ^ n4);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) v[4 * q + s] = rb[s * 256];
+  }
+}
+__device__ __forceinline__ void row_w1_put_inv(const double (&v)[16], double* R, int lane) {
+#pragma unroll
+  for (int n3 = 0; n3 < 16; ++n3) R[n3 * 64 + (lane ^ (n3 << 2))] = v[n3];
+}
+__device__ __forceinline__ void row_w1_get_inv(double (&v)[16], const double* R, int lane) {
+  const int n3 = lane >> 2, n4 = lane & 3;
+  const double* rb = R + n3 * 64 + n4;
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) v[k2] = rb[(k2 ^ n3) << 2];
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kRowNTH) void k_ntt_row(PassArgs a) {
+  static_assert(MODE == kFwd || MODE == kFused, "row passes run forward or fused");
+  __shared__ double xs[kBigE];
+  __shared__ BigTw ta[1024];
+  __shared__ double tb[2 * kBigTB];
+  const int pi = blockIdx.y;
+  const Prime P = a.prm[pi];
+  for (int e = threadIdx.x; e < 1024; e += kRowNTH) ta[e] = P.twL[e << (kLogE - 10)].x;
+  for (int e = threadIdx.x; e < 2 * kBigTB; e += kRowNTH)
+    tb[e] = root_pow((uint64_t)(e & (kBigTB - 1)), kBigLogE, e >= kBigTB, P).x;
+  uint64_t zoff = 0, soff = 0;
+  if (a.descs) {
+    const uint64_t zi = (uint64_t)blockIdx.z << a.zshift;
+    zoff = zi * a.zstride;
+    soff = ((zi >> a.sshift) << a.sshift) * a.zstride;
+  }
+  const double* src = a.src + zoff + pi * a.N;
+  double* dst = a.dst + zoff + pi * a.N;
+  const double* spec = MODE == kFused ? a.spec + soff + pi * a.N : nullptr;
+  const uint64_t ntiles = a.ntiles;
+
+  auto load_tile = [&](uint64_t w, double (&x)[16]) {
+    const double* s = src + (w << kBigLogE) + big_tid();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) x[r] = s[r << 10];
+  };
+
+  auto tile_step = [&](uint64_t w, double (&xr)[16], uint64_t wload) {
+    const uint64_t base = w << kBigLogE;
+    // fused: operand a's spectrum of this tile, loaded now, used after the
+    // forward stages
+    double sa[16];
+    if constexpr (MODE == kFused) {
+      const double* sp = spec + base + big_tid();
+#pragma unroll
+      for (int r = 0; r < 16; ++r) sa[r] = sp[r << 10];
+    }
+    double v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = xr[r];
+    dft16(v, P, false);
+    row_s1_twiddle(v, big_tid(), ta, tb, P, false);
+    __syncthreads();  // every wave is done with the previous tile's LDS
+    {
+      const int t = big_tid();
+      double* xw = xs + t;  // [k1][w][lane]
+#pragma unroll
+      for (int k1 = 0; k1 < 16; ++k1) xw[k1 << 10] = v[k1];
+    }
+    __syncthreads();
+    if (wload < ntiles) load_tile(wload, xr);
+    {
+      const int t = big_tid();
+      const int lane = t & 63;
+      const double* xw = xs + ((t >> 6) << 10) + lane;  // [w][n2][lane]
+#pragma unroll
+      for (int n2 = 0; n2 < 16; ++n2) v[n2] = xw[n2 << 6];
+      dft16(v, P, false);
+      row_s2_twiddle(v, lane, ta, P, false);
+      double* R = xs + ((t >> 6) << 10);
+      wave_lds_sync();
+      row_w1_put(v, R, lane);
+      wave_lds_sync();
+      row_w1_get(v, R, lane);
+    }
+    dft16(v, P, false);
+    row_s3_twiddle(v, big_tid() & 3, ta, P, false);
+    {
+      const int t = big_tid();
+      double* R = xs + ((t >> 6) << 10);
+      wave_lds_sync();
+      row_w2_put(v, R, t & 63);
+      wave_lds_sync();
+      row_w2_get(v, R, t & 63);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) dft_core<4>(v + 4 * s, P, false);
+    if constexpr (MODE == kFwd) {
+      double* d = dst + base + big_tid();
+#pragma unroll
+      for (int r = 0; r < 16; ++r) d[r << 10] = reduce(v[r], P.p, P.pinv);
+    } else {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const double x = mm_gen(v[r], sa[r], P.p, P.pinv);
+        v[r] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) dft_core<4>(v + 4 * s, P, true);
+      {
+        const int t = big_tid();
+        double* R = xs + ((t >> 6) << 10);
+        wave_lds_sync();
+        row_w2_put_inv(v, R, t & 63);
+        wave_lds_sync();
+        row_w2_get_inv(v, R, t & 63);
+      }
+      row_s3_twiddle(v, big_tid() & 3, ta, P, true);
+      dft16(v, P, true);
+      {
+        const int t = big_tid();
+        const int lane = t & 63;
+        double* R = xs + ((t >> 6) << 10);
+        wave_lds_sync();
+        row_w1_put_inv(v, R, lane);
+        wave_lds_sync();
+        row_w1_get_inv(v, R, lane);
+        row_s2_twiddle(v, lane, ta, P, true);
+      }
+      dft16(v, P, true);
+      __syncthreads();  // every wave is done with its own rows of LDS
+      {
+        const int t = big_tid();
+        double* xw = xs + ((t >> 6) << 10) + (t & 63);  // [w][n2][lane]
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) xw[n2 << 6] = v[n2];
+      }
+      __syncthreads();
+      {
+        const int t = big_tid();
+        const double* xw = xs + t;  // [k1][w][lane]
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) v[k1] = xw[k1 << 10];
+      }
+      row_s1_twiddle(v, big_tid(), ta, tb, P, true);
+      dft16(v, P, true);
+      double* d = dst + base + big_tid();
+#pragma unroll
+      for (int r = 0; r < 16; ++r) d[r << 10] = reduce(v[r], P.p, P.pinv);
+    }
+  };
+
+  // tile order as in k_ntt_col (XCD by XCD)
+  const uint64_t g = gridDim.x;
+  uint64_t w = blockIdx.x;
+  if ((g & 7) == 0) w = (uint64_t)(blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
+  if (w >= ntiles) return;
+  double xa[16];
+  load_tile(w, xa);
+  if constexpr (MODE == kFwd) {
+    // two tiles of loads in flight: xa holds tile w, xb tile w + g
+    double xb[16];
+    if (w + g < ntiles) load_tile(w + g, xb);
+    __syncthreads();  // tables
+    for (;;) {
+      tile_step(w, xa, w + 2 * g);
+      w += g;
+      if (w >= ntiles) break;
+      tile_step(w, xb, w + 2 * g);
+      w += g;
+      if (w >= ntiles) break;
+    }
+  } else {
+    __syncthreads();  // tables
+    for (;;) {
+      tile_step(w, xa, w + g);
+      w += g;
+      if (w >= ntiles) break;
+    }
+  }
+}
+
+}  // namespace pcg

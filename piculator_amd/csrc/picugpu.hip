@@ -29,6 +29,7 @@
 #include "df_kernels.hpp"
 #include "fft_kernels.hpp"
 #include "ntt_big.hpp"
+#include "ntt_row.hpp"
 #include "ntt_kernels.hpp"
 
 using namespace pcg;
@@ -89,7 +90,7 @@ struct PassList {
 struct Plan {
   int logN;
   int logE;
-  bool big = false;  // every pass a large-tile pass (k_ntt_big, 2^14 points per tile)
+  bool big = false;  // large-tile passes (k_ntt_col, k_ntt_row: 2^14 points per tile)
   PassList passes;
 };
 
@@ -108,7 +109,7 @@ int plan_passes(int logN, int maxcol) {
   return m;
 }
 
-// Large-tile plan (k_ntt_big): a column pass of 2^(logN-14) points (8 or
+// Large-tile plan (k_ntt_col, k_ntt_row): a column pass of 2^(logN-14) points (8 or
 // more adjacent columns per tile) and a row pass of 2^14: two sweeps.  Used
 // from 2^21 to 2^25 points, where it measured as fast or faster than the
 // 4096-point passes (2^22: 0.25 vs 0.31 ms per product; 2^20 and 2^26 are
@@ -235,11 +236,11 @@ PassFn row_pass_fn(int logL, int logLines) {
 template <int M>
 PassFn big_col_fn(int logL) {
   switch (logL) {
-    case 7: return &k_ntt_big<M, 7, false>;
-    case 8: return &k_ntt_big<M, 8, false>;
-    case 9: return &k_ntt_big<M, 9, false>;
-    case 10: return &k_ntt_big<M, 10, false>;
-    case 11: return &k_ntt_big<M, 11, false>;
+    case 7: return &k_ntt_col<M, 7>;
+    case 8: return &k_ntt_col<M, 8>;
+    case 9: return &k_ntt_col<M, 9>;
+    case 10: return &k_ntt_col<M, 10>;
+    case 11: return &k_ntt_col<M, 11>;
   }
   return nullptr;
 }
@@ -467,8 +468,8 @@ PassFn select_pass(int mode, const Plan& pl, const Pass& ps) {
   if (pl.big) {
     if (ps.logS == 0) {
       if (ps.logL != kBigLogE) return nullptr;
-      if (mode == kFwd) return &k_ntt_big<kFwd, kBigLogE, true>;
-      if (mode == kFused) return &k_ntt_big<kFused, kBigLogE, true>;
+      if (mode == kFwd) return &k_ntt_row<kFwd>;
+      if (mode == kFused) return &k_ntt_row<kFused>;
       return nullptr;
     }
     if (ps.logS < ps.logLines) return nullptr;  // a tile's columns within one block
