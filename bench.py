@@ -150,6 +150,27 @@ def probe_rates():
                      "FP64 FMAs, 16 workgroups of 256 threads per CU (piculator_amd/csrc/probe_mulmod.hip)"}
 
 
+STREAM_PATTERNS = ((0, 0), (0, 8), (8, 0), (8, 8), (0, 16), (16, 0), (16, 16), (32, 32), (4, 4))
+
+
+def stream_probe():
+    """Markdown table of the HBM access-pattern probe (libpcg_probe.so,
+    piculator_amd/csrc/probe_stream.hip): a 2^25-point, two-prime tile copy
+    with the large-tile passes' thread shape, each side contiguous (row
+    tiles) or strided with C adjacent columns (runs of 8 C bytes)."""
+    import ctypes as C
+
+    lib = C.CDLL(os.path.join(ROOT, "piculator_amd", "libpcg_probe.so"))
+    side = lambda c: "contiguous" if c == 0 else "strided, %d columns (%d-byte runs)" % (c, 8 * c)
+    print("| read side | write side | us | TB/s (read + write) |")
+    print("|---|---|---|---|")
+    for rc, wc in STREAM_PATTERNS:
+        us, bps = C.c_double(), C.c_double()
+        if lib.pcg_probe_stream(rc, wc, C.byref(us), C.byref(bps)) != 0:
+            raise RuntimeError("stream probe failed for pattern %d -> %d" % (rc, wc))
+        print("| %s | %s | %.1f | %.2f |" % (side(rc), side(wc), us.value, bps.value / 1e12), flush=True)
+
+
 def ntt_mul(logn: int, steps: int, warmup: int):
     """The 2^logn x 2^logn NTT multiply through pcg_mul_dev: mean time from HIP
     events over `steps` back-to-back multiplies, stage profile, product check."""
@@ -454,6 +475,8 @@ def main():
                     help="only the roofline multiply (for counter runs)")
     ap.add_argument("--sweep", action="store_true",
                     help="print the single-multiply table (sizes x shapes x engines) instead")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="print the HBM access-pattern probe table instead")
     ap.add_argument("--summarize-trace", metavar="DIR",
                     help="print the kernel table of a rocprofv3 --kernel-trace run (no GPU)")
     ap.add_argument("--summarize-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -487,6 +510,9 @@ def main():
         del src, dst
     if args.sweep:
         sweep(args.mul_steps)
+        return
+    if args.stream_probe:
+        stream_probe()
         return
 
     from piculator_amd import picugpu, pihost
