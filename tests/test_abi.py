@@ -27,6 +27,17 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
+def test_probe_library_exports_the_probes():
+    """The measurement probes bench.py loads (FP64 mulmod peak, HBM access
+    patterns) are built beside the engine; an unknown pattern is refused
+    without touching the GPU."""
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(picugpu.LIB_PATH), "libpcg_probe.so"))
+    assert hasattr(lib, "pcg_probe_fp64_rates")
+    assert hasattr(lib, "pcg_probe_stream")
+    us, bps = ctypes.c_double(), ctypes.c_double()
+    assert lib.pcg_probe_stream(3, 5, ctypes.byref(us), ctypes.byref(bps)) == 2
+
+
 def test_calls_before_init_fail_cleanly():
     lib = picugpu.load()
     assert lib.pcg_mul(None, 0, None, 0, None, 0) == 2  # PCG_ERR_NOT_INIT
