@@ -449,4 +449,155 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col(PassArgs a) {
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Inverse 2^11-point column pass over 16 adjacent columns (the last inverse
+// pass of a 2^25-point transform).
+//
+// A 2^14-point LDS tile holds 8 columns of 2^11 points, so k_ntt_col moves
+// 64-byte runs, half a 128-byte line per access.  This pass takes 16
+// adjacent columns per workgroup and runs them as two 8-column halves, one
+// after the other, through the stages of k_ntt_col<kInv, 11>; every global
+// access covers 4 rows x 16 columns (whole lines).  The lanes l and l ^ 8
+// hold adjacent rows of the same column in the stage layout, so each pair
+// of accesses is followed (loads) or preceded (stores) by one exchange
+// between them (a DPP row rotation by 8 lanes): lane l with bit 3 clear
+// keeps the even row's value of half 0 and trades its odd-row value for its
+// partner's even-row value of half 1.  Half 1's inputs wait in registers
+// while half 0 runs, and half 0's outputs while half 1 runs, so tiles are
+// not prefetched across tiles.  (The forward column pass keeps k_ntt_col:
+// its inputs are 4-byte limbs, and without its cross-tile prefetch it
+// measured slower.)
+
+constexpr int kCol16LogL = 11;
+
+// x from lane (lane ^ 8): within a 16-lane DPP row, rotation by 8 lanes
+__device__ __forceinline__ double lane_xor8(double x) {
+  const uint64_t u = (uint64_t)__double_as_longlong(x);
+  int lo = (int)(uint32_t)u, hi = (int)(uint32_t)(u >> 32);
+  lo = __builtin_amdgcn_update_dpp(0, lo, 0x128, 0xf, 0xf, false);  // row_ror:8
+  hi = __builtin_amdgcn_update_dpp(0, hi, 0x128, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+__global__ __launch_bounds__(kBigNTH) void k_ntt_col16_inv(PassArgs a) {
+  constexpr int LOGL = kCol16LogL;
+  using GE = BigGeo<LOGL>;  // one half: 8 adjacent columns
+  constexpr int LOGR1 = GE::LOGR1;
+  constexpr int R1 = 1 << LOGR1;
+  constexpr int NB1 = kBigEPT / R1;
+  constexpr int LOGTA = LOGL - 4;
+  static_assert(kBigEPT == 16 && kBigNBL == 1, "16 points per thread, one last-stage butterfly");
+  __shared__ double lds[GE::LDS];
+  __shared__ BigTw ta[1 << LOGTA];
+  __shared__ double tb[2 * kBigTB];
+  const int pi = blockIdx.y;
+  const Prime P = a.prm[pi];
+  big_tables<GE>(ta, tb, P);
+  const int logS = a.logS;
+  const uint64_t zoff = a.descs ? ((uint64_t)blockIdx.z << a.zshift) * a.zstride : 0;
+  const double* src = a.src + zoff + pi * a.N;
+  double* dst = a.dst + zoff + pi * a.N;
+  const int logRoot = LOGL + logS;
+  const uint64_t ntiles = a.ntiles;  // tiles of 16 columns
+
+  auto tile_base = [&](uint64_t t) -> uint64_t {
+    const int lgc = logS - 4;
+    return ((t >> lgc) << (LOGL + logS)) + ((t & ((1ull << lgc) - 1)) << 4);
+  };
+  // pre-twiddle bases of a half's first-stage butterflies (as k_ntt_col)
+  auto inv_twiddles = [&](uint64_t hbase, double2 (&cur)[NB1], double2 (&stp)[NB1]) {
+    const int bt = big_tid();
+#pragma unroll
+    for (int b = 0; b < NB1; ++b) {
+      int lam, n0;
+      big_pos<GE, LOGR1>(bt, b, 0, lam, n0);
+      const uint64_t c = (hbase & ((1ull << logS) - 1)) + (uint64_t)lam;
+      cur[b] = root_pow(c * (uint64_t)n0, logRoot, true, P);
+      stp[b] = root_pow(c * (uint64_t)(GE::L >> LOGR1), logRoot, true, P);
+    }
+  };
+  // Both halves' first-stage inputs of this thread: h[b R1 + r] is point
+  // n = (t >> 3) + 128 b + 256 r of column 8 h + (t & 7) (big_pos with
+  // R = R1).  Lane pair (l, l ^ 8): rows n0 = n - hb and n0 + 1, with
+  // hb = bit 3 of the lane; each lane loads both rows in its own half hb.
+  auto load_pair = [&](uint64_t b0, double (&h0)[kBigEPT], double (&h1)[kBigEPT]) {
+    const int bt = big_tid();
+    const int hb = (bt >> 3) & 1;
+    const uint64_t col = (uint64_t)((hb << 3) + (bt & 7));
+    const int n0 = (bt >> 3) & ~1;
+#pragma unroll
+    for (int b = 0; b < NB1; ++b)
+#pragma unroll
+      for (int r = 0; r < R1; ++r) {
+        const int n = n0 + (b << 7) + (r << 8);
+        const uint64_t gA = b0 + ((uint64_t)n << logS) + col;
+        const double A = src[gA];
+        const double B = src[gA + (1ull << logS)];
+        const double recv = lane_xor8(hb ? A : B);
+        h0[b * R1 + r] = hb ? recv : A;
+        h1[b * R1 + r] = hb ? B : recv;
+      }
+  };
+  // Both halves' outputs: o_h[r] is point (t >> 3) + 128 r of column
+  // 8 h + (t & 7) (big_pos with R = 16); the same pairing as the loads.
+  auto store_pair = [&](uint64_t b0, const double (&o0)[16], const double (&o1)[16]) {
+    const int bt = big_tid();
+    const int hb = (bt >> 3) & 1;
+    const uint64_t col = (uint64_t)((hb << 3) + (bt & 7));
+    const int n0 = (bt >> 3) & ~1;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const double recv = lane_xor8(hb ? o0[r] : o1[r]);
+      const uint64_t gA = b0 + ((uint64_t)(n0 + (r << 7)) << logS) + col;
+      dst[gA] = hb ? recv : o0[r];
+      dst[gA + (1ull << logS)] = hb ? o1[r] : recv;
+    }
+  };
+  // One 8-column half (columns hbase .. hbase + 7): the stages of
+  // k_ntt_col<kInv, 11> with inputs and outputs in registers; icur/istp
+  // hold this half's pre-twiddle bases, and after the first stage the
+  // next half's (when next).
+  auto half_step = [&](uint64_t hbase, const double (&xin)[kBigEPT], double (&out)[16], double2 (&icur)[NB1],
+                       double2 (&istp)[NB1], bool next) {
+    double v1[NB1][R1];
+#pragma unroll
+    for (int b = 0; b < NB1; ++b) {
+      double2 cur = icur[b];
+      const double2 stp = istp[b];
+#pragma unroll
+      for (int r = 0; r < R1; ++r) {
+        v1[b][r] = mm_shoup(xin[b * R1 + r], cur.x, cur.y, P.p);
+        const double wv = mm_shoup(cur.x, stp.x, stp.y, P.p);
+        cur = make_double2(wv, wv * P.pinv);
+      }
+    }
+    big_compute<GE, LOGR1, 0>(v1, ta, tb, P, true);
+    __syncthreads();  // the previous half's last-stage reads of LDS are done
+    big_write<GE, LOGR1, 0>(v1, lds);
+    __syncthreads();
+    if (next) inv_twiddles(hbase + 8, icur, istp);
+    big_middle<GE, LOGR1>(lds, ta, tb, P, true);
+    double y[kBigNBL][16];
+    big_last<GE>(lds, y, ta, tb, P, true);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) out[r] = reduce(y[0][r], P.p, P.pinv);
+  };
+
+  const uint64_t g = gridDim.x;
+  uint64_t w = blockIdx.x;
+  if ((g & 7) == 0) w = (uint64_t)(blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
+  for (; w < ntiles; w += g) {
+    const uint64_t b0 = tile_base(w);
+    double h0[kBigEPT], h1[kBigEPT];
+    load_pair(b0, h0, h1);
+    double2 icur[NB1], istp[NB1];
+    inv_twiddles(b0, icur, istp);
+    double o0[16], o1[16];
+    half_step(b0, h0, o0, icur, istp, true);
+    half_step(b0 + 8, h1, o1, icur, istp, false);
+    store_pair(b0, o0, o1);
+  }
+}
+
 }  // namespace pcg
