@@ -451,6 +451,132 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col(PassArgs a) {
 
 
 // ---------------------------------------------------------------------------
+// Forward column pass from the limbs of an operand that fills at most the
+// lower half of the transform (nlimbs <= N / 2: a balanced product's
+// operands, and the shorter one of any product).
+//
+// A column of L = 2^LOGL points then has inputs only at rows n < L / 2, so
+// its DFT splits into two DFTs of L / 2 points:
+//   Y[2j]     = sum_{n < L/2} x_n omega_{L/2}^{n j}
+//   Y[2j + 1] = sum_{n < L/2} (x_n omega_L^n) omega_{L/2}^{n j}.
+// A 2^14-point LDS tile holds L / 2 rows of 2^(15 - LOGL) adjacent columns
+// (16 for 2^11-point columns: every store covers whole 128-byte lines,
+// where k_ntt_col<kFwd, 11> stores 64-byte runs), and the two half-length
+// DFTs run one after the other through the stages of k_ntt_col on the
+// same 16 limbs per thread.  Each output row k = 2j + s gets the inter-pass
+// twiddle omega_{L s}^{c k} of k_ntt_col, so the output is the same
+// transform.  The limbs of the next tile (16 words per thread) are loaded
+// while the current tile's stages run.
+template <int LOGL>
+__global__ __launch_bounds__(kBigNTH) void k_ntt_col_half(PassArgs a) {
+  using GE = BigGeo<LOGL - 1>;  // one half-length line per column
+  constexpr int LH = GE::L;      // L / 2
+  constexpr int LOGR1 = GE::LOGR1;
+  constexpr int R1 = 1 << LOGR1;
+  constexpr int NB1 = kBigEPT / R1;
+  constexpr int LOGTA = LOGL - 1 - 4;
+  static_assert(kBigNBL == 1, "one last-stage butterfly per thread");
+  __shared__ double lds[GE::LDS];
+  __shared__ BigTw ta[1 << LOGTA];
+  __shared__ double tb[2 * kBigTB];
+  __shared__ double tp[LH];  // omega_L^n, n < L / 2 (odd rows' pre-twiddle)
+  const int pi = blockIdx.y;
+  const Prime P = a.prm[pi];
+  big_tables<GE>(ta, tb, P);
+  for (int e = threadIdx.x; e < LH; e += blockDim.x) tp[e] = root_pow((uint64_t)e, LOGL, false, P).x;
+  const int logS = a.logS;
+  const uint32_t* limbs = a.limbs;
+  const uint64_t nlimbs = a.nlimbs;
+  double* dst = a.dst + pi * a.N;
+  const int logRoot = LOGL + logS;
+  const uint64_t ntiles = a.ntiles;  // tiles of 2^(15 - LOGL) columns
+
+  auto tile_base = [&](uint64_t t) -> uint64_t {
+    const int lgc = logS - GE::LOGLINES;
+    return ((t >> lgc) << (LOGL + logS)) + ((t & ((1ull << lgc) - 1)) << GE::LOGLINES);
+  };
+  // limbs of this thread's first-stage inputs (rows n < L / 2); limbs past
+  // the operand read as zero (the clamped load stays in bounds)
+  auto load_tile = [&](uint64_t t, uint32_t (&x)[kBigEPT]) {
+    const uint64_t b0 = tile_base(t);
+    const int bt = big_tid();
+#pragma unroll
+    for (int b = 0; b < NB1; ++b)
+#pragma unroll
+      for (int r = 0; r < R1; ++r) {
+        int lam, n;
+        big_pos<GE, LOGR1>(bt, b, r, lam, n);
+        const uint64_t g = b0 + ((uint64_t)n << logS) + lam;
+        const uint32_t v = limbs[g < nlimbs ? g : nlimbs - 1];
+        x[b * R1 + r] = g < nlimbs ? v : 0u;
+      }
+  };
+
+  auto tile_step = [&](const uint64_t w, const uint32_t (&xc)[kBigEPT], uint32_t (&xn)[kBigEPT],
+                       const uint64_t wload) {
+    const uint64_t base = tile_base(w);
+    const int bt = big_tid();
+    int lam_o, n_o0;
+    big_pos<GE, 4>(bt, 0, 0, lam_o, n_o0);
+    const uint64_t c = (base & ((1ull << logS) - 1)) + (uint64_t)lam_o;
+    // output row k = 2 (n_o0 + r L / 32) + s: twiddle omega^{c k} from
+    // omega^{c (2 n_o0 + s)} by steps of omega^{c L / 16}
+    const double2 fstp = root_pow(c * (uint64_t)(GE::L >> 3), logRoot, false, P);
+#pragma unroll 1
+    for (int s = 0; s < 2; ++s) {
+      double2 cur = root_pow(c * (uint64_t)(2 * n_o0 + s), logRoot, false, P);
+      double v1[NB1][R1];
+#pragma unroll
+      for (int b = 0; b < NB1; ++b)
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+          const double x = (double)xc[b * R1 + r];
+          if (s == 0) {
+            v1[b][r] = x;
+          } else {
+            int lam, n;
+            big_pos<GE, LOGR1>(big_tid(), b, r, lam, n);
+            const double wv = tp[n];
+            v1[b][r] = mm_shoup(x, wv, wv * P.pinv, P.p);
+          }
+        }
+      big_compute<GE, LOGR1, 0>(v1, ta, tb, P, false);
+      __syncthreads();  // LDS readers of the previous half-length DFT are done
+      big_write<GE, LOGR1, 0>(v1, lds);
+      __syncthreads();
+      if (s == 0 && wload < ntiles) load_tile(wload, xn);
+      big_middle<GE, LOGR1>(lds, ta, tb, P, false);
+      double y[kBigNBL][16];
+      big_last<GE>(lds, y, ta, tb, P, false);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const double x = mm_shoup(y[0][r], cur.x, cur.y, P.p);
+        const double wv = mm_shoup(cur.x, fstp.x, fstp.y, P.p);
+        cur = make_double2(wv, wv * P.pinv);
+        const uint64_t k = (uint64_t)(2 * (n_o0 + r * (GE::L >> 4)) + s);
+        dst[base + (k << logS) + lam_o] = x;
+      }
+    }
+  };
+
+  // tile order as in k_ntt_col (XCD by XCD)
+  const uint64_t g = gridDim.x;
+  uint64_t w = blockIdx.x;
+  if ((g & 7) == 0) w = (uint64_t)(blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
+  if (w >= ntiles) return;
+  uint32_t xc[kBigEPT];
+  load_tile(w, xc);
+  for (;;) {
+    uint32_t xn[kBigEPT];
+    tile_step(w, xc, xn, w + g);
+    w += g;
+    if (w >= ntiles) break;
+#pragma unroll
+    for (int i = 0; i < kBigEPT; ++i) xc[i] = xn[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Inverse 2^11-point column pass over 16 adjacent columns (the last inverse
 // pass of a 2^25-point transform).
 //

@@ -245,6 +245,19 @@ PassFn big_col_fn(int logL) {
   return nullptr;
 }
 
+// Forward column passes from the limbs of an operand in the lower half of
+// the transform (two half-length DFTs per 2^14-point tile).
+PassFn big_col_half_fn(int logL) {
+  switch (logL) {
+    case 7: return &k_ntt_col_half<7>;
+    case 8: return &k_ntt_col_half<8>;
+    case 9: return &k_ntt_col_half<9>;
+    case 10: return &k_ntt_col_half<10>;
+    case 11: return &k_ntt_col_half<11>;
+  }
+  return nullptr;
+}
+
 constexpr int kMinLogN = 8;
 
 struct Ctx {
@@ -567,6 +580,14 @@ int launch_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double* 
   if (fn == &k_ntt_col16_inv) {
     if (!src) return fail(PCG_ERR_INTERNAL, "inverse column pass without input");
     a.ntiles /= 2;  // tiles of 16 columns
+  }
+  if (pl.big && mode == kFwd && !src && !descs && ps.logS >= kBigLogE + 1 - ps.logL && nlimbs >= 1 &&
+      2 * nlimbs <= a.N) {
+    // the operand fills at most the lower half of the transform
+    if (PassFn h = big_col_half_fn(ps.logL)) {
+      fn = h;
+      a.ntiles = a.N >> (kBigLogE + 1);  // tiles of 2^14 limbs (2^15 output points)
+    }
   }
   // Persistent tiles: as many workgroups as stay resident (occupancy of
   // this instance x compute units), spread over the primes and products.

@@ -7,7 +7,8 @@ import pytest
 
 from oracle import oracle as O
 from piculator_amd import workload
-from tests.helpers import exact_product, limb_digest, oracle_product, single_product_digest
+from tests.helpers import (exact_product, limb_digest, oracle_product, residues_match,
+                           single_product_digest)
 
 pytestmark = pytest.mark.gpu
 
@@ -75,6 +76,35 @@ def test_workload_exact(gpu, logn, unbalanced):
     if logn <= 16:
         assert np.array_equal(got, exact_product(a, b))
     assert limb_digest(got) == single_product_digest(n, nb)
+
+
+@pytest.mark.parametrize("logN", [21, 22, 23, 24, 25])
+def test_half_filled_operands(gpu, logN):
+    """Operands of at most N/2 limbs at every large-tile transform length
+    (2^21 .. 2^25 points: their forward column pass runs as two half-length
+    DFTs per tile): the NTT product equals the FFT engine's and matches the
+    check-prime residues; an operand one limb longer than N/2 (the full
+    column pass) too; all-ones operands of N/2 limbs square to the closed
+    form (2^(32n) - 1)^2."""
+    N = 1 << logN
+    rng = np.random.default_rng(logN)
+    for na, nb in ((N // 2, N // 2 - 12345), (N // 2 + 1, N // 4 + 3)):
+        a = rng.integers(0, 2**32, na, dtype=np.uint64).astype(np.uint32)
+        b = rng.integers(0, 2**32, nb, dtype=np.uint64).astype(np.uint32)
+        a[-1] |= 1
+        b[-1] |= 1
+        got = gpu.mul(a, b, gpu.ENGINE_NTT)
+        assert got.size == na + nb
+        assert residues_match(a, b, got), (na, nb)
+        assert np.array_equal(got, gpu.mul(a, b, gpu.ENGINE_FFT)), (na, nb)
+    n = N // 2
+    ones = np.full(n, M32, dtype=np.uint32)
+    got = gpu.mul(ones, ones.copy(), gpu.ENGINE_NTT)
+    want = np.zeros(2 * n, dtype=np.uint32)
+    want[0] = 1
+    want[n] = 0xFFFFFFFE
+    want[n + 1:] = M32
+    assert np.array_equal(got, want)
 
 
 def _low_limbs(a, b, k):
