@@ -28,7 +28,9 @@ constexpr int kLogE = 12;  // points staged per workgroup (max)
 constexpr int kE = 1 << kLogE;
 constexpr int kEPT = 8;    // points per thread per stage
 
-enum PassMode { kFwd = 0, kInv = 1, kFused = 2 };
+// kFusedA / kSquareA: the fused row pass of a large-tile plan that also runs
+// operand a's row pass (k_ntt_row_fa); kSquareA when a and b are one operand.
+enum PassMode { kFwd = 0, kInv = 1, kFused = 2, kFusedA = 3, kSquareA = 4 };
 
 // One product of a batched launch (device pointers).
 struct BatchDesc {

@@ -307,4 +307,178 @@ __global__ __launch_bounds__(kRowNTH) void k_ntt_row(PassArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused row pass that also runs operand a's row DFT.
+//
+// k_ntt_row<kFused> reads a's spectrum, which a separate k_ntt_row<kFwd>
+// pass wrote; here the pass reads a after its column pass instead and runs
+// a's row DFT in the tile (the spectrum stays in registers, in the layout
+// the pointwise product needs): one sweep of a's points fewer per product
+// (read 16 + write 16 bytes per point) for one more row DFT's arithmetic.
+// SQUARE: a and b are one operand (src == spec): one row DFT, the pointwise
+// square.  Per tile, the loads of b are issued before a's row DFT and the
+// next tile's a during b's cross-wave exchange.
+
+// Forward row DFT (S1 .. S4) of the 16 points t + 1024 r of thread t, into
+// the S4 layout; between() runs between the two workgroup barriers of the
+// cross-wave exchange.
+template <class F>
+__device__ __forceinline__ void row_forward(double (&v)[16], double* xs, const BigTw* ta, const double* tb,
+                                            const Prime& P, F&& between) {
+  dft16(v, P, false);
+  row_s1_twiddle(v, big_tid(), ta, tb, P, false);
+  __syncthreads();  // every wave is done with the LDS
+  {
+    const int t = big_tid();
+    double* xw = xs + t;  // [k1][w][lane]
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) xw[k1 << 10] = v[k1];
+  }
+  __syncthreads();
+  between();
+  {
+    const int t = big_tid();
+    const int lane = t & 63;
+    const double* xw = xs + ((t >> 6) << 10) + lane;  // [w][n2][lane]
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = xw[n2 << 6];
+    dft16(v, P, false);
+    row_s2_twiddle(v, lane, ta, P, false);
+    double* R = xs + ((t >> 6) << 10);
+    wave_lds_sync();
+    row_w1_put(v, R, lane);
+    wave_lds_sync();
+    row_w1_get(v, R, lane);
+  }
+  dft16(v, P, false);
+  row_s3_twiddle(v, big_tid() & 3, ta, P, false);
+  {
+    const int t = big_tid();
+    double* R = xs + ((t >> 6) << 10);
+    wave_lds_sync();
+    row_w2_put(v, R, t & 63);
+    wave_lds_sync();
+    row_w2_get(v, R, t & 63);
+  }
+#pragma unroll
+  for (int s = 0; s < 4; ++s) dft_core<4>(v + 4 * s, P, false);
+}
+
+// Inverse row DFT of the pointwise products in v (S4 layout), stored
+// reduced at points t + 1024 r of the tile at d.
+__device__ __forceinline__ void row_inverse_store(double (&v)[16], double* xs, const BigTw* ta,
+                                                  const double* tb, const Prime& P, double* d) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) dft_core<4>(v + 4 * s, P, true);
+  {
+    const int t = big_tid();
+    double* R = xs + ((t >> 6) << 10);
+    wave_lds_sync();
+    row_w2_put_inv(v, R, t & 63);
+    wave_lds_sync();
+    row_w2_get_inv(v, R, t & 63);
+  }
+  row_s3_twiddle(v, big_tid() & 3, ta, P, true);
+  dft16(v, P, true);
+  {
+    const int t = big_tid();
+    const int lane = t & 63;
+    double* R = xs + ((t >> 6) << 10);
+    wave_lds_sync();
+    row_w1_put_inv(v, R, lane);
+    wave_lds_sync();
+    row_w1_get_inv(v, R, lane);
+    row_s2_twiddle(v, lane, ta, P, true);
+  }
+  dft16(v, P, true);
+  __syncthreads();  // every wave is done with its own rows of LDS
+  {
+    const int t = big_tid();
+    double* xw = xs + ((t >> 6) << 10) + (t & 63);  // [w][n2][lane]
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) xw[n2 << 6] = v[n2];
+  }
+  __syncthreads();
+  {
+    const int t = big_tid();
+    const double* xw = xs + t;  // [k1][w][lane]
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) v[k1] = xw[k1 << 10];
+  }
+  row_s1_twiddle(v, big_tid(), ta, tb, P, true);
+  dft16(v, P, true);
+  double* dd = d + big_tid();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) dd[r << 10] = reduce(v[r], P.p, P.pinv);
+}
+
+template <bool SQUARE>
+__global__ __launch_bounds__(kRowNTH) void k_ntt_row_fa(PassArgs a) {
+  __shared__ double xs[kBigE];
+  __shared__ BigTw ta[1024];
+  __shared__ double tb[2 * kBigTB];
+  const int pi = blockIdx.y;
+  const Prime P = a.prm[pi];
+  for (int e = threadIdx.x; e < 1024; e += kRowNTH) ta[e] = P.twL[e << (kLogE - 10)].x;
+  for (int e = threadIdx.x; e < 2 * kBigTB; e += kRowNTH)
+    tb[e] = root_pow((uint64_t)(e & (kBigTB - 1)), kBigLogE, e >= kBigTB, P).x;
+  const double* src = a.src + pi * a.N;   // b after its column pass
+  const double* sra = a.spec + pi * a.N;  // a after its column pass (src when SQUARE)
+  double* dst = a.dst + pi * a.N;
+  const uint64_t ntiles = a.ntiles;
+
+  auto load_tile = [&](const double* s0, uint64_t w, double (&x)[16]) {
+    const double* s = s0 + (w << kBigLogE) + big_tid();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) x[r] = s[r << 10];
+  };
+
+  // tile order as in k_ntt_col (XCD by XCD)
+  const uint64_t g = gridDim.x;
+  uint64_t w = blockIdx.x;
+  if ((g & 7) == 0) w = (uint64_t)(blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
+  if (w >= ntiles) return;
+  double xa[16];
+  load_tile(sra, w, xa);
+  __syncthreads();  // tables
+  for (;;) {
+    const uint64_t wn = w + g;
+    double v[16];
+    if constexpr (SQUARE) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r] = xa[r];
+      row_forward(v, xs, ta, tb, P, [&] {
+        if (wn < ntiles) load_tile(sra, wn, xa);
+      });
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const double x = mm_gen(v[r], reduce(v[r], P.p, P.pinv), P.p, P.pinv);
+        v[r] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
+      }
+    } else {
+      double xb[16], sa[16];
+      load_tile(src, w, xb);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) sa[r] = xa[r];
+      row_forward(sa, xs, ta, tb, P, [] {});
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        sa[r] = reduce(sa[r], P.p, P.pinv);
+        v[r] = xb[r];
+      }
+      row_forward(v, xs, ta, tb, P, [&] {
+        if (wn < ntiles) load_tile(sra, wn, xa);
+      });
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const double x = mm_gen(v[r], sa[r], P.p, P.pinv);
+        v[r] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
+      }
+    }
+    row_inverse_store(v, xs, ta, tb, P, dst + (w << kBigLogE));
+    w = wn;
+    if (w >= ntiles) break;
+  }
+}
+
 }  // namespace pcg

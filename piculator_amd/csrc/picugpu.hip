@@ -483,6 +483,8 @@ PassFn select_pass(int mode, const Plan& pl, const Pass& ps) {
       if (ps.logL != kBigLogE) return nullptr;
       if (mode == kFwd) return &k_ntt_row<kFwd>;
       if (mode == kFused) return &k_ntt_row<kFused>;
+      if (mode == kFusedA) return &k_ntt_row_fa<false>;
+      if (mode == kSquareA) return &k_ntt_row_fa<true>;
       return nullptr;
     }
     if (ps.logS < ps.logLines) return nullptr;  // a tile's columns within one block
@@ -606,18 +608,24 @@ int ntt_log_points(uint64_t zL) {
   return logN < kMinLogN ? kMinLogN : logN;
 }
 
-// Forward transform of operand a into the spectrum A ([2][N]).
+// Passes of operand a's forward transform that run on their own: all of
+// them, but for large-tile plans the last (row) pass runs inside the fused
+// pass (k_ntt_row_fa), which reads a after its column pass.
+size_t a_own_passes(const Plan& pl) { return pl.big ? pl.passes.size() - 1 : pl.passes.size(); }
+
+// Forward transform of operand a into A ([2][N]): its spectrum, or for
+// large-tile plans a after its column pass (what the fused pass reads).
 int ntt_forward_into(Ctx& c, const Plan& pl, const uint32_t* d_a, uint64_t na, double* A) {
-  for (size_t j = 0; j < pl.passes.size(); ++j) {
+  for (size_t j = 0; j < a_own_passes(pl); ++j) {
     const int rc = launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : A, d_a, na, A, nullptr);
     if (rc) return rc;
   }
   return PCG_OK;
 }
 
-// With the spectrum of a in A: forward transform of b (its last pass fused
-// with the pointwise product and the first inverse pass) and the inverse
-// passes; B receives the convolution's residues ([2][N]).
+// With a's transform in A (ntt_forward_into): forward transform of b (its
+// last pass fused with the pointwise product and the first inverse pass)
+// and the inverse passes; B receives the convolution's residues ([2][N]).
 int ntt_tail_into(Ctx& c, const Plan& pl, const uint32_t* d_b, uint64_t nb, const double* A, double* B) {
   const size_t m = pl.passes.size();
   int rc;
@@ -625,7 +633,7 @@ int ntt_tail_into(Ctx& c, const Plan& pl, const uint32_t* d_b, uint64_t nb, cons
     rc = launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : B, d_b, nb, B, nullptr);
     if (rc) return rc;
   }
-  rc = launch_pass(c, kFused, pl, pl.passes[m - 1], m == 1 ? nullptr : B, d_b, nb, B, A);
+  rc = launch_pass(c, pl.big ? kFusedA : kFused, pl, pl.passes[m - 1], m == 1 ? nullptr : B, d_b, nb, B, A);
   if (rc) return rc;
   for (size_t j = m - 1; j-- > 0;) {
     rc = launch_pass(c, kInv, pl, pl.passes[j], B, nullptr, 0, B, nullptr);
@@ -814,22 +822,29 @@ int ntt_mul_enqueue(Ctx& c, const uint32_t* d_a, uint64_t na, const uint32_t* d_
   if (d_a == d_b && na == nb && m >= 2) {
     // a square: the fused pass takes a's forward transform but its last
     // pass as its input, so the last pass also writes into A (out of place)
-    // and b's forward passes are skipped (2 m instead of 3 m - 1 sweeps)
+    // and b's forward passes are skipped (2 m instead of 3 m - 1 sweeps).
+    // Large-tile plans run that last pass inside the fused pass (one row
+    // DFT, the pointwise square: 2 m - 1 sweeps); a's column pass writes A,
+    // which is a's transform as ntt_forward_into leaves it (PCG_KEEP_B).
     const uint64_t N = 1ull << pl.logN;
     double* A = c.spec;
     double* B = c.spec + 2 * N;
+    double* X = pl.big ? A : B;
     for (size_t j = 0; j + 1 < m; ++j) {
-      rc = launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : B, d_a, na, B, nullptr);
+      rc = launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : X, d_a, na, X, nullptr);
       if (rc) return rc;
     }
-    rc = launch_pass(c, kFwd, pl, pl.passes[m - 1], B, nullptr, 0, A, nullptr);
-    if (rc) return rc;
+    if (!pl.big) {
+      rc = launch_pass(c, kFwd, pl, pl.passes[m - 1], B, nullptr, 0, A, nullptr);
+      if (rc) return rc;
+    }
     if (profile) HIPCHK(hipEventRecord(c.ev[1], c.stream));
     if (mp.wp.use) {
       rc = ntt_wrap_top(c, mp.wp, d_a, na, d_b, nb);
       if (rc) return rc;
     }
-    rc = launch_pass(c, kFused, pl, pl.passes[m - 1], B, nullptr, 0, B, A);
+    rc = pl.big ? launch_pass(c, kSquareA, pl, pl.passes[m - 1], A, nullptr, 0, B, A)
+                : launch_pass(c, kFused, pl, pl.passes[m - 1], B, nullptr, 0, B, A);
     for (size_t j = m - 1; !rc && j-- > 0;) rc = launch_pass(c, kInv, pl, pl.passes[j], B, nullptr, 0, B, nullptr);
     if (rc) return rc;
     if (profile) HIPCHK(hipEventRecord(c.ev[2], c.stream));

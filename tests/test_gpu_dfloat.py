@@ -265,6 +265,44 @@ def test_kept_transform_reused(gpu, ns, nx, p):
         check(g, O.BF(*x).mul(O.BF(*s), p).tuple())
 
 
+@pytest.mark.parametrize("ns", [(1 << 21) - 3, (1 << 21) + 5])
+def test_kept_square_transform_reused(gpu, ns):
+    """s.mul(s) with PCG_KEEP_B (the square path), then x.mul(s) with
+    PCG_REUSE_B -- the 1/sqrt Newton step's r0 r0, then t r0 -- at a
+    large-tile transform length (2^22 points; the second size wraps onto a
+    2^22-point main transform): both results exact."""
+    from tests.helpers import CHECK_PRIMES, residue
+
+    rng = np.random.default_rng(ns)
+    s = rng.integers(0, 2**32, ns, dtype=np.uint64).astype(np.uint32)
+    x = rng.integers(0, 2**32, ns - 7, dtype=np.uint64).astype(np.uint32)
+    s[-1] |= 1
+    x[-1] |= 1
+    L, r = gpu.load(), gpu.C.byref
+
+    def products(flags):
+        sv, xv = gpu.df_from_tuple((True, 0, s)), gpu.df_from_tuple((False, 3, x))
+        z1, z2 = gpu.DFloat(), gpu.DFloat()
+        try:
+            gpu._check(L.pcg_df_mul(r(z1), r(sv), r(sv), 0, flags[0]))
+            gpu._check(L.pcg_df_mul(r(z2), r(xv), r(sv), 0, flags[1]))
+            return gpu.df_to_tuple(z1), gpu.df_to_tuple(z2)
+        finally:
+            for v in (sv, xv, z1, z2):
+                L.pcg_df_free(r(v))
+
+    got = products((gpu.KEEP_B, gpu.REUSE_B))
+    want = products((0, 0))
+    for g, w in zip(got, want):
+        check(g, w)
+    for (sign, exp, z), y, sgn in ((got[0], s, True), (got[1], x, False)):
+        z = np.asarray(z, dtype=np.uint32)
+        assert sign == sgn and exp == (0 if sgn else 3)
+        assert z.size == y.size + s.size or (z.size == y.size + s.size - 1)
+        for q in CHECK_PRIMES:
+            assert residue(z, q) == residue(y, q) * residue(s, q) % q
+
+
 def test_kept_transform_overwritten(gpu):
     """Another product between the keep and the reuse overwrites the kept
     transform: the reuse recomputes it, results exact."""
