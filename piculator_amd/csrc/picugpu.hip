@@ -110,21 +110,26 @@ int plan_passes(int logN, int maxcol) {
 }
 
 // Large-tile plan (k_ntt_col, k_ntt_row): a column pass of 2^(logN-14) points (8 or
-// more adjacent columns per tile) and a row pass of 2^14: two sweeps.  Used
-// from 2^21 to 2^25 points, where it measured as fast or faster than the
-// 4096-point passes (2^22: 0.25 vs 0.31 ms per product; 2^20 and 2^26 are
-// faster on three 4096-point passes).
+// more adjacent columns per tile) and a row pass of 2^14: two sweeps; from
+// 2^26 points two column passes of 2^6 .. 2^7 points and the row pass.  Used
+// from 2^21 to 2^28 points, where it measured faster than the 4096-point
+// passes (2^22: 0.25 vs 0.31 ms per product; 2^26: 4.46 vs 5.03 ms, 2^27:
+// 8.75 vs 10.04 ms); 2^20 is faster on two 4096-point passes.
 constexpr int kBigMinLogN = 21;
-constexpr int kBigMaxLogN = kBigLogE + 11;
+constexpr int kBigMaxLogN = 28;
 
 Plan make_big_plan(int logN) {
   Plan pl;
   pl.logN = logN;
   pl.logE = kBigLogE;
   pl.big = true;
-  const int ls[2] = {logN - kBigLogE, kBigLogE};
+  // two passes up to 2^25 points; above, two column passes of 2^6 .. 2^7
+  // points before the 2^14-point row pass
+  const int cols = logN - kBigLogE;
+  const int np = cols <= 11 ? 2 : 3;
+  const int ls[3] = {np == 2 ? cols : cols / 2, np == 2 ? kBigLogE : cols - cols / 2, kBigLogE};
   int rem = logN;
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < np; ++i) {
     Pass ps;
     ps.logL = ls[i];
     rem -= ps.logL;
@@ -236,6 +241,7 @@ PassFn row_pass_fn(int logL, int logLines) {
 template <int M>
 PassFn big_col_fn(int logL) {
   switch (logL) {
+    case 6: return &k_ntt_col<M, 6>;
     case 7: return &k_ntt_col<M, 7>;
     case 8: return &k_ntt_col<M, 8>;
     case 9: return &k_ntt_col<M, 9>;
@@ -249,6 +255,7 @@ PassFn big_col_fn(int logL) {
 // the transform (two half-length DFTs per 2^14-point tile).
 PassFn big_col_half_fn(int logL) {
   switch (logL) {
+    case 6: return &k_ntt_col_half<6>;
     case 7: return &k_ntt_col_half<7>;
     case 8: return &k_ntt_col_half<8>;
     case 9: return &k_ntt_col_half<9>;
