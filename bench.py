@@ -469,6 +469,8 @@ def main():
     ap.add_argument("--cpu-sample-digits", type=int, default=100000000)
     ap.add_argument("--no-fft", action="store_true", help="skip the FFT-engine multiply line")
     ap.add_argument("--no-probe", action="store_true", help="skip the FP64 mulmod peak probe")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay captured HIP graphs of the subtree level schedules (default: kernel by kernel)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run one device copy of 512 MiB (counter calibration)")
     ap.add_argument("--mul-only", action="store_true",
@@ -519,6 +521,7 @@ def main():
     from tests.helpers import pi_spot_checks
 
     picugpu.init(0)
+    picugpu.bsr_graphs(args.graph)
     roofline, mul_info = ntt_mul(args.logn, args.mul_steps, 3)
     rates = None
     if not args.no_probe:
@@ -592,7 +595,8 @@ def main():
                                "schedule, every big multiply on the GPU)" % D,
                    "digits": D, "hex_digits": hexd, "terms": terms, "precision_limbs": p,
                    "roofline_kernel": "2^%d x 2^%d-limb two-prime FP64 NTT multiply"
-                                      % (args.logn, args.logn)},
+                                      % (args.logn, args.logn),
+                   "subtree_schedule": "captured HIP graph" if args.graph else "kernel by kernel"},
         "roofline": roofline,
         "mulmod": mulmod,
         "step_s": walls,

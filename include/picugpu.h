@@ -167,6 +167,14 @@ int pcg_bsr_bound(uint32_t a, uint32_t b, uint64_t* bound);
  * first evaluation.  The reference has no plans (Pi_BSR recurses,
  * PI.h:64-112). */
 int pcg_bsr_prepare(const uint32_t* sizes, uint64_t n);
+/* Selects how pcg_bsr_exact and pcg_df_bsr_exact run a subtree's level
+ * schedule: enable = 1 records it once per plan as a HIP graph (captured
+ * on the plan's first evaluation, again after the batched scratch it names
+ * was reallocated) and replays the graph for every subtree of that plan;
+ * enable = 0 (the default) launches it kernel by kernel.  The results are
+ * the same.  The reference recurses instead (Pi_BSR, PI.h:93-112, one
+ * std::thread per branch); no counterpart there. */
+int pcg_bsr_graphs(int enable);
 /* Self-check, host only (no GPU call, no pcg_init needed): lays out the plan
  * of a subtree of `leaves` leaves whose largest leaf index is bmax (the
  * plans size every value from the leaf bounds at bmax; 0 = any leaf) as the

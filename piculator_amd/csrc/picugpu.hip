@@ -305,6 +305,9 @@ struct Ctx {
   CarryFn* bagg = nullptr;  // chunk carry functions of batched products
   uint64_t* bcin = nullptr;
   size_t bcarry_cap = 0;
+  // bumped whenever bspec, wrap_desc, bagg or bcin is reallocated (captured
+  // subtree schedules name them)
+  uint64_t scratch_gen = 1;
   // profiling
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t tev[2] = {nullptr, nullptr};
@@ -1105,20 +1108,34 @@ bool class_pairs(int cls) { return cls >= kPassClass || cls > kSmallNttMaxLog; }
 // kernels carry a product grid dimension; scratch is 4N doubles per product.
 // pairs: products 2k and 2k+1 have the same operand a, whose forward
 // transform runs once (count is even).
-int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, bool pairs = false,
-                      int logM = 0) {
-  const uint64_t N = 1ull << logN;
-  const uint64_t M = logM ? 1ull << logM : 0;
-  const uint64_t per = 4 * N + 4 * M;  // main spectra, then the top product's
-  if (pairs && (count & 1)) return fail(PCG_ERR_INTERNAL, "odd count of paired products");
-  size_t group_max = std::max<size_t>(1, ((size_t)1 << 28) / per);  // 2 GiB of scratch
-  if (pairs) group_max = std::max<size_t>(2, group_max & ~(size_t)1);
-  const size_t want = std::min(count, group_max) * per;
+// Group size, scratch per product and carry chunks per product of a batched
+// multi-pass launch at 2^logN points (wrapped: top products of 2^logM).
+struct BatchGeom {
+  uint64_t N, M, per;
+  size_t group_max;
+  uint32_t nch;
+};
+BatchGeom batch_geom(int logN, bool pairs, int logM) {
+  BatchGeom g;
+  g.N = 1ull << logN;
+  g.M = logM ? 1ull << logM : 0;
+  g.per = 4 * g.N + 4 * g.M;  // main spectra, then the top product's
+  g.group_max = std::max<size_t>(1, ((size_t)1 << 28) / g.per);  // 2 GiB of scratch
+  if (pairs) g.group_max = std::max<size_t>(2, g.group_max & ~(size_t)1);
   // chunk carry functions: (products per group) x (chunks per product); a
   // wrapped product has up to N + M / 2 + 1 limbs
-  const uint32_t nch = (uint32_t)((N + M / 2 + 1 + kCarryBlock - 1) / kCarryBlock);
-  const size_t carry_need = std::min(count, group_max) * (size_t)nch;
+  g.nch = (uint32_t)((g.N + g.M / 2 + 1 + kCarryBlock - 1) / kCarryBlock);
+  return g;
+}
+
+// Grows the batched scratch to what such a launch needs (synchronising the
+// stream first); nothing is allocated when it already suffices.
+int batched_reserve(Ctx& c, size_t count, int logN, bool pairs, int logM) {
+  const BatchGeom g = batch_geom(logN, pairs, logM);
+  const size_t want = std::min(count, g.group_max) * g.per;
+  const size_t carry_need = std::min(count, g.group_max) * (size_t)g.nch;
   if (carry_need > c.bcarry_cap) {
+    ++c.scratch_gen;
     HIPCHK(hipStreamSynchronize(c.stream));
     if (c.bagg) HIPCHK(hipFree(c.bagg));
     if (c.bcin) HIPCHK(hipFree(c.bcin));
@@ -1131,6 +1148,7 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, b
     c.bcarry_cap = cap;
   }
   if (want > c.bspec_doubles) {
+    ++c.scratch_gen;
     HIPCHK(hipStreamSynchronize(c.stream));
     if (c.bspec) HIPCHK(hipFree(c.bspec));
     c.bspec = nullptr;
@@ -1141,15 +1159,28 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, b
     }
     c.bspec_doubles = want;
   }
-  if (logM && std::min(count, group_max) > c.wrap_desc_cap) {
+  if (logM && std::min(count, g.group_max) > c.wrap_desc_cap) {
+    ++c.scratch_gen;
     HIPCHK(hipStreamSynchronize(c.stream));
     if (c.wrap_desc) HIPCHK(hipFree(c.wrap_desc));
     c.wrap_desc = nullptr;
     c.wrap_desc_cap = 0;
-    const size_t cap = std::max<size_t>(std::min(count, group_max), 1024);
+    const size_t cap = std::max<size_t>(std::min(count, g.group_max), 1024);
     HIPCHK(hipMalloc(&c.wrap_desc, cap * sizeof(BatchDesc)));
     c.wrap_desc_cap = cap;
   }
+  return PCG_OK;
+}
+
+int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, bool pairs = false,
+                      int logM = 0) {
+  if (pairs && (count & 1)) return fail(PCG_ERR_INTERNAL, "odd count of paired products");
+  const int rr = batched_reserve(c, count, logN, pairs, logM);
+  if (rr) return rr;
+  const BatchGeom g = batch_geom(logN, pairs, logM);
+  const uint64_t N = g.N, M = g.M, per = g.per;
+  const size_t group_max = g.group_max;
+  const uint32_t nch = g.nch;
   const Plan pl = make_plan(logN);
   const size_t m = pl.passes.size();
   double* A = c.bspec;
@@ -1617,9 +1648,23 @@ struct BsrPlan {
   size_t nslots = 0;
   CarryFn* add_agg = nullptr;       // per (large add, chunk)
   uint64_t* add_cin = nullptr;
+  uint32_t* a_dev = nullptr;  // the subtree's first index a (read by k_bsr_base)
+  // the level schedule captured as a graph (replayed while the plan and the
+  // batched scratch it names stay the same)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t graph_gen = 0;  // Ctx::scratch_gen at capture
 };
 
+void bsr_plan_drop_graph(BsrPlan& P) {
+  if (P.exec) (void)hipGraphExecDestroy(P.exec);
+  if (P.graph) (void)hipGraphDestroy(P.graph);
+  P.exec = nullptr;
+  P.graph = nullptr;
+}
+
 void bsr_plan_free(BsrPlan& P) {
+  bsr_plan_drop_graph(P);
   (void)hipFree(P.buf);
   (void)hipFree(P.arena);
   P = BsrPlan();
@@ -1990,8 +2035,9 @@ int bsr_plan_build(Ctx& c, uint32_t d, const LeafBits& leaf, const SmallTable& t
                       L.nadd * sizeof(BsrAdd) + L.nmul * sizeof(BatchDesc) +
                       S.max_large * sizeof(BsrAddSlot) +
                       S.max_large_chunks * (sizeof(CarryFn) + sizeof(uint64_t)) +
-                      ns * sizeof(BsrSizeInfo) + ns * ns * 4 + 13 * 256;
+                      ns * sizeof(BsrSizeInfo) + ns * ns * 4 + 4 + 14 * 256;
   HIPCHK(hipStreamSynchronize(c.stream));  // the slot's old plan: last use
+  bsr_plan_drop_graph(P);
   NP.buf = P.buf;
   NP.arena = P.arena;
   NP.buf_cap = P.buf_cap;
@@ -2030,6 +2076,7 @@ int bsr_plan_build(Ctx& c, uint32_t d, const LeafBits& leaf, const SmallTable& t
   P.add_slots = carve<BsrAddSlot>(p, S.max_large);
   P.add_agg = carve<CarryFn>(p, S.max_large_chunks);
   P.add_cin = carve<uint64_t>(p, S.max_large_chunks);
+  P.a_dev = carve<uint32_t>(p, 1);
   BsrSizeInfo* d_info = carve<BsrSizeInfo>(p, ns);
   uint32_t* d_cnt = carve<uint32_t>(p, ns * ns);
   uint32_t* d_err = carve<uint32_t>(p, 1);
@@ -2140,28 +2187,40 @@ int bsr_plan_get(Ctx& c, uint32_t d, const LeafBits& leaf, BsrPlan** out) {
   return PCG_OK;
 }
 
-// Evaluates the subtree [a, b) into the plan's arena; rv receives the
-// metadata of P, Q, R (their limbs stay in (*Pout)->arena until the next call).
-int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const uint64_t* caps,
-                    BsrPlan** Pout, BsrVal rv[3]) {
-  if (b <= a) return fail(PCG_ERR_INVALID, "empty range");
-  int rct = ensure_tuned(c);  // plans record the small-product classes
-  if (rct) return rct;
-  const uint32_t d = b - a;
-  BsrPlan* P = nullptr;
-  int rc = bsr_plan_get(c, d, leaf_bits_for(b), &P);  // the leaves are a + 1 .. b
-  if (rc) return rc;
-  if (P->maxb > max_limbs)
-    return fail(PCG_ERR_TOO_LARGE, "subtree values may reach %llu limbs > %llu",
-                (unsigned long long)P->maxb, (unsigned long long)max_limbs);
-  if (caps)
-    for (int j = 0; j < 3; ++j)
-      if (caps[j] < P->root_bound[j]) return fail(PCG_ERR_TOO_LARGE, "result buffer too small");
+// Grows the batched scratch to what every launch of the plan's level
+// schedule needs, so that the schedule itself allocates nothing (it can be
+// captured).
+int bsr_reserve(Ctx& c, const BsrPlan& P) {
+  for (int32_t h = 4; h <= P.H; ++h)
+    for (const auto& cl : P.hp[h].classes) {
+      int rc = PCG_OK;
+      if (cl.pairs || cl.wrap)
+        rc = batched_reserve(c, cl.n, cl.cls >= kPassClass ? cl.cls - kPassClass : cl.cls, cl.pairs, cl.wrap);
+      else if (cl.cls >= kPassClass)
+        rc = batched_reserve(c, cl.n, cl.cls - kPassClass, false, 0);
+      else if (cl.cls > kSmallNttMaxLog)
+        rc = batched_reserve(c, cl.n, cl.cls, false, cl.wrap);
+      if (rc) return rc;
+    }
+  return PCG_OK;
+}
+
+// The level schedule of a subtree on the plan (the first index a in
+// P.a_dev): leaves and nodes of up to four leaves, then per height the
+// operand descriptors, the products by class, the trims and the adds.
+// Launches only (after bsr_reserve: no allocation, no synchronisation).
+int bsr_schedule(Ctx& c, const BsrPlan& Pl) {
+  const BsrPlan* P = &Pl;
   // the large adds' slots start zeroed (k_bsr_add_finish clears each after use)
-  if (P->nslots) HIPCHK(hipMemsetAsync(P->add_slots, 0, P->nslots * sizeof(BsrAddSlot), c.stream));
+  if (P->nslots) {
+    static_assert(sizeof(BsrAddSlot) % 4 == 0, "slots clear as words");
+    const uint64_t words = P->nslots * sizeof(BsrAddSlot) / 4;
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, c.stream,
+                       reinterpret_cast<uint32_t*>(P->add_slots), words);
+  }
   // leaves and every node of height 2 and 3 in one launch
   hipLaunchKernelGGL(k_bsr_base, dim3((unsigned)((P->nleaves + 255) / 256)), dim3(256), 0, c.stream,
-                     P->leaf_rel, a, P->leaf_ids, P->nleaves, P->meta, P->arena);
+                     P->leaf_rel, P->a_dev, P->leaf_ids, P->nleaves, P->meta, P->arena);
   for (int32_t h = 4; h <= P->H; ++h) {
     const BsrHeight& hh = P->hp[h];
     const BsrMul* hm = P->muls + hh.mul0;
@@ -2209,6 +2268,71 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
     }
   }
   HIPCHK(hipGetLastError());
+  return PCG_OK;
+}
+
+__global__ void k_put_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+// Whether subtree evaluations replay their plan's captured level schedule
+// (pcg_bsr_graphs).  Off by default: the pi computation's subtrees mostly
+// have plans of their own (plans follow each subtree's leaf bounds), so
+// nearly every evaluation pays a capture and an instantiation on the host
+// while the GPU waits (pi to 1e9 digits 1.284-1.289 s with graphs against
+// 1.261 s kernel by kernel on one box).
+bool g_bsr_graphs = false;
+
+// Runs the level schedule of plan P: eagerly, or as the plan's graph
+// (captured on first use, and again after the batched scratch it names was
+// reallocated).
+int bsr_run_schedule(Ctx& c, BsrPlan& P) {
+  if (!g_bsr_graphs) return bsr_schedule(c, P);
+  if (P.exec && P.graph_gen != c.scratch_gen) bsr_plan_drop_graph(P);
+  if (!P.exec) {
+    HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    const int rc = bsr_schedule(c, P);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c.stream, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      return rc ? rc : fail(PCG_ERR_HIP, "capturing the subtree schedule: %s", hipGetErrorString(e));
+    }
+    hipGraphExec_t x = nullptr;
+    if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      return fail(PCG_ERR_HIP, "instantiating the subtree schedule");
+    }
+    P.graph = g;
+    P.exec = x;
+    P.graph_gen = c.scratch_gen;
+  }
+  HIPCHK(hipGraphLaunch(P.exec, c.stream));
+  return PCG_OK;
+}
+
+// Evaluates the subtree [a, b) into the plan's arena; rv receives the
+// metadata of P, Q, R (their limbs stay in (*Pout)->arena until the next call).
+int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const uint64_t* caps,
+                    BsrPlan** Pout, BsrVal rv[3]) {
+  if (b <= a) return fail(PCG_ERR_INVALID, "empty range");
+  int rct = ensure_tuned(c);  // plans record the small-product classes
+  if (rct) return rct;
+  const uint32_t d = b - a;
+  BsrPlan* P = nullptr;
+  int rc = bsr_plan_get(c, d, leaf_bits_for(b), &P);  // the leaves are a + 1 .. b
+  if (rc) return rc;
+  if (P->maxb > max_limbs)
+    return fail(PCG_ERR_TOO_LARGE, "subtree values may reach %llu limbs > %llu",
+                (unsigned long long)P->maxb, (unsigned long long)max_limbs);
+  if (caps)
+    for (int j = 0; j < 3; ++j)
+      if (caps[j] < P->root_bound[j]) return fail(PCG_ERR_TOO_LARGE, "result buffer too small");
+  rc = bsr_reserve(c, *P);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_put_u32, dim3(1), dim3(1), 0, c.stream, P->a_dev, a);
+  rc = bsr_run_schedule(c, *P);
+  if (rc) return rc;
   HIPCHK(hipMemcpyAsync(rv, P->meta, 3 * sizeof(BsrVal), hipMemcpyDeviceToHost, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
   *Pout = P;
@@ -3157,6 +3281,14 @@ int pcg_df_bsr_exact(uint32_t a, uint32_t b, uint64_t max_limbs, pcg_dfloat* P, 
                           c.stream));
     df_assign(out[j], buf, cap, rv[j].L, 0, rv[j].sign);
   }
+  return PCG_OK;
+}
+
+int pcg_bsr_graphs(int enable) {
+  DF_ENTER();
+  g_bsr_graphs = enable != 0;
+  if (!g_bsr_graphs)
+    for (BsrPlan& q : g_bsr_plans) bsr_plan_drop_graph(q);
   return PCG_OK;
 }
 

@@ -34,7 +34,7 @@ EXPORTS = [
     "pcg_dev_free", "pcg_host_alloc", "pcg_host_free", "pcg_memcpy_h2d", "pcg_memcpy_d2h",
     "pcg_fft_last_roundoff",
     "pcg_last_profile", "pcg_ntt_plan", "pcg_timer_start", "pcg_timer_stop",
-    "pcg_bsr_exact", "pcg_bsr_bound", "pcg_bsr_prepare", "pcg_df_free", "pcg_df_set",
+    "pcg_bsr_exact", "pcg_bsr_bound", "pcg_bsr_prepare", "pcg_bsr_graphs", "pcg_df_free", "pcg_df_set",
     "pcg_df_read", "pcg_df_copy", "pcg_df_ucmp", "pcg_df_add", "pcg_df_sub", "pcg_df_mul",
     "pcg_df_mul_pair", "pcg_df_mul_u32", "pcg_df_to_hex", "pcg_df_bsr_exact",
     "pcg_df_release_pool", "pcg_autotune", "pcg_batch_table", "pcg_bsr_plan_check",
@@ -100,6 +100,7 @@ def load():
     L.pcg_bsr_exact.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64] + [C.POINTER(BsrValue)] * 3
     L.pcg_bsr_bound.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
     L.pcg_bsr_prepare.argtypes = [C.POINTER(C.c_uint32), C.c_uint64]
+    L.pcg_bsr_graphs.argtypes = [C.c_int]
     L.pcg_bsr_plan_check.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_int), C.c_int]
     L.pcg_autotune.argtypes = [C.POINTER(C.c_int), C.c_int]
     L.pcg_batch_table.argtypes = [C.POINTER(C.c_int), C.c_int, C.c_int]
@@ -295,6 +296,13 @@ def bsr_prepare(sizes):
     ensure_init()
     arr = (C.c_uint32 * max(1, len(sizes)))(*sizes)
     _check(load().pcg_bsr_prepare(arr, len(sizes)))
+
+
+def bsr_graphs(enable: bool) -> None:
+    """Subtree schedules as replayed HIP graphs (True) or launched kernel by
+    kernel (False, the default)."""
+    ensure_init()
+    _check(load().pcg_bsr_graphs(1 if enable else 0))
 
 
 def bsr_plan_check(leaves: int, table, bmax: int = 0) -> None:

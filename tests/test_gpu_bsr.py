@@ -39,6 +39,30 @@ def test_wrapping_leaf_factors(gpu, a, b):
     assert got == [P.tuple(), Q.tuple(), R.tuple()]
 
 
+def test_captured_schedule_replays(gpu):
+    """Subtrees of one plan (same size, same leaf bounds near 7e7) run as
+    replays of the plan's captured level schedule, with other products and
+    synchronisations between the replays: every replay equals the
+    kernel-by-kernel evaluation, and that equals the oracle."""
+    import numpy as np
+
+    d = 20000
+    starts = [70_000_000 + k * d for k in range(3)]
+    try:
+        gpu.bsr_graphs(False)
+        want = [gpu.bsr_exact(a, a + d, BIG_P) for a in starts]
+        gpu.bsr_graphs(True)
+        other = (np.arange(1, 5000, dtype=np.uint32), np.arange(7, 3007, dtype=np.uint32))
+        for _ in range(2):
+            for a, w in zip(starts, want):
+                assert gpu.bsr_exact(a, a + d, BIG_P) == w, a
+                gpu.mul(*other)
+    finally:
+        gpu.bsr_graphs(False)
+    P, Q, R = O.pi_bsr(starts[1], starts[1] + d, BIG_P)
+    assert want[1] == [P.tuple(), Q.tuple(), R.tuple()]
+
+
 def test_window_too_small_is_refused(gpu):
     with pytest.raises(gpu.PcgError) as e:
         gpu.bsr_exact(0, 5000, 100)
