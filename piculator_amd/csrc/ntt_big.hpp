@@ -451,12 +451,13 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col(PassArgs a) {
 
 
 // ---------------------------------------------------------------------------
-// Forward column pass from the limbs of an operand that fills at most the
-// lower half of the transform (nlimbs <= N / 2: a balanced product's
-// operands, and the shorter one of any product).
+// First forward pass from an operand's limbs (column DFTs of L = 2^LOGL
+// points), as two half-length DFTs per tile.
 //
-// A column of L = 2^LOGL points then has inputs only at rows n < L / 2, so
-// its DFT splits into two DFTs of L / 2 points:
+// An operand that fills at most the lower half of the transform
+// (nlimbs <= N / 2: a balanced product's operands, the shorter one of any
+// product) has inputs only at rows n < L / 2 of each column, so its DFT
+// splits into two DFTs of L / 2 points:
 //   Y[2j]     = sum_{n < L/2} x_n omega_{L/2}^{n j}
 //   Y[2j + 1] = sum_{n < L/2} (x_n omega_L^n) omega_{L/2}^{n j}.
 // A 2^14-point LDS tile holds L / 2 rows of 2^(15 - LOGL) adjacent columns
@@ -467,8 +468,14 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col(PassArgs a) {
 // twiddle omega_{L s}^{c k} of k_ntt_col, so the output is the same
 // transform.  The limbs of the next tile (16 words per thread) are loaded
 // while the current tile's stages run.
+//
+// Batched launches (descs) take each product's operand and length.  A
+// product whose operand is longer than N / 2 has upper rows: each DFT's
+// inputs then add (x_n + x_{n+L/2}) or subtract (x_n - x_{n+L/2}) them,
+// the upper row's limbs loaded again for each (uniform per workgroup,
+// latency not hidden).
 template <int LOGL>
-__global__ __launch_bounds__(kBigNTH) void k_ntt_col_half(PassArgs a) {
+__global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs(PassArgs a) {
   using GE = BigGeo<LOGL - 1>;  // one half-length line per column
   constexpr int LH = GE::L;      // L / 2
   constexpr int LOGR1 = GE::LOGR1;
@@ -485,11 +492,21 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_half(PassArgs a) {
   big_tables<GE>(ta, tb, P);
   for (int e = threadIdx.x; e < LH; e += blockDim.x) tp[e] = root_pow((uint64_t)e, LOGL, false, P).x;
   const int logS = a.logS;
+  uint64_t zoff = 0;
   const uint32_t* limbs = a.limbs;
-  const uint64_t nlimbs = a.nlimbs;
-  double* dst = a.dst + pi * a.N;
+  uint64_t nlimbs = a.nlimbs;
+  if (a.descs) {
+    const uint64_t zi = (uint64_t)blockIdx.z << a.zshift;
+    const BatchDesc d = a.descs[zi];
+    zoff = zi * a.zstride;
+    limbs = a.operand ? d.b : d.a;
+    nlimbs = a.operand ? d.nb : d.na;
+  }
+  double* dst = a.dst + zoff + pi * a.N;
   const int logRoot = LOGL + logS;
   const uint64_t ntiles = a.ntiles;  // tiles of 2^(15 - LOGL) columns
+  const uint64_t half = (uint64_t)LH << logS;  // N / 2: the upper rows' offset
+  const bool full = nlimbs > half;
 
   auto tile_base = [&](uint64_t t) -> uint64_t {
     const int lgc = logS - GE::LOGLINES;
@@ -529,17 +546,32 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_half(PassArgs a) {
 #pragma unroll
       for (int b = 0; b < NB1; ++b)
 #pragma unroll
-        for (int r = 0; r < R1; ++r) {
-          const double x = (double)xc[b * R1 + r];
-          if (s == 0) {
-            v1[b][r] = x;
-          } else {
+        for (int r = 0; r < R1; ++r) v1[b][r] = (double)xc[b * R1 + r];
+      if (full) {
+        const int bt2 = big_tid();
+#pragma unroll
+        for (int b = 0; b < NB1; ++b)
+#pragma unroll
+          for (int r = 0; r < R1; ++r) {
+            int lam, n;
+            big_pos<GE, LOGR1>(bt2, b, r, lam, n);
+            const uint64_t gh = base + ((uint64_t)n << logS) + lam + half;
+            const uint32_t vh = limbs[gh < nlimbs ? gh : nlimbs - 1];
+            const double xh = gh < nlimbs ? (double)vh : 0.0;
+            v1[b][r] = s == 0 ? v1[b][r] + xh : v1[b][r] - xh;
+          }
+      }
+      if (s == 1) {
+#pragma unroll
+        for (int b = 0; b < NB1; ++b)
+#pragma unroll
+          for (int r = 0; r < R1; ++r) {
             int lam, n;
             big_pos<GE, LOGR1>(big_tid(), b, r, lam, n);
             const double wv = tp[n];
-            v1[b][r] = mm_shoup(x, wv, wv * P.pinv, P.p);
+            v1[b][r] = mm_shoup(v1[b][r], wv, wv * P.pinv, P.p);
           }
-        }
+      }
       big_compute<GE, LOGR1, 0>(v1, ta, tb, P, false);
       __syncthreads();  // LDS readers of the previous half-length DFT are done
       big_write<GE, LOGR1, 0>(v1, lds);

@@ -422,9 +422,15 @@ __global__ __launch_bounds__(kRowNTH) void k_ntt_row_fa(PassArgs a) {
   for (int e = threadIdx.x; e < 1024; e += kRowNTH) ta[e] = P.twL[e << (kLogE - 10)].x;
   for (int e = threadIdx.x; e < 2 * kBigTB; e += kRowNTH)
     tb[e] = root_pow((uint64_t)(e & (kBigTB - 1)), kBigLogE, e >= kBigTB, P).x;
-  const double* src = a.src + pi * a.N;   // b after its column pass
-  const double* sra = a.spec + pi * a.N;  // a after its column pass (src when SQUARE)
-  double* dst = a.dst + pi * a.N;
+  uint64_t zoff = 0, soff = 0;  // batched: as k_ntt_row
+  if (a.descs) {
+    const uint64_t zi = (uint64_t)blockIdx.z << a.zshift;
+    zoff = zi * a.zstride;
+    soff = ((zi >> a.sshift) << a.sshift) * a.zstride;
+  }
+  const double* src = a.src + zoff + pi * a.N;   // b after its column pass
+  const double* sra = a.spec + soff + pi * a.N;  // a after its column pass (src when SQUARE)
+  double* dst = a.dst + zoff + pi * a.N;
   const uint64_t ntiles = a.ntiles;
 
   auto load_tile = [&](const double* s0, uint64_t w, double (&x)[16]) {

@@ -251,16 +251,16 @@ PassFn big_col_fn(int logL) {
   return nullptr;
 }
 
-// Forward column passes from the limbs of an operand in the lower half of
-// the transform (two half-length DFTs per 2^14-point tile).
-PassFn big_col_half_fn(int logL) {
+// First forward passes from the limbs (two half-length DFTs per 2^14-point
+// tile).
+PassFn big_col_limbs_fn(int logL) {
   switch (logL) {
-    case 6: return &k_ntt_col_half<6>;
-    case 7: return &k_ntt_col_half<7>;
-    case 8: return &k_ntt_col_half<8>;
-    case 9: return &k_ntt_col_half<9>;
-    case 10: return &k_ntt_col_half<10>;
-    case 11: return &k_ntt_col_half<11>;
+    case 6: return &k_ntt_col_limbs<6>;
+    case 7: return &k_ntt_col_limbs<7>;
+    case 8: return &k_ntt_col_limbs<8>;
+    case 9: return &k_ntt_col_limbs<9>;
+    case 10: return &k_ntt_col_limbs<10>;
+    case 11: return &k_ntt_col_limbs<11>;
   }
   return nullptr;
 }
@@ -593,10 +593,9 @@ int launch_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double* 
     if (!src) return fail(PCG_ERR_INTERNAL, "inverse column pass without input");
     a.ntiles /= 2;  // tiles of 16 columns
   }
-  if (pl.big && mode == kFwd && !src && !descs && ps.logS >= kBigLogE + 1 - ps.logL && nlimbs >= 1 &&
-      2 * nlimbs <= a.N) {
-    // the operand fills at most the lower half of the transform
-    if (PassFn h = big_col_half_fn(ps.logL)) {
+  if (pl.big && mode == kFwd && !src && ps.logS >= kBigLogE + 1 - ps.logL && (descs || nlimbs >= 1)) {
+    // first pass from the limbs: two half-length DFTs per tile
+    if (PassFn h = big_col_limbs_fn(ps.logL)) {
       fn = h;
       a.ntiles = a.N >> (kBigLogE + 1);  // tiles of 2^14 limbs (2^15 output points)
     }
@@ -1190,7 +1189,7 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, b
     const BatchDesc* dd = d_desc + g0;
     int rc;
     const int sh = pairs ? 1 : 0;
-    for (size_t j = 0; j < m; ++j) {
+    for (size_t j = 0; j < a_own_passes(pl); ++j) {
       rc = launch_pass(c, kFwd, pl, pl.passes[j], j == 0 ? nullptr : A, nullptr, 0, A, nullptr,
                        dd, 0, per, k >> sh, sh, 0);
       if (rc) return rc;
@@ -1200,8 +1199,8 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, b
                        dd, 1, per, k);
       if (rc) return rc;
     }
-    rc = launch_pass(c, kFused, pl, pl.passes[m - 1], m == 1 ? nullptr : B, nullptr, 0, B, A, dd, 1,
-                     per, k, 0, sh);
+    rc = launch_pass(c, pl.big ? kFusedA : kFused, pl, pl.passes[m - 1], m == 1 ? nullptr : B, nullptr, 0,
+                     B, A, dd, 1, per, k, 0, sh);
     if (rc) return rc;
     for (size_t j = m - 1; j-- > 0;) {
       rc = launch_pass(c, kInv, pl, pl.passes[j], B, nullptr, 0, B, nullptr, dd, 1, per, k);
@@ -1216,7 +1215,7 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, b
       const size_t mm = pm.passes.size();
       double* A2 = c.bspec + 4 * N;
       double* B2 = A2 + 2 * M;
-      for (size_t j = 0; j < mm; ++j) {
+      for (size_t j = 0; j < a_own_passes(pm); ++j) {
         rc = launch_pass(c, kFwd, pm, pm.passes[j], j == 0 ? nullptr : A2, nullptr, 0, A2, nullptr,
                          c.wrap_desc, 0, per, k);
         if (rc) return rc;
@@ -1226,8 +1225,8 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, b
                          c.wrap_desc, 1, per, k);
         if (rc) return rc;
       }
-      rc = launch_pass(c, kFused, pm, pm.passes[mm - 1], mm == 1 ? nullptr : B2, nullptr, 0, B2, A2,
-                       c.wrap_desc, 1, per, k);
+      rc = launch_pass(c, pm.big ? kFusedA : kFused, pm, pm.passes[mm - 1], mm == 1 ? nullptr : B2, nullptr,
+                       0, B2, A2, c.wrap_desc, 1, per, k);
       if (rc) return rc;
       for (size_t j = mm - 1; j-- > 0;) {
         rc = launch_pass(c, kInv, pm, pm.passes[j], B2, nullptr, 0, B2, nullptr, c.wrap_desc, 1, per, k);
