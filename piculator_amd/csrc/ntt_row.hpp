@@ -316,8 +316,10 @@ __global__ __launch_bounds__(kRowNTH) void k_ntt_row(PassArgs a) {
 // the pointwise product needs): one sweep of a's points fewer per product
 // (read 16 + write 16 bytes per point) for one more row DFT's arithmetic.
 // SQUARE: a and b are one operand (src == spec): one row DFT, the pointwise
-// square.  Per tile, the loads of b are issued before a's row DFT and the
-// next tile's a during b's cross-wave exchange.
+// square.  LIMBS: a one-pass plan (N = 2^14, batched): both operands are
+// read from their limbs (descs), zero past their lengths.  Per tile, the
+// loads of b are issued before a's row DFT and the next tile's a during
+// b's cross-wave exchange.
 
 // Forward row DFT (S1 .. S4) of the 16 points t + 1024 r of thread t, into
 // the S4 layout; between() runs between the two workgroup barriers of the
@@ -412,8 +414,9 @@ __device__ __forceinline__ void row_inverse_store(double (&v)[16], double* xs, c
   for (int r = 0; r < 16; ++r) dd[r << 10] = reduce(v[r], P.p, P.pinv);
 }
 
-template <bool SQUARE>
+template <bool SQUARE, bool LIMBS>
 __global__ __launch_bounds__(kRowNTH) void k_ntt_row_fa(PassArgs a) {
+  static_assert(!(SQUARE && LIMBS), "squares take a's column pass output");
   __shared__ double xs[kBigE];
   __shared__ BigTw ta[1024];
   __shared__ double tb[2 * kBigTB];
@@ -423,20 +426,35 @@ __global__ __launch_bounds__(kRowNTH) void k_ntt_row_fa(PassArgs a) {
   for (int e = threadIdx.x; e < 2 * kBigTB; e += kRowNTH)
     tb[e] = root_pow((uint64_t)(e & (kBigTB - 1)), kBigLogE, e >= kBigTB, P).x;
   uint64_t zoff = 0, soff = 0;  // batched: as k_ntt_row
+  BatchDesc bd{};
   if (a.descs) {
     const uint64_t zi = (uint64_t)blockIdx.z << a.zshift;
     zoff = zi * a.zstride;
     soff = ((zi >> a.sshift) << a.sshift) * a.zstride;
+    if constexpr (LIMBS) bd = a.descs[zi];
   }
-  const double* src = a.src + zoff + pi * a.N;   // b after its column pass
-  const double* sra = a.spec + soff + pi * a.N;  // a after its column pass (src when SQUARE)
+  const double* src = LIMBS ? nullptr : a.src + zoff + pi * a.N;   // b after its column pass
+  const double* sra = LIMBS ? nullptr : a.spec + soff + pi * a.N;  // a after its column pass (src when SQUARE)
   double* dst = a.dst + zoff + pi * a.N;
   const uint64_t ntiles = a.ntiles;
 
-  auto load_tile = [&](const double* s0, uint64_t w, double (&x)[16]) {
-    const double* s = s0 + (w << kBigLogE) + big_tid();
+  // tile w of operand a (op 0) or b (op 1); LIMBS: the whole line
+  auto load_tile = [&](int op, uint64_t w, double (&x)[16]) {
+    if constexpr (LIMBS) {
+      const uint32_t* lp = op ? bd.b : bd.a;
+      const uint32_t nl = op ? bd.nb : bd.na;
+      const uint32_t t = (uint32_t)big_tid();
 #pragma unroll
-    for (int r = 0; r < 16; ++r) x[r] = s[r << 10];
+      for (int r = 0; r < 16; ++r) {
+        const uint32_t n = t + ((uint32_t)r << 10);
+        const uint32_t v = lp[n < nl ? n : nl - 1];
+        x[r] = n < nl ? (double)v : 0.0;
+      }
+    } else {
+      const double* s = (op ? src : sra) + (w << kBigLogE) + big_tid();
+#pragma unroll
+      for (int r = 0; r < 16; ++r) x[r] = s[r << 10];
+    }
   };
 
   // tile order as in k_ntt_col (XCD by XCD)
@@ -445,7 +463,7 @@ __global__ __launch_bounds__(kRowNTH) void k_ntt_row_fa(PassArgs a) {
   if ((g & 7) == 0) w = (uint64_t)(blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
   if (w >= ntiles) return;
   double xa[16];
-  load_tile(sra, w, xa);
+  load_tile(SQUARE ? 1 : 0, w, xa);
   __syncthreads();  // tables
   for (;;) {
     const uint64_t wn = w + g;
@@ -454,7 +472,7 @@ __global__ __launch_bounds__(kRowNTH) void k_ntt_row_fa(PassArgs a) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) v[r] = xa[r];
       row_forward(v, xs, ta, tb, P, [&] {
-        if (wn < ntiles) load_tile(sra, wn, xa);
+        if (wn < ntiles) load_tile(1, wn, xa);
       });
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
@@ -463,7 +481,7 @@ __global__ __launch_bounds__(kRowNTH) void k_ntt_row_fa(PassArgs a) {
       }
     } else {
       double xb[16], sa[16];
-      load_tile(src, w, xb);
+      load_tile(1, w, xb);
 #pragma unroll
       for (int r = 0; r < 16; ++r) sa[r] = xa[r];
       row_forward(sa, xs, ta, tb, P, [] {});
@@ -473,7 +491,7 @@ __global__ __launch_bounds__(kRowNTH) void k_ntt_row_fa(PassArgs a) {
         v[r] = xb[r];
       }
       row_forward(v, xs, ta, tb, P, [&] {
-        if (wn < ntiles) load_tile(sra, wn, xa);
+        if (wn < ntiles) load_tile(0, wn, xa);
       });
 #pragma unroll
       for (int r = 0; r < 16; ++r) {

@@ -142,6 +142,31 @@ Plan make_big_plan(int logN) {
   return pl;
 }
 
+// Plan of the products of a batched launch (many products, so a product's
+// own tile count matters less than for one product): one 2^14-point row
+// pass at 2^14 points (the fused pass transforms both operands from their
+// limbs), large tiles from 2^19.
+constexpr int kBatchBigMinLogN = 19;
+Plan make_plan(int logN);
+Plan make_batch_plan(int logN) {
+  if (logN == kBigLogE) {
+    Plan pl;
+    pl.logN = logN;
+    pl.logE = kBigLogE;
+    pl.big = true;
+    Pass ps;
+    ps.logL = kBigLogE;
+    ps.logS = 0;
+    ps.logLines = 0;
+    ps.logG = 0;
+    ps.logB = 0;
+    pl.passes.push_back(ps);
+    return pl;
+  }
+  if (logN >= kBatchBigMinLogN && logN <= kBigMaxLogN) return make_big_plan(logN);
+  return make_plan(logN);
+}
+
 Plan make_plan(int logN) {
   if (logN >= kBigMinLogN && logN <= kBigMaxLogN) return make_big_plan(logN);
   Plan pl;
@@ -241,6 +266,7 @@ PassFn row_pass_fn(int logL, int logLines) {
 template <int M>
 PassFn big_col_fn(int logL) {
   switch (logL) {
+    case 5: return &k_ntt_col<M, 5>;
     case 6: return &k_ntt_col<M, 6>;
     case 7: return &k_ntt_col<M, 7>;
     case 8: return &k_ntt_col<M, 8>;
@@ -493,8 +519,8 @@ PassFn select_pass(int mode, const Plan& pl, const Pass& ps) {
       if (ps.logL != kBigLogE) return nullptr;
       if (mode == kFwd) return &k_ntt_row<kFwd>;
       if (mode == kFused) return &k_ntt_row<kFused>;
-      if (mode == kFusedA) return &k_ntt_row_fa<false>;
-      if (mode == kSquareA) return &k_ntt_row_fa<true>;
+      if (mode == kFusedA) return &k_ntt_row_fa<false, false>;
+      if (mode == kSquareA) return &k_ntt_row_fa<true, false>;
       return nullptr;
     }
     if (ps.logS < ps.logLines) return nullptr;  // a tile's columns within one block
@@ -592,6 +618,11 @@ int launch_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double* 
   if (fn == &k_ntt_col16_inv) {
     if (!src) return fail(PCG_ERR_INTERNAL, "inverse column pass without input");
     a.ntiles /= 2;  // tiles of 16 columns
+  }
+  if (fn == &k_ntt_row_fa<false, false> && !src) {
+    // a one-pass plan: the fused pass transforms both operands from their limbs
+    if (!descs) return fail(PCG_ERR_INTERNAL, "one-pass large-tile products run batched only");
+    fn = &k_ntt_row_fa<false, true>;
   }
   if (pl.big && mode == kFwd && !src && ps.logS >= kBigLogE + 1 - ps.logL && (descs || nlimbs >= 1)) {
     // first pass from the limbs: two half-length DFTs per tile
@@ -1180,7 +1211,7 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, b
   const uint64_t N = g.N, M = g.M, per = g.per;
   const size_t group_max = g.group_max;
   const uint32_t nch = g.nch;
-  const Plan pl = make_plan(logN);
+  const Plan pl = make_batch_plan(logN);
   const size_t m = pl.passes.size();
   double* A = c.bspec;
   double* B = c.bspec + 2 * N;
