@@ -451,6 +451,92 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col(PassArgs a) {
 
 
 // ---------------------------------------------------------------------------
+// Column passes of 2 .. 16 points (batched products of 2^15 .. 2^18 points:
+// one of these and the 2^14-point row pass).  A line is short enough for
+// one thread: each of the 1024 threads of a 2^14-point tile runs the DFTs
+// of 16 / L adjacent-in-the-tile columns in registers (no LDS); lanes take
+// consecutive columns, so every access is 512 contiguous bytes per wave.
+// Forward: DFT, then the inter-pass twiddle omega_{L s}^{c k} of k_ntt_col
+// (from the limbs on the first pass); inverse: twiddle omega^{-c n}, then
+// the inverse DFT.
+template <int MODE, int LOGL>
+__global__ __launch_bounds__(kBigNTH) void k_ntt_col_reg(PassArgs a) {
+  static_assert(MODE == kFwd || MODE == kInv, "column passes run forward or inverse");
+  static_assert(LOGL >= 1 && LOGL <= 4, "lines of 2 .. 16 points");
+  constexpr int L = 1 << LOGL;
+  constexpr int CPT = kBigEPT / L;  // columns per thread
+  constexpr int LOGW = kBigLogE - LOGL;  // columns per tile
+  const int pi = blockIdx.y;
+  const Prime P = a.prm[pi];
+  const int logS = a.logS;
+  uint64_t zoff = 0;
+  const uint32_t* limbs = a.limbs;
+  uint64_t nlimbs = a.nlimbs;
+  if (a.descs) {
+    const uint64_t zi = (uint64_t)blockIdx.z << a.zshift;
+    const BatchDesc d = a.descs[zi];
+    zoff = zi * a.zstride;
+    limbs = a.operand ? d.b : d.a;
+    nlimbs = a.operand ? d.nb : d.na;
+  }
+  const double* src = a.src ? a.src + zoff + pi * a.N : nullptr;
+  double* dst = a.dst + zoff + pi * a.N;
+  const int logRoot = LOGL + logS;
+  const uint64_t ntiles = a.ntiles;
+  const uint64_t g = gridDim.x;
+  uint64_t w = blockIdx.x;
+  if ((g & 7) == 0) w = (uint64_t)(blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
+  for (; w < ntiles; w += g) {
+    const int lgc = logS - LOGW;
+    const uint64_t base = ((w >> lgc) << (LOGL + logS)) + ((w & ((1ull << lgc) - 1)) << LOGW);
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+      const uint64_t col = (uint64_t)threadIdx.x + ((uint64_t)j << 10);  // in the tile
+      const uint64_t c = (base & ((1ull << logS) - 1)) + col;             // in the stride block
+      double v[L];
+#pragma unroll
+      for (int n = 0; n < L; ++n) {
+        const uint64_t gi = base + ((uint64_t)n << logS) + col;
+        if (src) {
+          v[n] = src[gi];
+        } else {
+          const uint32_t x = limbs[gi < nlimbs ? gi : nlimbs - 1];
+          v[n] = gi < nlimbs ? (double)x : 0.0;
+        }
+      }
+      const double2 stp = root_pow(c, logRoot, MODE == kInv, P);  // omega^{+-c}
+      double2 cur = make_double2(1.0, 0.0);
+      if constexpr (MODE == kInv) {
+        // pre-twiddle omega^{-c n}
+#pragma unroll
+        for (int n = 1; n < L; ++n) {
+          const double wv = n == 1 ? stp.x : mm_shoup(cur.x, stp.x, stp.y, P.p);
+          cur = n == 1 ? stp : make_double2(wv, wv * P.pinv);
+          v[n] = mm_shoup(v[n], cur.x, cur.y, P.p);
+        }
+      }
+      big_dft<L>(v, P, MODE == kInv);
+#pragma unroll
+      for (int k = 0; k < L; ++k) {
+        double x = v[k];
+        if constexpr (MODE == kFwd) {
+          // post-twiddle omega^{c k}
+          if (k == 1) cur = stp;
+          if (k >= 2) {
+            const double wv = mm_shoup(cur.x, stp.x, stp.y, P.p);
+            cur = make_double2(wv, wv * P.pinv);
+          }
+          x = k == 0 ? reduce(x, P.p, P.pinv) : mm_shoup(x, cur.x, cur.y, P.p);
+        } else {
+          x = reduce(x, P.p, P.pinv);
+        }
+        dst[base + ((uint64_t)k << logS) + col] = x;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // First forward pass from an operand's limbs (column DFTs of L = 2^LOGL
 // points), as two half-length DFTs per tile.
 //

@@ -145,8 +145,11 @@ Plan make_big_plan(int logN) {
 // Plan of the products of a batched launch (many products, so a product's
 // own tile count matters less than for one product): one 2^14-point row
 // pass at 2^14 points (the fused pass transforms both operands from their
-// limbs), large tiles from 2^19.
-constexpr int kBatchBigMinLogN = 19;
+// limbs), large tiles from 2^17 (column passes of 8 and 16 points in
+// registers at 2^17 and 2^18).  Measured on pi to 1e9 digits with large
+// tiles from 2^15 / 2^16 / 2^17 / 2^18 / 2^19 points: 1.257 / 1.246 /
+// 1.239 / 1.239 / 1.245 s.
+constexpr int kBatchBigMinLogN = 17;
 Plan make_plan(int logN);
 Plan make_batch_plan(int logN) {
   if (logN == kBigLogE) {
@@ -266,6 +269,10 @@ PassFn row_pass_fn(int logL, int logLines) {
 template <int M>
 PassFn big_col_fn(int logL) {
   switch (logL) {
+    case 1: return &k_ntt_col_reg<M, 1>;
+    case 2: return &k_ntt_col_reg<M, 2>;
+    case 3: return &k_ntt_col_reg<M, 3>;
+    case 4: return &k_ntt_col_reg<M, 4>;
     case 5: return &k_ntt_col<M, 5>;
     case 6: return &k_ntt_col<M, 6>;
     case 7: return &k_ntt_col<M, 7>;
