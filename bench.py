@@ -473,6 +473,8 @@ def main():
                     help="replay captured HIP graphs of the subtree level schedules (default: kernel by kernel)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run one device copy of 512 MiB (counter calibration)")
+    ap.add_argument("--fft-only", action="store_true",
+                    help="only the FFT-engine multiply line (the operands of --logn), for profiling")
     ap.add_argument("--mul-only", action="store_true",
                     help="only the roofline multiply (for counter runs)")
     ap.add_argument("--sweep", action="store_true",
@@ -522,6 +524,9 @@ def main():
 
     picugpu.init(0)
     picugpu.bsr_graphs(args.graph)
+    if args.fft_only:
+        print(json.dumps({"fft_mul": fft_mul(args.logn, args.mul_steps)}), flush=True)
+        return
     roofline, mul_info = ntt_mul(args.logn, args.mul_steps, 3)
     rates = None
     if not args.no_probe:

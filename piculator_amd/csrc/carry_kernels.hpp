@@ -303,7 +303,13 @@ __global__ __launch_bounds__(kCarryThreads) void k_carry_local(CarryArgs a) {
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, d, 64));
-    if ((tid & 63) == 0) atomicMax(a.maxdist, (unsigned long long)__double_as_longlong(dmax));
+    // one contended address for all blocks: only a wave whose maximum
+    // exceeds the current one issues the atomic (a plain load first), so
+    // the atomics stay few instead of one per wave
+    if ((tid & 63) == 0) {
+      const unsigned long long v = (unsigned long long)__double_as_longlong(dmax);
+      if (v > __atomic_load_n(a.maxdist, __ATOMIC_RELAXED)) atomicMax(a.maxdist, v);
+    }
   } else {
     const Prime P0 = a.prm[0], P1 = a.prm[1];
     // coefficients 0 .. zL - 2 (the top limb only receives carries); from
