@@ -14,14 +14,13 @@
 // subproblem k1, so the exchanges before S3 and S4 stay inside the wave:
 // each wave rewrites only the 1024 LDS doubles it has just read, ordered by
 // the wave's own in-order LDS queue, with no workgroup barrier.  The
-// inverse (fused mode) runs the same stages mirrored: twiddle^-1 then DFT^-1
-// in each stage's layout, exchanges reversed.
+// inverse runs the same stages mirrored: twiddle^-1 then DFT^-1 in each
+// stage's layout, exchanges reversed.
 //
-// The spectrum is stored in the thread layout of S4: register s 4 + k4 of
-// thread (k1 = w, lane = 4 k2 + q) at position (s 4 + k4) 2^10 + t, which is
-// frequency k1 + 16 k2 + 256 (4 q + s) + 4096 k4 of the line.  Only this
-// pass and its fused inverse read or write spectra in that order; the
-// pointwise product does not depend on it.
+// The spectrum stays in the thread layout of S4: register s 4 + k4 of
+// thread (k1 = w, lane = 4 k2 + q) holds frequency k1 + 16 k2 +
+// 256 (4 q + s) + 4096 k4 of the line; the pointwise product does not
+// depend on the order, and the inverse starts from the same layout.
 //
 // LDS layouts (doubles; conflict-free for every 16- and 32-lane group of a
 // 64-bit access):
@@ -144,177 +143,15 @@ __device__ __forceinline__ void row_w1_get_inv(double (&v)[16], const double* R,
   for (int k2 = 0; k2 < 16; ++k2) v[k2] = rb[(k2 ^ n3) << 2];
 }
 
-template <int MODE>
-__global__ __launch_bounds__(kRowNTH) void k_ntt_row(PassArgs a) {
-  static_assert(MODE == kFwd || MODE == kFused, "row passes run forward or fused");
-  __shared__ double xs[kBigE];
-  __shared__ BigTw ta[1024];
-  __shared__ double tb[2 * kBigTB];
-  const int pi = blockIdx.y;
-  const Prime P = a.prm[pi];
-  for (int e = threadIdx.x; e < 1024; e += kRowNTH) ta[e] = P.twL[e << (kLogE - 10)].x;
-  for (int e = threadIdx.x; e < 2 * kBigTB; e += kRowNTH)
-    tb[e] = root_pow((uint64_t)(e & (kBigTB - 1)), kBigLogE, e >= kBigTB, P).x;
-  uint64_t zoff = 0, soff = 0;
-  if (a.descs) {
-    const uint64_t zi = (uint64_t)blockIdx.z << a.zshift;
-    zoff = zi * a.zstride;
-    soff = ((zi >> a.sshift) << a.sshift) * a.zstride;
-  }
-  const double* src = a.src + zoff + pi * a.N;
-  double* dst = a.dst + zoff + pi * a.N;
-  const double* spec = MODE == kFused ? a.spec + soff + pi * a.N : nullptr;
-  const uint64_t ntiles = a.ntiles;
-
-  auto load_tile = [&](uint64_t w, double (&x)[16]) {
-    const double* s = src + (w << kBigLogE) + big_tid();
-#pragma unroll
-    for (int r = 0; r < 16; ++r) x[r] = s[r << 10];
-  };
-
-  auto tile_step = [&](uint64_t w, double (&xr)[16], uint64_t wload) {
-    const uint64_t base = w << kBigLogE;
-    // fused: operand a's spectrum of this tile, loaded now, used after the
-    // forward stages
-    double sa[16];
-    if constexpr (MODE == kFused) {
-      const double* sp = spec + base + big_tid();
-#pragma unroll
-      for (int r = 0; r < 16; ++r) sa[r] = sp[r << 10];
-    }
-    double v[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = xr[r];
-    dft16(v, P, false);
-    row_s1_twiddle(v, big_tid(), ta, tb, P, false);
-    __syncthreads();  // every wave is done with the previous tile's LDS
-    {
-      const int t = big_tid();
-      double* xw = xs + t;  // [k1][w][lane]
-#pragma unroll
-      for (int k1 = 0; k1 < 16; ++k1) xw[k1 << 10] = v[k1];
-    }
-    __syncthreads();
-    if (wload < ntiles) load_tile(wload, xr);
-    {
-      const int t = big_tid();
-      const int lane = t & 63;
-      const double* xw = xs + ((t >> 6) << 10) + lane;  // [w][n2][lane]
-#pragma unroll
-      for (int n2 = 0; n2 < 16; ++n2) v[n2] = xw[n2 << 6];
-      dft16(v, P, false);
-      row_s2_twiddle(v, lane, ta, P, false);
-      double* R = xs + ((t >> 6) << 10);
-      wave_lds_sync();
-      row_w1_put(v, R, lane);
-      wave_lds_sync();
-      row_w1_get(v, R, lane);
-    }
-    dft16(v, P, false);
-    row_s3_twiddle(v, big_tid() & 3, ta, P, false);
-    {
-      const int t = big_tid();
-      double* R = xs + ((t >> 6) << 10);
-      wave_lds_sync();
-      row_w2_put(v, R, t & 63);
-      wave_lds_sync();
-      row_w2_get(v, R, t & 63);
-    }
-#pragma unroll
-    for (int s = 0; s < 4; ++s) dft_core<4>(v + 4 * s, P, false);
-    if constexpr (MODE == kFwd) {
-      double* d = dst + base + big_tid();
-#pragma unroll
-      for (int r = 0; r < 16; ++r) d[r << 10] = reduce(v[r], P.p, P.pinv);
-    } else {
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const double x = mm_gen(v[r], sa[r], P.p, P.pinv);
-        v[r] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
-      }
-#pragma unroll
-      for (int s = 0; s < 4; ++s) dft_core<4>(v + 4 * s, P, true);
-      {
-        const int t = big_tid();
-        double* R = xs + ((t >> 6) << 10);
-        wave_lds_sync();
-        row_w2_put_inv(v, R, t & 63);
-        wave_lds_sync();
-        row_w2_get_inv(v, R, t & 63);
-      }
-      row_s3_twiddle(v, big_tid() & 3, ta, P, true);
-      dft16(v, P, true);
-      {
-        const int t = big_tid();
-        const int lane = t & 63;
-        double* R = xs + ((t >> 6) << 10);
-        wave_lds_sync();
-        row_w1_put_inv(v, R, lane);
-        wave_lds_sync();
-        row_w1_get_inv(v, R, lane);
-        row_s2_twiddle(v, lane, ta, P, true);
-      }
-      dft16(v, P, true);
-      __syncthreads();  // every wave is done with its own rows of LDS
-      {
-        const int t = big_tid();
-        double* xw = xs + ((t >> 6) << 10) + (t & 63);  // [w][n2][lane]
-#pragma unroll
-        for (int n2 = 0; n2 < 16; ++n2) xw[n2 << 6] = v[n2];
-      }
-      __syncthreads();
-      {
-        const int t = big_tid();
-        const double* xw = xs + t;  // [k1][w][lane]
-#pragma unroll
-        for (int k1 = 0; k1 < 16; ++k1) v[k1] = xw[k1 << 10];
-      }
-      row_s1_twiddle(v, big_tid(), ta, tb, P, true);
-      dft16(v, P, true);
-      double* d = dst + base + big_tid();
-#pragma unroll
-      for (int r = 0; r < 16; ++r) d[r << 10] = reduce(v[r], P.p, P.pinv);
-    }
-  };
-
-  // tile order as in k_ntt_col (XCD by XCD)
-  const uint64_t g = gridDim.x;
-  uint64_t w = blockIdx.x;
-  if ((g & 7) == 0) w = (uint64_t)(blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
-  if (w >= ntiles) return;
-  double xa[16];
-  load_tile(w, xa);
-  if constexpr (MODE == kFwd) {
-    // two tiles of loads in flight: xa holds tile w, xb tile w + g
-    double xb[16];
-    if (w + g < ntiles) load_tile(w + g, xb);
-    __syncthreads();  // tables
-    for (;;) {
-      tile_step(w, xa, w + 2 * g);
-      w += g;
-      if (w >= ntiles) break;
-      tile_step(w, xb, w + 2 * g);
-      w += g;
-      if (w >= ntiles) break;
-    }
-  } else {
-    __syncthreads();  // tables
-    for (;;) {
-      tile_step(w, xa, w + g);
-      w += g;
-      if (w >= ntiles) break;
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
-// Fused row pass that also runs operand a's row DFT.
+// The fused row pass: the last forward pass of both operands, the pointwise
+// product and the first inverse pass.
 //
-// k_ntt_row<kFused> reads a's spectrum, which a separate k_ntt_row<kFwd>
-// pass wrote; here the pass reads a after its column pass instead and runs
-// a's row DFT in the tile (the spectrum stays in registers, in the layout
-// the pointwise product needs): one sweep of a's points fewer per product
-// (read 16 + write 16 bytes per point) for one more row DFT's arithmetic.
+// The pass reads a after its column pass and runs a's row DFT in the tile
+// (a's spectrum stays in registers, in the layout the pointwise product
+// needs), then b's: a's spectrum never goes through HBM, one sweep of a's
+// points fewer per product than a row pass of a of its own (read 16 +
+// write 16 bytes per point) for no more arithmetic.
 // SQUARE: a and b are one operand (src == spec): one row DFT, the pointwise
 // square.  LIMBS: a one-pass plan (N = 2^14, batched): both operands are
 // read from their limbs (descs), zero past their lengths.  Per tile, the

@@ -524,8 +524,6 @@ PassFn select_pass(int mode, const Plan& pl, const Pass& ps) {
   if (pl.big) {
     if (ps.logS == 0) {
       if (ps.logL != kBigLogE) return nullptr;
-      if (mode == kFwd) return &k_ntt_row<kFwd>;
-      if (mode == kFused) return &k_ntt_row<kFused>;
       if (mode == kFusedA) return &k_ntt_row_fa<false, false>;
       if (mode == kSquareA) return &k_ntt_row_fa<true, false>;
       return nullptr;
@@ -547,13 +545,14 @@ PassFn select_pass(int mode, const Plan& pl, const Pass& ps) {
   return nullptr;
 }
 
-// Every pass of the plan has a kernel instance for each mode it runs in.
+// Every pass of the plan has a kernel instance for each mode it runs in
+// (large-tile plans: the last pass only as the fused pass with a's row DFT).
 bool plan_supported(const Plan& pl) {
   const size_t m = pl.passes.size();
   for (size_t j = 0; j < m; ++j) {
-    if (!select_pass(kFwd, pl, pl.passes[j])) return false;
+    if ((j + 1 < m || !pl.big) && !select_pass(kFwd, pl, pl.passes[j])) return false;
     if (j + 1 < m && !select_pass(kInv, pl, pl.passes[j])) return false;
-    if (j + 1 == m && !select_pass(kFused, pl, pl.passes[j])) return false;
+    if (j + 1 == m && !select_pass(pl.big ? kFusedA : kFused, pl, pl.passes[j])) return false;
   }
   return true;
 }
