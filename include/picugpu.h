@@ -256,12 +256,15 @@ int pcg_df_release_pool(void);
  * class table[b]: 0 the schoolbook kernel (zL <= 1024), 1..3 the grouped
  * schoolbook (16 threads x 4 columns, zL <= 64; 64 x 4, zL <= 256; 64 x 8,
  * zL <= 512), 10..12 the single-workgroup NTT of 2^k points, 110..112 the
- * batched pass kernels at 2^(k-100) points.  The crossover is the autotuner's
- * (timed on the device at the first batched call, or by pcg_autotune,
- * which returns the table); pcg_batch_table reads it (set = 0) or installs
- * one (set = 1, every entry valid for its bucket).  Products are exact in
- * every class; the table only moves work between kernels.  The reference
- * picks its engine by size alone (BigFloat.cpp:386). */
+ * batched pass kernels at 2^(k-100) points.  Entries 13..20 (when n > 13)
+ * are the pass geometry of batched products at 2^b points: 0 the
+ * 4096-point passes, 1 the large-tile passes (2^14-point tiles; entry 13
+ * is always 0).  The crossovers are the autotuner's (timed on the device
+ * at the first batched call, or by pcg_autotune, which returns the table);
+ * pcg_batch_table reads it (set = 0) or installs one (set = 1, every entry
+ * valid for its bucket).  Products are exact in every class and geometry;
+ * the table only moves work between kernels.  The reference picks its
+ * engine by size alone (BigFloat.cpp:386). */
 int pcg_autotune(int* table, int n);
 int pcg_batch_table(int* table, int n, int set);
 

@@ -142,17 +142,20 @@ Plan make_big_plan(int logN) {
   return pl;
 }
 
-// Plan of the products of a batched launch (many products, so a product's
-// own tile count matters less than for one product): one 2^14-point row
-// pass at 2^14 points (the fused pass transforms both operands from their
-// limbs), large tiles from 2^17 (column passes of 8 and 16 points in
-// registers at 2^17 and 2^18).  Measured on pi to 1e9 digits with large
-// tiles from 2^15 / 2^16 / 2^17 / 2^18 / 2^19 points: 1.257 / 1.246 /
-// 1.239 / 1.239 / 1.245 s.
-constexpr int kBatchBigMinLogN = 17;
+// Plan of the products of a batched launch at 2^logN points: the large
+// tiles above 2^20 points (many products in flight, so a product's own
+// tile count matters less than for one product), below that the geometry
+// the autotuner picked (batch_geom): at 2^14 points one fused row pass
+// straight from both operands' limbs, from 2^15 a column pass (of 2 .. 16
+// points in registers up to 2^18) before the row pass.  Measured with a
+// fixed threshold for pi to 1e9 digits, large tiles from 2^15 / 2^16 /
+// 2^17 / 2^18 / 2^19 points: 1.257 / 1.246 / 1.239 / 1.239 / 1.245 s.
+constexpr int kGeomMinLog = 13, kGeomMaxLog = 20;  // autotuned geometries (batch_geom)
+int batch_geom(int logN);
 Plan make_plan(int logN);
 Plan make_batch_plan(int logN) {
-  if (logN == kBigLogE) {
+  const bool big = logN > kGeomMaxLog ? logN <= kBigMaxLogN : batch_geom(logN) == 1;
+  if (big && logN == kBigLogE) {
     Plan pl;
     pl.logN = logN;
     pl.logE = kBigLogE;
@@ -166,7 +169,7 @@ Plan make_batch_plan(int logN) {
     pl.passes.push_back(ps);
     return pl;
   }
-  if (logN >= kBatchBigMinLogN && logN <= kBigMaxLogN) return make_big_plan(logN);
+  if (big) return make_big_plan(logN);
   return make_plan(logN);
 }
 
@@ -1071,6 +1074,19 @@ int g_small_class[kSmallNttMaxLog + 1];
 bool g_small_ready = false;
 uint64_t g_table_gen = 1;  // bumped whenever g_small_class changes
 
+// Pass geometry of batched multi-pass products at 2^b points, b in
+// [kGeomMinLog, kGeomMaxLog] (the class table's entries 13 .. 20): 0 the
+// 4096-point passes (512 threads x 8 points), 1 the large-tile passes
+// (2^14-point tiles, 1024 threads x 16 points; at 2^14 points one fused
+// pass from the limbs).  The autotuner times both per size; below, only
+// the 4096-point passes exist, above only the large tiles.
+int batch_geom_default(int b) { return b == 14 || b >= 17 ? 1 : 0; }
+bool batch_geom_valid(int b, int g) { return g == 0 || (g == 1 && b > kGeomMinLog); }
+int g_batch_geom[kGeomMaxLog + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 1, 1, 1};
+int batch_geom(int logN) {
+  return logN >= kGeomMinLog && logN <= kGeomMaxLog ? g_batch_geom[logN] : (logN > kGeomMaxLog ? 1 : 0);
+}
+
 // Largest zL of a schoolbook class.
 int school_max(int cls) { return cls == 1 ? 64 : cls == 2 ? 256 : cls == 3 ? 512 : kSchoolMaxLimbs; }
 
@@ -1333,6 +1349,74 @@ __global__ void k_fill_pattern(uint32_t* p, uint64_t n) {
   if (i < n) p[i] = (uint32_t)(i * 2654435761u) ^ 0x9e3779b9u;
 }
 
+// Autotuner of the batched pass geometry (batch_geom): for each size
+// 2^b, b = 14 .. 20, it times one batched launch of products of
+// na = nb = 2^(b-1) - 1 limbs (2^23 operand limbs in all, at least 8
+// products) on the 4096-point and on the large-tile passes (best of three
+// launches after a first one, HIP events on the library stream) and keeps
+// the faster.  Products are exact on both; only the time differs.
+int tune_geom(Ctx& c) {
+  for (int b = 0; b <= kGeomMaxLog; ++b) g_batch_geom[b] = batch_geom_default(b);
+  constexpr size_t kLimbs = (size_t)1 << 23;
+  constexpr size_t kMaxCount = kLimbs >> (kGeomMinLog + 1);
+  uint32_t* in = nullptr;
+  uint32_t* out = nullptr;
+  BatchDesc* dd = nullptr;
+  if (hipMalloc(&in, kLimbs * 4) != hipSuccess || hipMalloc(&out, kLimbs * 4) != hipSuccess ||
+      hipMalloc(&dd, kMaxCount * sizeof(BatchDesc)) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(in);
+    (void)hipFree(out);
+    return fail(PCG_ERR_OOM, "cannot allocate the geometry autotuner's test batch");
+  }
+  hipLaunchKernelGGL(k_fill_pattern, dim3((unsigned)((kLimbs + 255) / 256)), dim3(256), 0, c.stream, in,
+                     (uint64_t)kLimbs);
+  std::vector<BatchDesc> h(kMaxCount);
+  int rc = PCG_OK;
+  for (int b = kGeomMinLog + 1; b <= kGeomMaxLog && rc == PCG_OK; ++b) {
+    const uint32_t n = (1u << (b - 1)) - 1;
+    const size_t count = std::min(kMaxCount, kLimbs / (2 * (size_t)n));
+    for (size_t i = 0; i < count; ++i) {
+      h[i].a = in + i * 2 * n;
+      h[i].b = h[i].a + n;
+      h[i].out = out + i * 2 * n;
+      h[i].na = n;
+      h[i].nb = n;
+    }
+    if (hipMemcpyAsync(dd, h.data(), count * sizeof(BatchDesc), hipMemcpyHostToDevice, c.stream) !=
+        hipSuccess) {
+      rc = fail(PCG_ERR_HIP, "geometry autotuner upload failed");
+      break;
+    }
+    float best_ms = 1e30f;
+    int best = batch_geom_default(b);
+    for (int g = 0; g <= 1 && rc == PCG_OK; ++g) {
+      g_batch_geom[b] = g;
+      float t = 1e30f;
+      for (int rep = 0; rep < 4 && rc == PCG_OK; ++rep) {
+        if (hipEventRecord(c.tev[0], c.stream) != hipSuccess) rc = fail(PCG_ERR_HIP, "event record failed");
+        if (!rc) rc = batched_multipass(c, dd, count, b);
+        float ms = 0;
+        if (!rc && (hipEventRecord(c.tev[1], c.stream) != hipSuccess ||
+                    hipEventSynchronize(c.tev[1]) != hipSuccess ||
+                    hipEventElapsedTime(&ms, c.tev[0], c.tev[1]) != hipSuccess))
+          rc = fail(PCG_ERR_HIP, "geometry autotuner timing failed");
+        if (!rc && rep > 0) t = std::min(t, ms);
+      }
+      if (!rc && t < best_ms) {
+        best_ms = t;
+        best = g;
+      }
+    }
+    g_batch_geom[b] = best;
+  }
+  (void)hipStreamSynchronize(c.stream);
+  (void)hipFree(in);
+  (void)hipFree(out);
+  (void)hipFree(dd);
+  return rc;
+}
+
 // Autotuner of the small-product classes: for each bucket b = 4..12 it
 // times a launch of 2048 to 16384 products (about the bucket's share of one
 // level of a large subtree) of na = nb = 3 * 2^(b-3) limbs (zL = 3/4 of the
@@ -1411,6 +1495,8 @@ int tune_small(Ctx& c) {
   (void)hipFree(in);
   (void)hipFree(out);
   (void)hipFree(dd);
+  if (rc) return rc;
+  rc = tune_geom(c);
   if (rc) return rc;
   g_small_ready = true;
   ++g_table_gen;
@@ -2858,6 +2944,7 @@ int init_locked(Ctx& c, int device) {
   for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&c.ev[i]));
   for (int i = 0; i < 2; ++i) HIPCHK(hipEventCreate(&c.tev[i]));
   for (int b = 0; b <= kSmallNttMaxLog; ++b) g_small_class[b] = small_class_default(b);
+  for (int b = 0; b <= kGeomMaxLog; ++b) g_batch_geom[b] = batch_geom_default(b);
   g_small_ready = false;
   // this device's copy of the allocation-free product's scratch array
   HIPCHK(hipGetSymbolAddress(reinterpret_cast<void**>(&c.noalloc), HIP_SYMBOL(scratch)));
@@ -3363,7 +3450,8 @@ int pcg_autotune(int* table, int n) {
   for (BsrPlan& q : g_bsr_plans) bsr_plan_free(q);  // plans record the classes
   int rc = tune_small(c);
   if (rc) return rc;
-  for (int b = 0; b < n && b <= kSmallNttMaxLog; ++b) table[b] = g_small_class[b];
+  for (int b = 0; b < n && b <= kGeomMaxLog; ++b)
+    table[b] = b <= kSmallNttMaxLog ? g_small_class[b] : g_batch_geom[b];
   return PCG_OK;
 }
 
@@ -3371,17 +3459,23 @@ int pcg_batch_table(int* table, int n, int set) {
   if (!table && n > 0) return fail(PCG_ERR_INVALID, "null pointer");
   DF_ENTER();
   if (set) {
-    for (int b = 0; b < n && b <= kSmallNttMaxLog; ++b)
-      if (!small_class_valid(b, table[b]))
+    for (int b = 0; b < n && b <= kGeomMaxLog; ++b)
+      if (b <= kSmallNttMaxLog ? !small_class_valid(b, table[b]) : !batch_geom_valid(b, table[b]))
         return fail(PCG_ERR_INVALID, "class %d is not valid for bucket %d", table[b], b);
     HIPCHK(hipStreamSynchronize(c.stream));
     for (BsrPlan& q : g_bsr_plans) bsr_plan_free(q);
-    for (int b = 0; b < n && b <= kSmallNttMaxLog; ++b) g_small_class[b] = table[b];
+    for (int b = 0; b < n && b <= kGeomMaxLog; ++b) {
+      if (b <= kSmallNttMaxLog)
+        g_small_class[b] = table[b];
+      else
+        g_batch_geom[b] = table[b];
+    }
     g_small_ready = true;
     ++g_table_gen;
     return PCG_OK;
   }
-  for (int b = 0; b < n && b <= kSmallNttMaxLog; ++b) table[b] = g_small_class[b];
+  for (int b = 0; b < n && b <= kGeomMaxLog; ++b)
+    table[b] = b <= kSmallNttMaxLog ? g_small_class[b] : g_batch_geom[b];
   return PCG_OK;
 }
 

@@ -397,20 +397,22 @@ def df_bsr_exact(a: int, b: int, max_limbs: int):
             load().pcg_df_free(C.byref(v))
 
 
-SMALL_BUCKETS = 13  # b = 0..12
+SMALL_BUCKETS = 13  # b = 0..12: small-product classes
+TABLE_ENTRIES = 21  # and b = 13..20: pass geometry of batched products at 2^b points
 
 
 def autotune():
-    """Re-runs the small-product autotuner; returns the class per bucket."""
+    """Re-runs the autotuner; returns the class per bucket (entries 0..12)
+    and the pass geometry per batched size (entries 13..20)."""
     ensure_init()
-    t = (C.c_int * SMALL_BUCKETS)()
-    _check(load().pcg_autotune(t, SMALL_BUCKETS))
+    t = (C.c_int * TABLE_ENTRIES)()
+    _check(load().pcg_autotune(t, TABLE_ENTRIES))
     return list(t)
 
 
 def batch_table(table=None):
-    """Reads the small-product class table, or installs `table`."""
+    """Reads the class / geometry table, or installs `table`."""
     ensure_init()
-    t = (C.c_int * SMALL_BUCKETS)(*(table or [0] * SMALL_BUCKETS))
-    _check(load().pcg_batch_table(t, SMALL_BUCKETS, 1 if table is not None else 0))
+    t = (C.c_int * TABLE_ENTRIES)(*(table or [0] * TABLE_ENTRIES))
+    _check(load().pcg_batch_table(t, TABLE_ENTRIES, 1 if table is not None else 0))
     return list(t)

@@ -96,12 +96,51 @@ def class_valid(b, c):
 
 def test_autotune_table_is_valid(gpu):
     t = gpu.autotune()
-    assert len(t) == 13
-    for b, c in enumerate(t):
+    assert len(t) == 21
+    for b, c in enumerate(t[:13]):
         assert class_valid(b, c), (b, c)
         if (1 << b) > 1024:
             assert c in (b, 100 + b)
+    # pass geometry per batched size: 0 the 4096-point passes, 1 large tiles
+    assert t[13] == 0 and all(g in (0, 1) for g in t[14:])
     assert gpu.batch_table() == t
+
+
+@pytest.mark.parametrize("geom", [0, 1])
+def test_forced_pass_geometry_exact(gpu, geom):
+    """Batched multi-pass products of 2^14 .. 2^17 points on each pass
+    geometry (4096-point passes; large tiles: one fused pass from the limbs
+    at 2^14 points, register column passes at 2^15 .. 2^17) are exact,
+    unbalanced and all-ones operands included."""
+    saved = gpu.batch_table()
+    gpu.batch_table(saved[:14] + [geom] * 7)
+    try:
+        rng = random.Random(77 + geom)
+        sizes = [(4000, 4100), (8191, 8190), (5000, 11000), (16000, 15000), (30000, 33000),
+                 (65000, 65000), (100, 60000), (12000, 4500)]
+        pairs = [(rand_limbs(rng, na, k % 3 == 1), rand_limbs(rng, nb, k % 3 == 1))
+                 for k, (na, nb) in enumerate(sizes)]
+        pairs = pairs * 2
+        outs = gpu.mul_batch(pairs)
+        for (a, b), o in zip(pairs, outs):
+            assert np.array_equal(o, exact_product(a, b)), (geom, a.size, b.size)
+    finally:
+        gpu.batch_table(saved)
+
+
+def test_subtree_on_each_pass_geometry(gpu):
+    """A subtree of 2^17 leaves near 7e7 (batched products up to 2^20 points)
+    gives the same P, Q, R on both pass geometries for every size."""
+    saved = gpu.batch_table()
+    a, b = 70_000_000, 70_000_000 + (1 << 17)
+    got = []
+    try:
+        for geom in (0, 1):
+            gpu.batch_table(saved[:14] + [geom] * 7)
+            got.append(gpu.bsr_exact(a, b, 1 << 28))
+    finally:
+        gpu.batch_table(saved)
+    assert got[0] == got[1]
 
 
 @pytest.mark.parametrize("forced", [0, 1, 2, 3, 10, 110])
