@@ -258,8 +258,9 @@ int pcg_df_release_pool(void);
  * zL <= 512), 10..12 the single-workgroup NTT of 2^k points, 110..112 the
  * batched pass kernels at 2^(k-100) points.  Entries 13..20 (when n > 13)
  * are the pass geometry of batched products at 2^b points: 0 the
- * 4096-point passes, 1 the large-tile passes (2^14-point tiles; entry 13
- * is always 0).  The crossovers are the autotuner's (timed on the device
+ * 4096-point passes, 1 the large-tile passes (2^14-point tiles; at 2^13
+ * points the one-pass 2^14-point product).  The crossovers are the
+ * autotuner's (timed on the device
  * at the first batched call, or by pcg_autotune, which returns the table);
  * pcg_batch_table reads it (set = 0) or installs one (set = 1, every entry
  * valid for its bucket).  Products are exact in every class and geometry;

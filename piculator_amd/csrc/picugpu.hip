@@ -151,24 +151,26 @@ Plan make_big_plan(int logN) {
 // fixed threshold for pi to 1e9 digits, large tiles from 2^15 / 2^16 /
 // 2^17 / 2^18 / 2^19 points: 1.257 / 1.246 / 1.239 / 1.239 / 1.245 s.
 constexpr int kGeomMinLog = 13, kGeomMaxLog = 20;  // autotuned geometries (batch_geom)
-int batch_geom(int logN);
+int batch_pass_geom(int logN);
 Plan make_plan(int logN);
+// One fused row pass of 2^14 points from both operands' limbs.
+Plan make_onepass_plan() {
+  Plan pl;
+  pl.logN = kBigLogE;
+  pl.logE = kBigLogE;
+  pl.big = true;
+  Pass ps;
+  ps.logL = kBigLogE;
+  ps.logS = 0;
+  ps.logLines = 0;
+  ps.logG = 0;
+  ps.logB = 0;
+  pl.passes.push_back(ps);
+  return pl;
+}
 Plan make_batch_plan(int logN) {
-  const bool big = logN > kGeomMaxLog ? logN <= kBigMaxLogN : batch_geom(logN) == 1;
-  if (big && logN == kBigLogE) {
-    Plan pl;
-    pl.logN = logN;
-    pl.logE = kBigLogE;
-    pl.big = true;
-    Pass ps;
-    ps.logL = kBigLogE;
-    ps.logS = 0;
-    ps.logLines = 0;
-    ps.logG = 0;
-    ps.logB = 0;
-    pl.passes.push_back(ps);
-    return pl;
-  }
+  const bool big = logN > kGeomMaxLog ? logN <= kBigMaxLogN : batch_pass_geom(logN) == 1;
+  if (big && logN == kBigLogE) return make_onepass_plan();
   if (big) return make_big_plan(logN);
   return make_plan(logN);
 }
@@ -1078,14 +1080,19 @@ uint64_t g_table_gen = 1;  // bumped whenever g_small_class changes
 // [kGeomMinLog, kGeomMaxLog] (the class table's entries 13 .. 20): 0 the
 // 4096-point passes (512 threads x 8 points), 1 the large-tile passes
 // (2^14-point tiles, 1024 threads x 16 points; at 2^14 points one fused
-// pass from the limbs).  The autotuner times both per size; below, only
-// the 4096-point passes exist, above only the large tiles.
+// pass from the limbs; at 2^13 points the same one-pass product at 2^14
+// points, twice the points in one sweep instead of five).  The autotuner
+// times both per size; below, only the 4096-point passes exist, above
+// only the large tiles.
 int batch_geom_default(int b) { return b == 14 || b >= 17 ? 1 : 0; }
-bool batch_geom_valid(int b, int g) { return g == 0 || (g == 1 && b > kGeomMinLog); }
+bool batch_geom_valid(int b, int g) { return g == 0 || g == 1; }
 int g_batch_geom[kGeomMaxLog + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 1, 1, 1};
-int batch_geom(int logN) {
+int batch_pass_geom(int logN) {
   return logN >= kGeomMinLog && logN <= kGeomMaxLog ? g_batch_geom[logN] : (logN > kGeomMaxLog ? 1 : 0);
 }
+// A batched launch of class 2^13 on geometry 1 runs at 2^14 points (its
+// wrapped products need no wrap there: zL - 1 <= 5/4 2^13).
+bool batch_onepass13(int logN) { return logN == kGeomMinLog && g_batch_geom[kGeomMinLog] == 1; }
 
 // Largest zL of a schoolbook class.
 int school_max(int cls) { return cls == 1 ? 64 : cls == 2 ? 256 : cls == 3 ? 512 : kSchoolMaxLimbs; }
@@ -1183,6 +1190,10 @@ BatchGeom batch_geom(int logN, bool pairs, int logM) {
 // Grows the batched scratch to what such a launch needs (synchronising the
 // stream first); nothing is allocated when it already suffices.
 int batched_reserve(Ctx& c, size_t count, int logN, bool pairs, int logM) {
+  if (batch_onepass13(logN)) {
+    logN = kBigLogE;
+    logM = 0;
+  }
   const BatchGeom g = batch_geom(logN, pairs, logM);
   const size_t want = std::min(count, g.group_max) * g.per;
   const size_t carry_need = std::min(count, g.group_max) * (size_t)g.nch;
@@ -1229,11 +1240,16 @@ int batched_multipass(Ctx& c, const BatchDesc* d_desc, size_t count, int logN, b
   if (pairs && (count & 1)) return fail(PCG_ERR_INTERNAL, "odd count of paired products");
   const int rr = batched_reserve(c, count, logN, pairs, logM);
   if (rr) return rr;
+  const bool onepass = batch_onepass13(logN);
+  if (onepass) {
+    logN = kBigLogE;
+    logM = 0;
+  }
   const BatchGeom g = batch_geom(logN, pairs, logM);
   const uint64_t N = g.N, M = g.M, per = g.per;
   const size_t group_max = g.group_max;
   const uint32_t nch = g.nch;
-  const Plan pl = make_batch_plan(logN);
+  const Plan pl = onepass ? make_onepass_plan() : make_batch_plan(logN);
   const size_t m = pl.passes.size();
   double* A = c.bspec;
   double* B = c.bspec + 2 * N;
@@ -1349,8 +1365,8 @@ __global__ void k_fill_pattern(uint32_t* p, uint64_t n) {
   if (i < n) p[i] = (uint32_t)(i * 2654435761u) ^ 0x9e3779b9u;
 }
 
-// Autotuner of the batched pass geometry (batch_geom): for each size
-// 2^b, b = 14 .. 20, it times one batched launch of products of
+// Autotuner of the batched pass geometry (batch_pass_geom): for each size
+// 2^b, b = 13 .. 20, it times one batched launch of products of
 // na = nb = 2^(b-1) - 1 limbs (2^23 operand limbs in all, at least 8
 // products) on the 4096-point and on the large-tile passes (best of three
 // launches after a first one, HIP events on the library stream) and keeps
@@ -1373,7 +1389,7 @@ int tune_geom(Ctx& c) {
                      (uint64_t)kLimbs);
   std::vector<BatchDesc> h(kMaxCount);
   int rc = PCG_OK;
-  for (int b = kGeomMinLog + 1; b <= kGeomMaxLog && rc == PCG_OK; ++b) {
+  for (int b = kGeomMinLog; b <= kGeomMaxLog && rc == PCG_OK; ++b) {
     const uint32_t n = (1u << (b - 1)) - 1;
     const size_t count = std::min(kMaxCount, kLimbs / (2 * (size_t)n));
     for (size_t i = 0; i < count; ++i) {

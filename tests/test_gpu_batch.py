@@ -102,18 +102,19 @@ def test_autotune_table_is_valid(gpu):
         if (1 << b) > 1024:
             assert c in (b, 100 + b)
     # pass geometry per batched size: 0 the 4096-point passes, 1 large tiles
-    assert t[13] == 0 and all(g in (0, 1) for g in t[14:])
+    assert all(g in (0, 1) for g in t[13:])
     assert gpu.batch_table() == t
 
 
 @pytest.mark.parametrize("geom", [0, 1])
 def test_forced_pass_geometry_exact(gpu, geom):
-    """Batched multi-pass products of 2^14 .. 2^17 points on each pass
+    """Batched multi-pass products of 2^13 .. 2^17 points on each pass
     geometry (4096-point passes; large tiles: one fused pass from the limbs
-    at 2^14 points, register column passes at 2^15 .. 2^17) are exact,
-    unbalanced and all-ones operands included."""
+    at 2^14 points -- and at 2^13, where it runs at 2^14 points --, register
+    column passes at 2^15 .. 2^17) are exact, unbalanced and all-ones
+    operands included."""
     saved = gpu.batch_table()
-    gpu.batch_table(saved[:14] + [geom] * 7)
+    gpu.batch_table(saved[:13] + [geom] * 8)
     try:
         rng = random.Random(77 + geom)
         sizes = [(4000, 4100), (8191, 8190), (5000, 11000), (16000, 15000), (30000, 33000),
@@ -136,7 +137,7 @@ def test_subtree_on_each_pass_geometry(gpu):
     got = []
     try:
         for geom in (0, 1):
-            gpu.batch_table(saved[:14] + [geom] * 7)
+            gpu.batch_table(saved[:13] + [geom] * 8)
             got.append(gpu.bsr_exact(a, b, 1 << 28))
     finally:
         gpu.batch_table(saved)
