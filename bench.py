@@ -256,6 +256,54 @@ def fft_mul(logn: int, steps: int):
     return res
 
 
+def class_sweep(count: int = 2048, reps: int = 3):
+    """One JSON line per small-product bucket b = 6..10: the time of one
+    pcg_mul_batch_dev call of `count` device-resident products of
+    na = nb = 3 * 2^(b-3) limbs in every class valid for the bucket (forced
+    through the class table: schoolbook 0..3, Karatsuba 4, single-workgroup
+    NTT 10, batched pass kernels 110), best of `reps` after one."""
+    import numpy as np
+
+    from piculator_amd import picugpu
+
+    picugpu.init(0)
+    saved = picugpu.batch_table()
+    lib = picugpu.load()
+    school_max = {0: 1024, 1: 64, 2: 256, 3: 512, 4: 1024}
+    try:
+        for b in range(6, 11):
+            n = 3 << (b - 3)
+            total = count * 2 * n
+            pool = picugpu.DeviceBuffer(8 * total + 64)
+            host = np.zeros(2 * total, dtype=np.uint32)
+            host[:total] = np.random.default_rng(b).integers(0, 2**32, total, dtype=np.uint64).astype(np.uint32)
+            pool.upload(host)
+            descs = (picugpu.MulDesc * count)()
+            for i in range(count):
+                descs[i] = picugpu.MulDesc(pool.ptr + 4 * (2 * n * i), n, pool.ptr + 4 * (2 * n * i + n), n,
+                                           pool.ptr + 4 * (total + 2 * n * i))
+            res = {}
+            for cls in (0, 1, 2, 3, 4, 10, 110):
+                if cls in school_max and (1 << b) > school_max[cls]:
+                    continue
+                t = list(saved)
+                t[b] = cls
+                picugpu.batch_table(t)
+                picugpu._check(lib.pcg_mul_batch_dev(descs, count, 0))
+                best = None
+                for _ in range(reps):
+                    picugpu.timer_start()
+                    picugpu._check(lib.pcg_mul_batch_dev(descs, count, 0))
+                    ms = picugpu.timer_stop()
+                    best = ms if best is None else min(best, ms)
+                res[str(cls)] = best
+            print(json.dumps({"bucket": b, "limbs": n, "products": count, "ms": res,
+                              "tuned_class": saved[b]}), flush=True)
+            pool.free()
+    finally:
+        picugpu.batch_table(saved)
+
+
 def batch_compare(count: int = 4096, reps: int = 3):
     """Uneven batch (lengths log-uniform in 2^4..2^14 limbs, seed 0xC0DE),
     operands resident in HBM: one pcg_mul_batch_dev call vs the plain form
@@ -479,6 +527,8 @@ def main():
                     help="only the roofline multiply (for counter runs)")
     ap.add_argument("--sweep", action="store_true",
                     help="print the single-multiply table (sizes x shapes x engines) instead")
+    ap.add_argument("--class-sweep", action="store_true",
+                    help="print the time of every small-product class per bucket instead")
     ap.add_argument("--stream-probe", action="store_true",
                     help="print the HBM access-pattern probe table instead")
     ap.add_argument("--summarize-trace", metavar="DIR",
@@ -517,6 +567,9 @@ def main():
         return
     if args.stream_probe:
         stream_probe()
+        return
+    if args.class_sweep:
+        class_sweep()
         return
 
     from piculator_amd import picugpu, pihost

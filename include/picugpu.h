@@ -255,7 +255,8 @@ int pcg_df_release_pool(void);
  * product of zL = na + nb limbs with 2^(b-1) < zL <= 2^b, b <= 12, runs in
  * class table[b]: 0 the schoolbook kernel (zL <= 1024), 1..3 the grouped
  * schoolbook (16 threads x 4 columns, zL <= 64; 64 x 4, zL <= 256; 64 x 8,
- * zL <= 512), 10..12 the single-workgroup NTT of 2^k points, 110..112 the
+ * zL <= 512), 4 one Karatsuba level over the schoolbook (zL <= 1024),
+ * 10..12 the single-workgroup NTT of 2^k points, 110..112 the
  * batched pass kernels at 2^(k-100) points.  Entries 13..20 (when n > 13)
  * are the pass geometry of batched products at 2^b points: 0 the
  * 4096-point passes, 1 the large-tile passes (2^14-point tiles; at 2^13

@@ -106,6 +106,129 @@ __global__ __launch_bounds__(256) void k_batch_school(const BatchDesc* descs) {
   }
 }
 
+// One Karatsuba level over the schoolbook column sums (the crossover class
+// of the schoolbook and the NTT), zL = na + nb <= 1024, one product per
+// workgroup of 256 threads.  With h = ceil(max(na, nb) / 2), a = a0 + a1 B^h
+// and b = b0 + b1 B^h (B = 2^32; a1 or b1 may be empty):
+//   a b = z0 + (z1 - z0 - z2) B^h + z2 B^{2h},
+//   z0 = a0 b0,  z2 = a1 b1,  z1 = (a0 + a1)(b0 + b1),
+// three products of about h x h limbs, 3/4 of the schoolbook's
+// multiply-adds.  The digit sums a0 + a1, b0 + b1 are resolved to 32-bit
+// limbs first (one carry scan each), so the three products run on the
+// schoolbook's 32 x 32-bit column sums; the columns combine column by
+// column in 128-bit arithmetic.  A combined column may be negative, so
+// every column k < zL carries the offset K B - K (K B in column 0), which
+// adds K B^zL to the total: above the product's limbs, so the carry
+// resolution of k_batch_school gives them unchanged.
+constexpr int kKaraMaxH = kSchoolMaxLimbs / 2;  // h <= 512
+constexpr uint64_t kKaraK = 1ull << 43;         // |negative part| < 2 h 2^64 <= 2^74 < K B - K
+
+__global__ __launch_bounds__(256) void k_batch_kara(const BatchDesc* descs) {
+  // a-side operands with one limb of slack (school_columns reads sa[ihi + 1]);
+  // b-side operands after four zeros and zero past their end
+  __shared__ uint32_t sa0[kKaraMaxH + 2], sa1[kKaraMaxH + 2], ssa[kKaraMaxH + 2];
+  __shared__ uint32_t sb0p[kKaraMaxH + 12], sb1p[kKaraMaxH + 12], ssbp[kKaraMaxH + 12];
+  __shared__ uint64_t clo[3][kSchoolMaxLimbs];  // column sums of z0, z2, z1
+  __shared__ uint32_t chi[3][kSchoolMaxLimbs];
+  __shared__ uint32_t sw1[kSchoolMaxLimbs], sw2[kSchoolMaxLimbs];
+  __shared__ CarryFn wtot[4];
+  const BatchDesc d = descs[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int na = (int)d.na, nb = (int)d.nb, zL = na + nb;
+  const int h = ((na > nb ? na : nb) + 1) >> 1;
+  const int la0 = na < h ? na : h, la1 = na - la0;
+  const int lb0 = nb < h ? nb : h, lb1 = nb - lb0;
+  for (int i = tid; i < kKaraMaxH + 2; i += 256) {
+    sa0[i] = i < la0 ? d.a[i] : 0u;
+    sa1[i] = i < la1 ? d.a[h + i] : 0u;
+    ssa[i] = 0u;
+  }
+  for (int i = tid; i < kKaraMaxH + 12; i += 256) {
+    const int j = i - 4;
+    sb0p[i] = (j >= 0 && j < lb0) ? d.b[j] : 0u;
+    sb1p[i] = (j >= 0 && j < lb1) ? d.b[h + j] : 0u;
+    ssbp[i] = 0u;
+  }
+  __syncthreads();
+  // digit sums (h limbs and a carry): thread t adds limbs 2t, 2t + 1; the
+  // carries are one scan of carry functions over the workgroup
+  auto digit_sum = [&](const uint32_t* x, int xo, const uint32_t* y, int yo, uint32_t* out, int oo) {
+    uint64_t sv[2];
+    CarryFn f;
+    f.base = 0;
+    f.T = 0;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int i = 2 * tid + q;
+      sv[q] = i < h ? (uint64_t)x[xo + i] + y[yo + i] : 0;
+      CarryFn g;
+      g.base = sv[q] >> 32;
+      g.T = (1ull << 32) - (sv[q] & 0xffffffffull);
+      f = cf_compose(f, g);
+    }
+    const CarryFn pre = cf_block_exclusive(f, wtot);
+    uint64_t c = cf_eval0(pre);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int i = 2 * tid + q;
+      const uint64_t t = (sv[q] & 0xffffffffull) + c;
+      if (i < h) out[oo + i] = (uint32_t)t;
+      c = (sv[q] >> 32) + (t >> 32);
+      if (i == h - 1) out[oo + h] = (uint32_t)c;  // the carry out of the top limb
+    }
+  };
+  digit_sum(sa0, 0, sa1, 0, ssa, 0);
+  digit_sum(sb0p, 4, sb1p, 4, ssbp, 4);
+  __syncthreads();
+  const int lsa = la1 ? h + 1 : la0, lsb = lb1 ? h + 1 : lb0;
+  // column sums of the three products, columns 4t .. 4t + 3 each
+  const int k0 = 4 * tid;
+#pragma unroll 1
+  for (int p = 0; p < 3; ++p) {
+    const uint32_t* x = p == 0 ? sa0 : (p == 1 ? sa1 : ssa);
+    const uint32_t* yp = p == 0 ? sb0p : (p == 1 ? sb1p : ssbp);
+    const int lx = p == 0 ? la0 : (p == 1 ? la1 : lsa);
+    const int ly = p == 0 ? lb0 : (p == 1 ? lb1 : lsb);
+    uint64_t lo[4] = {0, 0, 0, 0};
+    uint32_t hi[4] = {0, 0, 0, 0};
+    int ilo = k0 - ly + 1;
+    if (ilo < 0) ilo = 0;
+    const int ihi = k0 + 3 < lx - 1 ? k0 + 3 : lx - 1;
+    if (lx > 0 && ly > 0 && k0 < lx + ly - 1 && ilo <= ihi) school_columns<4>(x, yp, k0, ilo, ihi, lo, hi);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      clo[p][k0 + q] = lo[q];
+      chi[p][k0 + q] = hi[q];
+    }
+  }
+  __syncthreads();
+  auto col = [&](int p, int k) -> unsigned __int128 {
+    return (k >= 0 && k < kSchoolMaxLimbs) ? ((unsigned __int128)chi[p][k] << 64) | clo[p][k] : 0;
+  };
+  uint32_t w0[4], w1[4], w2[4], D[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int k = k0 + q;
+    unsigned __int128 acc = 0;
+    if (k < zL) {
+      acc = ((unsigned __int128)kKaraK << 32) - (k ? kKaraK : 0);
+      acc += col(0, k) + col(1, k - 2 * h);
+      acc += col(2, k - h);
+      acc -= col(0, k - h) + col(1, k - h);
+    }
+    w0[q] = (uint32_t)acc;
+    w1[q] = (uint32_t)(acc >> 32);
+    w2[q] = (uint32_t)(acc >> 64);
+  }
+  uint64_t c, spill;
+  block_carry<4>(w0, w1, w2, sw1, sw2, wtot, D, c, spill);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int k = k0 + q;
+    if (k < zL) d.out[k] = D[q];
+  }
+}
+
 // Schoolbook products of zL <= C G limbs with G threads per product (G = 16
 // or 64, a part of one wave), C adjacent columns per thread (4 or 8) and
 // 256 / G products per workgroup, for the many small products of the lower

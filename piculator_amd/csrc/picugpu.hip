@@ -1065,9 +1065,10 @@ int ensure_pool(Ctx& c, size_t limbs, size_t ndesc) {
 // (2^(b-1), 2^b], b <= 12) go to the class g_small_class[b]: 0 the
 // 256-thread schoolbook (zL <= 1024), 1, 2 and 3 the grouped schoolbook with
 // 16 threads x 4 columns, 64 x 4 and 64 x 8 per product (zL <= 64, <= 256,
-// <= 512), 10..12 the single-workgroup
-// NTT of 2^k points, 110..112 the batched pass kernels at 2^(k - 100)
-// points (spectra through HBM); the autotuner (tune_small) picks among them
+// <= 512), 4 one Karatsuba level over the schoolbook (zL <= 1024),
+// 10..12 the single-workgroup NTT of 2^k points, 110..112 the batched pass
+// kernels at 2^(k - 100) points (spectra through HBM); the autotuner
+// (tune_small) picks among them
 // per bucket on the device at first use.  Classes 13..17 are the multi-pass
 // NTT run for the whole class at once, -1 one product at a time.
 constexpr int kBatchMaxLog = 17;
@@ -1106,8 +1107,11 @@ int small_class_default(int b) {
 
 constexpr int kPassClass = 100;  // + log2 N: batched pass kernels
 
+constexpr int kKaraClass = 4;  // one Karatsuba level over the schoolbook (k_batch_kara)
+
 bool small_class_valid(int b, int cls) {
   if (cls >= 0 && cls <= 3) return (1 << b) <= school_max(cls);
+  if (cls == kKaraClass) return (1 << b) <= kSchoolMaxLimbs;
   if (cls >= kPassClass + kSmallNttMinLog && cls <= kPassClass + kSmallNttMaxLog) return cls - kPassClass >= b;
   return cls >= kSmallNttMinLog && cls <= kSmallNttMaxLog && cls >= b;
 }
@@ -1115,11 +1119,13 @@ bool small_class_valid(int b, int cls) {
 // Slot of a small class in launch order (0..9).
 int small_slot(int cls) {
   if (cls <= 3) return cls;
+  if (cls == kKaraClass) return 10;
   if (cls < kPassClass) return cls - kSmallNttMinLog + 4;
   return cls - kPassClass - kSmallNttMinLog + 7;
 }
 int small_slot_class(int slot) {
   if (slot <= 3) return slot;
+  if (slot == 10) return kKaraClass;
   if (slot <= 6) return slot - 4 + kSmallNttMinLog;
   return slot - 7 + kSmallNttMinLog + kPassClass;
 }
@@ -1339,6 +1345,9 @@ int launch_small_class(Ctx& c, int cls, const BatchDesc* base, size_t n) {
       hipLaunchKernelGGL((k_batch_school_g<64, 8>), dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.stream,
                          base, (uint32_t)n);
       break;
+    case kKaraClass:
+      hipLaunchKernelGGL(k_batch_kara, dim3((unsigned)n), dim3(256), 0, c.stream, base);
+      break;
     case 10:
       hipLaunchKernelGGL(k_batch_ntt<10>, dim3((unsigned)n), dim3((1 << 10) / kEPT), 0, c.stream, base,
                          c.d_prime + 10 * 2, c.crt);
@@ -1484,10 +1493,10 @@ int tune_small(Ctx& c) {
       break;
     }
     const int lg = b < kSmallNttMinLog ? kSmallNttMinLog : b;
-    const int cls[6] = {1, 2, 3, 0, lg, kPassClass + lg};
+    const int cls[7] = {1, 2, 3, 0, kKaraClass, lg, kPassClass + lg};
     float best_ms = 1e30f;
     int best_cls = small_class_default(b);
-    for (int m = 0; m < 6 && rc == PCG_OK; ++m) {
+    for (int m = 0; m < 7 && rc == PCG_OK; ++m) {
       if (!small_class_valid(b, cls[m])) continue;
       float best = 1e30f;
       for (int rep = 0; rep < 4 && rc == PCG_OK; ++rep) {
@@ -1523,9 +1532,9 @@ int ensure_tuned(Ctx& c) { return g_small_ready ? PCG_OK : tune_small(c); }
 
 int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, bool dev) {
   // class slots: 0..3 schoolbook classes 0..3, 4..6 single-workgroup NTT
-  // (2^10 .. 2^12), 7..9 batched passes at 2^10 .. 2^12, 10..14 batched
-  // multi-pass (2^13 .. 2^17)
-  constexpr int kSmallSlots = 10;
+  // (2^10 .. 2^12), 7..9 batched passes at 2^10 .. 2^12, 10 Karatsuba,
+  // 11..15 batched multi-pass (2^13 .. 2^17)
+  constexpr int kSmallSlots = 11;
   constexpr int kSlots = kSmallSlots + (kBatchMaxLog - kSmallNttMaxLog);
   int rc0 = ensure_tuned(c);
   if (rc0) return rc0;

@@ -139,6 +139,7 @@ _PLAN_TABLES = [
     [1, 1, 1, 1, 1, 1, 1, 2, 2, 0, 10, 11, 12],
     [1, 1, 1, 1, 1, 1, 1, 3, 3, 3, 110, 111, 112],
     [1, 1, 1, 1, 1, 1, 2, 2, 3, 110, 10, 11, 112],
+    [1, 1, 1, 1, 1, 1, 1, 2, 4, 4, 4, 11, 12],
 ]
 
 

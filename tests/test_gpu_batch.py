@@ -89,6 +89,8 @@ SCHOOL_MAX = {0: 1024, 1: 64, 2: 256, 3: 512}
 def class_valid(b, c):
     if c in SCHOOL_MAX:
         return (1 << b) <= SCHOOL_MAX[c]
+    if c == 4:  # one Karatsuba level over the schoolbook
+        return (1 << b) <= 1024
     if 110 <= c <= 112:  # batched pass kernels at 2^(c - 100) points
         return c - 100 >= b
     return 10 <= c <= 12 and c >= b
@@ -144,13 +146,14 @@ def test_subtree_on_each_pass_geometry(gpu):
     assert got[0] == got[1]
 
 
-@pytest.mark.parametrize("forced", [0, 1, 2, 3, 10, 110])
+@pytest.mark.parametrize("forced", [0, 1, 2, 3, 4, 10, 110])
 def test_forced_small_classes_exact(gpu, forced):
     """Every kernel of the tuned crossover (schoolbook with 256 threads, with
-    16 or 64 threads per product and 4 or 8 columns per thread, 2^10-point
-    single-workgroup NTT, batched pass kernels at 2^10 points) gives the
-    exact product in every bucket it may serve, including all-ones carry
-    chains and unbalanced shapes."""
+    16 or 64 threads per product and 4 or 8 columns per thread, one
+    Karatsuba level over the schoolbook, 2^10-point single-workgroup NTT,
+    batched pass kernels at 2^10 points) gives the exact product in every
+    bucket it may serve, including all-ones carry chains and unbalanced
+    shapes."""
     saved = gpu.batch_table()
     table = [forced if class_valid(b, forced) else c for b, c in enumerate(saved)]
     gpu.batch_table(table)
@@ -158,7 +161,8 @@ def test_forced_small_classes_exact(gpu, forced):
         rng = random.Random(5 + forced)
         sizes = [(1, 1), (2, 1), (3, 4), (7, 9), (16, 16), (33, 31), (64, 64), (100, 27),
                  (128, 128), (255, 257), (300, 212), (384, 384), (511, 513), (1, 1023),
-                 (1023, 1), (700, 324), (1, 63), (31, 33), (40, 24), (1, 255), (128, 127)]
+                 (1023, 1), (700, 324), (1, 63), (31, 33), (40, 24), (1, 255), (128, 127),
+                 (512, 512), (1000, 24), (513, 511), (24, 1000), (2, 1022)]
         pairs = []
         for k, (na, nb) in enumerate(sizes):
             ones = k % 3 == 2
