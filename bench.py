@@ -257,11 +257,11 @@ def fft_mul(logn: int, steps: int):
 
 
 def class_sweep(count: int = 2048, reps: int = 3):
-    """One JSON line per small-product bucket b = 6..10: the time of one
+    """One JSON line per small-product bucket b = 6..12: the time of one
     pcg_mul_batch_dev call of `count` device-resident products of
     na = nb = 3 * 2^(b-3) limbs in every class valid for the bucket (forced
     through the class table: schoolbook 0..3, Karatsuba 4, single-workgroup
-    NTT 10, batched pass kernels 110), best of `reps` after one."""
+    NTT 10..12, batched pass kernels 110..112), best of `reps` after one."""
     import numpy as np
 
     from piculator_amd import picugpu
@@ -271,7 +271,7 @@ def class_sweep(count: int = 2048, reps: int = 3):
     lib = picugpu.load()
     school_max = {0: 1024, 1: 64, 2: 256, 3: 512, 4: 1024}
     try:
-        for b in range(6, 11):
+        for b in range(6, 13):
             n = 3 << (b - 3)
             total = count * 2 * n
             pool = picugpu.DeviceBuffer(8 * total + 64)
@@ -283,7 +283,8 @@ def class_sweep(count: int = 2048, reps: int = 3):
                 descs[i] = picugpu.MulDesc(pool.ptr + 4 * (2 * n * i), n, pool.ptr + 4 * (2 * n * i + n), n,
                                            pool.ptr + 4 * (total + 2 * n * i))
             res = {}
-            for cls in (0, 1, 2, 3, 4, 10, 110):
+            lg = max(b, 10)
+            for cls in (0, 1, 2, 3, 4, lg, 100 + lg):
                 if cls in school_max and (1 << b) > school_max[cls]:
                     continue
                 t = list(saved)

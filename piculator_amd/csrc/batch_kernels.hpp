@@ -5,8 +5,8 @@
 //                   sums of limbs 4t..4t+3 (exact 96-bit accumulation of
 //                   32x32 products) from operands staged in LDS.
 //  k_batch_ntt    : zL <= 4096 limbs.  Two-prime NTT of N = 2^ceil(log2 zL)
-//                   points entirely in LDS (operands as two contiguous lines),
-//                   primes one after the other, CRT, carry.
+//                   points entirely in LDS (one line per prime, both primes
+//                   at once on separate waves), CRT, carry.
 #pragma once
 #include "carry_kernels.hpp"
 #include "ntt_kernels.hpp"
@@ -323,33 +323,62 @@ __global__ __launch_bounds__(256) void k_batch_school_g(const BatchDesc* descs, 
   }
 }
 
-// One product per workgroup of N / kEPT threads (N = 2^logN, 2^10 <= N <=
-// 2^12); all products of one launch share logN, prm points at the constants
-// for this N.  The operands are read once, in first-stage order, into
-// registers.  Per prime: the forward DFTs of a and b run from registers to
-// registers (one LDS line for the middle stages), the pointwise product with
-// N^-1 stays in registers, one pass through LDS puts it in first-stage order
-// for the inverse DFT; the prime-0 result waits in registers.  CRT words go
-// through LDS into limb order for the block carry.  LDS: one padded line (or
-// the 3 N CRT words) and one stage-twiddle table, reloaded for prime 1.
-// Up to 2^11 points the register budget is capped at three waves per SIMD
-// (measured 4-6% faster than two, with a few spilled registers at 2^11).
+// Two lines in LDS, one per prime, each thread working in one of them: the
+// first L/8 threads on prime 0's line, the next L/8 on prime 1's (whole
+// waves, so a wave's prime is uniform).  Butterfly bf = t + b NTH of a stage
+// is butterfly (t mod L/8) + b L/8 of line t / (L/8): within a line, the
+// thread-to-butterfly map of a one-line row tile.
+template <int LOGL>
+struct GeoPrimes {
+  static constexpr int L = 1 << LOGL;
+  static constexpr int LINES = 2;
+  static constexpr int E = 2 * L;
+  static constexpr int NTH = E / kEPT;  // L / 4
+  static constexpr int LP = L + L / 8;
+  static constexpr int LDS = 2 * LP;
+  __device__ static __forceinline__ int idx(int n, int lam) { return lam * LP + n + (n >> 3); }
+  __device__ static __forceinline__ void split(int bf, int logLR, int& lam, int& j) {
+    const int t = bf & (NTH - 1), b = bf >> (LOGL - 2);
+    lam = t >> (LOGL - 3);
+    j = (t & ((L >> 3) - 1)) + (b << (LOGL - 3));
+  }
+};
+
+// One product per workgroup (N = 2^logN, 2^10 <= N <= 2^12); all products
+// of one launch share logN, prm points at the constants for this N.  Both
+// primes at once: N / 4 threads, the first half on prime 0's transforms and
+// the second half on prime 1's (whole waves, so each wave holds one prime's
+// points and its twiddle table).  The operands are read once, in
+// first-stage order, into registers.  The forward DFTs of a and b run from
+// registers to registers (one LDS line per prime for the middle stages), the
+// pointwise product with N^-1 stays in registers, one pass through LDS puts
+// it in first-stage order for the inverse DFT.  The prime-1 results go
+// through LDS to the prime-0 threads for the CRT, whose words go through
+// LDS into limb order for the block carry over all threads.  LDS: two padded
+// lines (or the CRT staging) and a stage-twiddle table per prime.  Capped at
+// 128 VGPRs (four waves per SIMD): at 2^11 that fits two workgroups per CU.
 template <int LOGN>
-__global__ __launch_bounds__((1 << LOGN) / kEPT) __attribute__((amdgpu_waves_per_eu(LOGN <= 11 ? 3 : 1))) void k_batch_ntt(const BatchDesc* descs,
-                                                                  const Prime* prm, double2 crt) {
-  using GE = Geo<LOGN, 0, true>;
+__global__ __launch_bounds__((2 << LOGN) / kEPT) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_ntt(
+    const BatchDesc* descs, const Prime* prm, double2 crt) {
+  using GE = GeoPrimes<LOGN>;
   constexpr int N = 1 << LOGN;
   constexpr int LOGR1 = LOGN % 3 == 1 ? 1 : (LOGN % 3 == 2 ? 2 : 3);
   constexpr int R1 = 1 << LOGR1;
   constexpr int NB1 = kEPT / R1;
-  constexpr int LDSD = GE::LP > (3 * N) / 2 ? GE::LP : (3 * N) / 2;  // doubles
+  constexpr int NTH = GE::NTH;
+  constexpr int LDSD = GE::LDS > (5 * N) / 2 ? GE::LDS : (5 * N) / 2;  // doubles
   __shared__ double lds[LDSD];
-  __shared__ CarryFn wtot[(N / kEPT + 63) / 64];
-  __shared__ double2 ltw[lds_tw_size<LOGN>()];
+  __shared__ CarryFn wtot[NTH / 64];
+  __shared__ double2 ltw[2][lds_tw_size<LOGN>()];
   const BatchDesc d = descs[blockIdx.x];
   const int tid = threadIdx.x;
+  const int pw = __builtin_amdgcn_readfirstlane(tid >> (LOGN - 3));  // this wave's prime
   const int na = (int)d.na, nb = (int)d.nb, zL = na + nb;
   const Prime P0 = prm[0], P1 = prm[1];
+  const Prime& P = pw ? P1 : P0;
+  stage_twiddles<GE>(ltw[0], P0);
+  stage_twiddles<GE>(ltw[1], P1);
+  const double2* tw = ltw[pw];
   uint32_t ua[kEPT], ub[kEPT];
 #pragma unroll
   for (int b = 0; b < NB1; ++b)
@@ -360,12 +389,8 @@ __global__ __launch_bounds__((1 << LOGN) / kEPT) __attribute__((amdgpu_waves_per
       ua[b * R1 + r] = n < na ? d.a[n] : 0u;
       ub[b * R1 + r] = n < nb ? d.b[n] : 0u;
     }
-  double z0[8];
   double y[1][8];
-  for (int pi = 0; pi < 2; ++pi) {
-    const Prime& P = pi ? P1 : P0;
-    if (pi) __syncthreads();  // prime 0's last stage has read LDS and the table
-    const double2* tw = stage_twiddles<GE>(ltw, P);
+  {
     double x[kEPT], yb[1][8];
 #pragma unroll
     for (int q = 0; q < kEPT; ++q) x[q] = (double)ua[q];
@@ -374,7 +399,7 @@ __global__ __launch_bounds__((1 << LOGN) / kEPT) __attribute__((amdgpu_waves_per
     for (int q = 0; q < kEPT; ++q) x[q] = (double)ub[q];
     __syncthreads();
     line_dft_regs<GE>(x, yb, lds, tw, P, false);
-    // pointwise (natural order: point tid + r N / 8) with N^-1, then the
+    // pointwise (natural order: point t' + r N / 8) with N^-1, then the
     // inverse DFT's first-stage order through LDS
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
@@ -386,7 +411,7 @@ __global__ __launch_bounds__((1 << LOGN) / kEPT) __attribute__((amdgpu_waves_per
     for (int r = 0; r < 8; ++r) {
       int lam, n;
       stage_pos<GE, 3>(0, r, lam, n);
-      lds[GE::idx(n, 0)] = y[0][r];
+      lds[GE::idx(n, lam)] = y[0][r];
     }
     __syncthreads();
 #pragma unroll
@@ -395,43 +420,52 @@ __global__ __launch_bounds__((1 << LOGN) / kEPT) __attribute__((amdgpu_waves_per
       for (int r = 0; r < R1; ++r) {
         int lam, n;
         stage_pos<GE, LOGR1>(b, r, lam, n);
-        x[b * R1 + r] = lds[GE::idx(n, 0)];
+        x[b * R1 + r] = lds[GE::idx(n, lam)];
       }
     __syncthreads();
     line_dft_regs<GE>(x, y, lds, tw, P, true);
-    if (pi == 0) {
+  }
+  // CRT of coefficient n: prime 1's results by point, then the words
+  __syncthreads();
+  double* r1 = lds;                                     // N doubles
+  uint32_t* sw = reinterpret_cast<uint32_t*>(lds + N);  // w0 [0, N), w1 [N, 2N), w2 [2N, 3N)
+  if (pw == 1) {
 #pragma unroll
-      for (int r = 0; r < 8; ++r) z0[r] = y[0][r];
+    for (int r = 0; r < 8; ++r) {
+      int lam, n;
+      stage_pos<GE, 3>(0, r, lam, n);
+      r1[n] = y[0][r];
     }
   }
-  // CRT of coefficient n = tid + r N / 8; words to LDS, then limb order
   __syncthreads();
-  uint32_t* sw = reinterpret_cast<uint32_t*>(lds);  // w0 [0, N), w1 [N, 2N), w2 [2N, 3N)
+  if (pw == 0) {
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    int lam, n;
-    stage_pos<GE, 3>(0, r, lam, n);
-    uint32_t c0 = 0, c1 = 0, c2 = 0;
-    if (n < zL) crt_words(z0[r], y[0][r], P0, P1, crt, c0, c1, c2);
-    sw[n] = c0;
-    sw[N + n] = c1;
-    sw[2 * N + n] = c2;
+    for (int r = 0; r < 8; ++r) {
+      int lam, n;
+      stage_pos<GE, 3>(0, r, lam, n);
+      uint32_t c0 = 0, c1 = 0, c2 = 0;
+      if (n < zL) crt_words(y[0][r], r1[n], P0, P1, crt, c0, c1, c2);
+      sw[n] = c0;
+      sw[N + n] = c1;
+      sw[2 * N + n] = c2;
+    }
   }
   __syncthreads();
-  uint32_t w0[kEPT], w1[kEPT], w2[kEPT], D[kEPT];
+  constexpr int LPT = N / NTH;  // 4 limbs per thread
+  uint32_t w0[LPT], w1[LPT], w2[LPT], D[LPT];
 #pragma unroll
-  for (int q = 0; q < kEPT; ++q) {
-    const int k = tid * kEPT + q;
+  for (int q = 0; q < LPT; ++q) {
+    const int k = tid * LPT + q;
     w0[q] = sw[k];
     w1[q] = sw[N + k];
     w2[q] = sw[2 * N + k];
   }
   __syncthreads();  // block_carry's scratch reuses the LDS
   uint64_t c, spill;
-  block_carry<kEPT>(w0, w1, w2, sw, sw + N, wtot, D, c, spill);
+  block_carry<LPT>(w0, w1, w2, sw, sw + N, wtot, D, c, spill);
 #pragma unroll
-  for (int q = 0; q < kEPT; ++q) {
-    const int k = tid * kEPT + q;
+  for (int q = 0; q < LPT; ++q) {
+    const int k = tid * LPT + q;
     if (k < zL) d.out[k] = D[q];
   }
 }

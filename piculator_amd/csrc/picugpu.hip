@@ -1349,15 +1349,15 @@ int launch_small_class(Ctx& c, int cls, const BatchDesc* base, size_t n) {
       hipLaunchKernelGGL(k_batch_kara, dim3((unsigned)n), dim3(256), 0, c.stream, base);
       break;
     case 10:
-      hipLaunchKernelGGL(k_batch_ntt<10>, dim3((unsigned)n), dim3((1 << 10) / kEPT), 0, c.stream, base,
+      hipLaunchKernelGGL(k_batch_ntt<10>, dim3((unsigned)n), dim3((2 << 10) / kEPT), 0, c.stream, base,
                          c.d_prime + 10 * 2, c.crt);
       break;
     case 11:
-      hipLaunchKernelGGL(k_batch_ntt<11>, dim3((unsigned)n), dim3((1 << 11) / kEPT), 0, c.stream, base,
+      hipLaunchKernelGGL(k_batch_ntt<11>, dim3((unsigned)n), dim3((2 << 11) / kEPT), 0, c.stream, base,
                          c.d_prime + 11 * 2, c.crt);
       break;
     case 12:
-      hipLaunchKernelGGL(k_batch_ntt<12>, dim3((unsigned)n), dim3((1 << 12) / kEPT), 0, c.stream, base,
+      hipLaunchKernelGGL(k_batch_ntt<12>, dim3((unsigned)n), dim3((2 << 12) / kEPT), 0, c.stream, base,
                          c.d_prime + 12 * 2, c.crt);
       break;
     default:
