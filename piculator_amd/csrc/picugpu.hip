@@ -364,12 +364,28 @@ struct Ctx {
   size_t df_nblk_cap = 0;
   unsigned long long* df_slot = nullptr;  // [4] device
   unsigned long long* h_slot = nullptr;   // [4] pinned host
+  uint32_t* h_status = nullptr;           // pinned host copy of *status
+  hipEvent_t ev_wait = nullptr;           // polled by wait_stream
   // device address of the static scratch array of pcg_mul_dev_noalloc on
   // this context's device (hipGetSymbolAddress at pcg_init)
   unsigned char* noalloc = nullptr;
 };
 
 Ctx g_ctx;
+
+// Waits for everything enqueued on the library stream by polling an event
+// from this thread.  The runtime's stream synchronisation sleeps until an
+// interrupt wakes it; the device-resident operations read a length or a top
+// limb back after every few kernels, and that wake-up is a large share of
+// each such round trip while the GPU idles.
+int wait_stream(Ctx& c) {
+  HIPCHK(hipEventRecord(c.ev_wait, c.stream));
+  for (;;) {
+    const hipError_t e = hipEventQuery(c.ev_wait);
+    if (e == hipSuccess) return PCG_OK;
+    if (e != hipErrorNotReady) HIPCHK(e);
+  }
+}
 
 // ---- scratch of the allocation-free product (pcg_mul_dev_noalloc) ----
 // Products of na + nb <= 2^20 limbs run the plain (unwrapped) transform of
@@ -2481,7 +2497,8 @@ int bsr_eval_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, const ui
   rc = bsr_run_schedule(c, *P);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(rv, P->meta, 3 * sizeof(BsrVal), hipMemcpyDeviceToHost, c.stream));
-  HIPCHK(hipStreamSynchronize(c.stream));
+  rc = wait_stream(c);
+  if (rc) return rc;
   *Pout = P;
   return PCG_OK;
 }
@@ -2503,10 +2520,13 @@ int bsr_exact_locked(Ctx& c, uint32_t a, uint32_t b, uint64_t max_limbs, pcg_bsr
   return PCG_OK;
 }
 
+// Waits for the stream and checks the carry-overflow status, whose copy goes
+// through the same wait as whatever readback the caller has enqueued.
 int sync_locked(Ctx& c) {
-  HIPCHK(hipStreamSynchronize(c.stream));
-  uint32_t st = 0;
-  HIPCHK(hipMemcpy(&st, c.status, sizeof st, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(c.h_status, c.status, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
+  int rc = wait_stream(c);
+  if (rc) return rc;
+  const uint32_t st = *c.h_status;
   if (st) {
     uint32_t z = 0;
     HIPCHK(hipMemcpy(c.status, &z, sizeof z, hipMemcpyHostToDevice));
@@ -2613,13 +2633,13 @@ int df_ensure_blocks(Ctx& c, uint64_t nblk) {
 int df_read_slots(Ctx& c, int n) {
   HIPCHK(hipMemcpyAsync(c.h_slot, c.df_slot, n * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                         c.stream));
-  HIPCHK(hipStreamSynchronize(c.stream));
-  return PCG_OK;
+  return wait_stream(c);
 }
 
 int df_read_limb(Ctx& c, const uint32_t* p, uint32_t* v) {
   HIPCHK(hipMemcpyAsync(c.h_slot, p, sizeof(uint32_t), hipMemcpyDeviceToHost, c.stream));
-  HIPCHK(hipStreamSynchronize(c.stream));
+  const int rc = wait_stream(c);
+  if (rc) return rc;
   memcpy(v, c.h_slot, sizeof(uint32_t));
   return PCG_OK;
 }
@@ -2803,13 +2823,17 @@ int df_mul_locked(Ctx& c, pcg_dfloat* z, const pcg_dfloat* pa, const pcg_dfloat*
   } else {
     rc = mul_dev_locked(c, a.t, a.L, b.t, b.L, buf, flags & PCG_ENGINE_MASK);
   }
-  uint32_t topl = 0;
-  if (!rc) rc = df_read_limb(c, buf + a.L + b.L - 1, &topl);
+  // the top limb comes back through the status check's wait
+  if (!rc && hipMemcpyAsync(c.h_slot, buf + a.L + b.L - 1, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            c.stream) != hipSuccess)
+    rc = fail(PCG_ERR_HIP, "top limb readback failed");
   if (!rc) rc = sync_locked(c);
   if (rc) {
     df_put_back(buf, cap);
     return rc;
   }
+  uint32_t topl = 0;
+  memcpy(&topl, c.h_slot, sizeof topl);
   df_assign(z, buf, cap, a.L + b.L - (topl == 0 ? 1 : 0), a.exp + b.exp, (A.sign != 0) == (B.sign != 0));
   return PCG_OK;
 }
@@ -2966,6 +2990,8 @@ int init_locked(Ctx& c, int device) {
   HIPCHK(hipMemset(c.status, 0, sizeof(uint32_t)));
   HIPCHK(hipMalloc(&c.df_slot, 4 * sizeof(unsigned long long)));
   HIPCHK(hipHostMalloc(&c.h_slot, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc(&c.h_status, sizeof(uint32_t), hipHostMallocDefault));
+  HIPCHK(hipEventCreateWithFlags(&c.ev_wait, hipEventDisableTiming));
   for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&c.ev[i]));
   for (int i = 0; i < 2; ++i) HIPCHK(hipEventCreate(&c.tev[i]));
   for (int b = 0; b <= kSmallNttMaxLog; ++b) g_small_class[b] = small_class_default(b);
@@ -2998,8 +3024,10 @@ void release_locked(Ctx& c) {
                  c.d_pool, c.d_desc, c.bspec, c.wrap_desc, c.bagg, c.bcin, c.df_agg, c.df_cin,
                  c.df_slot};
   for (void* q : dev) (void)hipFree(q);
-  void* host[] = {c.h_pool, c.h_desc, c.h_slot};
+  void* host[] = {c.h_pool, c.h_desc, c.h_slot, c.h_status};
   for (void* q : host) (void)hipHostFree(q);
+  if (c.ev_wait) (void)hipEventDestroy(c.ev_wait);
+  c.ev_wait = nullptr;
   for (int i = 0; i < 4; ++i) {
     if (c.ev[i]) (void)hipEventDestroy(c.ev[i]);
     c.ev[i] = nullptr;
@@ -3044,6 +3072,7 @@ void release_locked(Ctx& c) {
   c.df_nblk_cap = 0;
   c.df_slot = nullptr;
   c.h_slot = nullptr;
+  c.h_status = nullptr;
   c.noalloc = nullptr;
 }
 
