@@ -236,10 +236,10 @@ __global__ __launch_bounds__(256) void k_batch_kara(const BatchDesc* descs) {
 // C limbs; the carries are resolved inside the group with shuffles: each
 // thread composes the carry functions of its C limbs, the group scans them
 // (width-G shuffles), and each thread adds its carry-in.
-__device__ __forceinline__ CarryFn cf_shfl_up_w(CarryFn f, int d, int width) {
-  CarryFn r;
+__device__ __forceinline__ LimbFn lf_shfl_up_w(LimbFn f, int d, int width) {
+  LimbFn r;
   r.base = __shfl_up(f.base, d, width);
-  r.T = __shfl_up(f.T, d, width);
+  r.tm = __shfl_up(f.tm, d, width);
   return r;
 }
 
@@ -296,25 +296,19 @@ __global__ __launch_bounds__(256) void k_batch_school_g(const BatchDesc* descs, 
   s[1] = (uint64_t)x0[1] + x1[0] + p23;
 #pragma unroll
   for (int q = 2; q < C; ++q) s[q] = (uint64_t)x0[q] + x1[q - 1] + x2[q - 2];
-  CarryFn f;
-  f.base = 0;
-  f.T = 0;
+  // s < 3 * 2^32: carries of at most 3, 32-bit limb functions
+  LimbFn f = lf_of_sum(s[0]);
 #pragma unroll
-  for (int q = 0; q < C; ++q) {
-    CarryFn g;
-    g.base = s[q] >> 32;
-    g.T = (1ull << 32) - (s[q] & 0xffffffffull);
-    f = cf_compose(f, g);
-  }
-  CarryFn incl = f;
+  for (int q = 1; q < C; ++q) f = lf_compose(f, lf_of_sum(s[q]));
+  LimbFn incl = f;
 #pragma unroll
   for (int dd = 1; dd < G; dd <<= 1) {
-    const CarryFn o = cf_shfl_up_w(incl, dd, G);
-    if (lane >= dd) incl = cf_compose(o, incl);
+    const LimbFn o = lf_shfl_up_w(incl, dd, G);
+    if (lane >= dd) incl = lf_compose(o, incl);
   }
-  CarryFn excl = cf_shfl_up_w(incl, 1, G);
-  if (lane == 0) excl.T = 0;
-  uint64_t c = cf_eval0(excl);
+  LimbFn excl = lf_shfl_up_w(incl, 1, G);
+  if (lane == 0) excl = lf_identity();
+  uint64_t c = lf_eval0(excl);
 #pragma unroll
   for (int q = 0; q < C; ++q) {
     const uint64_t t = (s[q] & 0xffffffffull) + c;

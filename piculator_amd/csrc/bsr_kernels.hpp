@@ -549,19 +549,16 @@ __global__ __launch_bounds__(kCarryThreads) void k_bsr_add_local(const BsrAdd* a
   }
   // Positions past the top pass the carry through unchanged (identity), so
   // the block function ends at the top limb: its value is the carry out.
-  CarryFn f;
-  f.base = 0;
-  f.T = 0;
+  // sums below 2^33: carries of at most 2, 32-bit limb functions (the
+  // scan's LDS scratch taken from wtot)
+  LimbFn f = lf_identity();
 #pragma unroll
   for (int q = 0; q < kCarryLPT; ++q) {
     if (base + (uint64_t)tid * kCarryLPT + q >= pl.LA) continue;
-    CarryFn g;
-    g.base = s[q] >> 32;
-    g.T = (1ull << 32) - (s[q] & 0xffffffffull);
-    f = cf_compose(f, g);
+    f = lf_compose(f, lf_of_sum(s[q]));
   }
-  const CarryFn pre = cf_block_exclusive(f, wtot);
-  uint64_t c = cf_eval0(pre);
+  const LimbFn pre = lf_block_exclusive(f, reinterpret_cast<LimbFn*>(wtot));
+  uint64_t c = lf_eval0(pre);
 #pragma unroll
   for (int q = 0; q < kCarryLPT; ++q) {
     const uint64_t k = base + (uint64_t)tid * kCarryLPT + q;
@@ -569,7 +566,7 @@ __global__ __launch_bounds__(kCarryThreads) void k_bsr_add_local(const BsrAdd* a
     if (k < pl.LA) pl.O[k] = (uint32_t)t;
     c = (s[q] >> 32) + (t >> 32);
   }
-  if (tid == kCarryThreads - 1) agg[(uint64_t)node * maxch + blockIdx.x] = cf_compose(pre, f);
+  if (tid == kCarryThreads - 1) agg[(uint64_t)node * maxch + blockIdx.x] = cf_of_limb(lf_compose(pre, f));
 }
 
 // One wave per node: exclusive scan of its chunk functions; the total is

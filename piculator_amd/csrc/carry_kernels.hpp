@@ -48,6 +48,85 @@ __device__ __forceinline__ CarryFn cf_compose(CarryFn f1, CarryFn f2) {
   return r;
 }
 
+// The same functions for carries between single limbs, in 32-bit fields:
+// a limb's carry-in c is at most 4, so f(c) = base + [c >= T] needs T only
+// up to 2^32 (T = 2^32 - lo(s) >= 1), and T = 2^32 acts as never.  Stored
+// as tm = T - 1 (0xFFFFFFFF: never); base == kLimbId marks the identity.
+constexpr uint32_t kLimbId = 0xFFFFFFFFu;
+constexpr uint32_t kLimbNever = 0xFFFFFFFFu;
+struct LimbFn {
+  uint32_t base, tm;
+};
+
+// The function of a limb whose sum (without carry-in) is s < 4 * 2^32.
+__device__ __forceinline__ LimbFn lf_of_sum(uint64_t s) {
+  LimbFn f;
+  f.base = (uint32_t)(s >> 32);
+  f.tm = ~(uint32_t)s;  // 2^32 - lo(s) - 1
+  return f;
+}
+
+__device__ __forceinline__ LimbFn lf_identity() {
+  LimbFn f;
+  f.base = kLimbId;
+  f.tm = kLimbNever;
+  return f;
+}
+
+// Selects only, field by field (a wave's lanes take different cases; a
+// select of whole structs would go through scratch memory).  When
+// f1.base > tm2 the test f1.base == tm2 fails, so tm becomes never.
+__device__ __forceinline__ LimbFn lf_compose(LimbFn f1, LimbFn f2) {
+  const bool id1 = f1.base == kLimbId, id2 = f2.base == kLimbId;
+  uint32_t base = f2.base + (f1.base > f2.tm ? 1u : 0u);
+  uint32_t tm = f1.base == f2.tm ? f1.tm : kLimbNever;
+  base = id2 ? f1.base : base;
+  tm = id2 ? f1.tm : tm;
+  LimbFn r;
+  r.base = id1 ? f2.base : base;
+  r.tm = id1 ? f2.tm : tm;
+  return r;
+}
+
+__device__ __forceinline__ uint32_t lf_eval0(LimbFn f) { return f.base == kLimbId ? 0u : f.base; }
+
+// The 64-bit form of a limb function, for block functions scanned across
+// blocks whose carries stay below 2^32 - 1 (T = 2^32 acts as never there).
+__device__ __forceinline__ CarryFn cf_of_limb(LimbFn f) {
+  const bool id = f.base == kLimbId;
+  CarryFn r;
+  r.base = id ? 0 : f.base;
+  r.T = id ? 0 : (f.tm == kLimbNever ? kInf : (uint64_t)f.tm + 1);
+  return r;
+}
+
+__device__ __forceinline__ LimbFn lf_shfl_up(LimbFn f, int d) {
+  LimbFn r;
+  r.base = __shfl_up(f.base, d, 64);
+  r.tm = __shfl_up(f.tm, d, 64);
+  return r;
+}
+
+// Exclusive scan of one LimbFn per thread across the workgroup (<= 1024
+// threads); `wave_tot` is LDS scratch of (threads / 64) entries.
+__device__ __forceinline__ LimbFn lf_block_exclusive(LimbFn f, LimbFn* wave_tot) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  LimbFn incl = f;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const LimbFn o = lf_shfl_up(incl, d);
+    if (lane >= d) incl = lf_compose(o, incl);
+  }
+  LimbFn excl = lf_shfl_up(incl, 1);
+  if (lane == 0) excl = lf_identity();
+  if (lane == 63) wave_tot[wv] = incl;
+  __syncthreads();
+  LimbFn pre = lf_identity();
+  for (int i = 0; i < wv; ++i) pre = lf_compose(pre, wave_tot[i]);
+  __syncthreads();
+  return lf_compose(pre, excl);
+}
+
 __device__ __forceinline__ uint64_t cf_eval0(CarryFn f) { return f.T == 0 ? 0 : f.base; }
 
 __device__ __forceinline__ CarryFn cf_shfl_up(CarryFn f, int d) {
@@ -196,13 +275,21 @@ __device__ __forceinline__ void fft_words(const double2* z, uint64_t N, uint64_t
   w2 = (uint32_t)(t >> 64);
 }
 
+// An integer 0 <= x < 2^52 held in a double, as an integer: x + 2^52 is
+// exact and its mantissa field is x.
+__device__ __forceinline__ uint64_t small_to_u64(double x) {
+  return (uint64_t)__double_as_longlong(x + 4503599627370496.0) & ((1ull << 52) - 1);
+}
+
+// c = r0 + p0 t with t = (r1 - r0) p0^-1 mod p1: only r0 (the base) and t
+// need their canonical residues; r1 enters through the difference, which the
+// Shoup product reduces whatever representative r1 is.
 __device__ __forceinline__ void crt_words(double r0, double r1, const Prime& P0, const Prime& P1,
                                           double2 crt, uint32_t& w0, uint32_t& w1, uint32_t& w2) {
   r0 = canon(r0, P0.p, P0.pinv);
-  r1 = canon(r1, P1.p, P1.pinv);
   double t = mm_shoup(r1 - r0, crt.x, crt.y, P1.p);
   t = canon(t, P1.p, P1.pinv);
-  const uint64_t tu = (uint64_t)t, r0u = (uint64_t)r0, p0u = (uint64_t)P0.p;
+  const uint64_t tu = small_to_u64(t), r0u = small_to_u64(r0), p0u = small_to_u64(P0.p);
   uint64_t lo = p0u * tu;
   uint64_t hi = __umul64hi(p0u, tu);
   lo += r0u;
@@ -227,18 +314,13 @@ __device__ __forceinline__ void crt_words(double r0, double r1, const Prime& P0,
 template <int LPT>
 __device__ __forceinline__ void block_carry_sums(const uint64_t* s, CarryFn* wtot, uint32_t* D,
                                                  uint64_t& cout) {
-  CarryFn f;
-  f.base = 0;
-  f.T = 0;
+  // carries between these limbs are at most 4: 32-bit limb functions, the
+  // scan's LDS scratch taken from wtot
+  LimbFn f = lf_of_sum(s[0]);
 #pragma unroll
-  for (int q = 0; q < LPT; ++q) {
-    CarryFn g;
-    g.base = s[q] >> 32;
-    g.T = (1ull << 32) - (s[q] & 0xffffffffull);
-    f = cf_compose(f, g);
-  }
-  const CarryFn pre = cf_block_exclusive(f, wtot);
-  uint64_t c = cf_eval0(pre);
+  for (int q = 1; q < LPT; ++q) f = lf_compose(f, lf_of_sum(s[q]));
+  const LimbFn pre = lf_block_exclusive(f, reinterpret_cast<LimbFn*>(wtot));
+  uint64_t c = lf_eval0(pre);
 #pragma unroll
   for (int q = 0; q < LPT; ++q) {
     const uint64_t t = (s[q] & 0xffffffffull) + c;
