@@ -46,46 +46,83 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Tables: ta[e] = omega_1024^e (e < 1024); tb[e] = omega_{2^14}^e and
-// tb[256 + e] = omega_{2^14}^-e (e < 256).
-__device__ __forceinline__ double2 row_ta(const BigTw* ta, int e, bool inv, const Prime& P) {
-  return tw_of(ta, inv ? (1024 - e) & 1023 : e, P);
+// Stage-twiddle tables in LDS (i = 0 forward, 1 inverse: exponent sign s =
+// +1, -1).  Each stage's table is laid out by (k, lane index) so that the
+// lanes of one read touch consecutive or equal entries (no bank conflicts;
+// a table indexed by k m2 would put lanes k entries apart) and its address
+// is a constant offset from the lane's base.
+//   ta[e]        = omega_1024^e                 (e < 1024)  S1, factor a
+//   t1[i][k][b]  = omega_{2^14}^{s k b}         (b < 16)    S1, factor b
+//   t2[i][k][m2] = omega_1024^{s k m2}          (m2 < 64)   S2
+//   t3[i][k][n4] = omega_64^{s k n4}            (n4 < 4)    S3
+struct RowTw {
+  BigTw ta[1024];
+  double t1[2][16][16];
+  double t2[2][16][64];
+  double t3[2][16][4];
+};
+
+// Fills the tables (all threads of the workgroup; a barrier must follow
+// before the first use).
+__device__ __forceinline__ void row_tw_init(RowTw& T, const Prime& P) {
+  const int nth = blockDim.x;
+  for (int e = threadIdx.x; e < 1024; e += nth) T.ta[e] = P.twL[e << (kLogE - 10)].x;
+  for (int x = threadIdx.x; x < 512; x += nth) {
+    const int i = x >> 8, k = (x >> 4) & 15, b = x & 15;
+    T.t1[i][k][b] = root_pow((uint64_t)(k * b), kBigLogE, i != 0, P).x;
+  }
+  for (int x = threadIdx.x; x < 2048; x += nth) {
+    const int i = x >> 10, k = (x >> 6) & 15, m2 = x & 63;
+    int e = (k * m2) & 1023;
+    if (i) e = (1024 - e) & 1023;
+    T.t2[i][k][m2] = P.twL[e << (kLogE - 10)].x;
+  }
+  for (int x = threadIdx.x; x < 128; x += nth) {
+    const int i = x >> 6, k = (x >> 2) & 15, n4 = x & 3;
+    int e = (16 * k * n4) & 1023;
+    if (i) e = (1024 - e) & 1023;
+    T.t3[i][k][n4] = P.twL[e << (kLogE - 10)].x;
+  }
 }
 
 // S1 twiddles: v[k1] *= omega_{2^14}^{+-k1 m}, m = 16 a + b, as
 // omega_{2^14}^{b k1} (b k1 < 256) times omega_1024^{a k1} (a k1 < 1024).
-__device__ __forceinline__ void row_s1_twiddle(double (&v)[16], int m, const BigTw* ta,
-                                               const double* tb, const Prime& P, bool inv) {
+__device__ __forceinline__ void row_s1_twiddle(double (&v)[16], int m, const RowTw& T, const Prime& P,
+                                               bool inv) {
   const int a = m >> 4, b = m & 15;
+  const int i = inv ? 1 : 0;
   v[0] = reduce(v[0], P.p, P.pinv);
 #pragma unroll
   for (int k = 1; k < 16; ++k) {
-    const double wb = tb[(inv ? kBigTB : 0) + b * k];
-    const double2 wa = row_ta(ta, a * k, inv, P);
+    const double wb = T.t1[i][k][b];
+    const int ea = (a * k) & 1023;
+    const double2 wa = tw_of(T.ta, inv ? (1024 - ea) & 1023 : ea, P);
     const double x = mm_shoup(v[k], wb, wb * P.pinv, P.p);
     v[k] = mm_shoup(x, wa.x, wa.y, P.p);
   }
 }
 
 // S2 twiddles: v[k2] *= omega_1024^{+-k2 m2}.
-__device__ __forceinline__ void row_s2_twiddle(double (&v)[16], int m2, const BigTw* ta,
-                                               const Prime& P, bool inv) {
+__device__ __forceinline__ void row_s2_twiddle(double (&v)[16], int m2, const RowTw& T, const Prime& P,
+                                               bool inv) {
+  const double* t = T.t2[inv ? 1 : 0][0] + m2;
   v[0] = reduce(v[0], P.p, P.pinv);
 #pragma unroll
   for (int k = 1; k < 16; ++k) {
-    const double2 w = row_ta(ta, k * m2, inv, P);
-    v[k] = mm_shoup(v[k], w.x, w.y, P.p);
+    const double w = t[k * 64];
+    v[k] = mm_shoup(v[k], w, w * P.pinv, P.p);
   }
 }
 
-// S3 twiddles: v[k3] *= omega_64^{+-k3 n4} = omega_1024^{+-16 k3 n4}.
-__device__ __forceinline__ void row_s3_twiddle(double (&v)[16], int n4, const BigTw* ta,
-                                               const Prime& P, bool inv) {
+// S3 twiddles: v[k3] *= omega_64^{+-k3 n4}.
+__device__ __forceinline__ void row_s3_twiddle(double (&v)[16], int n4, const RowTw& T, const Prime& P,
+                                               bool inv) {
+  const double* t = T.t3[inv ? 1 : 0][0] + n4;
   v[0] = reduce(v[0], P.p, P.pinv);
 #pragma unroll
   for (int k = 1; k < 16; ++k) {
-    const double2 w = row_ta(ta, 16 * k * n4, inv, P);
-    v[k] = mm_shoup(v[k], w.x, w.y, P.p);
+    const double w = t[k * 4];
+    v[k] = mm_shoup(v[k], w, w * P.pinv, P.p);
   }
 }
 
@@ -162,10 +199,10 @@ __device__ __forceinline__ void row_w1_get_inv(double (&v)[16], const double* R,
 // the S4 layout; between() runs between the two workgroup barriers of the
 // cross-wave exchange.
 template <class F>
-__device__ __forceinline__ void row_forward(double (&v)[16], double* xs, const BigTw* ta, const double* tb,
-                                            const Prime& P, F&& between) {
+__device__ __forceinline__ void row_forward(double (&v)[16], double* xs, const RowTw& T, const Prime& P,
+                                            F&& between) {
   dft16(v, P, false);
-  row_s1_twiddle(v, big_tid(), ta, tb, P, false);
+  row_s1_twiddle(v, big_tid(), T, P, false);
   __syncthreads();  // every wave is done with the LDS
   {
     const int t = big_tid();
@@ -182,7 +219,7 @@ __device__ __forceinline__ void row_forward(double (&v)[16], double* xs, const B
 #pragma unroll
     for (int n2 = 0; n2 < 16; ++n2) v[n2] = xw[n2 << 6];
     dft16(v, P, false);
-    row_s2_twiddle(v, lane, ta, P, false);
+    row_s2_twiddle(v, lane, T, P, false);
     double* R = xs + ((t >> 6) << 10);
     wave_lds_sync();
     row_w1_put(v, R, lane);
@@ -190,7 +227,7 @@ __device__ __forceinline__ void row_forward(double (&v)[16], double* xs, const B
     row_w1_get(v, R, lane);
   }
   dft16(v, P, false);
-  row_s3_twiddle(v, big_tid() & 3, ta, P, false);
+  row_s3_twiddle(v, big_tid() & 3, T, P, false);
   {
     const int t = big_tid();
     double* R = xs + ((t >> 6) << 10);
@@ -205,8 +242,8 @@ __device__ __forceinline__ void row_forward(double (&v)[16], double* xs, const B
 
 // Inverse row DFT of the pointwise products in v (S4 layout), stored
 // reduced at points t + 1024 r of the tile at d.
-__device__ __forceinline__ void row_inverse_store(double (&v)[16], double* xs, const BigTw* ta,
-                                                  const double* tb, const Prime& P, double* d) {
+__device__ __forceinline__ void row_inverse_store(double (&v)[16], double* xs, const RowTw& T, const Prime& P,
+                                                  double* d) {
 #pragma unroll
   for (int s = 0; s < 4; ++s) dft_core<4>(v + 4 * s, P, true);
   {
@@ -217,7 +254,7 @@ __device__ __forceinline__ void row_inverse_store(double (&v)[16], double* xs, c
     wave_lds_sync();
     row_w2_get_inv(v, R, t & 63);
   }
-  row_s3_twiddle(v, big_tid() & 3, ta, P, true);
+  row_s3_twiddle(v, big_tid() & 3, T, P, true);
   dft16(v, P, true);
   {
     const int t = big_tid();
@@ -227,7 +264,7 @@ __device__ __forceinline__ void row_inverse_store(double (&v)[16], double* xs, c
     row_w1_put_inv(v, R, lane);
     wave_lds_sync();
     row_w1_get_inv(v, R, lane);
-    row_s2_twiddle(v, lane, ta, P, true);
+    row_s2_twiddle(v, lane, T, P, true);
   }
   dft16(v, P, true);
   __syncthreads();  // every wave is done with its own rows of LDS
@@ -244,7 +281,7 @@ __device__ __forceinline__ void row_inverse_store(double (&v)[16], double* xs, c
 #pragma unroll
     for (int k1 = 0; k1 < 16; ++k1) v[k1] = xw[k1 << 10];
   }
-  row_s1_twiddle(v, big_tid(), ta, tb, P, true);
+  row_s1_twiddle(v, big_tid(), T, P, true);
   dft16(v, P, true);
   double* dd = d + big_tid();
 #pragma unroll
@@ -255,13 +292,10 @@ template <bool SQUARE, bool LIMBS>
 __global__ __launch_bounds__(kRowNTH) void k_ntt_row_fa(PassArgs a) {
   static_assert(!(SQUARE && LIMBS), "squares take a's column pass output");
   __shared__ double xs[kBigE];
-  __shared__ BigTw ta[1024];
-  __shared__ double tb[2 * kBigTB];
+  __shared__ RowTw T;
   const int pi = blockIdx.y;
   const Prime P = a.prm[pi];
-  for (int e = threadIdx.x; e < 1024; e += kRowNTH) ta[e] = P.twL[e << (kLogE - 10)].x;
-  for (int e = threadIdx.x; e < 2 * kBigTB; e += kRowNTH)
-    tb[e] = root_pow((uint64_t)(e & (kBigTB - 1)), kBigLogE, e >= kBigTB, P).x;
+  row_tw_init(T, P);
   uint64_t zoff = 0, soff = 0;  // batched: as k_ntt_row
   BatchDesc bd{};
   if (a.descs) {
@@ -308,7 +342,7 @@ __global__ __launch_bounds__(kRowNTH) void k_ntt_row_fa(PassArgs a) {
     if constexpr (SQUARE) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) v[r] = xa[r];
-      row_forward(v, xs, ta, tb, P, [&] {
+      row_forward(v, xs, T, P, [&] {
         if (wn < ntiles) load_tile(1, wn, xa);
       });
 #pragma unroll
@@ -321,13 +355,13 @@ __global__ __launch_bounds__(kRowNTH) void k_ntt_row_fa(PassArgs a) {
       load_tile(1, w, xb);
 #pragma unroll
       for (int r = 0; r < 16; ++r) sa[r] = xa[r];
-      row_forward(sa, xs, ta, tb, P, [] {});
+      row_forward(sa, xs, T, P, [] {});
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         sa[r] = reduce(sa[r], P.p, P.pinv);
         v[r] = xb[r];
       }
-      row_forward(v, xs, ta, tb, P, [&] {
+      row_forward(v, xs, T, P, [&] {
         if (wn < ntiles) load_tile(0, wn, xa);
       });
 #pragma unroll
@@ -336,7 +370,7 @@ __global__ __launch_bounds__(kRowNTH) void k_ntt_row_fa(PassArgs a) {
         v[r] = mm_shoup(x, P.ninv.x, P.ninv.y, P.p);
       }
     }
-    row_inverse_store(v, xs, ta, tb, P, dst + (w << kBigLogE));
+    row_inverse_store(v, xs, T, P, dst + (w << kBigLogE));
     w = wn;
     if (w >= ntiles) break;
   }
