@@ -260,6 +260,56 @@ __device__ __forceinline__ void big_last(const double* lds, double (&y)[kBigNBL]
   big_compute<GE, 4, LOGL - 4>(y, ta, tb, P, inv);
 }
 
+// The last stage of a forward line DFT with its twiddles omega_L^e (e =
+// jm r < L) from a table th[n] = omega_{2L}^n, n < L: omega_L^e =
+// omega_{2L}^{2e}, and for 2e >= L it is -th[2e - L].  One table read and
+// one modular product per twiddle instead of the two factors of big_last.
+template <class GE>
+__device__ __forceinline__ void big_last_th(const double* lds, double (&y)[kBigNBL][16], const double* th,
+                                            const Prime& P) {
+  constexpr int L = GE::L;
+  constexpr int LOGLR = __builtin_ctz(L) - 4;
+  constexpr int NS = L / 16;
+  big_read<GE, 4>(y, lds);
+  int lam, j;
+  GE::split(big_tid(), LOGLR, lam, j);
+  const int jm = j & (NS - 1);
+  y[0][0] = reduce(y[0][0], P.p, P.pinv);
+#pragma unroll
+  for (int r = 1; r < 16; ++r) {
+    const int e2 = 2 * jm * r;  // < 2L
+    const double t = th[e2 & (L - 1)];
+    const double w = (e2 & L) ? -t : t;
+    y[0][r] = mm_shoup(y[0][r], w, w * P.pinv, P.p);
+  }
+  dft16(y[0], P, false);
+}
+
+// The last stage of a line DFT (either direction) with its twiddles
+// omega_L^{+-e} (e = jm r < L) from a table th[n] = omega_L^n, n < L / 2:
+// omega_L^{n + L/2} = -omega_L^n.
+template <class GE>
+__device__ __forceinline__ void big_last_half(const double* lds, double (&y)[kBigNBL][16], const double* th,
+                                              const Prime& P, bool inv) {
+  constexpr int L = GE::L;
+  constexpr int LOGLR = __builtin_ctz(L) - 4;
+  constexpr int NS = L / 16;
+  big_read<GE, 4>(y, lds);
+  int lam, j;
+  GE::split(big_tid(), LOGLR, lam, j);
+  const int jm = j & (NS - 1);
+  y[0][0] = reduce(y[0][0], P.p, P.pinv);
+#pragma unroll
+  for (int r = 1; r < 16; ++r) {
+    int e = jm * r;  // < L
+    if (inv) e = (L - e) & (L - 1);
+    const double t = th[e & (L / 2 - 1)];
+    const double w = (e & (L / 2)) ? -t : t;
+    y[0][r] = mm_shoup(y[0][r], w, w * P.pinv, P.p);
+  }
+  dft16(y[0], P, inv);
+}
+
 // Fills the stage tables: ta[e] = omega_{L/16}^e (e < L/16; the inverse
 // direction negates the index), tb[e] = omega_L^e and tb[256 + e] =
 // omega_L^{-e} (e < 256).  Call before the first barrier.
@@ -665,7 +715,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs(PassArgs a) {
       if (s == 0 && wload < ntiles) load_tile(wload, xn);
       big_middle<GE, LOGR1>(lds, ta, tb, P, false);
       double y[kBigNBL][16];
-      big_last<GE>(lds, y, ta, tb, P, false);
+      big_last_th<GE>(lds, y, tp, P);  // tp[n] = omega_L^n = omega_{2 LH}^n
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const double x = mm_shoup(y[0][r], cur.x, cur.y, P.p);
@@ -734,10 +784,11 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col16_inv(PassArgs a) {
   static_assert(kBigEPT == 16 && kBigNBL == 1, "16 points per thread, one last-stage butterfly");
   __shared__ double lds[GE::LDS];
   __shared__ BigTw ta[1 << LOGTA];
-  __shared__ double tb[2 * kBigTB];
+  __shared__ double th[GE::L / 2];  // omega_L^n, n < L / 2 (last stage)
   const int pi = blockIdx.y;
   const Prime P = a.prm[pi];
-  big_tables<GE>(ta, tb, P);
+  for (int e = threadIdx.x; e < (1 << LOGTA); e += blockDim.x) ta[e] = P.twL[e << (kLogE - LOGTA)].x;
+  for (int e = threadIdx.x; e < GE::L / 2; e += blockDim.x) th[e] = P.twL[e << (kLogE - LOGL)].x;
   const int logS = a.logS;
   const uint64_t zoff = a.descs ? ((uint64_t)blockIdx.z << a.zshift) * a.zstride : 0;
   const double* src = a.src + zoff + pi * a.N;
@@ -816,14 +867,14 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col16_inv(PassArgs a) {
         cur = make_double2(wv, wv * P.pinv);
       }
     }
-    big_compute<GE, LOGR1, 0>(v1, ta, tb, P, true);
+    big_compute<GE, LOGR1, 0>(v1, ta, th, P, true);
     __syncthreads();  // the previous half's last-stage reads of LDS are done
     big_write<GE, LOGR1, 0>(v1, lds);
     __syncthreads();
     if (next) inv_twiddles(hbase + 8, icur, istp);
-    big_middle<GE, LOGR1>(lds, ta, tb, P, true);
+    big_middle<GE, LOGR1>(lds, ta, th, P, true);  // single-table stages (th unused)
     double y[kBigNBL][16];
-    big_last<GE>(lds, y, ta, tb, P, true);
+    big_last_half<GE>(lds, y, th, P, true);
 #pragma unroll
     for (int r = 0; r < 16; ++r) out[r] = reduce(y[0][r], P.p, P.pinv);
   };
