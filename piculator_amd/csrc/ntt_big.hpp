@@ -23,7 +23,6 @@ namespace pcg {
 
 constexpr int kBigLogE = 14;
 constexpr int kBigE = 1 << kBigLogE;
-constexpr int kBigTB = 256;  // entries of the last-stage table
 // 16 points per thread (1024 threads, at most 128 registers each); 32 points
 // per thread (512 threads) measured 19% slower at 2^25 points
 constexpr int kBigEPT = 16;
@@ -119,27 +118,16 @@ __device__ __forceinline__ int big_tid() {
   return t;
 }
 
-// Twiddle omega_{2^(LOGR+LOGNS)}^{e}, e = jm r, for a stage of the line DFT.
+// Twiddle omega_{2^(LOGR+LOGNS)}^{e}, e = jm r, for a stage of the line DFT
+// before the last one, from the table ta[e] = omega_{L/16}^e.
 template <int LOGL, int LOGR, int LOGNS>
-__device__ __forceinline__ double2 big_stage_tw(const BigTw* ta, const double* tb, int jm, int r,
-                                                const Prime& P, bool inv) {
+__device__ __forceinline__ double2 big_stage_tw(const BigTw* ta, int jm, int r, const Prime& P, bool inv) {
   constexpr int LOGTA = LOGL - 4;  // table omega_{L/16}
-  if constexpr (LOGR + LOGNS <= LOGTA) {
-    constexpr int SH = LOGTA - LOGR - LOGNS;
-    int e = (jm * r) << SH;
-    if (inv) e = ((1 << LOGTA) - e) & ((1 << LOGTA) - 1);
-    return tw_of(ta, e, P);
-  } else {
-    // last stage (LOGR + LOGNS == LOGL): jm = 16 a + b
-    static_assert(LOGR + LOGNS == LOGL, "stage beyond the line");
-    int ea = (jm >> 4) * r;        // < L / 16
-    const int eb = (jm & 15) * r;  // < 256
-    if (inv) ea = ((1 << LOGTA) - ea) & ((1 << LOGTA) - 1);
-    const double2 wa = tw_of(ta, ea, P);
-    const double wb = tb[(inv ? kBigTB : 0) + eb];  // omega_L^{+-eb}
-    const double w = mm_shoup(wb, wa.x, wa.y, P.p);
-    return make_double2(w, w * P.pinv);
-  }
+  static_assert(LOGR + LOGNS <= LOGTA, "stages before the last one only");
+  constexpr int SH = LOGTA - LOGR - LOGNS;
+  int e = (jm * r) << SH;
+  if (inv) e = ((1 << LOGTA) - e) & ((1 << LOGTA) - 1);
+  return tw_of(ta, e, P);
 }
 
 // Inputs of butterfly b of this thread in a stage of radix 2^LOGR: line lam,
@@ -152,10 +140,11 @@ __device__ __forceinline__ void big_pos(int t, int b, int r, int& lam, int& n) {
   n = j + (r << LOGLR);
 }
 
-// Twiddles and DFTs of a stage on the registers v (butterfly b, input r).
+// Twiddles and DFTs of a stage before the last one on the registers v
+// (butterfly b, input r).
 template <class GE, int LOGR, int LOGNS>
 __device__ __forceinline__ void big_compute(double (&v)[kBigEPT >> LOGR][1 << LOGR], const BigTw* ta,
-                                            const double* tb, const Prime& P, bool inv) {
+                                            const Prime& P, bool inv) {
   constexpr int R = 1 << LOGR;
   constexpr int LOGLR = __builtin_ctz(GE::L) - LOGR;
   constexpr int NS = 1 << LOGNS;
@@ -168,26 +157,10 @@ __device__ __forceinline__ void big_compute(double (&v)[kBigEPT >> LOGR][1 << LO
       const int jm = j & (NS - 1);
       v[b][0] = reduce(v[b][0], P.p, P.pinv);
       constexpr int LOGL = __builtin_ctz(GE::L);
-      constexpr int LOGTA = LOGL - 4;
-      if constexpr (LOGR + LOGNS <= LOGTA) {
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
-          const double2 t = big_stage_tw<LOGL, LOGR, LOGNS>(ta, tb, jm, r, P, inv);
-          v[b][r] = mm_shoup(v[b][r], t.x, t.y, P.p);
-        }
-      } else {
-        // last stage: omega_L^{jm r} = omega_{L/16}^{a r} omega_L^{b r}, jm = 16 a + b,
-        // applied as two factors (the same two modular products as forming
-        // the twiddle first, with fewer values live)
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          int ea = (jm >> 4) * r;
-          if (inv) ea = ((1 << LOGTA) - ea) & ((1 << LOGTA) - 1);
-          const double wb = tb[(inv ? kBigTB : 0) + (jm & 15) * r];
-          const double2 wa = tw_of(ta, ea, P);
-          const double x = mm_shoup(v[b][r], wb, wb * P.pinv, P.p);
-          v[b][r] = mm_shoup(x, wa.x, wa.y, P.p);
-        }
+      for (int r = 1; r < R; ++r) {
+        const double2 t = big_stage_tw<LOGL, LOGR, LOGNS>(ta, jm, r, P, inv);
+        v[b][r] = mm_shoup(v[b][r], t.x, t.y, P.p);
       }
     }
     big_dft<R>(v[b], P, inv);
@@ -228,11 +201,10 @@ __device__ __forceinline__ void big_read(double (&v)[kBigEPT >> LOGR][1 << LOGR]
 
 // A whole stage in LDS (read, compute, barrier, write, barrier).
 template <class GE, int LOGR, int LOGNS>
-__device__ __forceinline__ void big_stage(double* lds, const BigTw* ta, const double* tb,
-                                          const Prime& P, bool inv) {
+__device__ __forceinline__ void big_stage(double* lds, const BigTw* ta, const Prime& P, bool inv) {
   double v[kBigEPT >> LOGR][1 << LOGR];
   big_read<GE, LOGR>(v, lds);
-  big_compute<GE, LOGR, LOGNS>(v, ta, tb, P, inv);
+  big_compute<GE, LOGR, LOGNS>(v, ta, P, inv);
   __syncthreads();
   big_write<GE, LOGR, LOGNS>(v, lds);
   __syncthreads();
@@ -240,25 +212,19 @@ __device__ __forceinline__ void big_stage(double* lds, const BigTw* ta, const do
 
 // Radix-16 stages from Ns = 2^LOGNS up to (not including) the last one.
 template <class GE, int LOGNS>
-__device__ __forceinline__ void big_middle(double* lds, const BigTw* ta, const double* tb,
-                                           const Prime& P, bool inv) {
+__device__ __forceinline__ void big_middle(double* lds, const BigTw* ta, const Prime& P, bool inv) {
   if constexpr (LOGNS + 4 < __builtin_ctz(GE::L)) {
-    big_stage<GE, 4, LOGNS>(lds, ta, tb, P, inv);
-    big_middle<GE, LOGNS + 4>(lds, ta, tb, P, inv);
+    big_stage<GE, 4, LOGNS>(lds, ta, P, inv);
+    big_middle<GE, LOGNS + 4>(lds, ta, P, inv);
   }
 }
 
 constexpr int kBigNBL = kBigEPT / 16;  // last-stage butterflies per thread
 
-// The last (radix-16) stage: LDS -> registers; y[b][r] is output point
-// j_b + r L / 16 of butterfly b's line (the place its input r came from).
-template <class GE>
-__device__ __forceinline__ void big_last(const double* lds, double (&y)[kBigNBL][16], const BigTw* ta,
-                                         const double* tb, const Prime& P, bool inv) {
-  constexpr int LOGL = __builtin_ctz(GE::L);
-  big_read<GE, 4>(y, lds);
-  big_compute<GE, 4, LOGL - 4>(y, ta, tb, P, inv);
-}
+// The last (radix-16) stage reads LDS into registers: y[b][r] is output
+// point j_b + r L / 16 of butterfly b's line (the place its input r came
+// from).  Its twiddles omega_L^{jm r} take a table of omega_L or
+// omega_{2L} (big_last_half, big_last_th below).
 
 // The last stage of a forward line DFT with its twiddles omega_L^e (e =
 // jm r < L) from a table th[n] = omega_{2L}^n, n < L: omega_L^e =
@@ -310,20 +276,19 @@ __device__ __forceinline__ void big_last_half(const double* lds, double (&y)[kBi
   dft16(y[0], P, inv);
 }
 
-// Fills the stage tables: ta[e] = omega_{L/16}^e (e < L/16; the inverse
-// direction negates the index), tb[e] = omega_L^e and tb[256 + e] =
-// omega_L^{-e} (e < 256).  Call before the first barrier.
+// Fills the stage tables: ta[e] = omega_{L/16}^e (e < L/16, the stages
+// before the last; the inverse direction negates the index) and, when th
+// is given, th[n] = omega_L^n (n < L/2, the last stage).  Call before the
+// first barrier.
 template <class GE>
-__device__ __forceinline__ void big_tables(BigTw* ta, double* tb, const Prime& P) {
+__device__ __forceinline__ void big_tables(BigTw* ta, double* th, const Prime& P) {
   constexpr int LOGL = __builtin_ctz(GE::L);
   constexpr int LOGTA = LOGL - 4;
-  static_assert(LOGTA >= 1 && LOGTA <= kLogE, "line length out of range");
+  static_assert(LOGTA >= 1 && LOGL <= kLogE, "line length out of range");
   for (int e = threadIdx.x; e < (1 << LOGTA); e += blockDim.x)
     ta[e] = P.twL[e << (kLogE - LOGTA)].x;  // twL = (omega_4096^e, its quotient)
-  for (int e = threadIdx.x; e < 2 * kBigTB; e += blockDim.x) {
-    const double2 w = root_pow((uint64_t)(e & (kBigTB - 1)), LOGL, e >= kBigTB, P);
-    tb[e] = w.x;
-  }
+  if (th)
+    for (int e = threadIdx.x; e < GE::L / 2; e += blockDim.x) th[e] = P.twL[e << (kLogE - LOGL)].x;
 }
 
 template <int MODE, int LOGL>
@@ -336,10 +301,10 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col(PassArgs a) {
   constexpr int LOGTA = LOGL - 4;
   __shared__ double lds[GE::LDS];
   __shared__ BigTw ta[1 << LOGTA];
-  __shared__ double tb[2 * kBigTB];
+  __shared__ double th[GE::L / 2];  // omega_L^n, n < L / 2 (last stage)
   const int pi = blockIdx.y;
   const Prime P = a.prm[pi];
-  big_tables<GE>(ta, tb, P);
+  big_tables<GE>(ta, th, P);
   const int logS = a.logS;
   uint64_t zoff = 0;
   const uint32_t* limbs = a.limbs;
@@ -442,7 +407,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col(PassArgs a) {
       for (int b = 0; b < NB1; ++b)
 #pragma unroll
         for (int r = 0; r < R1; ++r) v1[b][r] = xr[b * R1 + r];
-      big_compute<GE, LOGR1, 0>(v1, ta, tb, P, MODE == kInv);
+      big_compute<GE, LOGR1, 0>(v1, ta, P, MODE == kInv);
       big_write<GE, LOGR1, 0>(v1, lds);
     }
     __syncthreads();
@@ -450,9 +415,9 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col(PassArgs a) {
       load_tile(wload, xr);
       inv_twiddles(wload, icur, istp);
     }
-    big_middle<GE, LOGR1>(lds, ta, tb, P, MODE == kInv);
+    big_middle<GE, LOGR1>(lds, ta, P, MODE == kInv);
     double y[kBigNBL][16];
-    big_last<GE>(lds, y, ta, tb, P, MODE == kInv);
+    big_last_half<GE>(lds, y, th, P, MODE == kInv);
 #pragma unroll
     for (int b = 0; b < kBigNBL; ++b) {
       int lam_o, n_o0;
@@ -621,11 +586,10 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs(PassArgs a) {
   static_assert(kBigNBL == 1, "one last-stage butterfly per thread");
   __shared__ double lds[GE::LDS];
   __shared__ BigTw ta[1 << LOGTA];
-  __shared__ double tb[2 * kBigTB];
-  __shared__ double tp[LH];  // omega_L^n, n < L / 2 (odd rows' pre-twiddle)
+  __shared__ double tp[LH];  // omega_L^n, n < L / 2 (odd rows' pre-twiddle, last stage)
   const int pi = blockIdx.y;
   const Prime P = a.prm[pi];
-  big_tables<GE>(ta, tb, P);
+  big_tables<GE>(ta, nullptr, P);
   for (int e = threadIdx.x; e < LH; e += blockDim.x) tp[e] = root_pow((uint64_t)e, LOGL, false, P).x;
   const int logS = a.logS;
   uint64_t zoff = 0;
@@ -708,12 +672,12 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs(PassArgs a) {
             v1[b][r] = mm_shoup(v1[b][r], wv, wv * P.pinv, P.p);
           }
       }
-      big_compute<GE, LOGR1, 0>(v1, ta, tb, P, false);
+      big_compute<GE, LOGR1, 0>(v1, ta, P, false);
       __syncthreads();  // LDS readers of the previous half-length DFT are done
       big_write<GE, LOGR1, 0>(v1, lds);
       __syncthreads();
       if (s == 0 && wload < ntiles) load_tile(wload, xn);
-      big_middle<GE, LOGR1>(lds, ta, tb, P, false);
+      big_middle<GE, LOGR1>(lds, ta, P, false);
       double y[kBigNBL][16];
       big_last_th<GE>(lds, y, tp, P);  // tp[n] = omega_L^n = omega_{2 LH}^n
 #pragma unroll
@@ -787,8 +751,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col16_inv(PassArgs a) {
   __shared__ double th[GE::L / 2];  // omega_L^n, n < L / 2 (last stage)
   const int pi = blockIdx.y;
   const Prime P = a.prm[pi];
-  for (int e = threadIdx.x; e < (1 << LOGTA); e += blockDim.x) ta[e] = P.twL[e << (kLogE - LOGTA)].x;
-  for (int e = threadIdx.x; e < GE::L / 2; e += blockDim.x) th[e] = P.twL[e << (kLogE - LOGL)].x;
+  big_tables<GE>(ta, th, P);
   const int logS = a.logS;
   const uint64_t zoff = a.descs ? ((uint64_t)blockIdx.z << a.zshift) * a.zstride : 0;
   const double* src = a.src + zoff + pi * a.N;
@@ -867,12 +830,12 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col16_inv(PassArgs a) {
         cur = make_double2(wv, wv * P.pinv);
       }
     }
-    big_compute<GE, LOGR1, 0>(v1, ta, th, P, true);
+    big_compute<GE, LOGR1, 0>(v1, ta, P, true);
     __syncthreads();  // the previous half's last-stage reads of LDS are done
     big_write<GE, LOGR1, 0>(v1, lds);
     __syncthreads();
     if (next) inv_twiddles(hbase + 8, icur, istp);
-    big_middle<GE, LOGR1>(lds, ta, th, P, true);  // single-table stages (th unused)
+    big_middle<GE, LOGR1>(lds, ta, P, true);
     double y[kBigNBL][16];
     big_last_half<GE>(lds, y, th, P, true);
 #pragma unroll
