@@ -1510,27 +1510,46 @@ int tune_small(Ctx& c) {
     }
     const int lg = b < kSmallNttMinLog ? kSmallNttMinLog : b;
     const int cls[7] = {1, 2, 3, 0, kKaraClass, lg, kPassClass + lg};
-    float best_ms = 1e30f;
-    int best_cls = small_class_default(b);
+    // one launch of class m, timed
+    auto time_class = [&](int m, float& ms) {
+      if (hipEventRecord(c.tev[0], c.stream) != hipSuccess) return fail(PCG_ERR_HIP, "event record failed");
+      int r = launch_small_class(c, cls[m], dd, kCount);
+      if (!r && (hipEventRecord(c.tev[1], c.stream) != hipSuccess || hipEventSynchronize(c.tev[1]) != hipSuccess ||
+                 hipEventElapsedTime(&ms, c.tev[0], c.tev[1]) != hipSuccess))
+        r = fail(PCG_ERR_HIP, "autotuner timing failed");
+      return r;
+    };
+    // every valid class: an untimed launch and a timed one; the two fastest
+    // then get two more timed launches each and the best time decides
+    float best[7];
+    int order[7], nvalid = 0;
     for (int m = 0; m < 7 && rc == PCG_OK; ++m) {
+      best[m] = 1e30f;
       if (!small_class_valid(b, cls[m])) continue;
-      float best = 1e30f;
-      for (int rep = 0; rep < 4 && rc == PCG_OK; ++rep) {
-        if (hipEventRecord(c.tev[0], c.stream) != hipSuccess) rc = fail(PCG_ERR_HIP, "event record failed");
-        if (!rc) rc = launch_small_class(c, cls[m], dd, kCount);
-        float ms = 0;
-        if (!rc && (hipEventRecord(c.tev[1], c.stream) != hipSuccess ||
-                    hipEventSynchronize(c.tev[1]) != hipSuccess ||
-                    hipEventElapsedTime(&ms, c.tev[0], c.tev[1]) != hipSuccess))
-          rc = fail(PCG_ERR_HIP, "autotuner timing failed");
-        if (!rc && rep > 0) best = std::min(best, ms);
-      }
-      if (!rc && best < best_ms) {
-        best_ms = best;
-        best_cls = cls[m];
-      }
+      float ms = 0;
+      rc = time_class(m, ms);
+      if (!rc) rc = time_class(m, ms);
+      if (rc) break;
+      best[m] = ms;
+      order[nvalid++] = m;
     }
-    if (!rc) g_small_class[b] = best_cls;
+    if (rc) break;
+    std::sort(order, order + nvalid, [&](int x, int y) { return best[x] < best[y]; });
+    for (int k = 0; k < std::min(nvalid, 2) && rc == PCG_OK; ++k)
+      for (int rep = 0; rep < 2 && rc == PCG_OK; ++rep) {
+        float ms = 0;
+        rc = time_class(order[k], ms);
+        if (!rc) best[order[k]] = std::min(best[order[k]], ms);
+      }
+    if (rc) break;
+    int best_cls = small_class_default(b);
+    float best_ms = 1e30f;
+    for (int k = 0; k < nvalid; ++k)
+      if (best[order[k]] < best_ms) {
+        best_ms = best[order[k]];
+        best_cls = cls[order[k]];
+      }
+    g_small_class[b] = best_cls;
   }
   (void)hipStreamSynchronize(c.stream);
   (void)hipFree(in);
