@@ -57,13 +57,13 @@ __device__ __forceinline__ double reduce(double x, double p, double pinv) {
   return __builtin_fma(-q, p, x);
 }
 
-// |x| < 2^52 -> [0, p)
+// |x| < 2^52 -> [0, p).  x * pinv is within 2^-46 of x / p, so the nearest
+// integer q leaves x - q p in (-p, p) (exact in the fma), and one
+// conditional + p finishes.
 __device__ __forceinline__ double canon(double x, double p, double pinv) {
-  double q = __builtin_floor(x * pinv);
-  double r = __builtin_fma(-q, p, x);
-  r = r < 0.0 ? r + p : r;
-  r = r >= p ? r - p : r;
-  return r;
+  const double q = __builtin_rint(x * pinv);
+  const double r = __builtin_fma(-q, p, x);
+  return r < 0.0 ? r + p : r;
 }
 
 }  // namespace pcg
