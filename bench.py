@@ -256,6 +256,44 @@ def fft_mul(logn: int, steps: int):
     return res
 
 
+def batch_size_sweep(total_limbs: int = 1 << 25, reps: int = 3):
+    """One JSON line per transform size 2^12 .. 2^24: one pcg_mul_batch_dev
+    call of equal device-resident products with na = nb = 2^(s-1) - 1 limbs
+    (the product fills 2^s points), as many as make `total_limbs` operand
+    limbs, best of `reps` after one untimed call; the algorithmic bytes are
+    those of the multiply roofline, 64 N_alg + 8 (na + nb) per product."""
+    import numpy as np
+
+    from piculator_amd import picugpu, workload
+
+    picugpu.init(0)
+    lib = picugpu.load()
+    for s in range(12, 25):
+        n = (1 << (s - 1)) - 1
+        count = max(1, total_limbs // (2 * n))
+        total = count * 2 * n
+        pool = picugpu.DeviceBuffer(8 * total + 64)
+        host = np.zeros(2 * total, dtype=np.uint32)
+        host[:total] = np.random.default_rng(s).integers(0, 2**32, total, dtype=np.uint64).astype(np.uint32)
+        pool.upload(host)
+        descs = (picugpu.MulDesc * count)()
+        for i in range(count):
+            descs[i] = picugpu.MulDesc(pool.ptr + 4 * (2 * n * i), n, pool.ptr + 4 * (2 * n * i + n), n,
+                                       pool.ptr + 4 * (total + 2 * n * i))
+        picugpu._check(lib.pcg_mul_batch_dev(descs, count, 0))
+        best = None
+        for _ in range(reps):
+            picugpu.timer_start()
+            picugpu._check(lib.pcg_mul_batch_dev(descs, count, 0))
+            ms = picugpu.timer_stop()
+            best = ms if best is None else min(best, ms)
+        alg = count * workload.bytes_alg(n, n)
+        print(json.dumps({"points": 1 << s, "limbs": n, "products": count, "ms": round(best, 4),
+                          "alg_GBs": round(alg / best / 1e6, 1), "hbm_frac": round(alg / best / 1e6 / 8000, 3)}),
+              flush=True)
+        pool.free()
+
+
 def class_sweep(count: int = 2048, reps: int = 3):
     """One JSON line per small-product bucket b = 6..12: the time of one
     pcg_mul_batch_dev call of `count` device-resident products of
@@ -530,6 +568,8 @@ def main():
                     help="print the single-multiply table (sizes x shapes x engines) instead")
     ap.add_argument("--class-sweep", action="store_true",
                     help="print the time of every small-product class per bucket instead")
+    ap.add_argument("--batch-sizes", action="store_true",
+                    help="print the batched-product throughput per transform size instead")
     ap.add_argument("--stream-probe", action="store_true",
                     help="print the HBM access-pattern probe table instead")
     ap.add_argument("--summarize-trace", metavar="DIR",
@@ -571,6 +611,9 @@ def main():
         return
     if args.class_sweep:
         class_sweep()
+        return
+    if args.batch_sizes:
+        batch_size_sweep()
         return
 
     from piculator_amd import picugpu, pihost

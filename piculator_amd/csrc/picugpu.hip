@@ -1085,9 +1085,11 @@ int ensure_pool(Ctx& c, size_t limbs, size_t ndesc) {
 // 10..12 the single-workgroup NTT of 2^k points, 110..112 the batched pass
 // kernels at 2^(k - 100) points (spectra through HBM); the autotuner
 // (tune_small) picks among them
-// per bucket on the device at first use.  Classes 13..17 are the multi-pass
-// NTT run for the whole class at once, -1 one product at a time.
-constexpr int kBatchMaxLog = 17;
+// per bucket on the device at first use.  Classes 13..21 are the multi-pass
+// NTT run for the whole class at once (in groups bounded by the batched
+// scratch), -1 one product at a time (from 2^22 points one product's tiles
+// fill the GPU by themselves).
+constexpr int kBatchMaxLog = 21;
 constexpr int kSmallNttMinLog = 10;
 int g_small_class[kSmallNttMaxLog + 1];
 bool g_small_ready = false;
@@ -1568,7 +1570,7 @@ int ensure_tuned(Ctx& c) { return g_small_ready ? PCG_OK : tune_small(c); }
 int batch_locked(Ctx& c, const pcg_mul_desc* d, uint64_t count, uint32_t flags, bool dev) {
   // class slots: 0..3 schoolbook classes 0..3, 4..6 single-workgroup NTT
   // (2^10 .. 2^12), 7..9 batched passes at 2^10 .. 2^12, 10 Karatsuba,
-  // 11..15 batched multi-pass (2^13 .. 2^17)
+  // 11..19 batched multi-pass (2^13 .. 2^21)
   constexpr int kSmallSlots = 11;
   constexpr int kSlots = kSmallSlots + (kBatchMaxLog - kSmallNttMaxLog);
   int rc0 = ensure_tuned(c);

@@ -51,6 +51,32 @@ def test_batch_equals_plain_on_uneven_workload(gpu):
         assert np.array_equal(g, exact_product(a, b))
 
 
+def test_mid_size_products_batched(gpu):
+    """Products of 2^18 .. 2^21 transform points run as batched classes (one
+    launch sequence per size class) and 2^22 as a single product: every
+    output equals the plain one-product-per-launch path, the smallest one the
+    exact product, all of them the product modulo the check primes."""
+    from tests.helpers import residues_match
+
+    rng = np.random.default_rng(2024)
+    sizes = [(70000, 65000), (70001, 64999), (150000, 110000), (300000, 200000), (600000, 400000),
+             (1000000, 900000), (1500000, 1500000)]
+    pairs = []
+    for k, (na, nb) in enumerate(sizes):
+        if k == 1:
+            pairs.append((np.full(na, M32, dtype=np.uint32), np.full(nb, M32, dtype=np.uint32)))
+        else:
+            pairs.append((rng.integers(0, 2**32, na, dtype=np.uint64).astype(np.uint32),
+                          rng.integers(0, 2**32, nb, dtype=np.uint64).astype(np.uint32)))
+    got = gpu.mul_batch(pairs)
+    plain = gpu.mul_batch(pairs, plain=True)
+    for (a, b), g, p in zip(pairs, got, plain):
+        assert np.array_equal(g, p), (a.size, b.size)
+        assert residues_match(a, b, g), (a.size, b.size)
+    for k in (0, 1):
+        assert np.array_equal(got[k], exact_product(*pairs[k]))
+
+
 def test_full_uneven_batch_4096(gpu):
     pairs = workload.batch_operands(4096)
     lens = workload.batch_lengths(4096)
