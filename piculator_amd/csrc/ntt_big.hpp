@@ -118,6 +118,15 @@ __device__ __forceinline__ int big_tid() {
   return t;
 }
 
+// Orders this wave's LDS accesses before and after it (the wave's LDS
+// operations execute in order, so no workgroup barrier is needed for an
+// exchange between its own lanes).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Twiddle omega_{2^(LOGR+LOGNS)}^{e}, e = jm r, for a stage of the line DFT
 // before the last one, from the table ta[e] = omega_{L/16}^e.
 template <int LOGL, int LOGR, int LOGNS>
@@ -705,6 +714,215 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs(PassArgs a) {
     if (w >= ntiles) break;
 #pragma unroll
     for (int i = 0; i < kBigEPT; ++i) xc[i] = xn[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// First forward pass from the limbs for 2^11-point columns with one
+// workgroup-wide exchange per half-length DFT (k_ntt_col_limbs<11> takes
+// three stages through LDS, four workgroup barriers each).
+//
+// A tile is 16 adjacent columns cl of 1024 rows (the radix-2 split of
+// k_ntt_col_limbs).  Each half-length DFT of 1024 points runs as
+//   S1: DFT16 over jj for each m (n = m + 64 jj), twiddle omega_1024^{k1 m}
+//   S2: DFT16 over i for each (k1, m0) (m = m0 + 4 i), twiddle omega_64^{k2 m0}
+//   S3: DFT4 over m0 for each (k1, k2)
+// giving Z[k1 + 16 k2 + 256 k3].  Thread t = 64 w + l runs S1 on column
+// t & 15, m = t >> 4 (its 16 loads: 4 rows x 16 columns per wave access);
+// one exchange between two workgroup barriers gives wave w all of k1 = w
+// for the 16 columns (lane l: column l & 15, m0 = l >> 4), and the
+// exchange before S3 stays inside the wave's own 1024 LDS doubles (ordered
+// by its in-order LDS queue).  After S3 lane l holds, for its column,
+// k2 = 4 (l >> 4) + s' and k3 = 0 .. 3: every store covers 4 rows x 16
+// columns (128-byte lines).  All LDS layouts are conflict-free for each
+// 32-lane half of a 64-bit access.
+// Global-memory views of pointers the compiler cannot place itself (they
+// come from argument structs or descriptors): loads through them are global
+// loads, which wait on the vector-memory counter only (flat loads also
+// hold up the LDS counter).
+typedef const __attribute__((address_space(1))) uint32_t GlobalU32;
+
+__device__ __forceinline__ double col11_tp(const double* tp, int e2) {
+  // omega_1024^e = omega_2048^{2e} from tp[n] = omega_2048^n (n < 1024):
+  // e2 = 2e < 2048, the upper half negated
+  const double t = tp[e2 & 1023];
+  return (e2 & 1024) ? -t : t;
+}
+
+__global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs11w(PassArgs a) {
+  constexpr int LOGL = 11;
+  constexpr int LH = 1 << (LOGL - 1);  // 1024 rows per tile
+  __shared__ double xs[kBigE];
+  __shared__ double tp[LH];      // omega_2048^n: the odd half's pre-twiddle, S1 twiddles
+  __shared__ BigTw t64[64];      // omega_64^e: S2 twiddles
+  const int pi = blockIdx.y;
+  const Prime P = a.prm[pi];
+  for (int e = threadIdx.x; e < LH; e += blockDim.x) tp[e] = root_pow((uint64_t)e, LOGL, false, P).x;
+  for (int e = threadIdx.x; e < 64; e += blockDim.x) t64[e] = P.twL[e << (kLogE - 6)].x;
+  const int logS = a.logS;
+  uint64_t zoff = 0;
+  const uint32_t* limbs = a.limbs;
+  uint64_t nlimbs = a.nlimbs;
+  if (a.descs) {
+    const uint64_t zi = (uint64_t)blockIdx.z << a.zshift;
+    const BatchDesc d = a.descs[zi];
+    zoff = zi * a.zstride;
+    limbs = a.operand ? d.b : d.a;
+    nlimbs = a.operand ? d.nb : d.na;
+  }
+  double* dst = a.dst + zoff + pi * a.N;
+  const int logRoot = LOGL + logS;
+  const uint64_t ntiles = a.ntiles;            // tiles of 16 columns
+  const uint64_t half = (uint64_t)LH << logS;  // N / 2: the upper rows' offset
+  const bool full = nlimbs > half;
+
+  auto tile_base = [&](uint64_t t) -> uint64_t {
+    const int lgc = logS - 4;
+    return ((t >> lgc) << (LOGL + logS)) + ((t & ((1ull << lgc) - 1)) << 4);
+  };
+  // S1 inputs of this thread: rows m + 64 jj of column t & 15.  A tile
+  // wholly inside the operand loads them plainly (all 16 in flight
+  // together); else limbs past the operand read as zero (the clamped load
+  // stays in bounds).  The limbs are global memory: global loads wait on
+  // the vector-memory counter only.
+  const GlobalU32* lp = (const GlobalU32*)limbs;  // address-space cast
+  // rows m + 64 jj (jj < 16) of this thread's column, the first at g0 in a
+  // block of rows whose last point is `last`
+  auto load_rows = [&](uint64_t g0, uint64_t last, uint32_t (&x)[16]) {
+    if (last < nlimbs) {
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) x[jj] = lp[g0 + ((uint64_t)jj << (logS + 6))];
+    } else {
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) {
+        const uint64_t g = g0 + ((uint64_t)jj << (logS + 6));
+        const uint32_t v = lp[g < nlimbs ? g : nlimbs - 1];
+        x[jj] = g < nlimbs ? v : 0u;
+      }
+    }
+  };
+  auto load_tile = [&](uint64_t t, uint32_t (&x)[16]) {
+    const uint64_t b0 = tile_base(t);
+    const int bt = big_tid();
+    load_rows(b0 + ((uint64_t)(bt >> 4) << logS) + (bt & 15), b0 + ((uint64_t)(LH - 1) << logS) + 15, x);
+  };
+
+  auto tile_step = [&](const uint64_t w, const uint32_t (&xc)[16], uint32_t (&xn)[16], const uint64_t wload) {
+    const uint64_t base = tile_base(w);
+#pragma unroll 1
+    for (int s = 0; s < 2; ++s) {
+      double v[16];
+      {
+        const int bt = big_tid();
+        const int m = bt >> 4;
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) v[jj] = (double)xc[jj];
+        if (full) {
+          uint32_t uh[16];
+          load_rows(base + ((uint64_t)m << logS) + (bt & 15) + half, base + ((uint64_t)(LH - 1) << logS) + 15 + half,
+                    uh);
+#pragma unroll
+          for (int jj = 0; jj < 16; ++jj) v[jj] = s == 0 ? v[jj] + (double)uh[jj] : v[jj] - (double)uh[jj];
+        }
+        if (s == 1) {
+#pragma unroll
+          for (int jj = 0; jj < 16; ++jj) {
+            const double wv = tp[m + 64 * jj];
+            v[jj] = mm_shoup(v[jj], wv, wv * P.pinv, P.p);
+          }
+        }
+        // S1 and its twiddles omega_1024^{k1 m}
+        dft16(v, P, false);
+        v[0] = reduce(v[0], P.p, P.pinv);
+#pragma unroll
+        for (int k1 = 1; k1 < 16; ++k1) {
+          const double wv = col11_tp(tp, 2 * m * k1);
+          v[k1] = mm_shoup(v[k1], wv, wv * P.pinv, P.p);
+        }
+      }
+      __syncthreads();  // every wave is done with the LDS (previous half)
+      {
+        double* xw = xs + big_tid();  // [k1][m][cl]
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) xw[k1 << 10] = v[k1];
+      }
+      __syncthreads();
+      if (s == 0 && wload < ntiles) load_tile(wload, xn);
+      const int bt = big_tid();
+      const int wv_ = bt >> 6, lane = bt & 63;
+      double* R = xs + (wv_ << 10);  // this wave's 1024 doubles: k1 = wave
+      {
+        // S2 over i (m = m0 + 4 i), twiddles omega_64^{k2 m0}
+        const int m0 = lane >> 4;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = R[lane + 64 * i];
+        dft16(v, P, false);
+        v[0] = reduce(v[0], P.p, P.pinv);
+#pragma unroll
+        for (int k2 = 1; k2 < 16; ++k2) {
+          const double2 t = tw_of(t64, m0 * k2, P);
+          v[k2] = mm_shoup(v[k2], t.x, t.y, P.p);
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) R[k2 * 64 + ((m0 ^ (k2 >> 2)) << 4) + (lane & 15)] = v[k2];
+        wave_lds_sync();
+        // S3: lane (column lane & 15, q = lane >> 4) takes k2 = 4 q + s'
+        const int q = lane >> 4;
+#pragma unroll
+        for (int sp = 0; sp < 4; ++sp) {
+#pragma unroll
+          for (int mm = 0; mm < 4; ++mm) v[4 * sp + mm] = R[(4 * q + sp) * 64 + ((mm ^ q) << 4) + (lane & 15)];
+          dft_core<4>(v + 4 * sp, P, false);
+        }
+      }
+      // v[4 s' + k3] is Z[j] of column lane & 15, j = wave + 16 (4 q + s') +
+      // 256 k3, i.e. output row k = 2 j + s; the inter-pass twiddle
+      // omega^{c k} along s' (step omega^{32 c}) and k3 (step omega^{512 c})
+      {
+        const int q = lane >> 4;
+        const uint64_t c = (base & ((1ull << logS) - 1)) + (uint64_t)(lane & 15);
+        const uint64_t k0 = (uint64_t)(2 * wv_ + 128 * q + s);
+        const double2 st = root_pow(c * 32, logRoot, false, P);
+        const double2 sk = root_pow(c * 512, logRoot, false, P);
+        double2 row = root_pow(c * k0, logRoot, false, P);
+        double* d0 = dst + base + (k0 << logS) + (uint64_t)(lane & 15);
+#pragma unroll
+        for (int k3 = 0; k3 < 4; ++k3) {
+          double2 cur = row;
+#pragma unroll
+          for (int sp = 0; sp < 4; ++sp) {
+            const double x = mm_shoup(v[4 * sp + k3], cur.x, cur.y, P.p);
+            d0[(uint64_t)(32 * sp + 512 * k3) << logS] = x;
+            if (sp < 3) {
+              const double wn = mm_shoup(cur.x, st.x, st.y, P.p);
+              cur = make_double2(wn, wn * P.pinv);
+            }
+          }
+          if (k3 < 3) {
+            const double wn = mm_shoup(row.x, sk.x, sk.y, P.p);
+            row = make_double2(wn, wn * P.pinv);
+          }
+        }
+      }
+    }
+  };
+
+  // tile order as in k_ntt_col (XCD by XCD)
+  const uint64_t g = gridDim.x;
+  uint64_t w = blockIdx.x;
+  if ((g & 7) == 0) w = (uint64_t)(blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
+  if (w >= ntiles) return;
+  uint32_t xc[16];
+  load_tile(w, xc);
+  __syncthreads();  // tables
+  for (;;) {
+    uint32_t xn[16];
+    tile_step(w, xc, xn, w + g);
+    w += g;
+    if (w >= ntiles) break;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) xc[i] = xn[i];
   }
 }
 

@@ -37,15 +37,6 @@ namespace pcg {
 constexpr int kRowNTH = kBigNTH;  // 1024 threads x 16 points
 static_assert(kBigEPT == 16, "the row pass holds 16 points per thread");
 
-// Orders this wave's LDS accesses before and after it (the wave's LDS
-// operations execute in order, so no workgroup barrier is needed for an
-// exchange between its own lanes).
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Stage-twiddle tables in LDS (i = 0 forward, 1 inverse: exponent sign s =
 // +1, -1).  Each stage's table is laid out by (k, lane index) so that the
 // lanes of one read touch consecutive or equal entries (no bank conflicts;

@@ -298,7 +298,7 @@ PassFn big_col_limbs_fn(int logL) {
     case 8: return &k_ntt_col_limbs<8>;
     case 9: return &k_ntt_col_limbs<9>;
     case 10: return &k_ntt_col_limbs<10>;
-    case 11: return &k_ntt_col_limbs<11>;
+    case 11: return &k_ntt_col_limbs11w;
   }
   return nullptr;
 }
