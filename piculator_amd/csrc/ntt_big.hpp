@@ -796,7 +796,9 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs11w(PassArgs a) {
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) {
         const uint64_t g = g0 + ((uint64_t)jj << (logS + 6));
-        const uint32_t v = lp[g < nlimbs ? g : nlimbs - 1];
+        // past the end: the last limb, or limb 0 of an empty operand (the
+        // empty top products of wrapped batches keep their operand pointer)
+        const uint32_t v = lp[g < nlimbs ? g : (nlimbs ? nlimbs - 1 : 0)];
         x[jj] = g < nlimbs ? v : 0u;
       }
     }
@@ -926,54 +928,62 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs11w(PassArgs a) {
   }
 }
 
-// ---------------------------------------------------------------------------
-// Inverse 2^11-point column pass over 16 adjacent columns (the last inverse
-// pass of a 2^25-point transform).
-//
-// A 2^14-point LDS tile holds 8 columns of 2^11 points, so k_ntt_col moves
-// 64-byte runs, half a 128-byte line per access.  This pass takes 16
-// adjacent columns per workgroup and runs them as two 8-column halves, one
-// after the other, through the stages of k_ntt_col<kInv, 11>; every global
-// access covers 4 rows x 16 columns (whole lines).  The lanes l and l ^ 8
-// hold adjacent rows of the same column in the stage layout, so each pair
-// of accesses is followed (loads) or preceded (stores) by one exchange
-// between them (a DPP row rotation by 8 lanes): lane l with bit 3 clear
-// keeps the even row's value of half 0 and trades its odd-row value for its
-// partner's even-row value of half 1.  Half 1's inputs wait in registers
-// while half 0 runs, and half 0's outputs while half 1 runs, so tiles are
-// not prefetched across tiles.  (The forward column pass keeps k_ntt_col:
-// its inputs are 4-byte limbs, and without its cross-tile prefetch it
-// measured slower.)
-
 constexpr int kCol16LogL = 11;
 
-// x from lane (lane ^ 8): within a 16-lane DPP row, rotation by 8 lanes
-__device__ __forceinline__ double lane_xor8(double x) {
-  const uint64_t u = (uint64_t)__double_as_longlong(x);
-  int lo = (int)(uint32_t)u, hi = (int)(uint32_t)(u >> 32);
-  lo = __builtin_amdgcn_update_dpp(0, lo, 0x128, 0xf, 0xf, false);  // row_ror:8
-  hi = __builtin_amdgcn_update_dpp(0, hi, 0x128, 0xf, 0xf, false);
-  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+// ---------------------------------------------------------------------------
+// Inverse 2^11-point column pass over 16 adjacent columns with one
+// workgroup-wide exchange per half-length DFT: the stages of
+// k_ntt_col_limbs11w run backwards in direction (inverse DFTs, conjugate
+// twiddles).  Its tiles are 16 adjacent columns (k_ntt_col<kInv, 11> takes
+// 8 per 2^14-point tile and moves 64-byte runs, half a 128-byte line per
+// access; it also takes three stages through LDS, four workgroup barriers
+// each).
+//
+// Column c of a tile holds x[n], n < 2048.  Its inverse DFT with the
+// inter-pass twiddle omega_N^{-c n} splits (decimation in frequency) into
+// two 1024-point inverse DFTs: output row 2 k + s is
+//   sum_{n < 1024} omega_1024^{-n k} y_s[n],
+//   y_s[n] = omega_N^{-(c + s S) n} (x[n] + (-1)^s u[n]),
+//   u[n] = omega_N^{-1024 c} x[n + 1024],
+// with S = 2^logS (omega_2048 = omega_N^S).  Thread t = 64 w + l loads rows
+// m + 64 jj and m + 64 jj + 1024 of column t & 15 (m = t >> 4; each wave
+// access is 4 rows x 16 columns, whole 128-byte lines) and forms y_0 and
+// y_1 in registers.  Each y_s then runs S1, the workgroup exchange, S2 and
+// S3 of k_ntt_col_limbs11w, and lane l stores its column's rows reduced.
+// The next tile's low rows are loaded after half 0's exchange and its high
+// rows after half 1's, so the loads overlap the wave-local stages.  The
+// global-address-space views keep the loads and stores off the LDS
+// counter: a flat access in flight would be waited for at every LDS wait.
+typedef const __attribute__((address_space(1))) double GlobalF64c;
+typedef __attribute__((address_space(1))) double GlobalF64;
+template <bool B>
+struct BoolTag {
+  static constexpr bool value = B;
+};
+
+// omega_1024^{-e} from t[n] = omega_1024^n, n < 512 (omega_1024^{n + 512}
+// = -omega_1024^n)
+__device__ __forceinline__ double w1024_conj(const double* t, int e) {
+  const int f = (-e) & 1023;
+  const double w = t[f & 511];
+  return (f & 512) ? -w : w;
 }
 
-__global__ __launch_bounds__(kBigNTH) void k_ntt_col16_inv(PassArgs a) {
+__global__ __launch_bounds__(kBigNTH) void k_ntt_col16_invw(PassArgs a) {
   constexpr int LOGL = kCol16LogL;
-  using GE = BigGeo<LOGL>;  // one half: 8 adjacent columns
-  constexpr int LOGR1 = GE::LOGR1;
-  constexpr int R1 = 1 << LOGR1;
-  constexpr int NB1 = kBigEPT / R1;
-  constexpr int LOGTA = LOGL - 4;
-  static_assert(kBigEPT == 16 && kBigNBL == 1, "16 points per thread, one last-stage butterfly");
-  __shared__ double lds[GE::LDS];
-  __shared__ BigTw ta[1 << LOGTA];
-  __shared__ double th[GE::L / 2];  // omega_L^n, n < L / 2 (last stage)
+  static_assert(LOGL == 11 && kBigE == (1 << 14) && kBigNTH == 1024, "tiles of 16 columns x 2048 rows");
+  __shared__ double xs[kBigE];
+  __shared__ double t1k[512];  // omega_1024^n: S1 twiddles
+  __shared__ BigTw t64[64];    // omega_64^e: S2 twiddles
   const int pi = blockIdx.y;
   const Prime P = a.prm[pi];
-  big_tables<GE>(ta, th, P);
+  for (int e = threadIdx.x; e < 512; e += blockDim.x) t1k[e] = P.twL[e << (kLogE - 10)].x;
+  for (int e = threadIdx.x; e < 64; e += blockDim.x) t64[e] = P.twL[e << (kLogE - 6)].x;
   const int logS = a.logS;
+  const uint64_t S = 1ull << logS;
   const uint64_t zoff = a.descs ? ((uint64_t)blockIdx.z << a.zshift) * a.zstride : 0;
-  const double* src = a.src + zoff + pi * a.N;
-  double* dst = a.dst + zoff + pi * a.N;
+  GlobalF64c* src = (GlobalF64c*)(a.src + zoff + pi * a.N);  // address-space casts
+  GlobalF64* dst = (GlobalF64*)(a.dst + zoff + pi * a.N);
   const int logRoot = LOGL + logS;
   const uint64_t ntiles = a.ntiles;  // tiles of 16 columns
 
@@ -981,99 +991,139 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col16_inv(PassArgs a) {
     const int lgc = logS - 4;
     return ((t >> lgc) << (LOGL + logS)) + ((t & ((1ull << lgc) - 1)) << 4);
   };
-  // pre-twiddle bases of a half's first-stage butterflies (as k_ntt_col)
-  auto inv_twiddles = [&](uint64_t hbase, double2 (&cur)[NB1], double2 (&stp)[NB1]) {
-    const int bt = big_tid();
-#pragma unroll
-    for (int b = 0; b < NB1; ++b) {
-      int lam, n0;
-      big_pos<GE, LOGR1>(bt, b, 0, lam, n0);
-      const uint64_t c = (hbase & ((1ull << logS) - 1)) + (uint64_t)lam;
-      cur[b] = root_pow(c * (uint64_t)n0, logRoot, true, P);
-      stp[b] = root_pow(c * (uint64_t)(GE::L >> LOGR1), logRoot, true, P);
-    }
+  // Addresses are a wave-uniform base plus one 32-bit byte offset per lane
+  // (a tile spans 2^(11 + logS) <= 2^28 points), the form of the global
+  // accesses with a scalar base: one VGPR of address per thread.
+  auto at = [](GlobalF64c* b, uint32_t boff) -> GlobalF64c* {
+    return (GlobalF64c*)((const __attribute__((address_space(1))) char*)b + boff);
   };
-  // Both halves' first-stage inputs of this thread: h[b R1 + r] is point
-  // n = (t >> 3) + 128 b + 256 r of column 8 h + (t & 7) (big_pos with
-  // R = R1).  Lane pair (l, l ^ 8): rows n0 = n - hb and n0 + 1, with
-  // hb = bit 3 of the lane; each lane loads both rows in its own half hb.
-  auto load_pair = [&](uint64_t b0, double (&h0)[kBigEPT], double (&h1)[kBigEPT]) {
+  // byte offset of row r, column cl in a tile
+  auto lane_off = [&](uint32_t r, int cl) -> uint32_t { return ((r << logS) + (uint32_t)cl) * 8u; };
+  // rows m + 64 jj + 1024 hi (jj < 16) of this thread's column
+  auto load_rows = [&](uint64_t t, int hi, double (&x)[16]) {
     const int bt = big_tid();
-    const int hb = (bt >> 3) & 1;
-    const uint64_t col = (uint64_t)((hb << 3) + (bt & 7));
-    const int n0 = (bt >> 3) & ~1;
+    const uint32_t off = lane_off((uint32_t)(bt >> 4), bt & 15);
+    GlobalF64c* p = src + tile_base(t) + ((uint64_t)hi << (10 + logS));
 #pragma unroll
-    for (int b = 0; b < NB1; ++b)
-#pragma unroll
-      for (int r = 0; r < R1; ++r) {
-        const int n = n0 + (b << 7) + (r << 8);
-        const uint64_t gA = b0 + ((uint64_t)n << logS) + col;
-        const double A = src[gA];
-        const double B = src[gA + (1ull << logS)];
-        const double recv = lane_xor8(hb ? A : B);
-        h0[b * R1 + r] = hb ? recv : A;
-        h1[b * R1 + r] = hb ? B : recv;
-      }
-  };
-  // Both halves' outputs: o_h[r] is point (t >> 3) + 128 r of column
-  // 8 h + (t & 7) (big_pos with R = 16); the same pairing as the loads.
-  auto store_pair = [&](uint64_t b0, const double (&o0)[16], const double (&o1)[16]) {
-    const int bt = big_tid();
-    const int hb = (bt >> 3) & 1;
-    const uint64_t col = (uint64_t)((hb << 3) + (bt & 7));
-    const int n0 = (bt >> 3) & ~1;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const double recv = lane_xor8(hb ? o0[r] : o1[r]);
-      const uint64_t gA = b0 + ((uint64_t)(n0 + (r << 7)) << logS) + col;
-      dst[gA] = hb ? recv : o0[r];
-      dst[gA + (1ull << logS)] = hb ? o1[r] : recv;
-    }
-  };
-  // One 8-column half (columns hbase .. hbase + 7): the stages of
-  // k_ntt_col<kInv, 11> with inputs and outputs in registers; icur/istp
-  // hold this half's pre-twiddle bases, and after the first stage the
-  // next half's (when next).
-  auto half_step = [&](uint64_t hbase, const double (&xin)[kBigEPT], double (&out)[16], double2 (&icur)[NB1],
-                       double2 (&istp)[NB1], bool next) {
-    double v1[NB1][R1];
-#pragma unroll
-    for (int b = 0; b < NB1; ++b) {
-      double2 cur = icur[b];
-      const double2 stp = istp[b];
-#pragma unroll
-      for (int r = 0; r < R1; ++r) {
-        v1[b][r] = mm_shoup(xin[b * R1 + r], cur.x, cur.y, P.p);
-        const double wv = mm_shoup(cur.x, stp.x, stp.y, P.p);
-        cur = make_double2(wv, wv * P.pinv);
-      }
-    }
-    big_compute<GE, LOGR1, 0>(v1, ta, P, true);
-    __syncthreads();  // the previous half's last-stage reads of LDS are done
-    big_write<GE, LOGR1, 0>(v1, lds);
-    __syncthreads();
-    if (next) inv_twiddles(hbase + 8, icur, istp);
-    big_middle<GE, LOGR1>(lds, ta, P, true);
-    double y[kBigNBL][16];
-    big_last_half<GE>(lds, y, th, P, true);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) out[r] = reduce(y[0][r], P.p, P.pinv);
+    for (int jj = 0; jj < 16; ++jj) x[jj] = *at(p + ((uint64_t)jj << (logS + 6)), off);
   };
 
+  // xl, xh: rows m + 64 jj and m + 64 jj + 1024 of tile w, replaced by
+  // y_0 and y_1 (in place); on return (PF), the same rows of tile wn
+  auto tile_step = [&](auto prefetch, const uint64_t w, double (&xl)[16], double (&xh)[16], const uint64_t wn) {
+    constexpr bool PF = decltype(prefetch)::value;
+    const uint64_t base = tile_base(w);
+    {
+      // y_0 (in xl) and y_1 (in xh) with their twiddles
+      // omega_N^{-(c + s S) n} along jj (steps omega_N^{-64 (c + s S)})
+      const int bt = big_tid();
+      const uint64_t m = (uint64_t)(bt >> 4);
+      const uint64_t c = (base & (S - 1)) + (uint64_t)(bt & 15);
+      const double2 th = root_pow(c << 10, logRoot, true, P);
+      const double2 st0 = root_pow(c << 6, logRoot, true, P);
+      const double2 st1 = root_pow((c + S) << 6, logRoot, true, P);
+      double2 c0 = root_pow(c * m, logRoot, true, P);
+      double2 c1 = root_pow((c + S) * m, logRoot, true, P);
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) {
+        const double u = mm_shoup(xh[jj], th.x, th.y, P.p);
+        const double lo = xl[jj];
+        xl[jj] = mm_shoup(lo + u, c0.x, c0.y, P.p);
+        xh[jj] = mm_shoup(lo - u, c1.x, c1.y, P.p);
+        if (jj < 15) {
+          const double w0 = mm_shoup(c0.x, st0.x, st0.y, P.p);
+          const double w1 = mm_shoup(c1.x, st1.x, st1.y, P.p);
+          c0 = make_double2(w0, w0 * P.pinv);
+          c1 = make_double2(w1, w1 * P.pinv);
+        }
+      }
+    }
+    // both halves unrolled: the next tile's rows land in registers that
+    // are dead from there on (a loop would keep xh live through half 0)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      double v[16];
+      {
+        const int m = big_tid() >> 4;
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) v[jj] = s == 0 ? xl[jj] : xh[jj];
+        // S1 and its twiddles omega_1024^{-k1 m}
+        dft16(v, P, true);
+        v[0] = reduce(v[0], P.p, P.pinv);
+#pragma unroll
+        for (int k1 = 1; k1 < 16; ++k1) {
+          const double wv = w1024_conj(t1k, m * k1);
+          v[k1] = mm_shoup(v[k1], wv, wv * P.pinv, P.p);
+        }
+      }
+      __syncthreads();  // every wave is done with the LDS (previous half)
+      {
+        double* xw = xs + big_tid();  // [k1][m][cl]
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) xw[k1 << 10] = v[k1];
+      }
+      __syncthreads();
+      if constexpr (PF) load_rows(wn, s, s == 0 ? xl : xh);
+      const int bt = big_tid();
+      const int wv_ = bt >> 6, lane = bt & 63;
+      double* R = xs + (wv_ << 10);  // this wave's 1024 doubles: k1 = wave
+      {
+        // S2 over i (m = m0 + 4 i), twiddles omega_64^{-k2 m0}
+        const int m0 = lane >> 4;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = R[lane + 64 * i];
+        dft16(v, P, true);
+        v[0] = reduce(v[0], P.p, P.pinv);
+#pragma unroll
+        for (int k2 = 1; k2 < 16; ++k2) {
+          const double2 t = tw_of(t64, (64 - m0 * k2) & 63, P);
+          v[k2] = mm_shoup(v[k2], t.x, t.y, P.p);
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) R[k2 * 64 + ((m0 ^ (k2 >> 2)) << 4) + (lane & 15)] = v[k2];
+        wave_lds_sync();
+        // S3: lane (column lane & 15, q = lane >> 4) takes k2 = 4 q + s'
+        const int q = lane >> 4;
+#pragma unroll
+        for (int sp = 0; sp < 4; ++sp) {
+#pragma unroll
+          for (int mm = 0; mm < 4; ++mm) v[4 * sp + mm] = R[(4 * q + sp) * 64 + ((mm ^ q) << 4) + (lane & 15)];
+          dft_core<4>(v + 4 * sp, P, true);
+        }
+      }
+      // v[4 s' + k3] is output row 2 (wave + 16 (4 q + s') + 256 k3) + s of
+      // column lane & 15
+      {
+        const int q = lane >> 4;
+        const uint32_t off = lane_off((uint32_t)(2 * wv_ + 128 * q + s), lane & 15);
+#pragma unroll
+        for (int k3 = 0; k3 < 4; ++k3)
+#pragma unroll
+          for (int sp = 0; sp < 4; ++sp) {
+            GlobalF64* d = dst + base + ((uint64_t)(32 * sp + 512 * k3) << logS);
+            *(GlobalF64*)((__attribute__((address_space(1))) char*)d + off) = reduce(v[4 * sp + k3], P.p, P.pinv);
+          }
+      }
+    }
+  };
+
+  // tile order as in k_ntt_col (XCD by XCD)
   const uint64_t g = gridDim.x;
   uint64_t w = blockIdx.x;
   if ((g & 7) == 0) w = (uint64_t)(blockIdx.x & 7) * (g >> 3) + (blockIdx.x >> 3);
-  for (; w < ntiles; w += g) {
-    const uint64_t b0 = tile_base(w);
-    double h0[kBigEPT], h1[kBigEPT];
-    load_pair(b0, h0, h1);
-    double2 icur[NB1], istp[NB1];
-    inv_twiddles(b0, icur, istp);
-    double o0[16], o1[16];
-    half_step(b0, h0, o0, icur, istp, true);
-    half_step(b0 + 8, h1, o1, icur, istp, false);
-    store_pair(b0, o0, o1);
+  if (w >= ntiles) return;
+  double xl[16], xh[16];
+  load_rows(w, 0, xl);
+  load_rows(w, 1, xh);
+  __syncthreads();  // tables
+  for (;;) {
+    const uint64_t wn = w + g;
+    if (wn >= ntiles) break;
+    tile_step(BoolTag<true>{}, w, xl, xh, wn);
+    w = wn;
   }
+  tile_step(BoolTag<false>{}, w, xl, xh, w);
 }
 
 }  // namespace pcg

@@ -550,8 +550,8 @@ PassFn select_pass(int mode, const Plan& pl, const Pass& ps) {
       return nullptr;
     }
     if (ps.logS < ps.logLines) return nullptr;  // a tile's columns within one block
-    if (mode == kInv && ps.logL == kCol16LogL && ps.logS >= 4)
-      return &k_ntt_col16_inv;  // 16 adjacent columns per tile (two halves)
+    if (mode == kInv && ps.logL == kCol16LogL && ps.logS >= 4 && ps.logS <= 17)
+      return &k_ntt_col16_invw;  // 16 adjacent columns per tile (32-bit byte offsets within it)
     if (mode == kFwd) return big_col_fn<kFwd>(ps.logL);
     if (mode == kInv) return big_col_fn<kInv>(ps.logL);
     return nullptr;
@@ -642,7 +642,7 @@ int launch_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double* 
   const int E = 1 << pl.logE;
   const int threads = E / (pl.big ? kBigEPT : kEPT);
   a.ntiles = a.N / E;
-  if (fn == &k_ntt_col16_inv) {
+  if (fn == &k_ntt_col16_invw) {
     if (!src) return fail(PCG_ERR_INTERNAL, "inverse column pass without input");
     a.ntiles /= 2;  // tiles of 16 columns
   }
