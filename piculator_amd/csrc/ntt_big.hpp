@@ -741,6 +741,20 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs(PassArgs a) {
 // loads, which wait on the vector-memory counter only (flat loads also
 // hold up the LDS counter).
 typedef const __attribute__((address_space(1))) uint32_t GlobalU32;
+typedef const __attribute__((address_space(1))) double GlobalF64c;
+typedef __attribute__((address_space(1))) double GlobalF64;
+// p advanced by boff bytes.  With p wave-uniform and boff one 32-bit
+// offset per lane this is the global access form with a scalar base: one
+// VGPR of address per thread instead of a 64-bit address per access.
+__device__ __forceinline__ const GlobalU32* at_bytes(const GlobalU32* p, uint32_t boff) {
+  return (const GlobalU32*)((const __attribute__((address_space(1))) char*)p + boff);
+}
+__device__ __forceinline__ GlobalF64c* at_bytes(GlobalF64c* p, uint32_t boff) {
+  return (GlobalF64c*)((const __attribute__((address_space(1))) char*)p + boff);
+}
+__device__ __forceinline__ GlobalF64* at_bytes(GlobalF64* p, uint32_t boff) {
+  return (GlobalF64*)((__attribute__((address_space(1))) char*)p + boff);
+}
 
 __device__ __forceinline__ double col11_tp(const double* tp, int e2) {
   // omega_1024^e = omega_2048^{2e} from tp[n] = omega_2048^n (n < 1024):
@@ -786,16 +800,19 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs11w(PassArgs a) {
   // stays in bounds).  The limbs are global memory: global loads wait on
   // the vector-memory counter only.
   const GlobalU32* lp = (const GlobalU32*)limbs;  // address-space cast
-  // rows m + 64 jj (jj < 16) of this thread's column, the first at g0 in a
-  // block of rows whose last point is `last`
-  auto load_rows = [&](uint64_t g0, uint64_t last, uint32_t (&x)[16]) {
-    if (last < nlimbs) {
+  // rows m + 64 jj (jj < 16) of this thread's column in the block of 1024
+  // rows at u (wave-uniform; a whole block loads at a scalar base plus one
+  // 32-bit lane offset)
+  auto load_rows = [&](uint64_t u, uint32_t (&x)[16]) {
+    const int bt = big_tid();
+    const uint32_t lo = ((uint32_t)(bt >> 4) << logS) + (uint32_t)(bt & 15);
+    if (u + ((uint64_t)(LH - 1) << logS) + 15 < nlimbs) {
 #pragma unroll
-      for (int jj = 0; jj < 16; ++jj) x[jj] = lp[g0 + ((uint64_t)jj << (logS + 6))];
+      for (int jj = 0; jj < 16; ++jj) x[jj] = *at_bytes(lp + u + ((uint64_t)jj << (logS + 6)), lo * 4u);
     } else {
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) {
-        const uint64_t g = g0 + ((uint64_t)jj << (logS + 6));
+        const uint64_t g = u + lo + ((uint64_t)jj << (logS + 6));
         // past the end: the last limb, or limb 0 of an empty operand (the
         // empty top products of wrapped batches keep their operand pointer)
         const uint32_t v = lp[g < nlimbs ? g : (nlimbs ? nlimbs - 1 : 0)];
@@ -803,11 +820,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs11w(PassArgs a) {
       }
     }
   };
-  auto load_tile = [&](uint64_t t, uint32_t (&x)[16]) {
-    const uint64_t b0 = tile_base(t);
-    const int bt = big_tid();
-    load_rows(b0 + ((uint64_t)(bt >> 4) << logS) + (bt & 15), b0 + ((uint64_t)(LH - 1) << logS) + 15, x);
-  };
+  auto load_tile = [&](uint64_t t, uint32_t (&x)[16]) { load_rows(tile_base(t), x); };
 
   auto tile_step = [&](const uint64_t w, const uint32_t (&xc)[16], uint32_t (&xn)[16], const uint64_t wload) {
     const uint64_t base = tile_base(w);
@@ -821,8 +834,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs11w(PassArgs a) {
         for (int jj = 0; jj < 16; ++jj) v[jj] = (double)xc[jj];
         if (full) {
           uint32_t uh[16];
-          load_rows(base + ((uint64_t)m << logS) + (bt & 15) + half, base + ((uint64_t)(LH - 1) << logS) + 15 + half,
-                    uh);
+          load_rows(base + half, uh);
 #pragma unroll
           for (int jj = 0; jj < 16; ++jj) v[jj] = s == 0 ? v[jj] + (double)uh[jj] : v[jj] - (double)uh[jj];
         }
@@ -888,14 +900,15 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs11w(PassArgs a) {
         const double2 st = root_pow(c * 32, logRoot, false, P);
         const double2 sk = root_pow(c * 512, logRoot, false, P);
         double2 row = root_pow(c * k0, logRoot, false, P);
-        double* d0 = dst + base + (k0 << logS) + (uint64_t)(lane & 15);
+        GlobalF64* db = (GlobalF64*)dst + base;  // address-space cast
+        const uint32_t so = (((uint32_t)k0 << logS) + (uint32_t)(lane & 15)) * 8u;
 #pragma unroll
         for (int k3 = 0; k3 < 4; ++k3) {
           double2 cur = row;
 #pragma unroll
           for (int sp = 0; sp < 4; ++sp) {
             const double x = mm_shoup(v[4 * sp + k3], cur.x, cur.y, P.p);
-            d0[(uint64_t)(32 * sp + 512 * k3) << logS] = x;
+            *at_bytes(db + ((uint64_t)(32 * sp + 512 * k3) << logS), so) = x;
             if (sp < 3) {
               const double wn = mm_shoup(cur.x, st.x, st.y, P.p);
               cur = make_double2(wn, wn * P.pinv);
@@ -954,8 +967,6 @@ constexpr int kCol16LogL = 11;
 // rows after half 1's, so the loads overlap the wave-local stages.  The
 // global-address-space views keep the loads and stores off the LDS
 // counter: a flat access in flight would be waited for at every LDS wait.
-typedef const __attribute__((address_space(1))) double GlobalF64c;
-typedef __attribute__((address_space(1))) double GlobalF64;
 template <bool B>
 struct BoolTag {
   static constexpr bool value = B;
@@ -992,11 +1003,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col16_invw(PassArgs a) {
     return ((t >> lgc) << (LOGL + logS)) + ((t & ((1ull << lgc) - 1)) << 4);
   };
   // Addresses are a wave-uniform base plus one 32-bit byte offset per lane
-  // (a tile spans 2^(11 + logS) <= 2^28 points), the form of the global
-  // accesses with a scalar base: one VGPR of address per thread.
-  auto at = [](GlobalF64c* b, uint32_t boff) -> GlobalF64c* {
-    return (GlobalF64c*)((const __attribute__((address_space(1))) char*)b + boff);
-  };
+  // (at_bytes; a tile spans 2^(11 + logS) <= 2^28 points).
   // byte offset of row r, column cl in a tile
   auto lane_off = [&](uint32_t r, int cl) -> uint32_t { return ((r << logS) + (uint32_t)cl) * 8u; };
   // rows m + 64 jj + 1024 hi (jj < 16) of this thread's column
@@ -1005,7 +1012,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col16_invw(PassArgs a) {
     const uint32_t off = lane_off((uint32_t)(bt >> 4), bt & 15);
     GlobalF64c* p = src + tile_base(t) + ((uint64_t)hi << (10 + logS));
 #pragma unroll
-    for (int jj = 0; jj < 16; ++jj) x[jj] = *at(p + ((uint64_t)jj << (logS + 6)), off);
+    for (int jj = 0; jj < 16; ++jj) x[jj] = *at_bytes(p + ((uint64_t)jj << (logS + 6)), off);
   };
 
   // xl, xh: rows m + 64 jj and m + 64 jj + 1024 of tile w, replaced by
@@ -1101,8 +1108,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col16_invw(PassArgs a) {
         for (int k3 = 0; k3 < 4; ++k3)
 #pragma unroll
           for (int sp = 0; sp < 4; ++sp) {
-            GlobalF64* d = dst + base + ((uint64_t)(32 * sp + 512 * k3) << logS);
-            *(GlobalF64*)((__attribute__((address_space(1))) char*)d + off) = reduce(v[4 * sp + k3], P.p, P.pinv);
+            *at_bytes(dst + base + ((uint64_t)(32 * sp + 512 * k3) << logS), off) = reduce(v[4 * sp + k3], P.p, P.pinv);
           }
       }
     }

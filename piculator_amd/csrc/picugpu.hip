@@ -290,8 +290,10 @@ PassFn big_col_fn(int logL) {
 }
 
 // First forward passes from the limbs (two half-length DFTs per 2^14-point
-// tile).
-PassFn big_col_limbs_fn(int logL) {
+// tile).  The 2^11 pass addresses a tile with 32-bit byte offsets (logS <=
+// 17).
+PassFn big_col_limbs_fn(int logL, int logS) {
+  if (logL == 11 && logS > 17) return nullptr;
   switch (logL) {
     case 6: return &k_ntt_col_limbs<6>;
     case 7: return &k_ntt_col_limbs<7>;
@@ -653,7 +655,7 @@ int launch_pass(Ctx& c, int mode, const Plan& pl, const Pass& ps, const double* 
   }
   if (pl.big && mode == kFwd && !src && ps.logS >= kBigLogE + 1 - ps.logL && (descs || nlimbs >= 1)) {
     // first pass from the limbs: two half-length DFTs per tile
-    if (PassFn h = big_col_limbs_fn(ps.logL)) {
+    if (PassFn h = big_col_limbs_fn(ps.logL, ps.logS)) {
       fn = h;
       a.ntiles = a.N >> (kBigLogE + 1);  // tiles of 2^14 limbs (2^15 output points)
     }
