@@ -374,15 +374,18 @@ __global__ __launch_bounds__((2 << LOGN) / kEPT) __attribute__((amdgpu_waves_per
   stage_twiddles<GE>(ltw[1], P1);
   const double2* tw = ltw[pw];
   uint32_t ua[kEPT], ub[kEPT];
+  {
+    const LimbBuf la(d.a, (uint32_t)na), lb(d.b, (uint32_t)nb);  // zero past na, nb
 #pragma unroll
-  for (int b = 0; b < NB1; ++b)
+    for (int b = 0; b < NB1; ++b)
 #pragma unroll
-    for (int r = 0; r < R1; ++r) {
-      int lam, n;
-      stage_pos<GE, LOGR1>(b, r, lam, n);
-      ua[b * R1 + r] = n < na ? d.a[n] : 0u;
-      ub[b * R1 + r] = n < nb ? d.b[n] : 0u;
-    }
+      for (int r = 0; r < R1; ++r) {
+        int lam, n;
+        stage_pos<GE, LOGR1>(b, r, lam, n);
+        ua[b * R1 + r] = la[(uint32_t)n];
+        ub[b * R1 + r] = lb[(uint32_t)n];
+      }
+  }
   double y[1][8];
   {
     double x[kEPT], yb[1][8];
