@@ -338,8 +338,8 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col(PassArgs a) {
     return base + ((uint64_t)n << logS) + lam;
   };
   // inputs of this thread's first-stage butterflies: x[b R1 + r]; limbs
-  // past the operand are zeros (the load stays in bounds and is discarded,
-  // so every load issues without a branch)
+  // past the operand are zeros (range-checked buffer loads)
+  const LimbBuf lb(limbs, (uint32_t)nlimbs);
   auto load_tile = [&](uint64_t t, double (&x)[kBigEPT]) {
     const uint64_t b0 = tile_base(t);
     const int bt = big_tid();
@@ -350,12 +350,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col(PassArgs a) {
         int lam, n;
         big_pos<GE, LOGR1>(bt, b, r, lam, n);
         const uint64_t g = gidx(b0, n, lam);
-        if (src) {
-          x[b * R1 + r] = src[g];
-        } else {
-          const uint32_t v = limbs[g < nlimbs ? g : nlimbs - 1];
-          x[b * R1 + r] = g < nlimbs ? (double)v : 0.0;
-        }
+        x[b * R1 + r] = src ? src[g] : (double)lb[(uint32_t)g];
       }
   };
   // inter-pass twiddle omega^{c n} of a column pass along the thread's points
@@ -505,6 +500,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_reg(PassArgs a) {
   }
   const double* src = a.src ? a.src + zoff + pi * a.N : nullptr;
   double* dst = a.dst + zoff + pi * a.N;
+  const LimbBuf lb(limbs, (uint32_t)nlimbs);  // first pass: zero past the operand
   const int logRoot = LOGL + logS;
   const uint64_t ntiles = a.ntiles;
   const uint64_t g = gridDim.x;
@@ -521,12 +517,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_reg(PassArgs a) {
 #pragma unroll
       for (int n = 0; n < L; ++n) {
         const uint64_t gi = base + ((uint64_t)n << logS) + col;
-        if (src) {
-          v[n] = src[gi];
-        } else {
-          const uint32_t x = limbs[gi < nlimbs ? gi : nlimbs - 1];
-          v[n] = gi < nlimbs ? (double)x : 0.0;
-        }
+        v[n] = src ? src[gi] : (double)lb[(uint32_t)gi];  // zero past the operand
       }
       const double2 stp = root_pow(c, logRoot, MODE == kInv, P);  // omega^{+-c}
       double2 cur = make_double2(1.0, 0.0);
@@ -622,7 +613,9 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs(PassArgs a) {
     return ((t >> lgc) << (LOGL + logS)) + ((t & ((1ull << lgc) - 1)) << GE::LOGLINES);
   };
   // limbs of this thread's first-stage inputs (rows n < L / 2); limbs past
-  // the operand read as zero (the clamped load stays in bounds)
+  // the operand read as zero (range-checked buffer loads, all in flight
+  // together; transforms of at most 2^28 points)
+  const LimbBuf lb(limbs, (uint32_t)nlimbs);
   auto load_tile = [&](uint64_t t, uint32_t (&x)[kBigEPT]) {
     const uint64_t b0 = tile_base(t);
     const int bt = big_tid();
@@ -632,9 +625,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs(PassArgs a) {
       for (int r = 0; r < R1; ++r) {
         int lam, n;
         big_pos<GE, LOGR1>(bt, b, r, lam, n);
-        const uint64_t g = b0 + ((uint64_t)n << logS) + lam;
-        const uint32_t v = limbs[g < nlimbs ? g : nlimbs - 1];
-        x[b * R1 + r] = g < nlimbs ? v : 0u;
+        x[b * R1 + r] = lb[(uint32_t)(b0 + ((uint64_t)n << logS) + lam)];
       }
   };
 
@@ -664,9 +655,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs(PassArgs a) {
           for (int r = 0; r < R1; ++r) {
             int lam, n;
             big_pos<GE, LOGR1>(bt2, b, r, lam, n);
-            const uint64_t gh = base + ((uint64_t)n << logS) + lam + half;
-            const uint32_t vh = limbs[gh < nlimbs ? gh : nlimbs - 1];
-            const double xh = gh < nlimbs ? (double)vh : 0.0;
+            const double xh = (double)lb[(uint32_t)(base + ((uint64_t)n << logS) + lam + half)];
             v1[b][r] = s == 0 ? v1[b][r] + xh : v1[b][r] - xh;
           }
       }
@@ -796,10 +785,11 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs11w(PassArgs a) {
   };
   // S1 inputs of this thread: rows m + 64 jj of column t & 15.  A tile
   // wholly inside the operand loads them plainly (all 16 in flight
-  // together); else limbs past the operand read as zero (the clamped load
-  // stays in bounds).  The limbs are global memory: global loads wait on
-  // the vector-memory counter only.
+  // together); else limbs past the operand read as zero (range-checked
+  // buffer loads).  The limbs are global memory: global loads wait on the
+  // vector-memory counter only.
   const GlobalU32* lp = (const GlobalU32*)limbs;  // address-space cast
+  const LimbBuf lb(limbs, (uint32_t)nlimbs);       // the ragged blocks: zero past the operand
   // rows m + 64 jj (jj < 16) of this thread's column in the block of 1024
   // rows at u (wave-uniform; a whole block loads at a scalar base plus one
   // 32-bit lane offset)
@@ -811,13 +801,7 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col_limbs11w(PassArgs a) {
       for (int jj = 0; jj < 16; ++jj) x[jj] = *at_bytes(lp + u + ((uint64_t)jj << (logS + 6)), lo * 4u);
     } else {
 #pragma unroll
-      for (int jj = 0; jj < 16; ++jj) {
-        const uint64_t g = u + lo + ((uint64_t)jj << (logS + 6));
-        // past the end: the last limb, or limb 0 of an empty operand (the
-        // empty top products of wrapped batches keep their operand pointer)
-        const uint32_t v = lp[g < nlimbs ? g : (nlimbs ? nlimbs - 1 : 0)];
-        x[jj] = g < nlimbs ? v : 0u;
-      }
+      for (int jj = 0; jj < 16; ++jj) x[jj] = lb[(uint32_t)(u + lo + ((uint64_t)jj << (logS + 6)))];
     }
   };
   auto load_tile = [&](uint64_t t, uint32_t (&x)[16]) { load_rows(tile_base(t), x); };

@@ -303,15 +303,10 @@ __global__ __launch_bounds__(kRowNTH) void k_ntt_row_fa(PassArgs a) {
   // tile w of operand a (op 0) or b (op 1); LIMBS: the whole line
   auto load_tile = [&](int op, uint64_t w, double (&x)[16]) {
     if constexpr (LIMBS) {
-      const uint32_t* lp = op ? bd.b : bd.a;
-      const uint32_t nl = op ? bd.nb : bd.na;
+      const LimbBuf lb(op ? bd.b : bd.a, op ? bd.nb : bd.na);  // zero past the operand
       const uint32_t t = (uint32_t)big_tid();
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const uint32_t n = t + ((uint32_t)r << 10);
-        const uint32_t v = lp[n < nl ? n : nl - 1];
-        x[r] = n < nl ? (double)v : 0.0;
-      }
+      for (int r = 0; r < 16; ++r) x[r] = (double)lb[t + ((uint32_t)r << 10)];
     } else {
       const double* s = (op ? src : sra) + (w << kBigLogE) + big_tid();
 #pragma unroll
