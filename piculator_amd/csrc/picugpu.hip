@@ -2405,7 +2405,8 @@ int bsr_schedule(Ctx& c, const BsrPlan& Pl) {
                        reinterpret_cast<uint32_t*>(P->add_slots), words);
   }
   // leaves and every node of height 2 and 3 in one launch
-  hipLaunchKernelGGL(k_bsr_base, dim3((unsigned)((P->nleaves + 255) / 256)), dim3(256), 0, c.stream,
+  hipLaunchKernelGGL(k_bsr_base, dim3((unsigned)((P->nleaves + kBsrBaseThreads - 1) / kBsrBaseThreads)),
+                     dim3(kBsrBaseThreads), 0, c.stream,
                      P->leaf_rel, P->a_dev, P->leaf_ids, P->nleaves, P->meta, P->arena);
   for (int32_t h = 4; h <= P->H; ++h) {
     const BsrHeight& hh = P->hp[h];
