@@ -308,9 +308,13 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col(PassArgs a) {
   constexpr int R1 = 1 << LOGR1;
   constexpr int NB1 = kBigEPT / R1;
   constexpr int LOGTA = LOGL - 4;
+  // inverse pre-twiddles from one power per thread (LOGL <= 8, see
+  // inv_twiddles): omega_N^{-lam} for the tile's columns lam
+  constexpr bool kPowTw = MODE == kInv && LOGL <= 8;
   __shared__ double lds[GE::LDS];
   __shared__ BigTw ta[1 << LOGTA];
   __shared__ double th[GE::L / 2];  // omega_L^n, n < L / 2 (last stage)
+  __shared__ double tlam[kPowTw ? GE::LINES : 1];
   const int pi = blockIdx.y;
   const Prime P = a.prm[pi];
   big_tables<GE>(ta, th, P);
@@ -329,6 +333,10 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col(PassArgs a) {
   double* dst = a.dst + zoff + pi * a.N;
   const int logRoot = LOGL + logS;
   const uint64_t ntiles = a.ntiles;
+  if constexpr (kPowTw) {
+    for (int e = threadIdx.x; e < GE::LINES; e += blockDim.x) tlam[e] = root_pow((uint64_t)e, logRoot, true, P).x;
+    __syncthreads();
+  }
 
   auto tile_base = [&](uint64_t t) -> uint64_t {
     const int lgc = logS - GE::LOGLINES;
@@ -364,9 +372,46 @@ __global__ __launch_bounds__(kBigNTH) void k_ntt_col(PassArgs a) {
 
   // bases of the inverse pre-twiddles of tile t's first-stage butterflies
   // (fetched with the tile's loads, so their table reads are in flight
-  // together with them)
+  // together with them).  Butterfly b of thread t is column lam of the
+  // tile (the same for every b) at point n0 + b 2^(L-4), n0 = j0 its first,
+  // stepping by L / R1 = 2^(L-4) NB1.  kPowTw: one power per thread,
+  // w = omega^{-c} = omega^{-c0} omega^{-lam} (c0 the tile's first column,
+  // the same for the workgroup, and the tlam table), then by squarings and
+  // products w^{j0}, q = w^{2^(L-4)} and the step q^{NB1}.  This replaces
+  // two lookups of the 2^30-th-root tables per butterfly: at the large
+  // strides of the last inverse pass of the three-pass plans (c up to
+  // 2^21) their lanes each read a different line of the tables, which the
+  // pass's streaming data keeps out of the caches.
   auto inv_twiddles = [&](uint64_t t, double2 (&cur)[NB1], double2 (&stp)[NB1]) {
-    if constexpr (MODE == kInv) {
+    if constexpr (kPowTw) {
+      const uint64_t b0 = tile_base(t);
+      int lam, j0;
+      big_pos<GE, LOGR1>(big_tid(), 0, 0, lam, j0);
+      const double2 u0 = root_pow(b0 & ((1ull << logS) - 1), logRoot, true, P);
+      const double w = mm_shoup(tlam[lam], u0.x, u0.y, P.p);
+      double2 pw = make_double2(w, w * P.pinv);
+      double2 cb = make_double2(1.0, P.pinv);
+#pragma unroll
+      for (int k = 0; k < LOGL - 4; ++k) {
+        const double m = mm_shoup(cb.x, pw.x, pw.y, P.p);
+        if ((j0 >> k) & 1) cb = make_double2(m, m * P.pinv);
+        const double sq = mm_shoup(pw.x, pw.x, pw.y, P.p);
+        pw = make_double2(sq, sq * P.pinv);
+      }
+      cur[0] = cb;  // omega^{-c j0}
+#pragma unroll
+      for (int b = 1; b < NB1; ++b) {
+        const double m = mm_shoup(cur[b - 1].x, pw.x, pw.y, P.p);
+        cur[b] = make_double2(m, m * P.pinv);
+      }
+#pragma unroll
+      for (int k = 0; k < 4 - LOGR1; ++k) {
+        const double sq = mm_shoup(pw.x, pw.x, pw.y, P.p);
+        pw = make_double2(sq, sq * P.pinv);
+      }
+#pragma unroll
+      for (int b = 0; b < NB1; ++b) stp[b] = pw;  // omega^{-c L / R1}
+    } else if constexpr (MODE == kInv) {
       const uint64_t b0 = tile_base(t);
       const int bt = big_tid();
 #pragma unroll
