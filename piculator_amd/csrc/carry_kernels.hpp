@@ -299,13 +299,6 @@ __device__ __forceinline__ void crt_words(double r0, double r1, const Prime& P0,
   w2 = (uint32_t)hi;
 }
 
-// Resolves the carries of LPT*blockDim.x consecutive limbs held LPT per
-// thread (thread t holds limbs [LPT t, LPT t + LPT)), with carry-in 0.
-// Inputs are the CRT words (w0, w1, w2) of each limb position; sw1/sw2 are
-// LDS scratch of LPT*blockDim.x words, wtot LDS scratch of blockDim.x/64.
-// On return D holds the limbs, and for the last thread `cout` is the carry
-// out of the top limb and `spill` the value of the words that spill past
-// the top limb.
 // Resolves the carries of LPT * blockDim.x consecutive limbs held LPT per
 // thread (thread t holds limbs [LPT t, LPT t + LPT)) given their sums
 // s < 4 * 2^32 (limb value plus everything added into that limb position)
@@ -331,36 +324,52 @@ __device__ __forceinline__ void block_carry_sums(const uint64_t* s, CarryFn* wto
 }
 
 // Same from CRT words: limb k receives s_k = w0_k + w1_{k-1} + w2_{k-2}.
-// sw1/sw2 are LDS scratch of LPT * blockDim.x words, wtot LDS scratch of
-// blockDim.x / 64.  A carry-in cin (< 2^64) may be folded into the first two
-// limbs: their sums stay below 3.5 * 2^32, so carries stay small.  For the
-// last thread `spill` is the value of the words that spill past the top limb.
+// Thread t's first two sums take its left neighbour's top words, by a
+// shuffle inside the wave and, for lane 0, through LDS from the previous
+// wave's lane 63 (sw1: blockDim.x / 64 words, sw2: twice that); the rest
+// come from its own registers.  (Exchanging every word through LDS at a
+// stride of LPT words per thread put a quarter of the LDS cycles of the
+// CRT/carry kernels in bank conflicts.)  wtot is LDS scratch of
+// blockDim.x / 64.  blockDim.x is a multiple of 64.  A carry-in cin
+// (< 2^64) is folded into the first two limbs: their sums stay below
+// 3.5 * 2^32, so carries stay small.  `spill` is the value of the words
+// that spill past the top limb (the same in every thread).
 template <int LPT>
 __device__ __forceinline__ void block_carry(const uint32_t* w0, const uint32_t* w1,
                                             const uint32_t* w2, uint32_t* sw1, uint32_t* sw2,
                                             CarryFn* wtot, uint32_t* D, uint64_t& cout,
                                             uint64_t& spill, uint64_t cin = 0) {
+  static_assert(LPT >= 2, "two words reach past a thread's first limb");
   const int tid = threadIdx.x;
-  const int total = LPT * blockDim.x;
-#pragma unroll
-  for (int q = 0; q < LPT; ++q) {
-    sw1[tid * LPT + q] = w1[q];
-    sw2[tid * LPT + q] = w2[q];
+  const int lane = tid & 63, wv = tid >> 6;
+  const int last = (blockDim.x >> 6) - 1;  // the last wave
+  uint32_t p1 = __shfl_up(w1[LPT - 1], 1, 64);
+  uint32_t p22 = __shfl_up(w2[LPT - 2], 1, 64);
+  uint32_t p23 = __shfl_up(w2[LPT - 1], 1, 64);
+  if (lane == 63) {
+    sw1[wv] = w1[LPT - 1];
+    sw2[2 * wv] = w2[LPT - 2];
+    sw2[2 * wv + 1] = w2[LPT - 1];
   }
   __syncthreads();
-  uint64_t s[LPT];
-#pragma unroll
-  for (int q = 0; q < LPT; ++q) {
-    const int li = tid * LPT + q;
-    uint64_t x = w0[q];
-    if (li >= 1) x += sw1[li - 1];
-    if (li >= 2) x += sw2[li - 2];
-    if (li == 0) x += cin & 0xffffffffull;
-    if (li == 1) x += cin >> 32;
-    s[q] = x;
+  if (lane == 0) {
+    if (wv > 0) {
+      p1 = sw1[wv - 1];
+      p22 = sw2[2 * wv - 2];
+      p23 = sw2[2 * wv - 1];
+    } else {
+      p1 = (uint32_t)cin;  // the carry-in takes the first thread's neighbour's place
+      p22 = 0;
+      p23 = (uint32_t)(cin >> 32);
+    }
   }
+  spill = (uint64_t)sw1[last] + (uint64_t)sw2[2 * last] + ((uint64_t)sw2[2 * last + 1] << 32);
+  uint64_t s[LPT];
+  s[0] = (uint64_t)w0[0] + p1 + p22;
+  s[1] = (uint64_t)w0[1] + w1[0] + p23;
+#pragma unroll
+  for (int q = 2; q < LPT; ++q) s[q] = (uint64_t)w0[q] + w1[q - 1] + w2[q - 2];
   block_carry_sums<LPT>(s, wtot, D, cout);
-  spill = (uint64_t)sw1[total - 1] + (uint64_t)sw2[total - 2] + ((uint64_t)sw2[total - 1] << 32);
 }
 
 template <bool FFT>
