@@ -16,6 +16,7 @@
 // its two lowest limbs are 0xFFFFFFFF), so blocks are resolved locally with
 // carry-in 0, block functions are scanned, and each block adds its carry-in.
 #pragma once
+#include "limb_buf.hpp"
 #include "modfp64.hpp"
 
 #pragma clang fp contract(off)
@@ -506,6 +507,35 @@ __global__ __launch_bounds__(256) void k_carry_fix(CarryArgs a) {
   uint64_t k = blk * kCarryBlock;
   uint64_t end = k + kCarryBlock;
   if (end > a.zL) end = a.zL;
+  // The first two limbs take c < 2^64; past them the carry is 0 or 1.
+  for (int i = 0; i < 2 && c != 0 && k < end; ++i, ++k) {
+    const uint64_t t = (uint64_t)a.out[k] + (c & 0xffffffffull);
+    a.out[k] = (uint32_t)t;
+    c = (c >> 32) + (t >> 32);
+  }
+  if (c == 1 && a.zL < (1ull << 29)) {
+    // A carry of 1 turns limbs equal to 2^32 - 1 into 0 and stops at the
+    // first other limb.  Such runs can fill whole blocks (a borrow through
+    // the zero limbs of a short minuend), so 16 limbs per round, their
+    // loads in flight together (range-checked: zero past zL), instead of
+    // one dependent load and store per limb.
+    const LimbBuf lb(a.out, (uint32_t)a.zL);
+    while (c != 0 && k < end) {
+      uint32_t v[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) v[j] = lb[(uint32_t)(k + j)];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        if (c != 0 && k + j < end) {
+          const bool ones = v[j] == 0xffffffffu;
+          a.out[k + j] = ones ? 0u : v[j] + 1u;
+          if (!ones) c = 0;
+        }
+      }
+      k += 16;
+    }
+    if (k > end) k = end;
+  }
   while (c != 0 && k < end) {
     const uint64_t t = (uint64_t)a.out[k] + (c & 0xffffffffull);
     a.out[k] = (uint32_t)t;

@@ -18,6 +18,7 @@
 // or radix-4 stage when l_j is not a multiple of 3) with each thread holding
 // kEPT (8) points in registers per stage.
 #pragma once
+#include "limb_buf.hpp"
 #include "modfp64.hpp"
 
 #pragma clang fp contract(off)
@@ -40,23 +41,6 @@ struct BatchDesc {
   uint32_t na, nb;
 };
 
-// Word 3 of a raw buffer resource on gfx9-family GPUs: 32-bit data format,
-// no swizzle, no thread-id offset.
-constexpr int kBufWord3 = 0x00020000;
-
-// Raw buffer view of n 32-bit limbs at a wave-uniform address.  A load past
-// the n limbs returns 0 (the buffer unit's range check, no memory access),
-// so a ragged operand loads without a select or a branch per limb and a
-// thread's loads all stay in flight together (a select after each flat
-// load had the compiler wait for every load in turn).  n < 2^29.
-struct LimbBuf {
-  __amdgpu_buffer_rsrc_t r;
-  __device__ __forceinline__ LimbBuf(const uint32_t* p, uint32_t n)
-      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(p), (short)0, (int)(n * 4u), kBufWord3)) {}
-  __device__ __forceinline__ uint32_t operator[](uint32_t i) const {
-    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4u), 0, 0);
-  }
-};
 
 struct PassArgs {
   const Prime* prm;        // device: the two primes' constants for this N
