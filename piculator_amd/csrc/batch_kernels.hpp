@@ -496,6 +496,17 @@ __global__ __launch_bounds__(kCarryThreads) void k_batch_carry_local(
   const Prime P0 = prm[0], P1 = prm[1];
   const int tid = threadIdx.x;
   uint32_t w0[kCarryLPT], w1[kCarryLPT], w2[kCarryLPT], D[kCarryLPT];
+  // both residues of the thread's coefficients below N, loaded before any
+  // branch so that all of them are in flight together (the index clamped,
+  // the value of a clamped load unused)
+  double m0[kCarryLPT], m1[kCarryLPT];
+#pragma unroll
+  for (int q = 0; q < kCarryLPT; ++q) {
+    const uint64_t k = base + tid * kCarryLPT + q;
+    const uint64_t kc = k < N ? k : N - 1;
+    m0[q] = rz[kc];
+    m1[q] = rz[N + kc];
+  }
 #pragma unroll
   for (int q = 0; q < kCarryLPT; ++q) {
     const uint64_t k = base + tid * kCarryLPT + q;
@@ -503,8 +514,8 @@ __global__ __launch_bounds__(kCarryThreads) void k_batch_carry_local(
     if (k < kend) {
       double r0, r1;
       if (k < N) {
-        r0 = rz[k];
-        r1 = rz[N + k];
+        r0 = m0[q];
+        r1 = m1[q];
         if (k < w) {
           r0 -= tz[k + w - 1];
           r1 -= tz[M + k + w - 1];

@@ -228,12 +228,12 @@ struct CarryArgs {
   uint64_t top_w;
 };
 
-// Residues of coefficient k (k < zL - 1) of an NTT product, both primes,
-// with the wrapped coefficients of a short cyclic convolution taken out.
-__device__ __forceinline__ void coeff_residues(const CarryArgs& a, uint64_t k, double& r0, double& r1) {
+// Residues of coefficient k of a product in a.r (two primes, N each) given
+// r0, r1 = a.r[k], a.r[N + k] when k < N (loaded by the caller): a wrapped
+// product (a.top) subtracts its short product's coefficient k + w - 1 below
+// w and takes coefficient k - N + w - 1 of it from N up.
+__device__ __forceinline__ void coeff_residues_top(const CarryArgs& a, uint64_t k, double& r0, double& r1) {
   if (k < a.N) {
-    r0 = a.r[k];
-    r1 = a.r[a.N + k];
     if (a.top && k < a.top_w) {
       const uint64_t t = k + a.top_w - 1;
       r0 -= a.top[t];
@@ -407,13 +407,24 @@ __global__ __launch_bounds__(kCarryThreads) void k_carry_local(CarryArgs a) {
     // coefficients 0 .. zL - 2 (the top limb only receives carries); from
     // N up they exist only in a wrapped product
     const uint64_t kend = a.top ? a.zL - 1 : (a.zL - 1 < a.N ? a.zL - 1 : a.N);
+    // both residues of the thread's coefficients below N, loaded before any
+    // branch so that all of them are in flight together (the index
+    // clamped, the value of a clamped load unused)
+    double m0[kCarryLPT], m1[kCarryLPT];
+#pragma unroll
+    for (int q = 0; q < kCarryLPT; ++q) {
+      const uint64_t k = base + tid * kCarryLPT + q;
+      const uint64_t kc = k < a.N ? k : a.N - 1;
+      m0[q] = a.r[kc];
+      m1[q] = a.r[a.N + kc];
+    }
 #pragma unroll
     for (int q = 0; q < kCarryLPT; ++q) {
       const uint64_t k = base + tid * kCarryLPT + q;
       w0[q] = w1[q] = w2[q] = 0;
       if (k < kend) {
-        double r0, r1;
-        coeff_residues(a, k, r0, r1);
+        double r0 = m0[q], r1 = m1[q];
+        coeff_residues_top(a, k, r0, r1);
         crt_words(r0, r1, P0, P1, a.crt, w0[q], w1[q], w2[q]);
       }
     }
