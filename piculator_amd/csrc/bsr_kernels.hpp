@@ -261,12 +261,13 @@ __host__ __device__ inline void bsr_base_entry(const uint32_t* rel, uint32_t a, 
 // The subtree's first index a comes from device memory (set before each
 // evaluation), so that a captured schedule serves every subtree of its plan.
 // Launched with kBsrBaseThreads threads per block: the bound lets an entry's
-// node arithmetic (up to ~200 registers per thread) stay in registers, where
-// the default 1024-thread bound capped it at 128 and spilled the rest.
+// node arithmetic stay (almost) in registers, where the default 1024-thread
+// bound capped it at 128 and spilled 82 registers.  Three waves per SIMD
+// (168 registers, 8 of them spilled) measured faster than two without
+// spills (173 registers).
 constexpr int kBsrBaseThreads = 256;
-__global__ __launch_bounds__(kBsrBaseThreads) void k_bsr_base(const uint32_t* rel, const uint32_t* pa,
-                                                               const uint32_t* ids, uint64_t n, BsrVal* meta,
-                                                               uint32_t* arena) {
+__global__ __launch_bounds__(kBsrBaseThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_bsr_base(
+    const uint32_t* rel, const uint32_t* pa, const uint32_t* ids, uint64_t n, BsrVal* meta, uint32_t* arena) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) bsr_base_entry(rel, *pa, ids, i, meta, arena);
 }
